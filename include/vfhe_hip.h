@@ -1,0 +1,256 @@
+/*
+ * vfhe_hip.h -- C ABI of the `hip` engine for vfhe's RNS polynomial hot path.
+ *
+ * The engine is a drop-in for the reference's arith + mlwe native surface on
+ * the hot path: the functions below keep the reference's names, argument
+ * order, argument meaning and error behaviour (failed preconditions abort, as
+ * the reference's live asserts do; allocation failure exits), and the structs
+ * keep the reference's public field layout so that Python code that casts a
+ * handle and reads `rns_mask`, `coeffs`, `l` or `w` keeps working.
+ *
+ * Memory model.  Every RNS polynomial owns a device buffer of allocated_l
+ * limb rows of N uint64 each (limb-major, 256-byte aligned rows) in HBM.  The
+ * `coeffs[i]` rows of the public struct are host mirror rows.  All compute
+ * entry points run on the device on one in-order HIP stream and return without
+ * waiting; a polynomial's host rows are refreshed only by
+ * polynomial_sync_to_host(), and host writes into `coeffs` become visible to
+ * the device after polynomial_mark_host_dirty().  Those two functions, the
+ * vfhe_hip_* functions and the batched entry points have no counterpart in the
+ * reference: INTEGRATION.md lists the two-line change they need on the Python
+ * side (modules/arith/python/src/vfhe/arith/polynomial.py:437-469).
+ *
+ * Standalone single-limb kernels (ntt_forward, ntt_reverse, mod_eltwise_*)
+ * accept host pointers (staged through the device, synchronous, as the
+ * reference's C tests call them) or device pointers (asynchronous).
+ */
+#ifndef VFHE_HIP_H
+#define VFHE_HIP_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Types (public prefix identical to the reference)                         */
+/* ------------------------------------------------------------------------ */
+
+/* Per-(n, q) NTT context.  Opaque to Python (modules/arith/python/cdef/arith.cdef:8);
+   replaces struct _NTT_proc of modules/arith/c/include/arith.h:29-46.  The
+   engine keeps n, q and the roots first, then its own device tables. */
+typedef struct _NTT_proc *NTT_proc;
+
+/* Prime pool for one (N, split degree): modules/arith/c/include/arith.h:48-54,
+   cdef modules/arith/python/cdef/arith.cdef:10-16.  w[i][k] = psi_i^(2 brv(k)+1). */
+typedef struct _incNTT {
+    NTT_proc *ntt;
+    uint64_t split_degree;
+    uint64_t **w;
+    uint64_t N, l;
+} *incNTT;
+
+/* RNS polynomial: modules/arith/c/include/arith.h:72-87, cdef arith.cdef:18-23.
+   coeffs[i] are host mirror rows (see the memory model above). */
+typedef struct _RNS_Polynomial {
+    uint64_t **coeffs;
+    incNTT ntt;
+    uint64_t rns_mask;
+    uint64_t allocated_l;
+} *RNS_Polynomial;
+typedef RNS_Polynomial RNSc_Polynomial;
+
+/* MLWE sample: modules/mlwe/c/include/mlwe.h:45-55, cdef mlwe.cdef:27-31 */
+typedef struct _RNS_MLWE {
+    RNS_Polynomial *a, b;
+    uint64_t r;
+} *RNS_MLWE;
+typedef RNS_MLWE RNSc_MLWE;
+
+/* MLWE key: modules/mlwe/c/include/mlwe.h:37-43, cdef mlwe.cdef:33-40 */
+typedef struct _IntPolynomial {
+    uint64_t *coeffs;
+    uint64_t N;
+} *IntPolynomial;
+typedef struct _RNS_MLWE_Key {
+    IntPolynomial *s;
+    RNS_Polynomial *s_RNS;
+    uint64_t N, l, r;
+    double sigma;
+} *RNS_MLWE_Key;
+
+/* Fast base conversion parameters: modules/arith/c/include/arith.h:211-221 */
+typedef struct _RNS_BaseConversionParams {
+    uint64_t in_mask;
+    uint64_t out_mask;
+    uint32_t w;
+    uint32_t v;
+    uint32_t *D;
+    uint32_t *P;
+    uint64_t *Dhat;
+    uint64_t **D_mod_p;
+} *RNS_BaseConversionParams;
+
+/* ------------------------------------------------------------------------ */
+/* Number theory (host): modules/arith/c/src/number_theory.c:6-217           */
+/* ------------------------------------------------------------------------ */
+uint64_t power_mod(uint64_t base, uint64_t exp, uint64_t mod);       /* number_theory.c:6-20 */
+uint64_t inverse_mod(uint64_t a, uint64_t m);                        /* number_theory.c:103-109 */
+uint64_t inverse_mod_eea(uint64_t a, uint64_t p);                    /* number_theory.c:22-101 */
+bool is_prime(uint64_t n);                                           /* number_theory.c:111-152 */
+uint64_t generate_Nth_root_of_unity(uint64_t q, uint64_t n);         /* number_theory.c:154-177 */
+uint64_t next_special_prime(uint64_t x, uint64_t n, bool primitive); /* number_theory.c:179-217 */
+void bit_rev(uint64_t *out, uint64_t *in, uint64_t n, uint64_t log_n); /* misc.c:186-192 */
+
+/* ------------------------------------------------------------------------ */
+/* Single-limb kernels: modules/arith/c/include/arith.h:123-156              */
+/* ------------------------------------------------------------------------ */
+NTT_proc ntt_new_proc(uint64_t n, uint64_t q);                       /* ntt.c:265-313 */
+void ntt_free_proc(NTT_proc proc);                                   /* ntt.c:347-352 */
+void ntt_forward(uint64_t *out, uint64_t *in, NTT_proc proc);        /* ntt.c:315-329 */
+void ntt_reverse(uint64_t *out, uint64_t *in, NTT_proc proc);        /* ntt.c:331-345 */
+NTT_proc *new_ntt_list(uint64_t *primes, uint64_t N, uint64_t l);    /* misc.c:118-126 */
+
+uint64_t add_modq(uint64_t a, uint64_t b, uint64_t q);               /* mod.c:54-58 */
+uint64_t sub_modq(uint64_t a, uint64_t b, uint64_t q);               /* mod.c:60 */
+uint64_t negate_modq(uint64_t a, uint64_t q);                        /* mod.c:62 */
+uint64_t mul_modq(uint64_t a, uint64_t b, NTT_proc proc);            /* mod.c:64-67 */
+
+void mod_eltwise_mul(uint64_t *out, uint64_t *in1, uint64_t *in2, uint64_t n, NTT_proc proc);       /* mod.c:69-83 */
+void mod_eltwise_mul_addto(uint64_t *out, uint64_t *in1, uint64_t *in2, uint64_t n, NTT_proc proc); /* mod.c:85-99 */
+void mod_eltwise_mul_subto(uint64_t *out, uint64_t *in1, uint64_t *in2, uint64_t n, NTT_proc proc); /* mod.c:101-115 */
+void mod_eltwise_scale(uint64_t *out, uint64_t *in, uint64_t scale, uint64_t n, NTT_proc proc);     /* mod.c:117-131 */
+void mod_eltwise_fma(uint64_t *out, uint64_t *in, uint64_t scale, uint64_t n, NTT_proc proc);       /* mod.c:133-147 */
+void mod_eltwise_add_scalar(uint64_t *out, uint64_t *in, uint64_t scalar, uint64_t n, NTT_proc proc); /* mod.c:149-163 */
+void mod_eltwise_sub_scalar(uint64_t *out, uint64_t *in, uint64_t scalar, uint64_t n, NTT_proc proc); /* mod.c:165-179 */
+void mod_eltwise_negate(uint64_t *out, uint64_t *in, uint64_t n, NTT_proc proc);                    /* mod.c:181-195 */
+void mod_eltwise_add(uint64_t *out, uint64_t *in1, uint64_t *in2, uint64_t n, NTT_proc proc);       /* mod.c:197-211 */
+void mod_eltwise_sub(uint64_t *out, uint64_t *in1, uint64_t *in2, uint64_t n, NTT_proc proc);       /* mod.c:213-227 */
+void mod_eltwise_reduce(uint64_t *out, uint64_t *in, uint64_t n, NTT_proc proc);                    /* mod.c:229-243 */
+void mod_eltwise_reduce_signed(uint64_t *out, int64_t *in, uint64_t n, NTT_proc proc);              /* mod.c:245-259 */
+void mod_reduce_array_mp(uint64_t *out, uint64_t *in_high, uint64_t *in_low, uint64_t n, NTT_proc proc); /* mod.c:261-276 */
+
+/* ------------------------------------------------------------------------ */
+/* RNS pool and polynomials: modules/arith/c/src/rns_polynomial.c            */
+/* ------------------------------------------------------------------------ */
+incNTT new_incomplete_ntt_list(uint64_t *primes, uint64_t split_degree, uint64_t N, uint64_t l); /* :7-31 */
+void incNTT_extend_with_primes(incNTT ntt, uint64_t *new_primes, uint64_t count);               /* :33-61 */
+uint64_t **incNTT_get_rou_matrix(incNTT ntt);                                                  /* :63 */
+
+RNS_Polynomial polynomial_new_RNS_polynomial(uint64_t N, uint64_t rns_mask, incNTT ntt);        /* :65-78 */
+RNS_Polynomial *polynomial_new_RNS_polynomial_array(uint64_t size, uint64_t N, uint64_t rns_mask, incNTT ntt); /* :80-90 */
+RNS_Polynomial *polynomial_new_array_of_RNS_polynomials(uint64_t N, uint64_t rns_mask, uint64_t size, incNTT ntt); /* :174-181 */
+void free_RNS_polynomial(void *p);                                                              /* :154-163 */
+void free_RNS_polynomial_array(uint64_t size, RNS_Polynomial *p);                               /* :165-172 */
+void polynomial_copy_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in);                     /* :126-136 */
+void polynomial_copy_RNSc_polynomial(RNSc_Polynomial out, RNSc_Polynomial in);                  /* :138-141 */
+void polynomial_RNS_zero(RNS_Polynomial p);                                                     /* :143-152 */
+bool polynomial_eq(RNS_Polynomial a, RNS_Polynomial b);                                         /* :92-123 */
+void int_array_to_RNS(RNS_Polynomial out, uint64_t *in);                                        /* :204-223 */
+void array_to_RNS(RNS_Polynomial out, uint64_t **in);                                           /* :225-240 */
+
+void polynomial_RNSc_to_RNS(RNS_Polynomial out, RNSc_Polynomial in);                            /* :582-597 */
+void polynomial_RNS_to_RNSc(RNSc_Polynomial out, RNS_Polynomial in);                            /* :599-614 */
+
+void polynomial_mul_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in1, RNS_Polynomial in2);       /* :311-356 */
+void polynomial_mul_addto_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in1, RNS_Polynomial in2); /* :358-396 */
+void polynomial_mul_subto_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in1, RNS_Polynomial in2); /* :398-435 */
+void polynomial_multo_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in);                          /* :273-309 */
+void polynomial_add_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in1, RNS_Polynomial in2);       /* :468-472 */
+void polynomial_add_RNSc_polynomial(RNSc_Polynomial out, RNSc_Polynomial in1, RNSc_Polynomial in2);   /* :455-466 */
+void polynomial_sub_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in1, RNS_Polynomial in2);       /* :437-448 */
+void polynomial_sub_RNSc_polynomial(RNSc_Polynomial out, RNSc_Polynomial in1, RNSc_Polynomial in2);   /* :450-453 */
+void polynomial_RNSc_negate(RNSc_Polynomial out, RNSc_Polynomial in);                                 /* :565-575 */
+void polynomial_RNS_negate(RNS_Polynomial out, RNS_Polynomial in);                                    /* :577-580 */
+void polynomial_scale_RNSc_polynomial(RNSc_Polynomial out, RNSc_Polynomial in1, uint64_t scale);      /* :516-526 */
+void polynomial_scale_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in1, uint64_t scale);         /* :547-550 */
+void polynomial_scale_RNS_polynomial_RNS(RNS_Polynomial out, RNS_Polynomial in1, uint64_t *scale);    /* :552-563 */
+void polynomial_scale_addto_RNSc_polynomial(RNSc_Polynomial out, RNSc_Polynomial in1, uint64_t scale); /* :528-540 */
+void polynomial_scale_addto_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in1, uint64_t scale);   /* :542-545 */
+void polynomial_RNSc_add_integer(RNSc_Polynomial out, RNSc_Polynomial in1, uint64_t in2);              /* :474-488 */
+void polynomial_RNS_add_integer(RNS_Polynomial out, RNS_Polynomial in1, uint64_t in2);                /* :490-514 */
+
+void polynomial_RNSc_mod_reduce_lifted(RNSc_Polynomial out, RNSc_Polynomial in, uint64_t idx);        /* :824-843 (ModUp digit) */
+void polynomial_RNSc_mod_reduce(RNSc_Polynomial out, RNSc_Polynomial in);                             /* :845-854 */
+void polynomial_round_division_RNSc_wo_free(RNSc_Polynomial out, uint64_t divide_mask);               /* :939-976 (rescale, ModDown) */
+void polynomial_floor_division_RNSc_wo_free(RNSc_Polynomial out, uint64_t divide_mask);               /* :905-937 */
+void polynomial_round_division_RNSc(RNSc_Polynomial out);                                             /* :987-994 */
+void polynomial_floor_division_RNSc(RNSc_Polynomial out);                                             /* :978-985 */
+
+RNS_BaseConversionParams init_base_conversion_params(incNTT ntt, uint64_t in_mask, uint64_t out_mask); /* :638-715 */
+void free_base_conversion_params(RNS_BaseConversionParams params);                                    /* :717-736 */
+void polynomial_base_conversion_RNSc(RNSc_Polynomial out, RNSc_Polynomial in, RNS_BaseConversionParams params); /* :738-822 */
+void rns_compute_scaling_factors(uint64_t *delta_out, incNTT ntt, uint64_t in_mask, uint64_t out_mask); /* :1463-1494 */
+void polynomial_RNSc_scaled_lift(RNSc_Polynomial out, RNSc_Polynomial in, uint64_t *delta);           /* :1497-1525 */
+void polynomial_RNSc_permute(RNSc_Polynomial out, RNSc_Polynomial in, uint64_t gen);                  /* :996-1035 */
+
+/* ------------------------------------------------------------------------ */
+/* MLWE: modules/mlwe/c/src/mlwe.c, gadget.c                                 */
+/* ------------------------------------------------------------------------ */
+RNS_MLWE mlwe_alloc_RNS_sample(uint64_t N, uint64_t r, uint64_t mask, incNTT ntt);                    /* mlwe.c:98-106 */
+RNSc_MLWE mlwe_alloc_RNSc_sample(uint64_t N, uint64_t r, uint64_t mask, incNTT ntt);                  /* mlwe.c:108-111 */
+RNS_MLWE *mlwe_alloc_RNS_sample_array(uint64_t size, uint64_t N, uint64_t r, uint64_t mask, incNTT ntt); /* mlwe.c:129-139 */
+RNS_MLWE *mlwe_create_copy_array(RNS_MLWE *in, uint64_t size);                                        /* mlwe.c:121-127 */
+void free_mlwe_RNS_sample(void *p);                                                                   /* mlwe.c:182-188 */
+void free_RNS_mlwe_array(uint64_t size, RNS_MLWE *v);                                                 /* mlwe.c:152-159 */
+void mlwe_copy_RNS_sample(RNS_MLWE out, RNS_MLWE in);                                                 /* mlwe.c:168-175 */
+void mlwe_RNS_trivial_sample_of_zero(RNS_MLWE out);                                                   /* mlwe.c:302-312 */
+void mlwe_RNSc_to_RNS(RNS_MLWE out, RNSc_MLWE in);                                                    /* mlwe.c:513-520 */
+void mlwe_RNS_to_RNSc(RNSc_MLWE out, RNS_MLWE in);                                                    /* mlwe.c:522-529 */
+void mlwe_RNS_mul_by_poly(RNS_MLWE out, RNS_MLWE in, RNS_Polynomial poly);                            /* mlwe.c:254-261 */
+void mlwe_RNS_mul_addto_by_poly(RNS_MLWE out, RNS_MLWE in, RNS_Polynomial poly);                      /* mlwe.c:263-270 */
+void mlwe_RNS_mul_subto_by_poly(RNS_MLWE out, RNS_MLWE in, RNS_Polynomial poly);                      /* mlwe.c:272-279 */
+void mlwe_add_RNSc_sample(RNSc_MLWE out, RNSc_MLWE in1, RNSc_MLWE in2);                               /* mlwe.c:446-453 */
+void mlwe_sub_RNSc_sample(RNSc_MLWE out, RNSc_MLWE in1, RNSc_MLWE in2);                               /* mlwe.c:484-491 */
+void mlwe_scale_RNS_mlwe_RNS(RNS_MLWE c, uint64_t *scale);                                            /* mlwe.c:217-225 */
+void mlwe_round_division_RNSc(RNSc_MLWE out, uint64_t divide_mask);                                   /* mlwe.c:762-772 */
+uint64_t mlwe_extended_rank(uint64_t r);                                                              /* mlwe.c:659-663 */
+void mlwe_tensor_product(RNS_Polynomial *out, RNS_MLWE in1, RNS_MLWE in2);                            /* mlwe.c:665-709 */
+void mlwe_RNSc_GHS_hybrid_keyswitch(RNSc_MLWE out, RNSc_MLWE in, RNS_MLWE **ksk, uint64_t lvl);       /* mlwe.c:557-598 + gadget.c:22-37 */
+void mlwe_multiply(RNS_MLWE out, RNS_MLWE in1, RNS_MLWE in2, RNS_MLWE **ksk);                         /* mlwe.c:711-760 */
+void mlwe_automorphism_RNSc_GHS(RNSc_MLWE out, RNSc_MLWE in, uint64_t gen, RNS_MLWE **ksk, uint64_t lvl); /* mlwe.c:314-326 */
+RNS_MLWE_Key mlwe_new_RNS_key_from_array(uint64_t *array, uint64_t N, uint64_t r, uint64_t l, incNTT ntt, double sigma); /* mlwe.c:28-38 */
+void free_mlwe_RNS_key(RNS_MLWE_Key key);                                                             /* mlwe.c:21-26 */
+void mlwe_RNS_phase(RNS_Polynomial out, RNS_MLWE in, RNS_MLWE_Key key);                               /* mlwe.c:243-252 */
+
+/* ------------------------------------------------------------------------ */
+/* Engine identity: modules/misc/python/cdef/misc.cdef:8, vfhe_cpu.cdef:6     */
+/* ------------------------------------------------------------------------ */
+const char *vfhe_engine_active(void);        /* returns "hip" */
+int vfhe_cpu_supports(const char *capability); /* "hip": a gfx950 device is visible */
+
+/* ------------------------------------------------------------------------ */
+/* Engine extensions (no counterpart in the reference)                       */
+/* ------------------------------------------------------------------------ */
+
+/* Residency: refresh the host rows from the device (waits), or declare that the
+   host rows were written and must be uploaded before the next device use. */
+void polynomial_sync_to_host(RNS_Polynomial p);
+void polynomial_mark_host_dirty(RNS_Polynomial p);
+/* Raw device address of limb row `limb` (for zero-copy interop, e.g. DLPack). */
+uint64_t *polynomial_device_row(RNS_Polynomial p, uint64_t limb);
+
+/* Runtime: wait for all queued device work; return the last HIP error code
+   (0 when none) and clear it; device name. */
+void vfhe_hip_synchronize(void);
+int vfhe_hip_last_error(void);
+const char *vfhe_hip_device_name(void);
+
+/* Batched hot path: one launch sequence for many polynomials.
+   direction 0 = forward (RNSc -> RNS), 1 = inverse. polys[i] is transformed in place. */
+void vfhe_hip_ntt_batch(RNS_Polynomial *polys, uint64_t count, int direction);
+/* z[i] = INTT(NTT(x[i]) * NTT(y[i])) on the active limbs of x[i] & y[i]
+   (coefficient form in, coefficient form out); x and y are left unchanged. */
+void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polynomial *y, uint64_t count);
+/* CKKS product for `count` ciphertext pairs sharing one relinearization key:
+   out[i] = rescale(relinearize(in1[i] * in2[i])) when divide_mask != 0 (outputs
+   in coefficient form on the rescaled mask), or mlwe_multiply semantics (NTT form)
+   when divide_mask == 0. */
+void vfhe_hip_mlwe_multiply_batch(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLWE *in2, uint64_t count,
+                                  RNS_MLWE **rlk, uint64_t divide_mask);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VFHE_HIP_H */

@@ -1,0 +1,148 @@
+"""GPU parity: tensor product, GHS key-switch, relinearized multiply, CKKS rescale
+and automorphism of the hip engine against the oracle, bit-exact.
+
+Key-switch keys here are uniform random NTT-form samples: bit-exactness of the
+arithmetic does not depend on the key being a valid encryption (decryption
+correctness with real keys is covered by tests/test_gpu_ckks.py).  The pool
+layout follows modules/mlwe/python/src/vfhe/mlwe/mlwe.py:57-73: level-0 primes
+first, the special primes on top.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vfhe_amd import _native, runtime as rt
+
+pytestmark = pytest.mark.gpu
+
+
+class Setup:
+    def __init__(self, N, L, K, seed, split=1):
+        self.N, self.L, self.K = N, L, K
+        self.primes = O.gen_primes(N, [50] * (L + K), split)
+        self.pool = rt.Pool(N, self.primes, split)
+        self.octx = O.Ctx(N, self.primes, split)
+        self.rng = np.random.default_rng(seed)
+        self.lvl_mask = (1 << L) - 1
+        self.ext_mask = (1 << (L + K)) - 1
+        self.keep = []
+
+    def rows(self, mask):
+        return rt.random_rows(self.rng, self.primes, self.N, mask)
+
+    def ct(self, r, mask):
+        data = np.stack([self.rows(mask) for _ in range(r + 1)])
+        return data, self.pool.sample(r, mask, data)
+
+    def key(self, ndig, r):
+        """One key-switch key: ndig entries of (r+1) NTT-form polys on the extended mask."""
+        data = np.stack([np.stack([self.rows(self.ext_mask) for _ in range(r + 1)]) for _ in range(ndig)])
+        entries = [self.pool.sample(r, self.ext_mask, data[d]) for d in range(ndig)]
+        arr = _native.voidp_array(entries)
+        self.keep += [entries, arr]
+        return data, arr
+
+    def outer(self, keys):
+        arr = _native.voidp_array([k[1] if k is not None else None for k in keys])
+        self.keep.append(arr)
+        return arr
+
+    def check(self, got, ref, mask):
+        for c in range(ref.shape[0]):
+            for i in range(ref.shape[1]):
+                if (mask >> i) & 1:
+                    assert np.array_equal(got[c][i], ref[c][i]), f"component {c} limb {i}"
+
+
+@pytest.mark.parametrize("r", [1, 2])
+def test_tensor_product(lib, r):
+    s = Setup(4096, 4, 0, 11)
+    m = s.lvl_mask
+    d1, c1 = s.ct(r, m)
+    d2, c2 = s.ct(r, m)
+    R = int(lib.mlwe_extended_rank(r))
+    outs = [s.pool.poly() for _ in range(R + 1)]
+    lib.mlwe_tensor_product(_native.voidp_array(outs), c1, c2)
+    ref = np.zeros((R + 1, s.L, s.N), dtype=np.uint64)
+    mref = s.octx.tensor_product(ref, d1, d2, r, m)
+    got = np.stack([s.pool.read(p) for p in outs])
+    s.check(got, ref, mref)
+
+
+@pytest.mark.parametrize("N,L,K,r,relin", [(4096, 4, 1, 1, True), (4096, 3, 2, 1, False),
+                                            (2048, 3, 2, 2, True), (1024, 2, 1, 1, False)])
+def test_ghs_keyswitch(lib, N, L, K, r, relin):
+    s = Setup(N, L, K, 12 + N + r)
+    Rin = int(lib.mlwe_extended_rank(r)) if relin else r
+    nkeys = Rin - r if relin else Rin
+    keys = [s.key(L, r) for _ in range(nkeys)] + ([None] * r if relin else [])
+    din, cin = s.ct(Rin, s.lvl_mask)
+    out = s.pool.sample(r, s.ext_mask)
+    lib.mlwe_RNSc_GHS_hybrid_keyswitch(out, cin, s.outer(keys), 0)
+    ref = np.zeros((r + 1, s.L + s.K, s.N), dtype=np.uint64)
+    mref = s.octx.ghs_keyswitch(ref, r, din, Rin, s.lvl_mask, [k[0] if k else None for k in keys], s.ext_mask)
+    assert s.pool.sample_masks(out) == [mref] * (r + 1)
+    s.check(s.pool.read_sample(out), ref, mref)
+
+
+def test_multiply_and_batched_rescale(lib):
+    """mlwe_multiply (mlwe.c:711-760) and the fused multiply+relinearize+rescale batch."""
+    s = Setup(4096, 4, 1, 21)
+    r, m = 1, s.lvl_mask
+    rlk = [s.key(s.L, r), None]
+    rlk_arr = s.outer(rlk)
+    rlk_np = [rlk[0][0], None]
+    cts = [(s.ct(r, m), s.ct(r, m)) for _ in range(3)]
+    top = 1 << (s.L - 1)
+    outs_single, outs_batch = [], []
+    for (d1, c1), (d2, c2) in cts:
+        o = s.pool.sample(r, s.ext_mask)
+        lib.mlwe_multiply(o, c1, c2, rlk_arr)
+        outs_single.append(o)
+        outs_batch.append(s.pool.sample(r, s.ext_mask))
+    lib.vfhe_hip_mlwe_multiply_batch(_native.voidp_array(outs_batch), _native.voidp_array([c[0][1] for c in cts]),
+                                     _native.voidp_array([c[1][1] for c in cts]), len(cts), rlk_arr, top)
+    for k, ((d1, c1), (d2, c2)) in enumerate(cts):
+        ref = np.zeros((r + 1, s.L + s.K, s.N), dtype=np.uint64)
+        mref = s.octx.multiply(ref, d1, d2, r, m, rlk_np, s.ext_mask)
+        assert mref == m
+        s.check(s.pool.read_sample(outs_single[k]), ref, mref)
+        # CKKS rescale (ckks.py:128-148): to coefficient form, round-divide by the top prime
+        for c in range(r + 1):
+            s.octx.to_coeff(ref[c], ref[c], mref)
+        m2 = s.octx.mlwe_round_division(ref, r, mref, top)
+        assert s.pool.sample_masks(outs_batch[k]) == [m2] * (r + 1)
+        s.check(s.pool.read_sample(outs_batch[k]), ref, m2)
+
+
+def test_automorphism(lib):
+    s = Setup(2048, 3, 1, 31)
+    r = 1
+    keys = [s.key(s.L, r)]
+    din, cin = s.ct(r, s.lvl_mask)
+    out = s.pool.sample(r, s.ext_mask)
+    gen = pow(5, 3, 2 * s.N)
+    lib.mlwe_automorphism_RNSc_GHS(out, cin, gen, s.outer(keys), 0)
+    ref = np.zeros((r + 1, s.L + s.K, s.N), dtype=np.uint64)
+    mref = s.octx.automorphism(ref, din, r, s.lvl_mask, gen, [keys[0][0]], s.ext_mask)
+    s.check(s.pool.read_sample(out), ref, mref)
+
+
+@pytest.mark.slow
+def test_ckks_config3_shape_multiply_rescale(lib):
+    """BASELINE config 3 shape: N = 2^16, L = 24 level limbs + K = 4 special limbs, rank 1."""
+    s = Setup(65536, 24, 4, 41)
+    r, m = 1, s.lvl_mask
+    rlk = [s.key(s.L, r), None]
+    rlk_arr = s.outer(rlk)
+    (d1, c1), (d2, c2) = s.ct(r, m), s.ct(r, m)
+    top = 1 << (s.L - 1)
+    o = s.pool.sample(r, s.ext_mask)
+    lib.vfhe_hip_mlwe_multiply_batch(_native.voidp_array([o]), _native.voidp_array([c1]), _native.voidp_array([c2]),
+                                     1, rlk_arr, top)
+    ref = np.zeros((r + 1, s.L + s.K, s.N), dtype=np.uint64)
+    mref = s.octx.multiply(ref, d1, d2, r, m, [rlk[0][0], None], s.ext_mask)
+    for c in range(r + 1):
+        s.octx.to_coeff(ref[c], ref[c], mref)
+    m2 = s.octx.mlwe_round_division(ref, r, mref, top)
+    s.check(s.pool.read_sample(o), ref, m2)

@@ -1,0 +1,192 @@
+"""ctypes binding of the hip engine's C ABI (include/vfhe_hip.h).
+
+This plays the role of the reference's cffi extension `_vfhe_native_<engine>`
+(modules/misc/python/src/_vfhe_native.py:29-56): it loads the engine library
+and declares every entry point with the argument types of the cdefs
+(modules/arith/python/cdef/arith.cdef, modules/mlwe/python/cdef/mlwe.cdef).
+Handles are passed as `void *`, as there.
+
+The library must have been built (`__graft_entry__.build()` or
+`make -C vfhe_amd/csrc`); loading fails loudly otherwise, there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvfhe_hip.so")
+
+U64 = ctypes.c_uint64
+U64P = ctypes.POINTER(ctypes.c_uint64)
+VP = ctypes.c_void_p
+VPP = ctypes.POINTER(ctypes.c_void_p)
+I32 = ctypes.c_int
+
+
+class IncNTT(ctypes.Structure):
+    """struct _incNTT (modules/arith/python/cdef/arith.cdef:10-16)."""
+    _fields_ = [("ntt", VPP), ("split_degree", U64), ("w", ctypes.POINTER(U64P)), ("N", U64), ("l", U64)]
+
+
+class RNSPolynomial(ctypes.Structure):
+    """struct _RNS_Polynomial (modules/arith/python/cdef/arith.cdef:18-23)."""
+    _fields_ = [("coeffs", ctypes.POINTER(U64P)), ("ntt", ctypes.POINTER(IncNTT)), ("rns_mask", U64),
+                ("allocated_l", U64)]
+
+
+class RNSMLWE(ctypes.Structure):
+    """struct _RNS_MLWE (modules/mlwe/python/cdef/mlwe.cdef:27-31)."""
+    _fields_ = [("a", VPP), ("b", VP), ("r", U64)]
+
+
+# name: (restype, argtypes)
+_SIGNATURES = {
+    # number theory / misc
+    "power_mod": (U64, [U64, U64, U64]),
+    "inverse_mod": (U64, [U64, U64]),
+    "inverse_mod_eea": (U64, [U64, U64]),
+    "is_prime": (ctypes.c_bool, [U64]),
+    "generate_Nth_root_of_unity": (U64, [U64, U64]),
+    "next_special_prime": (U64, [U64, U64, ctypes.c_bool]),
+    "bit_rev": (None, [U64P, U64P, U64, U64]),
+    "vfhe_engine_active": (ctypes.c_char_p, []),
+    "vfhe_cpu_supports": (I32, [ctypes.c_char_p]),
+    # single-limb kernels
+    "ntt_new_proc": (VP, [U64, U64]),
+    "ntt_free_proc": (None, [VP]),
+    "ntt_forward": (None, [VP, VP, VP]),
+    "ntt_reverse": (None, [VP, VP, VP]),
+    "new_ntt_list": (VPP, [U64P, U64, U64]),
+    "add_modq": (U64, [U64, U64, U64]),
+    "sub_modq": (U64, [U64, U64, U64]),
+    "negate_modq": (U64, [U64, U64]),
+    "mul_modq": (U64, [U64, U64, VP]),
+    "mod_eltwise_mul": (None, [VP, VP, VP, U64, VP]),
+    "mod_eltwise_mul_addto": (None, [VP, VP, VP, U64, VP]),
+    "mod_eltwise_mul_subto": (None, [VP, VP, VP, U64, VP]),
+    "mod_eltwise_scale": (None, [VP, VP, U64, U64, VP]),
+    "mod_eltwise_fma": (None, [VP, VP, U64, U64, VP]),
+    "mod_eltwise_add_scalar": (None, [VP, VP, U64, U64, VP]),
+    "mod_eltwise_sub_scalar": (None, [VP, VP, U64, U64, VP]),
+    "mod_eltwise_negate": (None, [VP, VP, U64, VP]),
+    "mod_eltwise_add": (None, [VP, VP, VP, U64, VP]),
+    "mod_eltwise_sub": (None, [VP, VP, VP, U64, VP]),
+    "mod_eltwise_reduce": (None, [VP, VP, U64, VP]),
+    "mod_eltwise_reduce_signed": (None, [VP, VP, U64, VP]),
+    "mod_reduce_array_mp": (None, [VP, VP, VP, U64, VP]),
+    # pool and polynomials
+    "new_incomplete_ntt_list": (VP, [U64P, U64, U64, U64]),
+    "incNTT_extend_with_primes": (None, [VP, U64P, U64]),
+    "incNTT_get_rou_matrix": (ctypes.POINTER(U64P), [VP]),
+    "polynomial_new_RNS_polynomial": (VP, [U64, U64, VP]),
+    "polynomial_new_RNS_polynomial_array": (VPP, [U64, U64, U64, VP]),
+    "free_RNS_polynomial": (None, [VP]),
+    "free_RNS_polynomial_array": (None, [U64, VPP]),
+    "polynomial_copy_RNS_polynomial": (None, [VP, VP]),
+    "polynomial_copy_RNSc_polynomial": (None, [VP, VP]),
+    "polynomial_RNS_zero": (None, [VP]),
+    "polynomial_eq": (ctypes.c_bool, [VP, VP]),
+    "int_array_to_RNS": (None, [VP, U64P]),
+    "array_to_RNS": (None, [VP, ctypes.POINTER(U64P)]),
+    "polynomial_RNSc_to_RNS": (None, [VP, VP]),
+    "polynomial_RNS_to_RNSc": (None, [VP, VP]),
+    "polynomial_mul_RNS_polynomial": (None, [VP, VP, VP]),
+    "polynomial_mul_addto_RNS_polynomial": (None, [VP, VP, VP]),
+    "polynomial_mul_subto_RNS_polynomial": (None, [VP, VP, VP]),
+    "polynomial_multo_RNS_polynomial": (None, [VP, VP]),
+    "polynomial_add_RNS_polynomial": (None, [VP, VP, VP]),
+    "polynomial_add_RNSc_polynomial": (None, [VP, VP, VP]),
+    "polynomial_sub_RNS_polynomial": (None, [VP, VP, VP]),
+    "polynomial_sub_RNSc_polynomial": (None, [VP, VP, VP]),
+    "polynomial_RNSc_negate": (None, [VP, VP]),
+    "polynomial_RNS_negate": (None, [VP, VP]),
+    "polynomial_scale_RNSc_polynomial": (None, [VP, VP, U64]),
+    "polynomial_scale_RNS_polynomial": (None, [VP, VP, U64]),
+    "polynomial_scale_RNS_polynomial_RNS": (None, [VP, VP, U64P]),
+    "polynomial_scale_addto_RNSc_polynomial": (None, [VP, VP, U64]),
+    "polynomial_scale_addto_RNS_polynomial": (None, [VP, VP, U64]),
+    "polynomial_RNSc_add_integer": (None, [VP, VP, U64]),
+    "polynomial_RNS_add_integer": (None, [VP, VP, U64]),
+    "polynomial_RNSc_mod_reduce_lifted": (None, [VP, VP, U64]),
+    "polynomial_RNSc_mod_reduce": (None, [VP, VP]),
+    "polynomial_round_division_RNSc_wo_free": (None, [VP, U64]),
+    "polynomial_floor_division_RNSc_wo_free": (None, [VP, U64]),
+    "polynomial_round_division_RNSc": (None, [VP]),
+    "polynomial_floor_division_RNSc": (None, [VP]),
+    "init_base_conversion_params": (VP, [VP, U64, U64]),
+    "free_base_conversion_params": (None, [VP]),
+    "polynomial_base_conversion_RNSc": (None, [VP, VP, VP]),
+    "rns_compute_scaling_factors": (None, [U64P, VP, U64, U64]),
+    "polynomial_RNSc_scaled_lift": (None, [VP, VP, U64P]),
+    "polynomial_RNSc_permute": (None, [VP, VP, U64]),
+    # mlwe
+    "mlwe_alloc_RNS_sample": (VP, [U64, U64, U64, VP]),
+    "mlwe_alloc_RNSc_sample": (VP, [U64, U64, U64, VP]),
+    "mlwe_alloc_RNS_sample_array": (VPP, [U64, U64, U64, U64, VP]),
+    "mlwe_create_copy_array": (VPP, [VPP, U64]),
+    "free_mlwe_RNS_sample": (None, [VP]),
+    "free_RNS_mlwe_array": (None, [U64, VPP]),
+    "mlwe_copy_RNS_sample": (None, [VP, VP]),
+    "mlwe_RNS_trivial_sample_of_zero": (None, [VP]),
+    "mlwe_RNSc_to_RNS": (None, [VP, VP]),
+    "mlwe_RNS_to_RNSc": (None, [VP, VP]),
+    "mlwe_RNS_mul_by_poly": (None, [VP, VP, VP]),
+    "mlwe_RNS_mul_addto_by_poly": (None, [VP, VP, VP]),
+    "mlwe_RNS_mul_subto_by_poly": (None, [VP, VP, VP]),
+    "mlwe_add_RNSc_sample": (None, [VP, VP, VP]),
+    "mlwe_sub_RNSc_sample": (None, [VP, VP, VP]),
+    "mlwe_scale_RNS_mlwe_RNS": (None, [VP, U64P]),
+    "mlwe_round_division_RNSc": (None, [VP, U64]),
+    "mlwe_extended_rank": (U64, [U64]),
+    "mlwe_tensor_product": (None, [VPP, VP, VP]),
+    "mlwe_RNSc_GHS_hybrid_keyswitch": (None, [VP, VP, VPP, U64]),
+    "mlwe_multiply": (None, [VP, VP, VP, VPP]),
+    "mlwe_automorphism_RNSc_GHS": (None, [VP, VP, U64, VPP, U64]),
+    "mlwe_new_RNS_key_from_array": (VP, [U64P, U64, U64, U64, VP, ctypes.c_double]),
+    "free_mlwe_RNS_key": (None, [VP]),
+    "mlwe_RNS_phase": (None, [VP, VP, VP]),
+    # engine extensions
+    "polynomial_sync_to_host": (None, [VP]),
+    "polynomial_mark_host_dirty": (None, [VP]),
+    "polynomial_device_row": (U64P, [VP, U64]),
+    "vfhe_hip_synchronize": (None, []),
+    "vfhe_hip_last_error": (I32, []),
+    "vfhe_hip_device_name": (ctypes.c_char_p, []),
+    "vfhe_hip_ntt_batch": (None, [VPP, U64, I32]),
+    "vfhe_hip_ntt_mul_intt_batch": (None, [VPP, VPP, VPP, U64]),
+    "vfhe_hip_mlwe_multiply_batch": (None, [VPP, VPP, VPP, U64, VPP, U64]),
+}
+
+_lib = None
+
+
+def symbols() -> list[str]:
+    """Every entry point this binding declares (all are exported by the library)."""
+    return list(_SIGNATURES)
+
+
+def load():
+    """Load the engine library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"hip engine not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def voidp_array(items) -> ctypes.Array:
+    """A `void *[]` of handles (ints, c_void_p, or nested ctypes arrays); None -> NULL."""
+    arr = (VP * max(1, len(items)))()
+    for i, x in enumerate(items):
+        if isinstance(x, ctypes.Array):
+            x = ctypes.cast(x, VP).value
+        arr[i] = x
+    return arr

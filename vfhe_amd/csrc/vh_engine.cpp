@@ -1,0 +1,1804 @@
+// vh_engine.cpp -- host runtime of the hip engine.
+//
+// Implements the reference's C ABI (include/vfhe_hip.h cites each function's
+// reference counterpart) over device-resident polynomials.  Host code only
+// builds work lists and constant tables; all coefficient arithmetic runs in
+// the kernels of vh_kernels.hip, enqueued on one in-order HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/vfhe_hip.h"
+#include "vh_kernels.h"
+
+typedef unsigned __int128 u128;
+
+// ---------------------------------------------------------------------------
+// Errors: the reference aborts on a failed assert (live in release builds,
+// meson.build:17) and exits on allocation failure (misc_tp.c:37-46).
+// ---------------------------------------------------------------------------
+[[noreturn]] static void vh_die(const char *what, const char *file, int line)
+{
+    fprintf(stderr, "vfhe hip engine: %s (%s:%d)\n", what, file, line);
+    fflush(stderr);
+    abort();
+}
+[[noreturn]] static void vh_hip_fail(hipError_t e, const char *call, const char *file, int line)
+{
+    fprintf(stderr, "vfhe hip engine: %s failed: %s (%s:%d)\n", call, hipGetErrorString(e), file, line);
+    fflush(stderr);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) exit(EXIT_FAILURE);
+    abort();
+}
+#define VH_CHECK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) vh_hip_fail(e_, #x, __FILE__, __LINE__); } while (0)
+#define VH_REQUIRE(c) do { if (!(c)) vh_die("precondition failed: " #c, __FILE__, __LINE__); } while (0)
+#define ACTIVE(mask, i) ((((mask) >> (i)) & 1ULL) != 0)
+
+// ---------------------------------------------------------------------------
+// Engine-private tails of the public structs
+// ---------------------------------------------------------------------------
+struct _NTT_proc {
+    uint64_t n, q, root_of_unity, inv_root_of_unity;  // as the reference's first fields
+    vh_limb h;          // host copy; its table pointers are device addresses
+    vh_limb *d_self;    // device copy (a one-entry limb table)
+    void *d_tab;        // device tables
+};
+
+struct vh_pool {
+    struct _incNTT pub;
+    vh_limb *d_limbs;                // device limb table, one entry per pool prime
+    std::vector<vh_limb> h_limbs;    // host copy
+    std::vector<uint64_t *> d_w;     // device w rows (split_degree > 1 only)
+    uint64_t n, logn;                // transform size N/split and its log
+};
+
+struct vh_poly {
+    struct _RNS_Polynomial pub;
+    uint64_t *d;        // device rows [allocated_l][N]
+    uint64_t *hblock;   // host mirror block behind coeffs[]
+    int host_dirty;
+};
+
+static inline vh_poly *VP(RNS_Polynomial p) { return reinterpret_cast<vh_poly *>(p); }
+static inline vh_pool *VPOOL(incNTT t) { return reinterpret_cast<vh_pool *>(t); }
+
+// ---------------------------------------------------------------------------
+// Runtime context: one device, one stream, a staging ring for work lists.
+// ---------------------------------------------------------------------------
+struct vh_ctx {
+    bool ready = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    char *h_ring = nullptr, *d_ring = nullptr;
+    size_t ring_size = 0, ring_off = 0;
+    int cur_half = 0;
+    hipEvent_t half_ev[2];
+    bool half_used[2] = {false, false};
+    char name[128] = {0};
+};
+static vh_ctx C;
+static std::recursive_mutex vh_mu;
+
+static void vh_ensure()
+{
+    if (C.ready) return;
+    int n = 0;
+    VH_CHECK(hipGetDeviceCount(&n));
+    VH_REQUIRE(n > 0);
+    VH_CHECK(hipGetDevice(&C.device));
+    hipDeviceProp_t prop;
+    VH_CHECK(hipGetDeviceProperties(&prop, C.device));
+    snprintf(C.name, sizeof(C.name), "%s", prop.gcnArchName);
+    VH_CHECK(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, C.device) == hipSuccess) {
+        uint64_t thr = UINT64_MAX;
+        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    (void)hipGetLastError();
+    C.ring_size = (size_t)64 << 20;
+    VH_CHECK(hipHostMalloc((void **)&C.h_ring, C.ring_size, hipHostMallocDefault));
+    VH_CHECK(hipMalloc((void **)&C.d_ring, C.ring_size));
+    VH_CHECK(hipEventCreateWithFlags(&C.half_ev[0], hipEventDisableTiming));
+    VH_CHECK(hipEventCreateWithFlags(&C.half_ev[1], hipEventDisableTiming));
+    C.ready = true;
+}
+
+#define VH_ENTER std::lock_guard<std::recursive_mutex> vh_lock_(vh_mu); vh_ensure()
+
+// Copy a small host array to device memory, ordered on the stream.
+static void *vh_stage(const void *src, size_t bytes)
+{
+    const size_t need = (bytes + 255) & ~(size_t)255, half = C.ring_size / 2;
+    VH_REQUIRE(need <= half);
+    if (C.ring_off + need > (size_t)(C.cur_half + 1) * half) {
+        VH_CHECK(hipEventRecord(C.half_ev[C.cur_half], C.stream));
+        C.half_used[C.cur_half] = true;
+        const int nh = 1 - C.cur_half;
+        if (C.half_used[nh]) VH_CHECK(hipEventSynchronize(C.half_ev[nh]));
+        C.cur_half = nh;
+        C.ring_off = (size_t)nh * half;
+    }
+    const size_t off = C.ring_off;
+    memcpy(C.h_ring + off, src, bytes);
+    VH_CHECK(hipMemcpyAsync(C.d_ring + off, C.h_ring + off, bytes, hipMemcpyHostToDevice, C.stream));
+    C.ring_off += need;
+    return C.d_ring + off;
+}
+
+template <typename T> static T *vh_stage_vec(const std::vector<T> &v)
+{
+    return reinterpret_cast<T *>(vh_stage(v.data(), v.size() * sizeof(T)));
+}
+
+static void *vh_dmalloc(size_t bytes)
+{
+    void *p = nullptr;
+    VH_CHECK(hipMallocAsync(&p, bytes ? bytes : 256, C.stream));
+    return p;
+}
+static void vh_dfree(void *p)
+{
+    if (p) VH_CHECK(hipFreeAsync(p, C.stream));
+}
+
+// ---------------------------------------------------------------------------
+// Number theory (host): the reference's number_theory.c contract
+// ---------------------------------------------------------------------------
+static inline uint64_t mulmod_h(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)(((u128)a * b) % q); }
+static inline uint64_t shoup_of(uint64_t w, uint64_t q) { return (uint64_t)(((u128)(w % q) << 64) / q); }
+
+extern "C" uint64_t power_mod(uint64_t base, uint64_t exp, uint64_t mod)
+{
+    uint64_t r = 1 % mod;
+    base %= mod;
+    while (exp) {
+        if (exp & 1) r = mulmod_h(r, base, mod);
+        base = mulmod_h(base, base, mod);
+        exp >>= 1;
+    }
+    return r;
+}
+
+extern "C" uint64_t inverse_mod_eea(uint64_t a, uint64_t p)
+{
+    __int128 t = 0, nt = 1, r = p, nr = a % p;
+    while (nr != 0) {
+        __int128 qq = r / nr, tmp;
+        tmp = t - qq * nt; t = nt; nt = tmp;
+        tmp = r - qq * nr; r = nr; nr = tmp;
+    }
+    if (t < 0) t += p;
+    return (uint64_t)t;
+}
+
+extern "C" uint64_t inverse_mod(uint64_t a, uint64_t m)
+{
+    a %= m;
+    if (a == 0) return 0;
+    return inverse_mod_eea(a, m);
+}
+
+extern "C" bool is_prime(uint64_t n)
+{
+    static const uint64_t B[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    if (n < 2) return false;
+    for (uint64_t b : B) {
+        if (n == b) return true;
+        if (n % b == 0) return false;
+    }
+    uint64_t d = n - 1;
+    int s = 0;
+    while (!(d & 1)) { d >>= 1; s++; }
+    for (uint64_t b : B) {
+        uint64_t x = power_mod(b, d, n);
+        if (x == 1 || x == n - 1) continue;
+        bool comp = true;
+        for (int r = 1; r < s && comp; r++) {
+            x = mulmod_h(x, x, n);
+            if (x == n - 1) comp = false;
+        }
+        if (comp) return false;
+    }
+    return true;
+}
+
+extern "C" uint64_t generate_Nth_root_of_unity(uint64_t q, uint64_t n)
+{
+    for (uint64_t g = 2;; g++) {
+        uint64_t root = power_mod(g, (q - 1) / n, q);
+        if (power_mod(root, n / 2, q) == q - 1) return root;
+    }
+}
+
+extern "C" uint64_t next_special_prime(uint64_t x, uint64_t n, bool primitive)
+{
+    uint64_t ord = 2 * n, a = x / ord;
+    if (a * ord + 1 <= x) a++;
+    if (primitive && a % 2 == 0) a++;
+    for (;;) {
+        uint64_t c = a * ord + 1;
+        if (is_prime(c) && (!primitive || c % (4 * n) != 1)) return c;
+        a += primitive ? 2 : 1;
+    }
+}
+
+static uint64_t brv(uint64_t x, int bits)
+{
+    uint64_t r = 0;
+    for (int i = 0; i < bits; i++) { r = (r << 1) | (x & 1); x >>= 1; }
+    return r;
+}
+
+extern "C" void bit_rev(uint64_t *out, uint64_t *in, uint64_t n, uint64_t log_n)
+{
+    for (uint64_t i = 0; i < n; i++) out[i] = in[brv(i, (int)log_n)];
+}
+
+extern "C" uint64_t add_modq(uint64_t a, uint64_t b, uint64_t q) { uint64_t s = a + b; return s >= q ? s - q : s; }
+extern "C" uint64_t sub_modq(uint64_t a, uint64_t b, uint64_t q) { return a >= b ? a - b : a + q - b; }
+extern "C" uint64_t negate_modq(uint64_t a, uint64_t q) { return a == 0 ? 0 : q - a; }
+extern "C" uint64_t mul_modq(uint64_t a, uint64_t b, NTT_proc proc) { return mulmod_h(a, b, proc->q); }
+
+static int ilog2(uint64_t n)
+{
+    int l = 0;
+    while ((1ULL << l) < n) l++;
+    return l;
+}
+
+// ---------------------------------------------------------------------------
+// NTT_proc: roots (ntt.c:265-313) and the engine's device tables
+// ---------------------------------------------------------------------------
+static double centred(uint64_t w, uint64_t q) { return w > q / 2 ? -(double)(q - w) : (double)w; }
+
+static void fill_limb_consts(vh_limb &L, uint64_t n, uint64_t q)
+{
+    memset(&L, 0, sizeof(L));
+    L.q = q;
+    L.mu64 = UINT64_MAX / q;
+    uint32_t s = 0;
+    while (s < 64 && (q >> s)) s++;
+    L.bar_s = s;
+    L.bar_mu = (uint64_t)(((u128)1 << (2 * s)) / q);
+    L.fp64 = q < (1ULL << 50);
+    L.qd = (double)q;
+    L.qinv = 1.0 / (double)q;
+    L.ninv = inverse_mod(n, q);
+    L.ninv_sh = shoup_of(L.ninv, q);
+    L.ninv_d = centred(L.ninv, q);
+    L.ninv_q = L.ninv_d / (double)q;
+}
+
+extern "C" NTT_proc ntt_new_proc(uint64_t n, uint64_t q)
+{
+    VH_ENTER;
+    VH_REQUIRE(n >= 2 && (n & (n - 1)) == 0 && n <= (1ULL << 17));
+    VH_REQUIRE(q > 2 && q <= UINT64_MAX / 3 && (q - 1) % (2 * n) == 0);  // 3q < 2^64 (Barrett bound)
+    NTT_proc p = new _NTT_proc();
+    p->n = n;
+    p->q = q;
+    // ntt.c:271-281: first g = 2, 3, ... whose g^((q-1)/2n) has order exactly 2n
+    uint64_t root = 0;
+    for (uint64_t g = 2; g < q; g++) {
+        uint64_t c = power_mod(g, (q - 1) / (2 * n), q);
+        if (power_mod(c, n, q) == q - 1) { root = c; break; }
+    }
+    VH_REQUIRE(root != 0);
+    p->root_of_unity = root;
+    p->inv_root_of_unity = inverse_mod(root, q);
+    fill_limb_consts(p->h, n, q);
+
+    // Tables in the reference's ws order: tab[brv(i)] = psi^(+-i) (ntt_portable.c:25-33)
+    const int logn = ilog2(n);
+    std::vector<uint64_t> fw(n), iw(n);
+    uint64_t a = 1, b = 1;
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t r = brv(i, logn);
+        fw[r] = a;
+        iw[r] = b;
+        a = mulmod_h(a, root, q);
+        b = mulmod_h(b, p->inv_root_of_unity, q);
+    }
+    const size_t bytes = 2 * n * 16;
+    VH_CHECK(hipMalloc(&p->d_tab, bytes));
+    if (p->h.fp64) {
+        std::vector<double2> t(2 * n);
+        for (uint64_t i = 0; i < n; i++) {
+            double wf = centred(fw[i], q), wi = centred(iw[i], q);
+            t[i] = make_double2(wf, wf / (double)q);
+            t[n + i] = make_double2(wi, wi / (double)q);
+        }
+        VH_CHECK(hipMemcpy(p->d_tab, t.data(), bytes, hipMemcpyHostToDevice));
+        p->h.tw_fwd = reinterpret_cast<const double2 *>(p->d_tab);
+        p->h.tw_inv = p->h.tw_fwd + n;
+    } else {
+        std::vector<ulonglong2> t(2 * n);
+        for (uint64_t i = 0; i < n; i++) {
+            t[i] = make_ulonglong2(fw[i], shoup_of(fw[i], q));
+            t[n + i] = make_ulonglong2(iw[i], shoup_of(iw[i], q));
+        }
+        VH_CHECK(hipMemcpy(p->d_tab, t.data(), bytes, hipMemcpyHostToDevice));
+        p->h.twi_fwd = reinterpret_cast<const ulonglong2 *>(p->d_tab);
+        p->h.twi_inv = p->h.twi_fwd + n;
+    }
+    VH_CHECK(hipMalloc((void **)&p->d_self, sizeof(vh_limb)));
+    VH_CHECK(hipMemcpy(p->d_self, &p->h, sizeof(vh_limb), hipMemcpyHostToDevice));
+    return p;
+}
+
+extern "C" void ntt_free_proc(NTT_proc proc)
+{
+    if (!proc) return;
+    VH_ENTER;
+    VH_CHECK(hipStreamSynchronize(C.stream));
+    VH_CHECK(hipFree(proc->d_tab));
+    VH_CHECK(hipFree(proc->d_self));
+    delete proc;
+}
+
+extern "C" NTT_proc *new_ntt_list(uint64_t *primes, uint64_t N, uint64_t l)
+{
+    NTT_proc *v = (NTT_proc *)malloc(sizeof(NTT_proc) * l);
+    if (!v) exit(EXIT_FAILURE);
+    for (uint64_t i = 0; i < l; i++) v[i] = ntt_new_proc(N, primes[i]);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+static const unsigned VH_MAX_GRID_Y = 65535;
+
+// Forward (dir 0) or inverse (dir 1) transforms of n-point rows.
+static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const vh_limb *h_limbs, uint64_t n, int dir)
+{
+    if (items.empty()) return;
+    const int logn = ilog2(n);
+    std::vector<vh_item> grp[2];
+    for (auto &it : items) grp[h_limbs[it.limb].fp64 ? 1 : 0].push_back(it);
+    for (int fp = 0; fp < 2; fp++) {
+        auto &v = grp[fp];
+        for (size_t off = 0; off < v.size(); off += VH_MAX_GRID_Y) {
+            const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, v.size() - off);
+            std::vector<vh_item> chunk(v.begin() + off, v.begin() + off + cnt);
+            const vh_item *d_items = vh_stage_vec(chunk);
+            if (logn <= 12) {
+                VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, dir, logn, 0, logn, 0, 1, 1, 1));
+            } else {
+                const int R1 = logn - 8, R2 = 8;
+                const int Gc = std::min(1 << (logn - R1), 4096 >> R1);
+                const int Gr = std::min(1 << R1, 16);
+                if (dir == 0) {
+                    VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 0, logn, 0, R1, 1, Gc, 1, 0));
+                    VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 0, logn, R1, R2, 0, Gr, 0, 1));
+                } else {
+                    VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 1, logn, R1, R2, 0, Gr, 1, 0));
+                    VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 1, logn, 0, R1, 1, Gc, 0, 1));
+                }
+            }
+        }
+    }
+}
+
+static void run_eltwise(std::vector<vh_item> &items, const vh_limb *d_limbs, int op, uint64_t n)
+{
+    for (size_t off = 0; off < items.size(); off += VH_MAX_GRID_Y) {
+        const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, items.size() - off);
+        std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
+        VH_CHECK(vh_launch_eltwise(C.stream, vh_stage_vec(chunk), cnt, d_limbs, op, n));
+    }
+}
+
+static vh_item mk_item(const uint64_t *src, uint64_t *dst, uint32_t limb, const uint64_t *src2 = nullptr,
+                       uint64_t scalar = 0, uint64_t scalar_sh = 0, uint32_t aux = 0)
+{
+    vh_item it;
+    it.src = src;
+    it.dst = dst;
+    it.limb = limb;
+    it.aux = aux;
+    it.src2 = src2;
+    it.scalar = scalar;
+    it.scalar_sh = scalar_sh;
+    return it;
+}
+
+// ---------------------------------------------------------------------------
+// Standalone single-limb kernels on host or device pointers
+// ---------------------------------------------------------------------------
+static bool is_device_ptr(const void *p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) { (void)hipGetLastError(); return false; }
+    return a.type == hipMemoryTypeDevice;
+}
+
+// Runs fn(device pointers...) on arrays of n uint64 that may live on the host.
+struct vh_args {
+    uint64_t *dev[3] = {nullptr, nullptr, nullptr};
+    uint64_t *host[3] = {nullptr, nullptr, nullptr};
+    bool staged = false;
+    uint64_t n = 0;
+};
+
+static vh_args stage_args(uint64_t n, uint64_t *out, const uint64_t *in1, const uint64_t *in2, bool out_is_input)
+{
+    vh_args a;
+    a.n = n;
+    const uint64_t *ptrs[3] = {out, in1, in2};
+    bool host = false;
+    for (int i = 0; i < 3; i++) if (ptrs[i] && !is_device_ptr(ptrs[i])) host = true;
+    if (!host) {
+        a.dev[0] = out; a.dev[1] = const_cast<uint64_t *>(in1); a.dev[2] = const_cast<uint64_t *>(in2);
+        return a;
+    }
+    a.staged = true;
+    for (int i = 0; i < 3; i++) {
+        if (!ptrs[i]) continue;
+        a.host[i] = const_cast<uint64_t *>(ptrs[i]);
+        a.dev[i] = (uint64_t *)vh_dmalloc(n * 8);
+        if (i > 0 || out_is_input)
+            VH_CHECK(hipMemcpyAsync(a.dev[i], ptrs[i], n * 8, hipMemcpyHostToDevice, C.stream));
+    }
+    return a;
+}
+
+static void finish_args(vh_args &a)
+{
+    if (!a.staged) return;
+    VH_CHECK(hipMemcpyAsync(a.host[0], a.dev[0], a.n * 8, hipMemcpyDeviceToHost, C.stream));
+    VH_CHECK(hipStreamSynchronize(C.stream));
+    for (int i = 0; i < 3; i++) vh_dfree(a.dev[i]);
+}
+
+static void single_ntt(uint64_t *out, uint64_t *in, NTT_proc proc, int dir)
+{
+    VH_ENTER;
+    vh_args a = stage_args(proc->n, out, in, nullptr, false);
+    std::vector<vh_item> items{mk_item(a.dev[1], a.dev[0], 0)};
+    run_ntt(items, proc->d_self, &proc->h, proc->n, dir);
+    finish_args(a);
+}
+
+extern "C" void ntt_forward(uint64_t *out, uint64_t *in, NTT_proc proc) { single_ntt(out, in, proc, 0); }
+extern "C" void ntt_reverse(uint64_t *out, uint64_t *in, NTT_proc proc) { single_ntt(out, in, proc, 1); }
+
+static void single_eltwise(int op, uint64_t *out, const uint64_t *in1, const uint64_t *in2, uint64_t scalar,
+                           uint64_t n, NTT_proc proc, bool out_is_input)
+{
+    VH_ENTER;
+    if (n == 0) return;
+    const uint64_t q = proc->q;
+    vh_args a = stage_args(n, out, in1, in2, out_is_input);
+    uint64_t s = scalar, ssh = 0;
+    if (op == VH_OP_SCALE || op == VH_OP_FMA) { s = scalar % q; ssh = shoup_of(s, q); }
+    if (op == VH_OP_ADD_SCALAR || op == VH_OP_SUB_SCALAR) s = scalar % q;
+    if (op == VH_OP_REDUCE_MP) { s = (uint64_t)(((u128)1 << 64) % q); ssh = shoup_of(s, q); }
+    std::vector<vh_item> items{mk_item(a.dev[1], a.dev[0], 0, a.dev[2], s, ssh)};
+    run_eltwise(items, proc->d_self, op, n);
+    finish_args(a);
+}
+
+extern "C" void mod_eltwise_mul(uint64_t *out, uint64_t *in1, uint64_t *in2, uint64_t n, NTT_proc proc)
+{ single_eltwise(VH_OP_MUL, out, in1, in2, 0, n, proc, false); }
+extern "C" void mod_eltwise_mul_addto(uint64_t *out, uint64_t *in1, uint64_t *in2, uint64_t n, NTT_proc proc)
+{ single_eltwise(VH_OP_MUL_ADDTO, out, in1, in2, 0, n, proc, true); }
+extern "C" void mod_eltwise_mul_subto(uint64_t *out, uint64_t *in1, uint64_t *in2, uint64_t n, NTT_proc proc)
+{ single_eltwise(VH_OP_MUL_SUBTO, out, in1, in2, 0, n, proc, true); }
+extern "C" void mod_eltwise_scale(uint64_t *out, uint64_t *in, uint64_t scale, uint64_t n, NTT_proc proc)
+{ single_eltwise(VH_OP_SCALE, out, in, nullptr, scale, n, proc, false); }
+extern "C" void mod_eltwise_fma(uint64_t *out, uint64_t *in, uint64_t scale, uint64_t n, NTT_proc proc)
+{ single_eltwise(VH_OP_FMA, out, in, nullptr, scale, n, proc, true); }
+extern "C" void mod_eltwise_add_scalar(uint64_t *out, uint64_t *in, uint64_t scalar, uint64_t n, NTT_proc proc)
+{ single_eltwise(VH_OP_ADD_SCALAR, out, in, nullptr, scalar, n, proc, false); }
+extern "C" void mod_eltwise_sub_scalar(uint64_t *out, uint64_t *in, uint64_t scalar, uint64_t n, NTT_proc proc)
+{ single_eltwise(VH_OP_SUB_SCALAR, out, in, nullptr, scalar, n, proc, false); }
+extern "C" void mod_eltwise_negate(uint64_t *out, uint64_t *in, uint64_t n, NTT_proc proc)
+{ single_eltwise(VH_OP_NEGATE, out, in, nullptr, 0, n, proc, false); }
+extern "C" void mod_eltwise_add(uint64_t *out, uint64_t *in1, uint64_t *in2, uint64_t n, NTT_proc proc)
+{ single_eltwise(VH_OP_ADD, out, in1, in2, 0, n, proc, false); }
+extern "C" void mod_eltwise_sub(uint64_t *out, uint64_t *in1, uint64_t *in2, uint64_t n, NTT_proc proc)
+{ single_eltwise(VH_OP_SUB, out, in1, in2, 0, n, proc, false); }
+extern "C" void mod_eltwise_reduce(uint64_t *out, uint64_t *in, uint64_t n, NTT_proc proc)
+{ single_eltwise(VH_OP_REDUCE, out, in, nullptr, 0, n, proc, false); }
+extern "C" void mod_eltwise_reduce_signed(uint64_t *out, int64_t *in, uint64_t n, NTT_proc proc)
+{ single_eltwise(VH_OP_REDUCE_SIGNED, out, (const uint64_t *)in, nullptr, 0, n, proc, false); }
+extern "C" void mod_reduce_array_mp(uint64_t *out, uint64_t *in_high, uint64_t *in_low, uint64_t n, NTT_proc proc)
+{ single_eltwise(VH_OP_REDUCE_MP, out, in_low, in_high, 0, n, proc, false); }
+
+// ---------------------------------------------------------------------------
+// incNTT pool (rns_polynomial.c:7-61)
+// ---------------------------------------------------------------------------
+static void pool_upload(vh_pool *P)
+{
+    vh_limb *old = P->d_limbs;
+    P->d_limbs = (vh_limb *)vh_dmalloc(P->h_limbs.size() * sizeof(vh_limb));
+    VH_CHECK(hipMemcpyAsync(P->d_limbs, P->h_limbs.data(), P->h_limbs.size() * sizeof(vh_limb),
+                            hipMemcpyHostToDevice, C.stream));
+    VH_CHECK(hipStreamSynchronize(C.stream));
+    vh_dfree(old);
+}
+
+static void pool_add_primes(vh_pool *P, const uint64_t *primes, uint64_t count)
+{
+    incNTT t = &P->pub;
+    const uint64_t n = P->n, old_l = t->l, new_l = old_l + count;
+    t->ntt = (NTT_proc *)realloc(t->ntt, sizeof(NTT_proc) * new_l);
+    t->w = (uint64_t **)realloc(t->w, sizeof(uint64_t *) * new_l);
+    if (!t->ntt || !t->w) exit(EXIT_FAILURE);
+    std::vector<uint64_t> wp(2 * n);
+    for (uint64_t i = old_l; i < new_l; i++) {
+        const uint64_t q = primes[i - old_l];
+        t->ntt[i] = ntt_new_proc(n, q);
+        // rns_polynomial.c:21-27: w[k] = psi^(2 brv(k) + 1)
+        const uint64_t psi = t->ntt[i]->root_of_unity;
+        wp[0] = psi;
+        for (uint64_t j = 1; j < 2 * n; j++) wp[j] = mulmod_h(wp[j - 1], psi, q);
+        uint64_t *w = (uint64_t *)aligned_alloc(64, ((n * 8 + 63) / 64) * 64);
+        if (!w) exit(EXIT_FAILURE);
+        for (uint64_t k = 0; k < n; k++) w[k] = wp[2 * brv(k, (int)P->logn)];
+        t->w[i] = w;
+        P->h_limbs.push_back(t->ntt[i]->h);
+        uint64_t *dw = nullptr;
+        if (t->split_degree > 1) {
+            VH_CHECK(hipMalloc((void **)&dw, n * 8));
+            VH_CHECK(hipMemcpy(dw, w, n * 8, hipMemcpyHostToDevice));
+        }
+        P->d_w.push_back(dw);
+    }
+    t->l = new_l;
+    pool_upload(P);
+}
+
+extern "C" incNTT new_incomplete_ntt_list(uint64_t *primes, uint64_t split_degree, uint64_t N, uint64_t l)
+{
+    VH_ENTER;
+    VH_REQUIRE(split_degree >= 1 && (split_degree & (split_degree - 1)) == 0 && N % split_degree == 0);
+    VH_REQUIRE(l <= VH_MAX_LIMBS);
+    vh_pool *P = new vh_pool();
+    P->pub.ntt = nullptr;
+    P->pub.w = nullptr;
+    P->pub.split_degree = split_degree;
+    P->pub.N = N;
+    P->pub.l = 0;
+    P->n = N / split_degree;
+    P->logn = ilog2(P->n);
+    P->d_limbs = nullptr;
+    pool_add_primes(P, primes, l);
+    return &P->pub;
+}
+
+extern "C" void incNTT_extend_with_primes(incNTT ntt, uint64_t *new_primes, uint64_t count)
+{
+    if (count == 0) return;
+    VH_ENTER;
+    VH_REQUIRE(ntt->l + count <= VH_MAX_LIMBS);
+    pool_add_primes(VPOOL(ntt), new_primes, count);
+}
+
+extern "C" uint64_t **incNTT_get_rou_matrix(incNTT ntt) { return ntt->w; }
+
+// ---------------------------------------------------------------------------
+// Polynomials (rns_polynomial.c:65-181)
+// ---------------------------------------------------------------------------
+static void dev_ready(RNS_Polynomial p)
+{
+    vh_poly *v = VP(p);
+    if (!v->host_dirty) return;
+    VH_CHECK(hipMemcpyAsync(v->d, v->hblock, p->allocated_l * p->ntt->N * 8, hipMemcpyHostToDevice, C.stream));
+    VH_CHECK(hipStreamSynchronize(C.stream));
+    v->host_dirty = 0;
+}
+
+static inline uint64_t *drow(RNS_Polynomial p, uint64_t limb) { return VP(p)->d + limb * p->ntt->N; }
+static inline const vh_limb *DL(RNS_Polynomial p) { return VPOOL(p->ntt)->d_limbs; }
+
+// Every active limb must have a row (the reference indexes coeffs[i] unchecked).
+static void check_rows(RNS_Polynomial p, uint64_t mask)
+{
+    VH_REQUIRE(p->allocated_l >= 64 || (mask >> p->allocated_l) == 0);
+}
+
+extern "C" RNS_Polynomial polynomial_new_RNS_polynomial(uint64_t N, uint64_t rns_mask, incNTT ntt)
+{
+    VH_ENTER;
+    VH_REQUIRE(N == ntt->N);
+    vh_poly *v = new vh_poly();
+    const uint64_t l = ntt->l;
+    v->pub.ntt = ntt;
+    v->pub.rns_mask = rns_mask;
+    v->pub.allocated_l = l;
+    v->pub.coeffs = (uint64_t **)malloc(sizeof(uint64_t *) * (l ? l : 1));
+    size_t hb = ((l * N * 8 + 63) / 64) * 64;
+    v->hblock = (uint64_t *)aligned_alloc(64, hb ? hb : 64);
+    if (!v->pub.coeffs || !v->hblock) exit(EXIT_FAILURE);
+    for (uint64_t i = 0; i < l; i++) v->pub.coeffs[i] = v->hblock + i * N;
+    v->d = (uint64_t *)vh_dmalloc(l * N * 8);
+    VH_CHECK(hipMemsetAsync(v->d, 0, l * N * 8, C.stream));
+    v->host_dirty = 0;
+    return &v->pub;
+}
+
+extern "C" RNS_Polynomial *polynomial_new_RNS_polynomial_array(uint64_t size, uint64_t N, uint64_t rns_mask, incNTT ntt)
+{
+    RNS_Polynomial *r = (RNS_Polynomial *)malloc(sizeof(RNS_Polynomial) * (size ? size : 1));
+    if (!r) exit(EXIT_FAILURE);
+    for (uint64_t i = 0; i < size; i++) r[i] = polynomial_new_RNS_polynomial(N, rns_mask, ntt);
+    return r;
+}
+
+extern "C" RNS_Polynomial *polynomial_new_array_of_RNS_polynomials(uint64_t N, uint64_t rns_mask, uint64_t size, incNTT ntt)
+{
+    return polynomial_new_RNS_polynomial_array(size, N, rns_mask, ntt);
+}
+
+extern "C" void free_RNS_polynomial(void *p)
+{
+    if (!p) return;
+    VH_ENTER;
+    vh_poly *v = reinterpret_cast<vh_poly *>(p);
+    vh_dfree(v->d);
+    free(v->hblock);
+    free(v->pub.coeffs);
+    delete v;
+}
+
+extern "C" void free_RNS_polynomial_array(uint64_t size, RNS_Polynomial *p)
+{
+    for (uint64_t i = 0; i < size; i++) free_RNS_polynomial(p[i]);
+    free(p);
+}
+
+extern "C" void polynomial_sync_to_host(RNS_Polynomial p)
+{
+    VH_ENTER;
+    vh_poly *v = VP(p);
+    if (v->host_dirty) return;
+    VH_CHECK(hipMemcpyAsync(v->hblock, v->d, p->allocated_l * p->ntt->N * 8, hipMemcpyDeviceToHost, C.stream));
+    VH_CHECK(hipStreamSynchronize(C.stream));
+}
+
+extern "C" void polynomial_mark_host_dirty(RNS_Polynomial p)
+{
+    VH_ENTER;
+    VP(p)->host_dirty = 1;
+}
+
+extern "C" uint64_t *polynomial_device_row(RNS_Polynomial p, uint64_t limb)
+{
+    VH_ENTER;
+    VH_REQUIRE(limb < p->allocated_l);
+    dev_ready(p);
+    return drow(p, limb);
+}
+
+extern "C" void vfhe_hip_synchronize(void)
+{
+    VH_ENTER;
+    VH_CHECK(hipStreamSynchronize(C.stream));
+}
+
+extern "C" int vfhe_hip_last_error(void)
+{
+    VH_ENTER;
+    return (int)hipGetLastError();
+}
+
+extern "C" const char *vfhe_hip_device_name(void)
+{
+    VH_ENTER;
+    return C.name;
+}
+
+extern "C" const char *vfhe_engine_active(void) { return "hip"; }
+
+extern "C" int vfhe_cpu_supports(const char *capability)
+{
+    if (!capability) return 0;
+    if (strcmp(capability, "hip") != 0) return 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { (void)hipGetLastError(); return 0; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+    return strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+// Row-wise helpers over active limbs ------------------------------------------------
+template <typename F> static void for_active(uint64_t mask, uint64_t l, F f)
+{
+    for (uint64_t i = 0; i < l; i++) if (ACTIVE(mask, i)) f(i);
+}
+
+static void copy_rows(RNS_Polynomial out, RNS_Polynomial in, uint64_t mask)
+{
+    if (out == in) return;
+    const uint64_t N = out->ntt->N;
+    std::vector<vh_item> items;
+    for_active(mask, out->ntt->l, [&](uint64_t i) { items.push_back(mk_item(drow(in, i), drow(out, i), i)); });
+    run_eltwise(items, DL(out), VH_OP_COPY, N);
+}
+
+extern "C" void polynomial_copy_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in)
+{
+    VH_ENTER;
+    dev_ready(in);
+    dev_ready(out);
+    out->rns_mask = in->rns_mask;
+    check_rows(out, out->rns_mask);
+    copy_rows(out, in, out->rns_mask);
+}
+
+extern "C" void polynomial_copy_RNSc_polynomial(RNSc_Polynomial out, RNSc_Polynomial in)
+{
+    polynomial_copy_RNS_polynomial(out, in);
+}
+
+static void zero_rows(RNS_Polynomial p, uint64_t mask)
+{
+    std::vector<vh_item> items;
+    for_active(mask, std::min(p->ntt->l, p->allocated_l), [&](uint64_t i) { items.push_back(mk_item(nullptr, drow(p, i), i)); });
+    run_eltwise(items, DL(p), VH_OP_ZERO, p->ntt->N);
+}
+
+extern "C" void polynomial_RNS_zero(RNS_Polynomial p)
+{
+    VH_ENTER;
+    dev_ready(p);
+    zero_rows(p, p->rns_mask);
+}
+
+extern "C" bool polynomial_eq(RNS_Polynomial a, RNS_Polynomial b)
+{
+    VH_ENTER;
+    polynomial_sync_to_host(a);
+    polynomial_sync_to_host(b);
+    const uint64_t la = a->ntt->l, lb = b->ntt->l, ml = std::max(la, lb);
+    for (uint64_t i = 0; i < ml; i++) {
+        bool aa = i < la && ACTIVE(a->rns_mask, i), ab = i < lb && ACTIVE(b->rns_mask, i);
+        if (aa && ab) {
+            if (memcmp(a->coeffs[i], b->coeffs[i], a->ntt->N * 8) != 0) return false;
+        } else if (aa) {
+            for (uint64_t j = 0; j < a->ntt->N; j++) if (a->coeffs[i][j]) return false;
+        } else if (ab) {
+            for (uint64_t j = 0; j < b->ntt->N; j++) if (b->coeffs[i][j]) return false;
+        }
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Domain conversion (rns_polynomial.c:582-614)
+// ---------------------------------------------------------------------------
+static void ntt_rows(std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pairs, int dir)
+{
+    if (pairs.empty()) return;
+    incNTT t = pairs[0].first->ntt;
+    vh_pool *P = VPOOL(t);
+    std::vector<vh_item> items;
+    for (auto &pr : pairs) {
+        RNS_Polynomial out = pr.first, in = pr.second;
+        VH_REQUIRE(out->ntt == t && in->ntt == t);
+        check_rows(out, in->rns_mask);
+        for_active(in->rns_mask, t->l, [&](uint64_t i) {
+            for (uint64_t k = 0; k < t->split_degree; k++)
+                items.push_back(mk_item(drow(in, i) + k * P->n, drow(out, i) + k * P->n, i));
+        });
+    }
+    run_ntt(items, P->d_limbs, P->h_limbs.data(), P->n, dir);
+}
+
+extern "C" void polynomial_RNSc_to_RNS(RNS_Polynomial out, RNSc_Polynomial in)
+{
+    VH_ENTER;
+    dev_ready(in);
+    dev_ready(out);
+    out->rns_mask = in->rns_mask;
+    ntt_rows({{out, in}}, 0);
+}
+
+extern "C" void polynomial_RNS_to_RNSc(RNSc_Polynomial out, RNS_Polynomial in)
+{
+    VH_ENTER;
+    dev_ready(in);
+    dev_ready(out);
+    out->rns_mask = in->rns_mask;
+    ntt_rows({{out, in}}, 1);
+}
+
+// int_array_to_RNS (rns_polynomial.c:204-223) and array_to_RNS (:225-240)
+extern "C" void int_array_to_RNS(RNS_Polynomial out, uint64_t *in)
+{
+    VH_ENTER;
+    dev_ready(out);
+    const uint64_t N = out->ntt->N;
+    uint64_t *tmp = (uint64_t *)vh_dmalloc(N * 8);
+    VH_CHECK(hipMemcpyAsync(tmp, in, N * 8, hipMemcpyHostToDevice, C.stream));
+    VH_CHECK(hipStreamSynchronize(C.stream));
+    std::vector<vh_item> items;
+    for_active(out->rns_mask, out->ntt->l, [&](uint64_t i) {
+        items.push_back(mk_item(tmp, drow(out, i), i, nullptr, out->ntt->split_degree));
+    });
+    if (!items.empty())
+        VH_CHECK(vh_launch_scatter_split(C.stream, vh_stage_vec(items), (unsigned)items.size(), DL(out), N, 1));
+    vh_dfree(tmp);
+    ntt_rows({{out, out}}, 0);
+}
+
+extern "C" void array_to_RNS(RNS_Polynomial out, uint64_t **in)
+{
+    VH_ENTER;
+    dev_ready(out);
+    const uint64_t N = out->ntt->N;
+    std::vector<uint64_t> hostbuf;
+    std::vector<vh_item> items;
+    uint64_t cnt = 0;
+    for_active(out->rns_mask, out->ntt->l, [&](uint64_t) { cnt++; });
+    uint64_t *tmp = (uint64_t *)vh_dmalloc(cnt * N * 8);
+    uint64_t k = 0;
+    for_active(out->rns_mask, out->ntt->l, [&](uint64_t i) {
+        VH_CHECK(hipMemcpyAsync(tmp + k * N, in[i], N * 8, hipMemcpyHostToDevice, C.stream));
+        items.push_back(mk_item(tmp + k * N, drow(out, i), i, nullptr, out->ntt->split_degree));
+        k++;
+    });
+    VH_CHECK(hipStreamSynchronize(C.stream));
+    if (!items.empty())
+        VH_CHECK(vh_launch_scatter_split(C.stream, vh_stage_vec(items), (unsigned)items.size(), DL(out), N, 0));
+    vh_dfree(tmp);
+    ntt_rows({{out, out}}, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Pointwise products with the split-degree wrap (rns_polynomial.c:273-435)
+// ---------------------------------------------------------------------------
+// mode 0: out = a*b, 1: out += a*b, 2: out -= a*b, on the active limbs of mask.
+struct vh_mulreq { RNS_Polynomial out, a, b; uint64_t mask; };
+
+static void run_muls(const std::vector<vh_mulreq> &reqs, int mode)
+{
+    if (reqs.empty()) return;
+    incNTT t = reqs[0].out->ntt;
+    vh_pool *P = VPOOL(t);
+    std::vector<vh_item> items;
+    for (auto &r : reqs) {
+        check_rows(r.out, r.mask);
+        for_active(r.mask, t->l, [&](uint64_t i) {
+            vh_item it = mk_item(drow(r.a, i), drow(r.out, i), i, drow(r.b, i));
+            if (t->split_degree > 1) {
+                it.aux = (uint32_t)t->split_degree;
+                it.scalar = (uint64_t)(uintptr_t)P->d_w[i];
+            }
+            items.push_back(it);
+        });
+    }
+    if (t->split_degree == 1) {
+        const int op = mode == 0 ? VH_OP_MUL : (mode == 1 ? VH_OP_MUL_ADDTO : VH_OP_MUL_SUBTO);
+        run_eltwise(items, P->d_limbs, op, t->N);
+    } else {
+        for (size_t off = 0; off < items.size(); off += VH_MAX_GRID_Y) {
+            const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, items.size() - off);
+            std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
+            VH_CHECK(vh_launch_split_mul(C.stream, vh_stage_vec(chunk), cnt, P->d_limbs, mode, P->n, t->split_degree));
+        }
+    }
+}
+
+extern "C" void polynomial_mul_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in1, RNS_Polynomial in2)
+{
+    VH_ENTER;
+    VH_REQUIRE(out != in1 && out != in2);
+    dev_ready(in1); dev_ready(in2); dev_ready(out);
+    out->rns_mask = in1->rns_mask & in2->rns_mask;
+    run_muls({{out, in1, in2, out->rns_mask}}, 0);
+}
+
+extern "C" void polynomial_mul_addto_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in1, RNS_Polynomial in2)
+{
+    VH_ENTER;
+    VH_REQUIRE(out != in1 && out != in2);
+    dev_ready(in1); dev_ready(in2); dev_ready(out);
+    out->rns_mask = in1->rns_mask & in2->rns_mask;
+    run_muls({{out, in1, in2, out->rns_mask}}, 1);
+}
+
+extern "C" void polynomial_mul_subto_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in1, RNS_Polynomial in2)
+{
+    VH_ENTER;
+    VH_REQUIRE(out != in1 && out != in2);
+    dev_ready(in1); dev_ready(in2); dev_ready(out);
+    out->rns_mask = in1->rns_mask & in2->rns_mask;
+    run_muls({{out, in1, in2, out->rns_mask}}, 2);
+}
+
+extern "C" void polynomial_multo_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in)
+{
+    VH_ENTER;
+    dev_ready(in); dev_ready(out);
+    out->rns_mask &= in->rns_mask;
+    if (out->ntt->split_degree == 1) {
+        run_muls({{out, out, in, out->rns_mask}}, 0);
+        return;
+    }
+    RNS_Polynomial tmp = polynomial_new_RNS_polynomial(out->ntt->N, out->rns_mask, out->ntt);
+    copy_rows(tmp, out, out->rns_mask);
+    run_muls({{out, in, tmp, out->rns_mask}}, 0);
+    free_RNS_polynomial(tmp);
+}
+
+// ---------------------------------------------------------------------------
+// Additive and scalar ops (rns_polynomial.c:437-580)
+// ---------------------------------------------------------------------------
+static void binop(RNS_Polynomial out, RNS_Polynomial a, RNS_Polynomial b, int op)
+{
+    dev_ready(a); dev_ready(b); dev_ready(out);
+    const uint64_t m = a->rns_mask & b->rns_mask;
+    out->rns_mask = m;
+    check_rows(out, m);
+    std::vector<vh_item> items;
+    for_active(m, out->ntt->l, [&](uint64_t i) { items.push_back(mk_item(drow(a, i), drow(out, i), i, drow(b, i))); });
+    run_eltwise(items, DL(out), op, out->ntt->N);
+}
+
+extern "C" void polynomial_add_RNSc_polynomial(RNSc_Polynomial out, RNSc_Polynomial in1, RNSc_Polynomial in2)
+{ VH_ENTER; binop(out, in1, in2, VH_OP_ADD); }
+extern "C" void polynomial_add_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in1, RNS_Polynomial in2)
+{ VH_ENTER; binop(out, in1, in2, VH_OP_ADD); }
+extern "C" void polynomial_sub_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in1, RNS_Polynomial in2)
+{ VH_ENTER; binop(out, in1, in2, VH_OP_SUB); }
+extern "C" void polynomial_sub_RNSc_polynomial(RNSc_Polynomial out, RNSc_Polynomial in1, RNSc_Polynomial in2)
+{ VH_ENTER; binop(out, in1, in2, VH_OP_SUB); }
+
+extern "C" void polynomial_RNSc_negate(RNSc_Polynomial out, RNSc_Polynomial in)
+{
+    VH_ENTER;
+    dev_ready(in); dev_ready(out);
+    out->rns_mask = in->rns_mask;
+    std::vector<vh_item> items;
+    for_active(out->rns_mask, out->ntt->l, [&](uint64_t i) { items.push_back(mk_item(drow(in, i), drow(out, i), i)); });
+    run_eltwise(items, DL(out), VH_OP_NEGATE, out->ntt->N);
+}
+extern "C" void polynomial_RNS_negate(RNS_Polynomial out, RNS_Polynomial in) { polynomial_RNSc_negate(out, in); }
+
+static void scale_op(RNS_Polynomial out, RNS_Polynomial in, const uint64_t *scales, uint64_t scale, int op)
+{
+    dev_ready(in); dev_ready(out);
+    out->rns_mask = in->rns_mask;
+    std::vector<vh_item> items;
+    for_active(out->rns_mask, out->ntt->l, [&](uint64_t i) {
+        const uint64_t q = out->ntt->ntt[i]->q, s = (scales ? scales[i] : scale) % q;
+        items.push_back(mk_item(drow(in, i), drow(out, i), i, nullptr, s, shoup_of(s, q)));
+    });
+    run_eltwise(items, DL(out), op, out->ntt->N);
+}
+
+extern "C" void polynomial_scale_RNSc_polynomial(RNSc_Polynomial out, RNSc_Polynomial in1, uint64_t scale)
+{ VH_ENTER; scale_op(out, in1, nullptr, scale, VH_OP_SCALE); }
+extern "C" void polynomial_scale_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in1, uint64_t scale)
+{ VH_ENTER; scale_op(out, in1, nullptr, scale, VH_OP_SCALE); }
+extern "C" void polynomial_scale_RNS_polynomial_RNS(RNS_Polynomial out, RNS_Polynomial in1, uint64_t *scale)
+{ VH_ENTER; scale_op(out, in1, scale, 0, VH_OP_SCALE); }
+extern "C" void polynomial_scale_addto_RNSc_polynomial(RNSc_Polynomial out, RNSc_Polynomial in1, uint64_t scale)
+{ VH_ENTER; scale_op(out, in1, nullptr, scale, VH_OP_FMA); }
+extern "C" void polynomial_scale_addto_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in1, uint64_t scale)
+{ VH_ENTER; scale_op(out, in1, nullptr, scale, VH_OP_FMA); }
+
+extern "C" void polynomial_RNSc_add_integer(RNSc_Polynomial out, RNSc_Polynomial in1, uint64_t in2)
+{
+    VH_ENTER;
+    dev_ready(in1); dev_ready(out);
+    out->rns_mask = in1->rns_mask;
+    std::vector<vh_item> items;
+    for_active(out->rns_mask, out->ntt->l, [&](uint64_t i) {
+        const uint64_t q = out->ntt->ntt[i]->q;
+        const uint64_t v = (in2 >> 63) ? (q - ((0 - in2) % q)) % q : in2 % q;
+        items.push_back(mk_item(drow(in1, i), drow(out, i), i, nullptr, v));
+    });
+    run_eltwise(items, DL(out), VH_OP_ADD_AT0, out->ntt->N);
+}
+
+extern "C" void polynomial_RNS_add_integer(RNS_Polynomial out, RNS_Polynomial in1, uint64_t in2)
+{
+    VH_ENTER;
+    dev_ready(in1); dev_ready(out);
+    out->rns_mask = in1->rns_mask;
+    copy_rows(out, in1, out->rns_mask);
+    std::vector<vh_item> items;
+    for_active(out->rns_mask, out->ntt->l, [&](uint64_t i) {
+        const uint64_t q = out->ntt->ntt[i]->q;
+        const uint64_t v = (in2 >> 63) ? negate_modq((0 - in2) % q, q) : in2 % q;
+        items.push_back(mk_item(drow(out, i), drow(out, i), i, nullptr, v));
+    });
+    run_eltwise(items, DL(out), VH_OP_ADD_SCALAR, VPOOL(out->ntt)->n);
+}
+
+// ---------------------------------------------------------------------------
+// ModUp digit and limb copies (rns_polynomial.c:824-854)
+// ---------------------------------------------------------------------------
+extern "C" void polynomial_RNSc_mod_reduce_lifted(RNSc_Polynomial out, RNSc_Polynomial in, uint64_t idx)
+{
+    VH_ENTER;
+    dev_ready(in); dev_ready(out);
+    std::vector<vh_item> items;
+    for_active(out->rns_mask, out->ntt->l, [&](uint64_t i) { items.push_back(mk_item(drow(in, idx), drow(out, i), i)); });
+    run_eltwise(items, DL(out), VH_OP_REDUCE, out->ntt->N);
+}
+
+extern "C" void polynomial_RNSc_mod_reduce(RNSc_Polynomial out, RNSc_Polynomial in)
+{
+    VH_ENTER;
+    dev_ready(in); dev_ready(out);
+    std::vector<vh_item> items;
+    for_active(out->rns_mask, out->ntt->l, [&](uint64_t i) {
+        if (drow(in, i) != drow(out, i)) items.push_back(mk_item(drow(in, i), drow(out, i), i));
+    });
+    run_eltwise(items, DL(out), VH_OP_COPY, out->ntt->N);
+}
+
+// ---------------------------------------------------------------------------
+// Rescale / ModDown: sequential division by the primes of divide_mask
+// (rns_polynomial.c:905-976), all passes fused into one launch per <=16 primes.
+// ---------------------------------------------------------------------------
+struct vh_divc_h { uint64_t h, inv, inv_sh; };
+
+static void run_division(const std::vector<RNS_Polynomial> &polys, uint64_t divide_mask, bool round)
+{
+    if (polys.empty()) return;
+    incNTT t = polys[0]->ntt;
+    const uint64_t l = t->l;
+    // Group polynomials by active mask (they normally share one).
+    std::map<uint64_t, std::vector<RNS_Polynomial>> groups;
+    for (auto p : polys) {
+        VH_REQUIRE(p->ntt == t);
+        dev_ready(p);
+        groups[p->rns_mask].push_back(p);
+    }
+    for (auto &g : groups) {
+        uint64_t mask = g.first;
+        uint64_t dm = divide_mask & mask;
+        while (dm) {
+            std::vector<int> divs;
+            for (uint64_t i = 0; i < l && divs.size() < VH_MAX_DIV; i++) if (ACTIVE(dm, i)) divs.push_back((int)i);
+            uint64_t chunk = 0;
+            for (int d : divs) chunk |= 1ULL << d;
+            std::vector<int> outs;
+            for (uint64_t i = 0; i < l; i++) if (ACTIVE(mask, i) && !ACTIVE(chunk, i)) outs.push_back((int)i);
+            std::vector<vh_divc_h> consts(divs.size() * l);
+            for (size_t k = 0; k < divs.size(); k++) {
+                const uint64_t p = t->ntt[divs[k]]->q, half = round ? p / 2 : 0;
+                for (uint64_t i = 0; i < l; i++) {
+                    if (!ACTIVE(mask, i)) continue;
+                    const uint64_t q = t->ntt[i]->q;
+                    vh_divc_h &c = consts[k * l + i];
+                    c.h = half % q;
+                    c.inv = (i == (uint64_t)divs[k]) ? 0 : inverse_mod(p, q);
+                    c.inv_sh = shoup_of(c.inv, q);
+                }
+            }
+            std::vector<vh_div_job> jobs;
+            for (auto p : g.second) {
+                vh_div_job jb;
+                memset(&jb, 0, sizeof(jb));
+                jb.base = VP(p)->d;
+                jb.stride = t->N;
+                jb.K = (int)divs.size();
+                jb.nl = (int)l;
+                for (size_t k = 0; k < divs.size(); k++) jb.div_limb[k] = divs[k];
+                for (size_t k = 0; k < outs.size(); k++) jb.out_limb[k] = outs[k];
+                jobs.push_back(jb);
+            }
+            const void *d_consts = vh_stage(consts.data(), consts.size() * sizeof(vh_divc_h));
+            if (!outs.empty()) {
+                for (size_t off = 0; off < jobs.size(); off += VH_MAX_GRID_Y) {
+                    const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, jobs.size() - off);
+                    std::vector<vh_div_job> chunkv(jobs.begin() + off, jobs.begin() + off + cnt);
+                    VH_CHECK(vh_launch_division(C.stream, vh_stage_vec(chunkv), cnt, (unsigned)outs.size(),
+                                                VPOOL(t)->d_limbs, d_consts, t->N));
+                }
+            }
+            std::vector<vh_item> zi;
+            for (auto p : g.second) for (int d : divs) zi.push_back(mk_item(nullptr, drow(p, d), d));
+            run_eltwise(zi, VPOOL(t)->d_limbs, VH_OP_ZERO, t->N);
+            mask &= ~chunk;
+            dm &= ~chunk;
+        }
+        for (auto p : g.second) p->rns_mask = mask;
+    }
+}
+
+extern "C" void polynomial_round_division_RNSc_wo_free(RNSc_Polynomial out, uint64_t divide_mask)
+{
+    VH_ENTER;
+    if ((divide_mask & out->rns_mask) == 0) return;
+    run_division({out}, divide_mask, true);
+}
+
+extern "C" void polynomial_floor_division_RNSc_wo_free(RNSc_Polynomial out, uint64_t divide_mask)
+{
+    VH_ENTER;
+    if ((divide_mask & out->rns_mask) == 0) return;
+    run_division({out}, divide_mask, false);
+}
+
+static int last_active(uint64_t mask)
+{
+    for (int i = 63; i >= 0; i--) if (ACTIVE(mask, i)) return i;
+    return -1;
+}
+
+extern "C" void polynomial_round_division_RNSc(RNSc_Polynomial out)
+{
+    int la = last_active(out->rns_mask);
+    if (la >= 0) polynomial_round_division_RNSc_wo_free(out, 1ULL << la);
+}
+
+extern "C" void polynomial_floor_division_RNSc(RNSc_Polynomial out)
+{
+    int la = last_active(out->rns_mask);
+    if (la >= 0) polynomial_floor_division_RNSc_wo_free(out, 1ULL << la);
+}
+
+// ---------------------------------------------------------------------------
+// Fast base extension (rns_polynomial.c:638-822) and scaled lift (:1463-1525)
+// ---------------------------------------------------------------------------
+extern "C" RNS_BaseConversionParams init_base_conversion_params(incNTT ntt, uint64_t in_mask, uint64_t out_mask)
+{
+    RNS_BaseConversionParams p = (RNS_BaseConversionParams)calloc(1, sizeof(*p));
+    if (!p) exit(EXIT_FAILURE);
+    p->in_mask = in_mask;
+    p->out_mask = out_mask;
+    p->D = (uint32_t *)malloc(sizeof(uint32_t) * (ntt->l ? ntt->l : 1));
+    p->P = (uint32_t *)malloc(sizeof(uint32_t) * (ntt->l ? ntt->l : 1));
+    for (uint64_t i = 0; i < ntt->l; i++) if (ACTIVE(in_mask, i)) p->D[p->w++] = (uint32_t)i;
+    for (uint64_t i = 0; i < ntt->l; i++) if (ACTIVE(out_mask, i) && !ACTIVE(in_mask, i)) p->P[p->v++] = (uint32_t)i;
+    if (p->v == 0) return p;
+    VH_REQUIRE(p->w > 0);
+    p->Dhat = (uint64_t *)malloc(sizeof(uint64_t) * p->w);
+    for (uint32_t j = 0; j < p->w; j++) {
+        const uint64_t qj = ntt->ntt[p->D[j]]->q;
+        uint64_t prod = 1;
+        for (uint32_t k = 0; k < p->w; k++) if (k != j) prod = mulmod_h(prod, ntt->ntt[p->D[k]]->q % qj, qj);
+        p->Dhat[j] = inverse_mod(prod, qj);
+    }
+    p->D_mod_p = (uint64_t **)malloc(sizeof(uint64_t *) * p->v);
+    for (uint32_t i = 0; i < p->v; i++) {
+        p->D_mod_p[i] = (uint64_t *)malloc(sizeof(uint64_t) * p->w);
+        const uint64_t pi = ntt->ntt[p->P[i]]->q;
+        for (uint32_t j = 0; j < p->w; j++) {
+            uint64_t prod = 1;
+            for (uint32_t k = 0; k < p->w; k++) if (k != j) prod = mulmod_h(prod, ntt->ntt[p->D[k]]->q % pi, pi);
+            p->D_mod_p[i][j] = prod;
+        }
+    }
+    return p;
+}
+
+extern "C" void free_base_conversion_params(RNS_BaseConversionParams p)
+{
+    if (!p) return;
+    if (p->D_mod_p) {
+        for (uint32_t i = 0; i < p->v; i++) free(p->D_mod_p[i]);
+        free(p->D_mod_p);
+    }
+    free(p->Dhat);
+    free(p->D);
+    free(p->P);
+    free(p);
+}
+
+extern "C" void polynomial_base_conversion_RNSc(RNSc_Polynomial out, RNSc_Polynomial in, RNS_BaseConversionParams params)
+{
+    VH_ENTER;
+    VH_REQUIRE(in->ntt->l == out->ntt->l && in->ntt->N == out->ntt->N);
+    dev_ready(in); dev_ready(out);
+    const uint64_t in_mask = in->rns_mask, out_mask = out->rns_mask;
+    copy_rows(out, in, in_mask & out_mask);
+    RNS_BaseConversionParams p = params;
+    if (!p) p = init_base_conversion_params(in->ntt, in_mask, out_mask);
+    else VH_REQUIRE(p->in_mask == in_mask && p->out_mask == out_mask);
+    if (p->v > 0) {
+        std::vector<ulonglong2> consts(p->w + (size_t)p->v * p->w);
+        for (uint32_t j = 0; j < p->w; j++) {
+            const uint64_t qj = in->ntt->ntt[p->D[j]]->q;
+            consts[j] = make_ulonglong2(p->Dhat[j], shoup_of(p->Dhat[j], qj));
+        }
+        for (uint32_t t = 0; t < p->v; t++) {
+            const uint64_t pt = out->ntt->ntt[p->P[t]]->q;
+            for (uint32_t j = 0; j < p->w; j++) {
+                const uint64_t d = p->D_mod_p[t][j] % pt;
+                consts[p->w + (size_t)t * p->w + j] = make_ulonglong2(d, shoup_of(d, pt));
+            }
+        }
+        vh_bconv_job jb;
+        memset(&jb, 0, sizeof(jb));
+        jb.in = VP(in)->d;
+        jb.out = VP(out)->d;
+        jb.stride = in->ntt->N;
+        jb.w = (int)p->w;
+        jb.v = (int)p->v;
+        for (uint32_t j = 0; j < p->w; j++) jb.D[j] = (int)p->D[j];
+        for (uint32_t t = 0; t < p->v; t++) jb.P[t] = (int)p->P[t];
+        const void *d_consts = vh_stage(consts.data(), consts.size() * sizeof(ulonglong2));
+        const vh_bconv_job *d_job = (const vh_bconv_job *)vh_stage(&jb, sizeof(jb));
+        VH_CHECK(vh_launch_base_conversion(C.stream, d_job, 1, p->v, DL(out), d_consts, out->ntt->N));
+    }
+    if (!params) free_base_conversion_params(p);
+    out->rns_mask = out_mask;
+}
+
+extern "C" void rns_compute_scaling_factors(uint64_t *delta_out, incNTT ntt, uint64_t in_mask, uint64_t out_mask)
+{
+    const uint64_t diff = out_mask & ~in_mask;
+    for (uint64_t i = 0; i < ntt->l; i++) {
+        delta_out[i] = 0;
+        if (!ACTIVE(out_mask, i) || !ACTIVE(in_mask, i)) continue;
+        const uint64_t qi = ntt->ntt[i]->q;
+        uint64_t d = 1 % qi;
+        for (uint64_t j = 0; j < ntt->l; j++) if (ACTIVE(diff, j)) d = mulmod_h(d, ntt->ntt[j]->q % qi, qi);
+        delta_out[i] = d;
+    }
+}
+
+extern "C" void polynomial_RNSc_scaled_lift(RNSc_Polynomial out, RNSc_Polynomial in, uint64_t *delta)
+{
+    VH_ENTER;
+    VH_REQUIRE(in->ntt->N == out->ntt->N);
+    dev_ready(in); dev_ready(out);
+    uint64_t local[64];
+    if (!delta) { rns_compute_scaling_factors(local, out->ntt, in->rns_mask, out->rns_mask); delta = local; }
+    std::vector<vh_item> scale, zero;
+    for_active(out->rns_mask, out->ntt->l, [&](uint64_t i) {
+        const uint64_t q = out->ntt->ntt[i]->q;
+        if (ACTIVE(in->rns_mask, i)) {
+            const uint64_t s = delta[i] % q;
+            scale.push_back(mk_item(drow(in, i), drow(out, i), i, nullptr, s, shoup_of(s, q)));
+        } else {
+            zero.push_back(mk_item(nullptr, drow(out, i), i));
+        }
+    });
+    run_eltwise(scale, DL(out), VH_OP_SCALE, out->ntt->N);
+    run_eltwise(zero, DL(out), VH_OP_ZERO, out->ntt->N);
+}
+
+// ---------------------------------------------------------------------------
+// Automorphism (rns_polynomial.c:996-1035)
+// ---------------------------------------------------------------------------
+static void permute_into(RNS_Polynomial out, RNS_Polynomial in, uint64_t gen)
+{
+    VH_REQUIRE(out != in && gen > 0 && gen < 2 * out->ntt->N);
+    dev_ready(in); dev_ready(out);
+    out->rns_mask = in->rns_mask;
+    std::vector<vh_item> perm, zero;
+    const uint64_t l = std::min(out->ntt->l, out->allocated_l);
+    for (uint64_t j = 0; j < l; j++) {
+        if (ACTIVE(out->rns_mask, j))
+            perm.push_back(mk_item(drow(in, j), drow(out, j), (uint32_t)j, nullptr, out->ntt->split_degree, 0, (uint32_t)gen));
+        else
+            zero.push_back(mk_item(nullptr, drow(out, j), (uint32_t)j));
+    }
+    if (!perm.empty())
+        VH_CHECK(vh_launch_permute(C.stream, vh_stage_vec(perm), (unsigned)perm.size(), DL(out), out->ntt->N));
+    run_eltwise(zero, DL(out), VH_OP_ZERO, out->ntt->N);
+}
+
+extern "C" void polynomial_RNSc_permute(RNSc_Polynomial out, RNSc_Polynomial in, uint64_t gen)
+{
+    VH_ENTER;
+    permute_into(out, in, gen);
+}
+
+// ---------------------------------------------------------------------------
+// MLWE samples (mlwe.c:98-312)
+// ---------------------------------------------------------------------------
+extern "C" RNS_MLWE mlwe_alloc_RNS_sample(uint64_t N, uint64_t r, uint64_t mask, incNTT ntt)
+{
+    RNS_MLWE c = (RNS_MLWE)malloc(sizeof(*c));
+    if (!c) exit(EXIT_FAILURE);
+    c->a = polynomial_new_RNS_polynomial_array(r, N, mask, ntt);
+    c->b = polynomial_new_RNS_polynomial(N, mask, ntt);
+    c->r = r;
+    return c;
+}
+
+extern "C" RNSc_MLWE mlwe_alloc_RNSc_sample(uint64_t N, uint64_t r, uint64_t mask, incNTT ntt)
+{
+    return mlwe_alloc_RNS_sample(N, r, mask, ntt);
+}
+
+extern "C" RNS_MLWE *mlwe_alloc_RNS_sample_array(uint64_t size, uint64_t N, uint64_t r, uint64_t mask, incNTT ntt)
+{
+    RNS_MLWE *v = (RNS_MLWE *)malloc(sizeof(RNS_MLWE) * (size ? size : 1));
+    if (!v) exit(EXIT_FAILURE);
+    for (uint64_t i = 0; i < size; i++) v[i] = mlwe_alloc_RNS_sample(N, r, mask, ntt);
+    return v;
+}
+
+extern "C" void mlwe_copy_RNS_sample(RNS_MLWE out, RNS_MLWE in)
+{
+    for (uint64_t i = 0; i < out->r; i++) polynomial_copy_RNS_polynomial(out->a[i], in->a[i]);
+    polynomial_copy_RNS_polynomial(out->b, in->b);
+}
+
+extern "C" RNS_MLWE *mlwe_create_copy_array(RNS_MLWE *in, uint64_t size)
+{
+    RNS_MLWE *v = mlwe_alloc_RNS_sample_array(size, in[0]->b->ntt->N, in[0]->r, in[0]->b->rns_mask, in[0]->b->ntt);
+    for (uint64_t i = 0; i < size; i++) mlwe_copy_RNS_sample(v[i], in[i]);
+    return v;
+}
+
+extern "C" void free_mlwe_RNS_sample(void *p)
+{
+    if (!p) return;
+    RNS_MLWE c = (RNS_MLWE)p;
+    free_RNS_polynomial_array(c->r, c->a);
+    free_RNS_polynomial(c->b);
+    free(c);
+}
+
+extern "C" void free_RNS_mlwe_array(uint64_t size, RNS_MLWE *v)
+{
+    for (uint64_t i = 0; i < size; i++) free_mlwe_RNS_sample(v[i]);
+    free(v);
+}
+
+static std::vector<RNS_Polynomial> comps(RNS_MLWE c)
+{
+    std::vector<RNS_Polynomial> v(c->a, c->a + c->r);
+    v.push_back(c->b);
+    return v;
+}
+
+extern "C" void mlwe_RNS_trivial_sample_of_zero(RNS_MLWE out)
+{
+    VH_ENTER;
+    std::vector<vh_item> items;
+    for (auto p : comps(out)) {
+        dev_ready(p);
+        for (uint64_t i = 0; i < std::min(p->ntt->l, p->allocated_l); i++) items.push_back(mk_item(nullptr, drow(p, i), (uint32_t)i));
+    }
+    run_eltwise(items, DL(out->b), VH_OP_ZERO, out->b->ntt->N);
+}
+
+static void mlwe_ntt(RNS_MLWE out, RNS_MLWE in, int dir)
+{
+    VH_ENTER;
+    std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pairs;
+    VH_REQUIRE(out->r == in->r);
+    auto o = comps(out), i = comps(in);
+    for (size_t k = 0; k < o.size(); k++) {
+        RNS_Polynomial src = i[k];
+        dev_ready(src); dev_ready(o[k]);
+        o[k]->rns_mask = src->rns_mask;
+        pairs.push_back({o[k], src});
+    }
+    ntt_rows(pairs, dir);
+}
+
+extern "C" void mlwe_RNSc_to_RNS(RNS_MLWE out, RNSc_MLWE in) { mlwe_ntt(out, in, 0); }
+extern "C" void mlwe_RNS_to_RNSc(RNSc_MLWE out, RNS_MLWE in) { mlwe_ntt(out, in, 1); }
+
+static void mlwe_mul_by_poly_mode(RNS_MLWE out, RNS_MLWE in, RNS_Polynomial poly, int mode)
+{
+    VH_ENTER;
+    std::vector<vh_mulreq> reqs;
+    auto o = comps(out);
+    dev_ready(poly);
+    for (size_t k = 0; k < o.size(); k++) {
+        RNS_Polynomial src = k < out->r ? in->a[k] : in->b;
+        VH_REQUIRE(o[k] != src && o[k] != poly);
+        dev_ready(src); dev_ready(o[k]);
+        o[k]->rns_mask = src->rns_mask & poly->rns_mask;
+        reqs.push_back({o[k], src, poly, o[k]->rns_mask});
+    }
+    run_muls(reqs, mode);
+}
+
+extern "C" void mlwe_RNS_mul_by_poly(RNS_MLWE out, RNS_MLWE in, RNS_Polynomial poly) { mlwe_mul_by_poly_mode(out, in, poly, 0); }
+extern "C" void mlwe_RNS_mul_addto_by_poly(RNS_MLWE out, RNS_MLWE in, RNS_Polynomial poly) { mlwe_mul_by_poly_mode(out, in, poly, 1); }
+extern "C" void mlwe_RNS_mul_subto_by_poly(RNS_MLWE out, RNS_MLWE in, RNS_Polynomial poly) { mlwe_mul_by_poly_mode(out, in, poly, 2); }
+
+static void mlwe_binop(RNS_MLWE out, RNS_MLWE a, RNS_MLWE b, int op)
+{
+    VH_ENTER;
+    std::vector<vh_item> items;
+    for (uint64_t k = 0; k <= out->r; k++) {
+        RNS_Polynomial o = k < out->r ? out->a[k] : out->b, x = k < out->r ? a->a[k] : a->b, y = k < out->r ? b->a[k] : b->b;
+        dev_ready(o); dev_ready(x); dev_ready(y);
+        o->rns_mask = x->rns_mask & y->rns_mask;
+        for_active(o->rns_mask, o->ntt->l, [&](uint64_t i) { items.push_back(mk_item(drow(x, i), drow(o, i), (uint32_t)i, drow(y, i))); });
+    }
+    run_eltwise(items, DL(out->b), op, out->b->ntt->N);
+}
+
+extern "C" void mlwe_add_RNSc_sample(RNSc_MLWE out, RNSc_MLWE in1, RNSc_MLWE in2) { mlwe_binop(out, in1, in2, VH_OP_ADD); }
+extern "C" void mlwe_sub_RNSc_sample(RNSc_MLWE out, RNSc_MLWE in1, RNSc_MLWE in2) { mlwe_binop(out, in1, in2, VH_OP_SUB); }
+
+extern "C" void mlwe_scale_RNS_mlwe_RNS(RNS_MLWE c, uint64_t *scale)
+{
+    for (uint64_t i = 0; i < c->r; i++) polynomial_scale_RNS_polynomial_RNS(c->a[i], c->a[i], scale);
+    polynomial_scale_RNS_polynomial_RNS(c->b, c->b, scale);
+}
+
+extern "C" void mlwe_round_division_RNSc(RNSc_MLWE out, uint64_t divide_mask)
+{
+    VH_ENTER;
+    if (divide_mask == 0) return;
+    std::vector<RNS_Polynomial> ps;
+    for (auto p : comps(out)) if (divide_mask & p->rns_mask) ps.push_back(p);
+    run_division(ps, divide_mask, true);
+}
+
+extern "C" uint64_t mlwe_extended_rank(uint64_t r) { return r * (r + 3) / 2; }
+
+// mlwe_tensor_product (mlwe.c:665-709): quadratic slots q_ij (i <= j), linear slots, constant.
+static void tensor_into(const std::vector<RNS_Polynomial> &out, RNS_MLWE in1, RNS_MLWE in2)
+{
+    const uint64_t r = in1->r, R = mlwe_extended_rank(r);
+    VH_REQUIRE(in2->r == r && out.size() == R + 1);
+    std::vector<vh_mulreq> first, second;
+    auto m = [](RNS_Polynomial o, RNS_Polynomial a, RNS_Polynomial b) {
+        dev_ready(a); dev_ready(b); dev_ready(o);
+        o->rns_mask = a->rns_mask & b->rns_mask;
+        return vh_mulreq{o, a, b, o->rns_mask};
+    };
+    uint64_t k = 0;
+    for (uint64_t i = 0; i < r; i++)
+        for (uint64_t j = i; j < r; j++) {
+            first.push_back(m(out[k], in1->a[i], in2->a[j]));
+            if (i != j) second.push_back(m(out[k], in1->a[j], in2->a[i]));
+            k++;
+        }
+    for (uint64_t i = 0; i < r; i++) {
+        first.push_back(m(out[k], in1->a[i], in2->b));
+        second.push_back(m(out[k], in1->b, in2->a[i]));
+        k++;
+    }
+    first.push_back(m(out[R], in1->b, in2->b));
+    run_muls(first, 0);
+    run_muls(second, 1);
+}
+
+extern "C" void mlwe_tensor_product(RNS_Polynomial *out, RNS_MLWE in1, RNS_MLWE in2)
+{
+    VH_ENTER;
+    const uint64_t R = mlwe_extended_rank(in1->r);
+    std::vector<RNS_Polynomial> o(out, out + R + 1);
+    tensor_into(o, in1, in2);
+}
+
+// ---------------------------------------------------------------------------
+// GHS hybrid key-switch (mlwe.c:557-598, gadget.c:22-37), batched over
+// `count` inputs that share one key.  For every non-NULL ksk[i] and every
+// active limb j of in->a[i] (ascending; the k-th such digit consumes
+// ksk[i][k]), the digit is lifted to all extended limbs, forward-transformed
+// and accumulated as out -= ksk[i][k] * NTT(lift(a_i mod q_j)).  Then the
+// output is inverse-transformed, divided by the special primes in ascending
+// order with rounding, and the pass-through components are added.
+// ---------------------------------------------------------------------------
+static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &ins, RNS_MLWE **ksk)
+{
+    const size_t B = outs.size();
+    VH_REQUIRE(B == ins.size() && B > 0);
+    const uint64_t Rin = ins[0]->r, r = outs[0]->r;
+    RNS_MLWE first = nullptr;
+    for (uint64_t i = 0; i < Rin && !first; i++) if (ksk[i]) first = ksk[i][0];
+    VH_REQUIRE(first != nullptr);
+    const uint64_t ext = first->b->rns_mask;
+    incNTT t = first->b->ntt;
+    vh_pool *P = VPOOL(t);
+    const uint64_t N = t->N, l = t->l, Cc = r + 1;
+    VH_REQUIRE(Cc <= VH_MAX_COMPS);
+    std::vector<int> E;
+    for (uint64_t i = 0; i < l; i++) if (ACTIVE(ext, i)) E.push_back((int)i);
+    const uint64_t next = E.size();
+    const uint64_t in_mask = ins[0]->b->rns_mask;
+
+    // Digits: (component i, limb j, key entry k)
+    struct digit { uint64_t comp, limb, entry; };
+    std::vector<digit> D;
+    for (uint64_t i = 0; i < Rin; i++) {
+        if (!ksk[i]) continue;
+        uint64_t k = 0;
+        for (uint64_t j = 0; j < l; j++) if (ACTIVE(ins[0]->a[i]->rns_mask, j)) D.push_back({i, j, k++});
+    }
+    const uint64_t ND = D.size();
+
+    for (size_t bi = 0; bi < B; bi++) {
+        VH_REQUIRE(outs[bi] != ins[bi] && ins[bi]->r == Rin && outs[bi]->r == r);
+        for (uint64_t i = 0; i < Rin; i++) VH_REQUIRE(ins[bi]->a[i]->rns_mask == ins[0]->a[i]->rns_mask);
+        for (auto p : comps(ins[bi])) dev_ready(p);
+        for (auto p : comps(outs[bi])) { dev_ready(p); p->rns_mask = ext; check_rows(p, ext); }
+    }
+    // Key rows: kptr[(ti * ND + d) * Cc + c]
+    std::vector<const uint64_t *> kptr(next * ND * Cc);
+    for (uint64_t d = 0; d < ND; d++) {
+        RNS_MLWE e = ksk[D[d].comp][D[d].entry];
+        VH_REQUIRE(e->r == r);
+        for (uint64_t c = 0; c < Cc; c++) {
+            RNS_Polynomial kp = c < r ? e->a[c] : e->b;
+            dev_ready(kp);
+            for (uint64_t ti = 0; ti < next; ti++) kptr[(ti * ND + d) * Cc + c] = drow(kp, E[ti]);
+        }
+    }
+
+    if (ND > 0) {
+        // ModUp: lift every digit to every extended limb, then forward NTT.
+        uint64_t *modup = (uint64_t *)vh_dmalloc(B * ND * next * N * 8);
+        std::vector<vh_item> lift;
+        lift.reserve(B * ND * next);
+        for (size_t bi = 0; bi < B; bi++)
+            for (uint64_t d = 0; d < ND; d++) {
+                const uint64_t *src = drow(ins[bi]->a[D[d].comp], D[d].limb);
+                for (uint64_t ti = 0; ti < next; ti++)
+                    lift.push_back(mk_item(src, modup + ((bi * ND + d) * next + ti) * N, (uint32_t)E[ti]));
+            }
+        run_eltwise(lift, P->d_limbs, VH_OP_REDUCE, N);
+        std::vector<vh_item> nt;
+        nt.reserve(lift.size() * t->split_degree);
+        for (auto &it : lift)
+            for (uint64_t k = 0; k < t->split_degree; k++)
+                nt.push_back(mk_item(it.dst + k * P->n, it.dst + k * P->n, it.limb));
+        run_ntt(nt, P->d_limbs, P->h_limbs.data(), P->n, 0);
+
+        if (t->split_degree == 1) {
+            const uint64_t *const *d_kptr = vh_stage_vec(kptr);
+            std::vector<vh_mac_job> jobs(B);
+            for (size_t bi = 0; bi < B; bi++) {
+                vh_mac_job &jb = jobs[bi];
+                memset(&jb, 0, sizeof(jb));
+                auto oc = comps(outs[bi]);
+                for (uint64_t c = 0; c < Cc; c++) jb.out[c] = VP(oc[c])->d;
+                jb.modup = modup + bi * ND * next * N;
+                jb.kptr_off = 0;
+                jb.comps = (int)Cc;
+                jb.digits = (int)ND;
+                jb.next = (int)next;
+                for (uint64_t ti = 0; ti < next; ti++) jb.limb[ti] = E[ti];
+            }
+            for (size_t off = 0; off < B; off += VH_MAX_GRID_Y) {
+                const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, B - off);
+                std::vector<vh_mac_job> chunk(jobs.begin() + off, jobs.begin() + off + cnt);
+                VH_CHECK(vh_launch_keyswitch_mac(C.stream, vh_stage_vec(chunk), cnt, (unsigned)next, P->d_limbs, d_kptr, N));
+            }
+        } else {
+            // Split-degree rings: zero the outputs, then accumulate one digit at a time
+            // with the wrap-aware product (rns_polynomial.c:398-435).
+            std::vector<vh_item> z;
+            for (size_t bi = 0; bi < B; bi++)
+                for (auto p : comps(outs[bi]))
+                    for (uint64_t i = 0; i < std::min(l, p->allocated_l); i++) z.push_back(mk_item(nullptr, drow(p, i), (uint32_t)i));
+            run_eltwise(z, P->d_limbs, VH_OP_ZERO, N);
+            std::vector<vh_item> items;
+            for (size_t bi = 0; bi < B; bi++) {
+                auto oc = comps(outs[bi]);
+                for (uint64_t d = 0; d < ND; d++)
+                    for (uint64_t c = 0; c < Cc; c++)
+                        for (uint64_t ti = 0; ti < next; ti++) {
+                            vh_item it = mk_item(kptr[(ti * ND + d) * Cc + c], drow(oc[c], E[ti]), (uint32_t)E[ti],
+                                                 modup + ((bi * ND + d) * next + ti) * N);
+                            it.aux = (uint32_t)t->split_degree;
+                            it.scalar = (uint64_t)(uintptr_t)P->d_w[E[ti]];
+                            items.push_back(it);
+                        }
+                // One digit's rows may not be accumulated concurrently into one output row:
+                // launch per digit so the read-modify-write of out is ordered.
+                const size_t per = Cc * next;
+                for (uint64_t d = 0; d < ND; d++) {
+                    std::vector<vh_item> chunk(items.begin() + d * per, items.begin() + (d + 1) * per);
+                    VH_CHECK(vh_launch_split_mul(C.stream, vh_stage_vec(chunk), (unsigned)chunk.size(), P->d_limbs, 2, P->n, t->split_degree));
+                }
+                items.clear();
+            }
+        }
+        vh_dfree(modup);
+    } else {
+        std::vector<vh_item> z;
+        for (size_t bi = 0; bi < B; bi++)
+            for (auto p : comps(outs[bi]))
+                for (uint64_t i = 0; i < std::min(l, p->allocated_l); i++) z.push_back(mk_item(nullptr, drow(p, i), (uint32_t)i));
+        run_eltwise(z, P->d_limbs, VH_OP_ZERO, N);
+    }
+
+    // Back to coefficient form, then ModDown by the special primes.
+    std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pairs;
+    std::vector<RNS_Polynomial> all;
+    for (size_t bi = 0; bi < B; bi++)
+        for (auto p : comps(outs[bi])) { pairs.push_back({p, p}); all.push_back(p); }
+    ntt_rows(pairs, 1);
+    const uint64_t divide_mask = ext & ~in_mask;
+    if (divide_mask) run_division(all, divide_mask, true);
+
+    // Pass-through components keep their key: added after the rescale (mlwe.c:588-597).
+    std::vector<vh_item> adds;
+    for (size_t bi = 0; bi < B; bi++) {
+        uint64_t keep = 0;
+        std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pt;
+        for (uint64_t i = 0; i < Rin; i++) if (!ksk[i]) pt.push_back({outs[bi]->a[keep++], ins[bi]->a[i]});
+        pt.push_back({outs[bi]->b, ins[bi]->b});
+        for (auto &pr : pt) {
+            RNS_Polynomial o = pr.first, x = pr.second;
+            const uint64_t m = o->rns_mask & x->rns_mask;
+            o->rns_mask = m;
+            for_active(m, l, [&](uint64_t i) { adds.push_back(mk_item(drow(o, i), drow(o, i), (uint32_t)i, drow(x, i))); });
+        }
+    }
+    run_eltwise(adds, P->d_limbs, VH_OP_ADD, N);
+}
+
+extern "C" void mlwe_RNSc_GHS_hybrid_keyswitch(RNSc_MLWE out, RNSc_MLWE in, RNS_MLWE **ksk, uint64_t lvl)
+{
+    (void)lvl;
+    VH_ENTER;
+    keyswitch_batch({out}, {in}, ksk);
+}
+
+extern "C" void mlwe_automorphism_RNSc_GHS(RNSc_MLWE out, RNSc_MLWE in, uint64_t gen, RNS_MLWE **ksk, uint64_t lvl)
+{
+    (void)lvl;
+    VH_ENTER;
+    RNSc_MLWE tmp = mlwe_alloc_RNS_sample(out->a[0]->ntt->N, out->r, out->a[0]->rns_mask, out->a[0]->ntt);
+    for (uint64_t i = 0; i < out->r; i++) permute_into(tmp->a[i], in->a[i], gen);
+    permute_into(tmp->b, in->b, gen);
+    keyswitch_batch({out}, {tmp}, ksk);
+    free_mlwe_RNS_sample(tmp);
+}
+
+// mlwe_multiply (mlwe.c:711-760), batched: tensor, INTT, key-switch, then either
+// NTT (mlwe_multiply's contract) or the CKKS rescale by divide_mask directly
+// (INTT(NTT(x)) == x exactly, so the pair the reference runs between the two
+// calls is skipped without changing a bit).
+static void multiply_batch(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &in1,
+                           const std::vector<RNS_MLWE> &in2, RNS_MLWE **ksk, uint64_t divide_mask)
+{
+    const size_t B = outs.size();
+    VH_REQUIRE(B > 0 && in1.size() == B && in2.size() == B);
+    const uint64_t N = in1[0]->b->ntt->N, r = in1[0]->r, R = mlwe_extended_rank(r);
+    incNTT t = in1[0]->b->ntt;
+    if (!ksk) {
+        VH_REQUIRE(divide_mask == 0);
+        for (size_t bi = 0; bi < B; bi++) {
+            VH_REQUIRE(outs[bi]->r == R);
+            tensor_into(comps(outs[bi]), in1[bi], in2[bi]);
+        }
+        return;
+    }
+    std::vector<RNS_MLWE> ext(B);
+    std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pairs;
+    for (size_t bi = 0; bi < B; bi++) {
+        VH_REQUIRE(outs[bi]->r == r);
+        ext[bi] = mlwe_alloc_RNS_sample(N, R, in1[bi]->b->rns_mask, t);
+        tensor_into(comps(ext[bi]), in1[bi], in2[bi]);
+        for (auto p : comps(ext[bi])) pairs.push_back({p, p});
+    }
+    ntt_rows(pairs, 1);
+    keyswitch_batch(outs, ext, ksk);
+    if (divide_mask == 0) {
+        std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> o;
+        for (size_t bi = 0; bi < B; bi++) for (auto p : comps(outs[bi])) o.push_back({p, p});
+        ntt_rows(o, 0);
+    } else {
+        std::vector<RNS_Polynomial> all;
+        for (size_t bi = 0; bi < B; bi++) for (auto p : comps(outs[bi])) all.push_back(p);
+        run_division(all, divide_mask, true);
+    }
+    for (auto e : ext) free_mlwe_RNS_sample(e);
+}
+
+extern "C" void mlwe_multiply(RNS_MLWE out, RNS_MLWE in1, RNS_MLWE in2, RNS_MLWE **ksk)
+{
+    VH_ENTER;
+    multiply_batch({out}, {in1}, {in2}, ksk, 0);
+}
+
+extern "C" void vfhe_hip_mlwe_multiply_batch(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLWE *in2, uint64_t count,
+                                             RNS_MLWE **rlk, uint64_t divide_mask)
+{
+    VH_ENTER;
+    if (count == 0) return;
+    multiply_batch(std::vector<RNS_MLWE>(out, out + count), std::vector<RNS_MLWE>(in1, in1 + count),
+                   std::vector<RNS_MLWE>(in2, in2 + count), rlk, divide_mask);
+}
+
+// ---------------------------------------------------------------------------
+// Keys and phase (mlwe.c:8-38, 243-252)
+// ---------------------------------------------------------------------------
+extern "C" RNS_MLWE_Key mlwe_new_RNS_key_from_array(uint64_t *array, uint64_t N, uint64_t r, uint64_t l, incNTT ntt, double sigma)
+{
+    VH_ENTER;
+    RNS_MLWE_Key k = (RNS_MLWE_Key)malloc(sizeof(*k));
+    if (!k) exit(EXIT_FAILURE);
+    k->N = N; k->l = l; k->r = r; k->sigma = sigma;
+    k->s = (IntPolynomial *)malloc(sizeof(IntPolynomial) * (r ? r : 1));
+    k->s_RNS = polynomial_new_RNS_polynomial_array(r, N, l >= 64 ? ~0ULL : ((1ULL << l) - 1), ntt);
+    for (uint64_t i = 0; i < r; i++) {
+        k->s[i] = (IntPolynomial)malloc(sizeof(*k->s[i]));
+        k->s[i]->N = N;
+        k->s[i]->coeffs = (uint64_t *)malloc(N * 8);
+        memcpy(k->s[i]->coeffs, array + i * N, N * 8);
+        int_array_to_RNS(k->s_RNS[i], k->s[i]->coeffs);
+    }
+    return k;
+}
+
+extern "C" void free_mlwe_RNS_key(RNS_MLWE_Key key)
+{
+    if (!key) return;
+    for (uint64_t i = 0; i < key->r; i++) { free(key->s[i]->coeffs); free(key->s[i]); }
+    free(key->s);
+    free_RNS_polynomial_array(key->r, key->s_RNS);
+    free(key);
+}
+
+extern "C" void mlwe_RNS_phase(RNS_Polynomial out, RNS_MLWE in, RNS_MLWE_Key key)
+{
+    polynomial_mul_RNS_polynomial(out, in->a[0], key->s_RNS[0]);
+    for (uint64_t i = 1; i < in->r; i++) polynomial_mul_addto_RNS_polynomial(out, in->a[i], key->s_RNS[i]);
+    polynomial_sub_RNS_polynomial(out, in->b, out);
+}
+
+// ---------------------------------------------------------------------------
+// Batched transforms
+// ---------------------------------------------------------------------------
+extern "C" void vfhe_hip_ntt_batch(RNS_Polynomial *polys, uint64_t count, int direction)
+{
+    VH_ENTER;
+    std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pairs;
+    for (uint64_t i = 0; i < count; i++) { dev_ready(polys[i]); pairs.push_back({polys[i], polys[i]}); }
+    ntt_rows(pairs, direction ? 1 : 0);
+}
+
+extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polynomial *y, uint64_t count)
+{
+    VH_ENTER;
+    if (count == 0) return;
+    incNTT t = z[0]->ntt;
+    const uint64_t N = t->N;
+    std::vector<RNS_Polynomial> tmp(count);
+    std::vector<vh_mulreq> muls;
+    for (uint64_t i = 0; i < count; i++) {
+        VH_REQUIRE(z[i] != x[i] && z[i] != y[i]);
+        dev_ready(x[i]); dev_ready(y[i]); dev_ready(z[i]);
+        const uint64_t m = x[i]->rns_mask & y[i]->rns_mask;
+        check_rows(z[i], m);
+        tmp[i] = polynomial_new_RNS_polynomial(N, m, t);
+        z[i]->rns_mask = m;
+    }
+    {
+        // forward transforms of x into z and of y into a temporary, common limbs only
+        std::vector<vh_item> items;
+        vh_pool *P = VPOOL(t);
+        for (uint64_t i = 0; i < count; i++) {
+            const uint64_t m = z[i]->rns_mask;
+            for_active(m, t->l, [&](uint64_t li) {
+                for (uint64_t k = 0; k < t->split_degree; k++) {
+                    items.push_back(mk_item(drow(x[i], li) + k * P->n, drow(z[i], li) + k * P->n, (uint32_t)li));
+                    items.push_back(mk_item(drow(y[i], li) + k * P->n, drow(tmp[i], li) + k * P->n, (uint32_t)li));
+                }
+            });
+        }
+        run_ntt(items, P->d_limbs, P->h_limbs.data(), P->n, 0);
+    }
+    for (uint64_t i = 0; i < count; i++) muls.push_back({z[i], z[i], tmp[i], z[i]->rns_mask});
+    if (t->split_degree == 1) {
+        run_muls(muls, 0);
+    } else {
+        // the wrap-aware product needs an operand distinct from its output
+        for (uint64_t i = 0; i < count; i++) {
+            RNS_Polynomial zz = polynomial_new_RNS_polynomial(N, z[i]->rns_mask, t);
+            copy_rows(zz, z[i], z[i]->rns_mask);
+            run_muls({{z[i], zz, tmp[i], z[i]->rns_mask}}, 0);
+            free_RNS_polynomial(zz);
+        }
+    }
+    std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> inv;
+    for (uint64_t i = 0; i < count; i++) inv.push_back({z[i], z[i]});
+    ntt_rows(inv, 1);
+    for (auto p : tmp) free_RNS_polynomial(p);
+}

@@ -1,0 +1,58 @@
+// vh_kernels.h -- launch interface between the host runtime and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "vh_device.h"
+
+#define VH_MAX_DIV 16     // divide primes handled by one fused division launch
+#define VH_MAX_LIMBS 64   // masks are uint64_t, as in the reference
+#define VH_MAX_COMPS 8    // MLWE components (rank + 1) of one key-switch output
+
+enum {
+    VH_OP_MUL = 0, VH_OP_MUL_ADDTO, VH_OP_MUL_SUBTO, VH_OP_SCALE, VH_OP_FMA, VH_OP_ADD, VH_OP_SUB,
+    VH_OP_NEGATE, VH_OP_REDUCE, VH_OP_REDUCE_SIGNED, VH_OP_ADD_SCALAR, VH_OP_SUB_SCALAR,
+    VH_OP_COPY, VH_OP_ZERO, VH_OP_REDUCE_MP, VH_OP_ADD_AT0
+};
+
+// One polynomial's sequential division job.
+struct vh_div_job {
+    uint64_t *base;      // device rows of the polynomial, limb-major
+    uint64_t stride;     // row length (N)
+    int K;               // number of divide primes
+    int nl;              // pool size (row count of the constant table)
+    int div_limb[VH_MAX_DIV];
+    int out_limb[VH_MAX_LIMBS];
+};
+
+struct vh_bconv_job {
+    const uint64_t *in;
+    uint64_t *out;
+    uint64_t stride;
+    int w, v;
+    int D[VH_MAX_LIMBS];
+    int P[VH_MAX_LIMBS];
+};
+
+struct vh_mac_job {
+    uint64_t *out[VH_MAX_COMPS];  // device base of each output component
+    const uint64_t *modup;        // [digits][next][n] lifted, forward-transformed digits
+    uint64_t kptr_off;            // offset of this job's key-row table in kptr
+    int comps, digits, next;
+    int limb[VH_MAX_LIMBS];       // extended limb list
+};
+
+hipError_t vh_launch_ntt_pass(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
+                              int fp, int dir, int logn, int s0, int R, int col, int G, int first, int last);
+hipError_t vh_launch_eltwise(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
+                             int op, uint64_t n);
+hipError_t vh_launch_split_mul(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
+                               int mode, uint64_t n, uint64_t S);
+hipError_t vh_launch_division(hipStream_t st, const vh_div_job *jobs, unsigned njobs, unsigned nout,
+                              const vh_limb *limbs, const void *consts, uint64_t n);
+hipError_t vh_launch_base_conversion(hipStream_t st, const vh_bconv_job *jobs, unsigned njobs, unsigned nout,
+                                     const vh_limb *limbs, const void *consts, uint64_t n);
+hipError_t vh_launch_permute(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, uint64_t N);
+hipError_t vh_launch_keyswitch_mac(hipStream_t st, const vh_mac_job *jobs, unsigned njobs, unsigned next,
+                                   const vh_limb *limbs, const uint64_t *const *kptr, uint64_t n);
+hipError_t vh_launch_scatter_split(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
+                                   uint64_t N, int signed_in);
