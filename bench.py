@@ -1,0 +1,312 @@
+#!/usr/bin/env python3
+"""Headline benchmark of the hip engine on one MI355X.
+
+Metric: NTT round-trips/s at N = 2^16, L = 24 (one round trip = forward plus
+inverse transform of every limb of one polynomial), on the batched workload of
+64 polynomials whose 24 limbs are ~50-bit primes q = 1 mod 2^17 from
+Ring.gen_prime.  A step is one pass of the hot path over the batch: one
+forward launch sequence and one inverse launch sequence over all 64 x 24
+limb rows, inputs already resident in HBM.
+
+Also reported on the same line:
+  * roofline  -- the NTT pass kernels' algorithmic HBM rate, timed live with HIP
+                 events on the engine stream (vfhe_hip_profile_*), against 8 TB/s;
+  * cpu_baseline -- the reference's own ntt_forward/ntt_reverse (compiled from
+                 its sources into oracle/_ref) on this host's cores, bounded sample;
+  * fused     -- z = INTT(NTT(x) * NTT(y)) over 64 pairs (products/s);
+  * plain     -- the same round trips as one C-ABI call per polynomial, with and
+                 without host residency;
+  * ckks      -- CKKS multiply + relinearize + rescale at N = 2^16, L = 24 + 4
+                 special limbs, batch of 32 ciphertext pairs (mults/s, p50/p99).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+N, L, POLYS = 1 << 16, 24, 64
+HBM_PEAK_GBS = 8000.0
+PROF_TAGS = 6
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=1.5, help="wall time of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ckks", action="store_true")
+    ap.add_argument("--ckks-batch", type=int, default=32)
+    ap.add_argument("--ckks-steps", type=int, default=5)
+    return ap.parse_args()
+
+
+class Prof:
+    def __init__(self, lib):
+        self.lib = lib
+
+    def __enter__(self):
+        self.lib.vfhe_hip_profile_start()
+        return self
+
+    def __exit__(self, *exc):
+        ms = (ctypes.c_double * PROF_TAGS)()
+        n = (ctypes.c_uint64 * PROF_TAGS)()
+        self.lib.vfhe_hip_profile_stop(ms, n)
+        self.ms = list(ms)
+        self.launches = [int(x) for x in n]
+
+
+def timed(lib, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    lib.vfhe_hip_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    lib.vfhe_hip_synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def cpu_baseline(primes, seconds):
+    """Reference ntt_forward + ntt_reverse on this host's cores (oracle/_ref)."""
+    from oracle import oracle as O
+
+    engines = O.ref_engines()
+    if not engines:
+        return None
+    eng = engines[0]
+    procs = [O.RefProc(eng, N, q) for q in primes]
+    threads = max(1, min(16, os.cpu_count() or 1))
+    counts = [0] * threads
+    rng = np.random.default_rng(5)
+    xs = [rng.integers(0, q, N, dtype=np.uint64) for q in primes]
+    stop = time.perf_counter() + seconds
+    lib = procs[0].L
+
+    def work(t):
+        a, b = O.aligned(N), O.aligned(N)
+        k = t
+        while time.perf_counter() < stop:
+            limb = k % L
+            a[:] = xs[limb]
+            lib.ntt_forward(O.ptr(b), O.ptr(a), procs[limb].h)
+            lib.ntt_reverse(O.ptr(a), O.ptr(b), procs[limb].h)
+            counts[t] += 1
+            k += threads
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    wall = time.perf_counter() - t0
+    limb_rt = sum(counts)
+    return {
+        "value": limb_rt / L / wall,
+        "unit": "round-trips/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": f"{limb_rt} limb round trips (ntt_forward+ntt_reverse, reference {eng} engine, N=2^16, "
+                  f"the 24 ~50-bit primes) over {wall:.2f} s on {threads} threads; value = limb round trips / 24 / s",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; run with --gpus 1 "
+              "(the multi-GPU sharding of the path is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from vfhe_amd import _native, params, runtime as rt
+
+    lib = _native.load()
+    primes = params.gen_primes(N, [50] * L)
+    log("device", lib.vfhe_hip_device_name().decode(), "primes", primes[0], "...", primes[-1])
+    pool = rt.Pool(N, primes)
+    rng = np.random.default_rng(20261202)
+    polys = []
+    for _ in range(POLYS):
+        polys.append(pool.poly(rt.random_rows(rng, primes, N)))
+    arr = _native.voidp_array(polys)
+    ref0 = pool.read(polys[0])
+
+    def round_trip():
+        lib.vfhe_hip_ntt_batch(arr, POLYS, 0)
+        lib.vfhe_hip_ntt_batch(arr, POLYS, 1)
+
+    log("unit A: warmup", args.warmup, "steps", args.steps)
+    for _ in range(args.warmup):
+        round_trip()
+    lib.vfhe_hip_synchronize()
+    with Prof(lib) as prof:
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            round_trip()
+        lib.vfhe_hip_synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+    assert np.array_equal(pool.read(polys[0]), ref0), "round trip changed the data"
+    value = POLYS / dt
+    ntt_ms, ntt_launches = prof.ms[0], prof.launches[0]
+    avg_launch_s = ntt_ms / 1e3 / max(1, ntt_launches)
+    # 4 pass launches per step; a transform's 16*N bytes per limb are shared by its two passes
+    bytes_per_launch = POLYS * L * 16 * N / 2
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "ntt_pass_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    log(f"unit A: {dt * 1e3:.3f} ms/step, {value:.1f} round-trips/s, NTT launches {ntt_launches}, "
+        f"avg {avg_launch_s * 1e3:.3f} ms, {achieved:.0f} GB/s algorithmic")
+
+    # fused product (unit B)
+    xs = polys[:POLYS]
+    ys = [pool.poly(rt.random_rows(rng, primes, N)) for _ in range(POLYS)]
+    zs = [pool.poly() for _ in range(POLYS)]
+    ax, ay, az = _native.voidp_array(xs), _native.voidp_array(ys), _native.voidp_array(zs)
+    fused_dt = timed(lib, lambda: lib.vfhe_hip_ntt_mul_intt_batch(az, ax, ay, POLYS), args.steps, 2)
+    log(f"unit B: {fused_dt * 1e3:.3f} ms per 64 products")
+
+    # plain one-call-per-polynomial forms (device resident, and host resident)
+    def plain_dev():
+        for p in polys:
+            lib.polynomial_RNSc_to_RNS(p, p)
+            lib.vfhe_hip_synchronize()
+            lib.polynomial_RNS_to_RNSc(p, p)
+            lib.vfhe_hip_synchronize()
+
+    plain_dev_dt = timed(lib, plain_dev, max(2, args.steps // 4), 1)
+    host_polys = polys[:4]
+
+    def plain_host():
+        for p in host_polys:
+            lib.polynomial_mark_host_dirty(p)
+            lib.polynomial_RNSc_to_RNS(p, p)
+            lib.polynomial_sync_to_host(p)
+            lib.polynomial_mark_host_dirty(p)
+            lib.polynomial_RNS_to_RNSc(p, p)
+            lib.polynomial_sync_to_host(p)
+
+    plain_host_dt = timed(lib, plain_host, 2, 1) / len(host_polys) * POLYS
+    log(f"plain: device-resident {POLYS / plain_dev_dt:.1f} rt/s, host-resident {POLYS / plain_host_dt:.1f} rt/s")
+    for p in ys + zs:
+        pool.free(p)
+
+    ckks = None
+    if not args.no_ckks:
+        ckks = bench_ckks(lib, _native, params, rt, args)
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline:", args.cpu_seconds, "s")
+        cpu = cpu_baseline(primes, args.cpu_seconds)
+
+    line = {
+        "metric": "NTT round-trips/s at N=2^16,L=24 (% HBM roofline); CKKS mults/s + p99 latency",
+        "value": round(value, 2),
+        "unit": "round-trips/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "batched NTT round trip, 64 polynomials x 24 limbs, N=2^16, ~50-bit primes "
+                               "q=1 mod 2^17 (Ring.gen_prime), uniform residues, seed 20261202",
+                   "N": N, "L": L, "polys": POLYS},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_ntt_pass (FP64 path)", "launches": ntt_launches,
+                     "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+                     "algorithmic_bytes_per_launch": int(bytes_per_launch)},
+        "cpu_baseline": cpu,
+        "fused": {"unit": "products/s", "value": round(POLYS / fused_dt, 2), "ms_per_batch": round(fused_dt * 1e3, 4),
+                  "workload": "z=INTT(NTT(x)*NTT(y)), 64 pairs x 24 limbs, N=2^16"},
+        "plain": {"device_resident_round_trips_per_s": round(POLYS / plain_dev_dt, 2),
+                  "host_resident_round_trips_per_s": round(POLYS / plain_host_dt, 2),
+                  "batched_over_device_plain": round(plain_dev_dt / dt, 2),
+                  "batched_over_host_plain": round(plain_host_dt / dt, 2)},
+        "ckks": ckks,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_ckks(lib, _native, params, rt, args):
+    """CKKS multiply + relinearize + rescale, N=2^16, L=24 level limbs + 4 special limbs, r=1."""
+    K, B = 4, args.ckks_batch
+    primes = params.gen_primes(N, [50] * (L + K))
+    pool = rt.Pool(N, primes)
+    lvl, ext = (1 << L) - 1, (1 << (L + K)) - 1
+    rng = np.random.default_rng(99)
+    log("ckks: building relinearization key (24 digits x 2 x 28 limbs)")
+    entries = [pool.sample(1, ext, np.stack([rt.random_rows(rng, primes, N, ext) for _ in range(2)])) for _ in range(L)]
+    key = _native.voidp_array(entries)
+    rlk = _native.voidp_array([key, None])
+    in1 = [pool.sample(1, lvl, np.stack([rt.random_rows(rng, primes, N, lvl) for _ in range(2)])) for _ in range(B)]
+    in2 = [pool.sample(1, lvl, np.stack([rt.random_rows(rng, primes, N, lvl) for _ in range(2)])) for _ in range(B)]
+    outs = [pool.sample(1, ext) for _ in range(B)]
+    a1, a2, ao = _native.voidp_array(in1), _native.voidp_array(in2), _native.voidp_array(outs)
+    top = 1 << (L - 1)
+    lat = []
+    lib.vfhe_hip_mlwe_multiply_batch(ao, a1, a2, B, rlk, top)
+    lib.vfhe_hip_synchronize()
+    with Prof(lib) as prof:
+        for _ in range(args.ckks_steps):
+            for c in outs:
+                pool.set_sample_mask(c, ext)
+            t0 = time.perf_counter()
+            lib.vfhe_hip_mlwe_multiply_batch(ao, a1, a2, B, rlk, top)
+            lib.vfhe_hip_synchronize()
+            lat.append(time.perf_counter() - t0)
+    one = []
+    o1 = _native.voidp_array(outs[:1])
+    for _ in range(3):
+        pool.set_sample_mask(outs[0], ext)
+        t0 = time.perf_counter()
+        lib.vfhe_hip_mlwe_multiply_batch(o1, _native.voidp_array(in1[:1]), _native.voidp_array(in2[:1]), 1, rlk, top)
+        lib.vfhe_hip_synchronize()
+        one.append(time.perf_counter() - t0)
+    lat_ms = np.array(lat) * 1e3
+    names = ["ntt_fp64", "ntt_int", "eltwise", "division", "keyswitch_mac", "other"]
+    res = {
+        "unit": "mults/s",
+        "value": round(B / float(np.median(lat)), 2),
+        "batch": B,
+        "batch_ms_p50": round(float(np.percentile(lat_ms, 50)), 3),
+        "batch_ms_p99": round(float(np.percentile(lat_ms, 99)), 3),
+        "request_latency_p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "unbatched_mults_per_s": round(1.0 / float(np.median(one)), 2),
+        "kernel_ms_per_batch": {n: round(m / args.ckks_steps, 3) for n, m in zip(names, prof.ms)},
+        "workload": "multiply+relinearize+rescale, N=2^16, L=24+4 special, rank 1, batch of 32 pairs, "
+                    "random rlk (throughput does not depend on key validity)",
+    }
+    log("ckks:", res)
+    for c in in1 + in2 + outs + entries:
+        lib.free_mlwe_RNS_sample(c)
+    return res
+
+
+if __name__ == "__main__":
+    main()

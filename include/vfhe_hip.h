@@ -237,6 +237,14 @@ void vfhe_hip_synchronize(void);
 int vfhe_hip_last_error(void);
 const char *vfhe_hip_device_name(void);
 
+/* Launch profiler: between start and stop every kernel launch is bracketed by
+   HIP events on the engine stream.  stop() waits, then writes the summed
+   milliseconds and launch counts of each kernel family into ms[6] and
+   launches[6]: 0 NTT pass (FP64 path), 1 NTT pass (integer path),
+   2 elementwise, 3 sequential division, 4 key-switch accumulation, 5 other. */
+void vfhe_hip_profile_start(void);
+void vfhe_hip_profile_stop(double *ms, uint64_t *launches);
+
 /* Batched hot path: one launch sequence for many polynomials.
    direction 0 = forward (RNSc -> RNS), 1 = inverse. polys[i] is transformed in place. */
 void vfhe_hip_ntt_batch(RNS_Polynomial *polys, uint64_t count, int direction);

@@ -1,0 +1,409 @@
+"""Pins the CPU oracle (oracle/vfhe_oracle.c) before it is trusted as the checker.
+
+Each block checks the oracle against something it does not itself compute:
+the reference's own test vectors and known answers, the reference's compiled
+kernels (oracle/_ref), closed forms, and big-integer CRT arithmetic.  A few
+tests plant a mistake in a copy of an oracle result to show the check would
+catch it.
+"""
+import math
+import random
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vfhe_amd import params
+
+R = random.Random(0x5EED)
+
+
+# --- number theory: modules/arith/c/test/test_number_theory.c ---------------------------------
+
+def test_number_theory_known_answers():
+    L = O.lib()
+    assert L.o_is_prime(562949954142209)            # test_number_theory.c:14-18
+    assert not L.o_is_prime(100)
+    n, start = 1024, 1 << 40                         # :20-29
+    sp = int(L.o_next_special_prime(start, n, 1))
+    assert sp > start and L.o_is_prime(sp) and sp % (2 * n) == 1 and sp % (4 * n) != 1
+    sp2 = int(L.o_next_special_prime(start, n, 0))   # :31-37
+    assert sp2 > start and L.o_is_prime(sp2) and sp2 % (2 * n) == 1
+    p = 1073739937                                   # :49-58
+    for a in range(1, 1000):
+        assert int(L.o_inverse_mod(a, p)) * a % p == 1
+    # root of unity for q = 769, n = 16 (:39-47): the order-2n root rule of ntt.c:271-281
+    psi = int(L.o_find_psi(8, 769))
+    assert pow(psi, 8, 769) == 768
+
+
+def test_primality_matches_python_bigint_rule():
+    """Oracle Miller-Rabin == the package's restatement == trial division on a sample."""
+    L = O.lib()
+    for _ in range(300):
+        n = R.randrange(2, 10**6)
+        td = n > 1 and all(n % d for d in range(2, int(math.isqrt(n)) + 1))
+        assert bool(L.o_is_prime(n)) == td == params.is_prime(n)
+    for _ in range(200):
+        n = R.randrange(2**40, 2**62) | 1
+        assert bool(L.o_is_prime(n)) == params.is_prime(n)
+
+
+def test_gen_prime_rule():
+    """Ring.gen_prime (polynomial.py:151-160): descending odd multiples, first candidate skipped."""
+    for N in (4096, 65536):
+        ps = O.gen_primes(N, [50] * 6)
+        assert ps == params.gen_primes(N, [50] * 6)
+        order = 2 * N
+        a0 = ((2**50 - 1) // order) | 1
+        prev = (a0 - 2) * order + 1 + 2 * order
+        for p in ps:
+            assert p < 2**50 and p % order == 1 and ((p - 1) // order) % 2 == 1 and params.is_prime(p)
+            assert p < prev
+            # no prime of the rule's form lies strictly between consecutive outputs
+            for c in range(prev - 2 * order, p, -2 * order):
+                assert not params.is_prime(c)
+            prev = p
+    assert len(set(ps)) == len(ps)
+
+
+def test_psi_known_answers():
+    """psi values the survey probe printed from the reference's ntt_new_proc at n = 4096."""
+    assert O.lib().o_find_psi(4096, 562949953216513) == 169658170966992
+    assert O.lib().o_find_psi(4096, 1125899906424833) == 371347710277771
+
+
+# --- NTT ----------------------------------------------------------------------------------------
+
+def brv(x, bits):
+    return int(format(x, f"0{bits}b")[::-1], 2)
+
+
+@pytest.mark.parametrize("n", [16, 64, 256])
+def test_ntt_closed_form(n):
+    """NTT(x)[i] = sum_j x_j psi^(j (2 brv(i) + 1)) mod q, brute force with Python ints."""
+    for q in (O.gen_prime(2 * n, 50), int(O.lib().o_next_special_prime(1 << 60, n, 1)), 7681 if n <= 256 else 12289):
+        if (q - 1) % (2 * n):
+            continue
+        psi = int(O.lib().o_find_psi(n, q))
+        x = [R.randrange(q) for _ in range(n)]
+        a = np.array(x, dtype=np.uint64)
+        O.lib().o_ntt_forward(O.ptr(a), n, q)
+        logn = n.bit_length() - 1
+        for i in range(n):
+            e = 2 * brv(i, logn) + 1
+            assert int(a[i]) == sum(x[j] * pow(psi, j * e, q) for j in range(n)) % q
+        # a planted mistake -- natural instead of bit-reversed output order -- fails the same check
+        assert any(int(a[i]) != sum(x[j] * pow(psi, j * (2 * i + 1), q) for j in range(n)) % q
+                   for i in range(1, n))
+
+
+@pytest.mark.parametrize("engine", O.ref_engines())
+def test_ntt_matches_reference_kernels(engine):
+    """Bit-exact against the reference's own ntt_forward / ntt_reverse (oracle/_ref)."""
+    rng = np.random.default_rng(3)
+    for n in (64, 1024, 4096, 16384, 65536):
+        for q in (O.gen_prime(2 * n, 50), O.gen_prime(2 * n, 40),
+                  int(O.lib().o_next_special_prime(1 << 30, n, 1)), int(O.lib().o_next_special_prime(1 << 60, n, 1))):
+            ref = O.RefProc(engine, n, q)
+            x = rng.integers(0, q, n, dtype=np.uint64)
+            a = x.copy()
+            O.lib().o_ntt_forward(O.ptr(a), n, q)
+            assert np.array_equal(a, ref.forward(x)), (engine, n, q)
+            b = a.copy()
+            O.lib().o_ntt_inverse(O.ptr(b), n, q)
+            assert np.array_equal(b, x)
+            y = rng.integers(0, q, n, dtype=np.uint64)
+            c = y.copy()
+            O.lib().o_ntt_inverse(O.ptr(c), n, q)
+            assert np.array_equal(c, ref.reverse(y)), (engine, n, q)
+
+
+def test_negacyclic_convolution():
+    """test_ntt.c:51-104 and test_arith.py:31-53: pointwise product of NTTs == schoolbook mod X^n + 1."""
+    for n, bits in ((64, 20), (64, 62), (256, 50)):
+        q = int(O.lib().o_next_special_prime(1 << bits, n, 1))
+        a = [(i + 123) % q for i in range(n)]
+        b = [(2 * i + 456) % q for i in range(n)]
+        ref = [0] * n
+        for i in range(n):
+            for j in range(n):
+                k = i + j
+                if k < n:
+                    ref[k] = (ref[k] + a[i] * b[j]) % q
+                else:
+                    ref[k - n] = (ref[k - n] - a[i] * b[j]) % q
+        na, nb = np.array(a, dtype=np.uint64), np.array(b, dtype=np.uint64)
+        O.lib().o_ntt_forward(O.ptr(na), n, q)
+        O.lib().o_ntt_forward(O.ptr(nb), n, q)
+        nr = np.zeros(n, dtype=np.uint64)
+        O.lib().o_eltwise(O.O_MUL, O.ptr(nr), O.ptr(na), O.ptr(nb), 0, n, q)
+        O.lib().o_ntt_inverse(O.ptr(nr), n, q)
+        assert [int(v) for v in nr] == ref
+
+
+# --- elementwise: test_mod.c:21-85 oracles with Python big ints, and the reference kernels -------
+
+@pytest.mark.parametrize("bits", [10, 20, 30, 40, 50, 60, 62])
+def test_eltwise_against_bigint_and_reference(bits):
+    n = 1024
+    q = int(O.lib().o_next_special_prime(1 << bits, n, 1))
+    i = np.arange(n, dtype=np.uint64)
+    in1 = ((np.uint64(0x123456789ABCDEF) ^ (i * np.uint64(0x1337BEEF))) % np.uint64(q)).astype(np.uint64)
+    in2 = ((np.uint64(0xFEDCBA987654321) ^ (i * np.uint64(0xDEADBEEF))) % np.uint64(q)).astype(np.uint64)
+    s = 0x123456789ABCDEF
+    big = {
+        O.O_MUL: lambda o, a, b: a * b % q,
+        O.O_MUL_ADDTO: lambda o, a, b: (o + a * b) % q,
+        O.O_MUL_SUBTO: lambda o, a, b: (o - a * b) % q,
+        O.O_SCALE: lambda o, a, b: a * (s % q) % q,
+        O.O_FMA: lambda o, a, b: (o + a * (s % q)) % q,
+        O.O_ADD: lambda o, a, b: (a + b) % q,
+        O.O_SUB: lambda o, a, b: (a - b) % q,
+        O.O_NEGATE: lambda o, a, b: (-a) % q,
+        O.O_REDUCE: lambda o, a, b: a % q,
+        O.O_ADD_SCALAR: lambda o, a, b: (a + s % q) % q,
+        O.O_SUB_SCALAR: lambda o, a, b: (a - s % q) % q,
+    }
+    start = in2.copy()
+    for op, f in big.items():
+        out = start.copy()
+        O.lib().o_eltwise(op, O.ptr(out), O.ptr(in1), O.ptr(in2), s, n, q)
+        exp = [f(int(o), int(a), int(b)) for o, a, b in zip(start, in1, in2)]
+        assert [int(v) for v in out] == exp, op
+    sig = np.array([0, -(2**63), -1, 1] + [(1 if k % 2 else -1) * k * 0x12345678 for k in range(4, n)], dtype=np.int64)
+    out = np.zeros(n, dtype=np.uint64)
+    O.lib().o_eltwise(O.O_REDUCE_SIGNED, O.ptr(out), O.ptr(sig.view(np.uint64)), O.ptr(in2), 0, n, q)
+    assert [int(v) for v in out] == [int(v) % q for v in sig]
+    for engine in O.ref_engines():
+        if bits < 20:
+            continue
+        L = O.ref_lib(engine)
+        proc = L.ntt_new_proc(n, q)
+        for op, name in ((O.O_MUL, "mod_eltwise_mul"), (O.O_ADD, "mod_eltwise_add"), (O.O_SUB, "mod_eltwise_sub")):
+            a, b, o1, o2 = O.aligned(n), O.aligned(n), O.aligned(n), O.aligned(n)
+            a[:], b[:] = in1, in2
+            getattr(L, name)(O.ptr(o1), O.ptr(a), O.ptr(b), n, proc)
+            O.lib().o_eltwise(op, O.ptr(o2), O.ptr(a), O.ptr(b), 0, n, q)
+            assert np.array_equal(o1, o2), (engine, name, q)
+        L.ntt_free_proc(proc)
+
+
+# --- RNS layer: big-integer CRT closed forms ----------------------------------------------------
+
+def crt(res, mods):
+    M = math.prod(mods)
+    return sum(r * (M // m) * pow(M // m, -1, m) for r, m in zip(res, mods)) % M
+
+
+def small_ctx(N=16, L=5, seed=1):
+    primes = O.gen_primes(N, [50] * L)
+    return primes, O.Ctx(N, primes), np.random.default_rng(seed)
+
+
+@pytest.mark.parametrize("rnd", [True, False])
+def test_division_closed_form(rnd):
+    """Dividing by the primes of D in ascending order equals, per coefficient X (CRT value in [0, Q)),
+    applying X <- floor((X + h_p) / p) with h_p = floor(p/2) (round) or 0 (floor), one prime at a time."""
+    primes, c, rng = small_ctx()
+    L, N = len(primes), 16
+    for dm in (0b10000, 0b11000, 0b01010, 0b11110):
+        x = np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in primes])
+        y = x.copy()
+        m = c.division(y, (1 << L) - 1, dm, rnd)
+        assert m == ((1 << L) - 1) & ~dm
+        for k in range(N):
+            mods = list(primes)
+            X = crt([int(x[i][k]) for i in range(L)], mods)
+            for idx in range(L):
+                if not (dm >> idx) & 1:
+                    continue
+                p = primes[idx]
+                X = (X + (p // 2 if rnd else 0)) // p
+                mods.remove(p)
+                X %= math.prod(mods)
+            for i in range(L):
+                if (m >> i) & 1:
+                    assert int(y[i][k]) == X % primes[i]
+                else:
+                    assert int(y[i][k]) == 0
+    # a planted mistake -- the rounding offset h_p dropped (or added for floor) -- fails the check
+    x = np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in primes])
+    y = x.copy()
+    c.division(y, (1 << L) - 1, 0b11000, rnd)
+    mism = 0
+    for k in range(N):
+        X = crt([int(x[i][k]) for i in range(L)], primes)
+        for p in (primes[3], primes[4]):
+            X = (X + (0 if rnd else p // 2)) // p
+        X %= math.prod(primes[:3])
+        mism += any(int(y[i][k]) != X % primes[i] for i in range(3))
+    assert mism > 0
+
+
+def test_mod_reduce_lifted_closed_form():
+    primes, c, rng = small_ctx()
+    x = np.stack([rng.integers(0, q, 16, dtype=np.uint64) for q in primes])
+    out = np.zeros_like(x)
+    c.mod_reduce_lifted(out, 0b11011, x, 2)
+    for i, q in enumerate(primes):
+        exp = [int(v) % q for v in x[2]] if (0b11011 >> i) & 1 else [0] * 16
+        assert [int(v) for v in out[i]] == exp
+
+
+def test_base_conversion_closed_form():
+    """out_p = sum_j [x_j Dhat_j]_{q_j} (D/q_j) mod p, i.e. X + u*D for some 0 <= u < w (no overflow correction)."""
+    primes, c, rng = small_ctx()
+    in_mask, out_mask = 0b00111, 0b11111
+    x = np.zeros((5, 16), dtype=np.uint64)
+    for i in range(3):
+        x[i] = rng.integers(0, primes[i], 16, dtype=np.uint64)
+    out = np.zeros_like(x)
+    c.base_conversion(out, out_mask, x, in_mask)
+    D = primes[:3]
+    Dp = math.prod(D)
+    for k in range(16):
+        X = crt([int(x[i][k]) for i in range(3)], D)
+        y = sum(int(x[j][k]) * pow(Dp // D[j], -1, D[j]) % D[j] * (Dp // D[j]) for j in range(3))
+        assert (y - X) % Dp == 0 and 0 <= (y - X) // Dp < 3
+        for t in (3, 4):
+            assert int(out[t][k]) == y % primes[t]
+        for i in range(3):
+            assert out[i][k] == x[i][k]
+
+
+def test_scaled_lift_closed_form():
+    primes, c, rng = small_ctx()
+    in_mask, out_mask = 0b00111, 0b11111
+    x = np.stack([rng.integers(0, q, 16, dtype=np.uint64) for q in primes])
+    out = np.zeros_like(x)
+    c.scaled_lift(out, out_mask, x, in_mask)
+    delta = primes[3] * primes[4]
+    for i in range(5):
+        for k in range(16):
+            exp = int(x[i][k]) * delta % primes[i] if i < 3 else 0
+            assert int(out[i][k]) == exp
+
+
+@pytest.mark.parametrize("split", [1, 4])
+def test_permute_closed_form(split):
+    """X -> X^gen on the integer polynomial (test_arith.py:83-88 composition, plus the direct map)."""
+    N = 16
+    primes = O.gen_primes(N, [50] * 2, split)
+    c = O.Ctx(N, primes, split)
+    n = N // split
+    rng = np.random.default_rng(9)
+    coeffs = [[R.randrange(q) for _ in range(N)] for q in primes]
+    x = np.zeros((2, N), dtype=np.uint64)
+    for i in range(2):
+        for j in range(N):
+            x[i][(j % split) * n + j // split] = coeffs[i][j]
+    out = np.zeros_like(x)
+    back = np.zeros_like(x)
+    gen = 3
+    c.permute(out, x, 0b11, gen)
+    c.permute(back, out, 0b11, 11)     # 3 * 11 = 33 = 1 mod 2N
+    assert np.array_equal(back, x)
+    for i, q in enumerate(primes):
+        exp = [0] * N
+        for j in range(N):
+            e = j * gen % (2 * N)
+            exp[e % N] = (coeffs[i][j] if e < N else -coeffs[i][j]) % q
+        got = [int(out[i][(j % split) * n + j // split]) for j in range(N)]
+        assert got == exp
+    del rng
+
+
+def test_split_product_is_ring_product():
+    """With split degree S, NTT-domain block products with the w wrap equal the product in Z_q[X]/(X^N+1)."""
+    N, S = 16, 4
+    primes = O.gen_primes(N, [50] * 2, S)
+    c = O.Ctx(N, primes, S)
+    n = N // S
+    a = [[R.randrange(q) for _ in range(N)] for q in primes]
+    b = [[R.randrange(q) for _ in range(N)] for q in primes]
+
+    def place(v):
+        out = np.zeros((2, N), dtype=np.uint64)
+        for i in range(2):
+            for j in range(N):
+                out[i][(j % S) * n + j // S] = v[i][j]
+        return out
+
+    pa, pb = place(a), place(b)
+    c.to_ntt(pa, pa, 0b11)
+    c.to_ntt(pb, pb, 0b11)
+    pr = np.zeros_like(pa)
+    c.mul(pr, pa, 0b11, pb, 0b11, 0)
+    c.to_coeff(pr, pr, 0b11)
+    for i, q in enumerate(primes):
+        ref = [0] * N
+        for x in range(N):
+            for y in range(N):
+                k = x + y
+                if k < N:
+                    ref[k] = (ref[k] + a[i][x] * b[i][y]) % q
+                else:
+                    ref[k - N] = (ref[k - N] - a[i][x] * b[i][y]) % q
+        got = [int(pr[i][(j % S) * n + j // S]) for j in range(N)]
+        assert got == ref
+
+
+def test_incntt_w_table():
+    """incNTT.w[k] = psi^(2 brv(k) + 1) (rns_polynomial.c:21-27), i.e. the slot value of X."""
+    N = 64
+    primes = O.gen_primes(N, [50] * 2)
+    c = O.Ctx(N, primes)
+    for i, q in enumerate(primes):
+        psi = c.psi(i)
+        w = c.w(i)
+        x = np.zeros(N, dtype=np.uint64)
+        x[1] = 1  # the polynomial X
+        O.lib().o_ntt_forward(O.ptr(x), N, q)
+        assert np.array_equal(x, w)
+        assert all(int(w[k]) == pow(psi, 2 * brv(k, 6) + 1, q) for k in range(N))
+
+
+# --- MLWE: tensor slot identity and decryption with valid keys -----------------------------------
+
+def test_tensor_product_slot_identity():
+    """test_mlwe.py:183-207: b1b2 - sum_slots slot*key == phase(c1)*phase(c2) (r = 2, NTT domain)."""
+    N, r = 16, 2
+    primes = O.gen_primes(N, [50] * 2)
+    c = O.Ctx(N, primes)
+    rng = np.random.default_rng(4)
+    m = 0b11
+
+    def rnd():
+        return np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in primes])
+
+    s = [rnd() for _ in range(r)]
+    c1 = np.stack([rnd() for _ in range(r + 1)])
+    c2 = np.stack([rnd() for _ in range(r + 1)])
+    R = int(O.lib().o_extended_rank(r))
+    out = np.zeros((R + 1, 2, N), dtype=np.uint64)
+    c.tensor_product(out, c1, c2, r, m)
+    tmp = np.zeros((2, N), dtype=np.uint64)
+
+    def phase(ct):
+        acc = ct[r].copy()
+        for i in range(r):
+            c.mul(tmp, ct[i], m, s[i], m, 0)
+            c.addsub(acc, acc, m, tmp, m, True)
+        return acc
+
+    exp = np.zeros((2, N), dtype=np.uint64)
+    c.mul(exp, phase(c1), m, phase(c2), m, 0)
+    keys = []
+    for i in range(r):
+        for j in range(i, r):
+            kk = np.zeros((2, N), dtype=np.uint64)
+            c.mul(kk, s[i], m, s[j], m, 0)
+            c.negate(kk, kk, m)
+            keys.append(kk)
+    keys += s
+    acc = out[R].copy()
+    for k in range(R):
+        c.mul(tmp, out[k], m, keys[k], m, 0)
+        c.addsub(acc, acc, m, tmp, m, True)
+    assert np.array_equal(acc, exp)

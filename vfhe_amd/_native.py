@@ -82,6 +82,7 @@ _SIGNATURES = {
     "incNTT_get_rou_matrix": (ctypes.POINTER(U64P), [VP]),
     "polynomial_new_RNS_polynomial": (VP, [U64, U64, VP]),
     "polynomial_new_RNS_polynomial_array": (VPP, [U64, U64, U64, VP]),
+    "polynomial_new_array_of_RNS_polynomials": (VPP, [U64, U64, U64, VP]),
     "free_RNS_polynomial": (None, [VP]),
     "free_RNS_polynomial_array": (None, [U64, VPP]),
     "polynomial_copy_RNS_polynomial": (None, [VP, VP]),
@@ -154,6 +155,8 @@ _SIGNATURES = {
     "vfhe_hip_synchronize": (None, []),
     "vfhe_hip_last_error": (I32, []),
     "vfhe_hip_device_name": (ctypes.c_char_p, []),
+    "vfhe_hip_profile_start": (None, []),
+    "vfhe_hip_profile_stop": (None, [ctypes.POINTER(ctypes.c_double), U64P]),
     "vfhe_hip_ntt_batch": (None, [VPP, U64, I32]),
     "vfhe_hip_ntt_mul_intt_batch": (None, [VPP, VPP, VPP, U64]),
     "vfhe_hip_mlwe_multiply_batch": (None, [VPP, VPP, VPP, U64, VPP, U64]),

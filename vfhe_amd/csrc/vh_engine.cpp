@@ -137,6 +137,39 @@ template <typename T> static T *vh_stage_vec(const std::vector<T> &v)
     return reinterpret_cast<T *>(vh_stage(v.data(), v.size() * sizeof(T)));
 }
 
+// ---------------------------------------------------------------------------
+// Launch profiler: when on, every kernel launch is bracketed by two HIP events
+// on the engine stream and its duration is summed per kernel family.
+// ---------------------------------------------------------------------------
+struct vh_prof_rec { int tag; hipEvent_t a, b; };
+static bool g_prof = false;
+static std::vector<vh_prof_rec> g_prof_recs;
+static std::vector<hipEvent_t> g_prof_free;
+
+static hipEvent_t prof_event()
+{
+    if (!g_prof_free.empty()) { hipEvent_t e = g_prof_free.back(); g_prof_free.pop_back(); return e; }
+    hipEvent_t e;
+    VH_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+struct vh_prof_scope {
+    int tag; hipEvent_t a = nullptr, b = nullptr;
+    explicit vh_prof_scope(int t) : tag(t)
+    {
+        if (!g_prof) return;
+        a = prof_event(); b = prof_event();
+        VH_CHECK(hipEventRecord(a, C.stream));
+    }
+    ~vh_prof_scope()
+    {
+        if (!a) return;
+        VH_CHECK(hipEventRecord(b, C.stream));
+        g_prof_recs.push_back({tag, a, b});
+    }
+};
+
 static void *vh_dmalloc(size_t bytes)
 {
     void *p = nullptr;
@@ -370,17 +403,17 @@ static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const v
             std::vector<vh_item> chunk(v.begin() + off, v.begin() + off + cnt);
             const vh_item *d_items = vh_stage_vec(chunk);
             if (logn <= 12) {
-                VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, dir, logn, 0, logn, 0, 1, 1, 1));
+                { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, dir, logn, 0, logn, 0, 1, 1, 1)); }
             } else {
                 const int R1 = logn - 8, R2 = 8;
                 const int Gc = std::min(1 << (logn - R1), 4096 >> R1);
                 const int Gr = std::min(1 << R1, 16);
                 if (dir == 0) {
-                    VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 0, logn, 0, R1, 1, Gc, 1, 0));
-                    VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 0, logn, R1, R2, 0, Gr, 0, 1));
+                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 0, logn, 0, R1, 1, Gc, 1, 0)); }
+                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 0, logn, R1, R2, 0, Gr, 0, 1)); }
                 } else {
-                    VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 1, logn, R1, R2, 0, Gr, 1, 0));
-                    VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 1, logn, 0, R1, 1, Gc, 0, 1));
+                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 1, logn, R1, R2, 0, Gr, 1, 0)); }
+                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 1, logn, 0, R1, 1, Gc, 0, 1)); }
                 }
             }
         }
@@ -392,7 +425,10 @@ static void run_eltwise(std::vector<vh_item> &items, const vh_limb *d_limbs, int
     for (size_t off = 0; off < items.size(); off += VH_MAX_GRID_Y) {
         const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, items.size() - off);
         std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
-        VH_CHECK(vh_launch_eltwise(C.stream, vh_stage_vec(chunk), cnt, d_limbs, op, n));
+        {
+            vh_prof_scope ps_(VH_PROF_ELTWISE);
+            VH_CHECK(vh_launch_eltwise(C.stream, vh_stage_vec(chunk), cnt, d_limbs, op, n));
+        }
     }
 }
 
@@ -828,7 +864,10 @@ extern "C" void int_array_to_RNS(RNS_Polynomial out, uint64_t *in)
         items.push_back(mk_item(tmp, drow(out, i), i, nullptr, out->ntt->split_degree));
     });
     if (!items.empty())
-        VH_CHECK(vh_launch_scatter_split(C.stream, vh_stage_vec(items), (unsigned)items.size(), DL(out), N, 1));
+        {
+            vh_prof_scope ps_(VH_PROF_OTHER);
+            VH_CHECK(vh_launch_scatter_split(C.stream, vh_stage_vec(items), (unsigned)items.size(), DL(out), N, 1));
+        }
     vh_dfree(tmp);
     ntt_rows({{out, out}}, 0);
 }
@@ -851,7 +890,10 @@ extern "C" void array_to_RNS(RNS_Polynomial out, uint64_t **in)
     });
     VH_CHECK(hipStreamSynchronize(C.stream));
     if (!items.empty())
-        VH_CHECK(vh_launch_scatter_split(C.stream, vh_stage_vec(items), (unsigned)items.size(), DL(out), N, 0));
+        {
+            vh_prof_scope ps_(VH_PROF_OTHER);
+            VH_CHECK(vh_launch_scatter_split(C.stream, vh_stage_vec(items), (unsigned)items.size(), DL(out), N, 0));
+        }
     vh_dfree(tmp);
     ntt_rows({{out, out}}, 0);
 }
@@ -886,7 +928,10 @@ static void run_muls(const std::vector<vh_mulreq> &reqs, int mode)
         for (size_t off = 0; off < items.size(); off += VH_MAX_GRID_Y) {
             const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, items.size() - off);
             std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
-            VH_CHECK(vh_launch_split_mul(C.stream, vh_stage_vec(chunk), cnt, P->d_limbs, mode, P->n, t->split_degree));
+            {
+                vh_prof_scope ps_(VH_PROF_ELTWISE);
+                VH_CHECK(vh_launch_split_mul(C.stream, vh_stage_vec(chunk), cnt, P->d_limbs, mode, P->n, t->split_degree));
+            }
         }
     }
 }
@@ -1099,8 +1144,11 @@ static void run_division(const std::vector<RNS_Polynomial> &polys, uint64_t divi
                 for (size_t off = 0; off < jobs.size(); off += VH_MAX_GRID_Y) {
                     const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, jobs.size() - off);
                     std::vector<vh_div_job> chunkv(jobs.begin() + off, jobs.begin() + off + cnt);
-                    VH_CHECK(vh_launch_division(C.stream, vh_stage_vec(chunkv), cnt, (unsigned)outs.size(),
+                    {
+                        vh_prof_scope ps_(VH_PROF_DIVISION);
+                        VH_CHECK(vh_launch_division(C.stream, vh_stage_vec(chunkv), cnt, (unsigned)outs.size(),
                                                 VPOOL(t)->d_limbs, d_consts, t->N));
+                    }
                 }
             }
             std::vector<vh_item> zi;
@@ -1227,7 +1275,10 @@ extern "C" void polynomial_base_conversion_RNSc(RNSc_Polynomial out, RNSc_Polyno
         for (uint32_t t = 0; t < p->v; t++) jb.P[t] = (int)p->P[t];
         const void *d_consts = vh_stage(consts.data(), consts.size() * sizeof(ulonglong2));
         const vh_bconv_job *d_job = (const vh_bconv_job *)vh_stage(&jb, sizeof(jb));
-        VH_CHECK(vh_launch_base_conversion(C.stream, d_job, 1, p->v, DL(out), d_consts, out->ntt->N));
+        {
+            vh_prof_scope ps_(VH_PROF_OTHER);
+            VH_CHECK(vh_launch_base_conversion(C.stream, d_job, 1, p->v, DL(out), d_consts, out->ntt->N));
+        }
     }
     if (!params) free_base_conversion_params(p);
     out->rns_mask = out_mask;
@@ -1284,7 +1335,10 @@ static void permute_into(RNS_Polynomial out, RNS_Polynomial in, uint64_t gen)
             zero.push_back(mk_item(nullptr, drow(out, j), (uint32_t)j));
     }
     if (!perm.empty())
-        VH_CHECK(vh_launch_permute(C.stream, vh_stage_vec(perm), (unsigned)perm.size(), DL(out), out->ntt->N));
+        {
+            vh_prof_scope ps_(VH_PROF_OTHER);
+            VH_CHECK(vh_launch_permute(C.stream, vh_stage_vec(perm), (unsigned)perm.size(), DL(out), out->ntt->N));
+        }
     run_eltwise(zero, DL(out), VH_OP_ZERO, out->ntt->N);
 }
 
@@ -1565,7 +1619,10 @@ static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector
             for (size_t off = 0; off < B; off += VH_MAX_GRID_Y) {
                 const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, B - off);
                 std::vector<vh_mac_job> chunk(jobs.begin() + off, jobs.begin() + off + cnt);
-                VH_CHECK(vh_launch_keyswitch_mac(C.stream, vh_stage_vec(chunk), cnt, (unsigned)next, P->d_limbs, d_kptr, N));
+                {
+                    vh_prof_scope ps_(VH_PROF_KSMAC);
+                    VH_CHECK(vh_launch_keyswitch_mac(C.stream, vh_stage_vec(chunk), cnt, (unsigned)next, P->d_limbs, d_kptr, N));
+                }
             }
         } else {
             // Split-degree rings: zero the outputs, then accumulate one digit at a time
@@ -1592,7 +1649,10 @@ static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector
                 const size_t per = Cc * next;
                 for (uint64_t d = 0; d < ND; d++) {
                     std::vector<vh_item> chunk(items.begin() + d * per, items.begin() + (d + 1) * per);
-                    VH_CHECK(vh_launch_split_mul(C.stream, vh_stage_vec(chunk), (unsigned)chunk.size(), P->d_limbs, 2, P->n, t->split_degree));
+                    {
+                        vh_prof_scope ps_(VH_PROF_ELTWISE);
+                        VH_CHECK(vh_launch_split_mul(C.stream, vh_stage_vec(chunk), (unsigned)chunk.size(), P->d_limbs, 2, P->n, t->split_degree));
+                    }
                 }
                 items.clear();
             }
@@ -1801,4 +1861,34 @@ extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x
     for (uint64_t i = 0; i < count; i++) inv.push_back({z[i], z[i]});
     ntt_rows(inv, 1);
     for (auto p : tmp) free_RNS_polynomial(p);
+}
+
+
+// ---------------------------------------------------------------------------
+// Profiler API
+// ---------------------------------------------------------------------------
+extern "C" void vfhe_hip_profile_start(void)
+{
+    VH_ENTER;
+    VH_CHECK(hipStreamSynchronize(C.stream));
+    for (auto &r : g_prof_recs) { g_prof_free.push_back(r.a); g_prof_free.push_back(r.b); }
+    g_prof_recs.clear();
+    g_prof = true;
+}
+
+extern "C" void vfhe_hip_profile_stop(double *ms, uint64_t *launches)
+{
+    VH_ENTER;
+    VH_CHECK(hipStreamSynchronize(C.stream));
+    for (int t = 0; t < VH_PROF_TAGS; t++) { ms[t] = 0; launches[t] = 0; }
+    for (auto &r : g_prof_recs) {
+        float e = 0;
+        VH_CHECK(hipEventElapsedTime(&e, r.a, r.b));
+        ms[r.tag] += e;
+        launches[r.tag]++;
+        g_prof_free.push_back(r.a);
+        g_prof_free.push_back(r.b);
+    }
+    g_prof_recs.clear();
+    g_prof = false;
 }

@@ -8,6 +8,12 @@
 #define VH_MAX_LIMBS 64   // masks are uint64_t, as in the reference
 #define VH_MAX_COMPS 8    // MLWE components (rank + 1) of one key-switch output
 
+// Kernel families reported by the engine's launch profiler (vfhe_hip_profile_stop).
+enum {
+    VH_PROF_NTT_FP64 = 0, VH_PROF_NTT_INT, VH_PROF_ELTWISE, VH_PROF_DIVISION, VH_PROF_KSMAC, VH_PROF_OTHER,
+    VH_PROF_TAGS
+};
+
 enum {
     VH_OP_MUL = 0, VH_OP_MUL_ADDTO, VH_OP_MUL_SUBTO, VH_OP_SCALE, VH_OP_FMA, VH_OP_ADD, VH_OP_SUB,
     VH_OP_NEGATE, VH_OP_REDUCE, VH_OP_REDUCE_SIGNED, VH_OP_ADD_SCALAR, VH_OP_SUB_SCALAR,
