@@ -54,6 +54,10 @@ def parse():
     ap.add_argument("--no-ckks", action="store_true")
     ap.add_argument("--ckks-batch", type=int, default=32)
     ap.add_argument("--ckks-steps", type=int, default=5)
+    ap.add_argument("--only-ntt", action="store_true",
+                    help="run unit A alone (the command profiled with rocprofv3)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also copy the 64-polynomial batch once per step (known-size copy for counter scaling)")
     return ap.parse_args()
 
 
@@ -141,6 +145,9 @@ def main():
     lib = _native.load()
     primes = params.gen_primes(N, [50] * L)
     log("device", lib.vfhe_hip_device_name().decode(), "primes", primes[0], "...", primes[-1])
+    sched = {k: int(lib.vfhe_hip_get_option(k.encode())) for k in
+             ("ntt.mode", "ntt.xcd_available", "ntt.xcd_lag", "ntt.xcd_occupancy", "ntt.chunk_mb")}
+    log("schedule", sched, "probe", [int(lib.vfhe_hip_get_option(f"probe.xcd{k}".encode())) for k in range(8)])
     pool = rt.Pool(N, primes)
     rng = np.random.default_rng(20261202)
     polys = []
@@ -167,8 +174,10 @@ def main():
     value = POLYS / dt
     ntt_ms, ntt_launches = prof.ms[0], prof.launches[0]
     avg_launch_s = ntt_ms / 1e3 / max(1, ntt_launches)
-    # 4 pass launches per step; a transform's 16*N bytes per limb are shared by its two passes
-    bytes_per_launch = POLYS * L * 16 * N / 2
+    # Algorithmic bytes of a step: every limb row is read and written once per direction
+    # (16*N bytes per limb per transform); the step's pass launches share them evenly.
+    step_bytes = POLYS * L * 2 * 16 * N
+    bytes_per_launch = step_bytes * args.steps / max(1, ntt_launches)
     achieved = bytes_per_launch / avg_launch_s / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "ntt_pass_traffic.json")
@@ -177,6 +186,18 @@ def main():
             traffic = json.load(f).get("hbm_bytes_per_launch")
     log(f"unit A: {dt * 1e3:.3f} ms/step, {value:.1f} round-trips/s, NTT launches {ntt_launches}, "
         f"avg {avg_launch_s * 1e3:.3f} ms, {achieved:.0f} GB/s algorithmic")
+
+    if args.calibrate_copy:
+        dst = [pool.poly() for _ in range(POLYS)]
+        for _ in range(args.steps):
+            for d, p in zip(dst, polys):
+                lib.polynomial_copy_RNS_polynomial(d, p)
+        lib.vfhe_hip_synchronize()
+        log(f"copy calibration: {args.steps} x {POLYS} copies of {L * N * 8} bytes")
+    if args.only_ntt:
+        print(json.dumps({"schedule": sched, "unit_a_ms_per_step": dt * 1e3, "value": value, "ntt_launches": ntt_launches,
+                          "avg_launch_ms": avg_launch_s * 1e3, "achieved_gbs": achieved}), flush=True)
+        return
 
     # fused product (unit B)
     xs = polys[:POLYS]
@@ -235,7 +256,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "batched NTT round trip, 64 polynomials x 24 limbs, N=2^16, ~50-bit primes "
                                "q=1 mod 2^17 (Ring.gen_prime), uniform residues, seed 20261202",
-                   "N": N, "L": L, "polys": POLYS},
+                   "N": N, "L": L, "polys": POLYS, "schedule": sched},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_ntt_pass (FP64 path)", "launches": ntt_launches,

@@ -203,6 +203,10 @@ void mlwe_RNS_mul_addto_by_poly(RNS_MLWE out, RNS_MLWE in, RNS_Polynomial poly);
 void mlwe_RNS_mul_subto_by_poly(RNS_MLWE out, RNS_MLWE in, RNS_Polynomial poly);                      /* mlwe.c:272-279 */
 void mlwe_add_RNSc_sample(RNSc_MLWE out, RNSc_MLWE in1, RNSc_MLWE in2);                               /* mlwe.c:446-453 */
 void mlwe_sub_RNSc_sample(RNSc_MLWE out, RNSc_MLWE in1, RNSc_MLWE in2);                               /* mlwe.c:484-491 */
+void mlwe_add_RNSc_polynomial(RNSc_MLWE out, RNSc_MLWE in1, RNSc_Polynomial in2);                     /* mlwe.c:464-467 */
+void mlwe_sub_RNSc_polynomial(RNSc_MLWE out, RNSc_MLWE in1, RNSc_Polynomial in2);                     /* mlwe.c:469-472 */
+void mlwe_RNS_add_polynomial(RNS_MLWE out, RNS_MLWE in1, RNS_Polynomial in2);                          /* mlwe.c:474-477 */
+void mlwe_RNS_sub_polynomial(RNS_MLWE out, RNS_MLWE in1, RNS_Polynomial in2);                          /* mlwe.c:479-482 */
 void mlwe_scale_RNS_mlwe_RNS(RNS_MLWE c, uint64_t *scale);                                            /* mlwe.c:217-225 */
 void mlwe_round_division_RNSc(RNSc_MLWE out, uint64_t divide_mask);                                   /* mlwe.c:762-772 */
 uint64_t mlwe_extended_rank(uint64_t r);                                                              /* mlwe.c:659-663 */
@@ -236,6 +240,19 @@ uint64_t *polynomial_device_row(RNS_Polynomial p, uint64_t limb);
 void vfhe_hip_synchronize(void);
 int vfhe_hip_last_error(void);
 const char *vfhe_hip_device_name(void);
+
+/* Tunables, also readable from the environment at first use (VFHE_HIP_NTT_MODE,
+   VFHE_HIP_NTT_CHUNK_MB, VFHE_HIP_XCD_LAG, VFHE_HIP_XCD_OCC):
+     "ntt.mode"          0: two launches per direction (default); 1: persistent per-XCD kernel
+     "ntt.chunk_mb"      two-launch mode: MiB of rows per sub-batch, 0 = whole batch
+     "ntt.xcd_lag"       persistent mode: queue slots between the two passes of a transform
+     "ntt.xcd_occupancy" persistent mode: workgroups per CU, 2 or 4
+     "ntt.spin_limit"    persistent mode: bound on a wait; exceeding it aborts at the next sync
+   get also answers "ntt.xcd_available" (1 when the placement probe found XCC ids 0..7)
+   and "probe.xcd0".."probe.xcd7" (workgroups of the probe placed on each XCC).
+   set returns 0, or -1 for an unknown key; get returns -1 for an unknown key. */
+int vfhe_hip_set_option(const char *key, long value);
+long vfhe_hip_get_option(const char *key);
 
 /* Launch profiler: between start and stop every kernel launch is bracketed by
    HIP events on the engine stream.  stop() waits, then writes the summed

@@ -138,6 +138,10 @@ _SIGNATURES = {
     "mlwe_RNS_mul_subto_by_poly": (None, [VP, VP, VP]),
     "mlwe_add_RNSc_sample": (None, [VP, VP, VP]),
     "mlwe_sub_RNSc_sample": (None, [VP, VP, VP]),
+    "mlwe_add_RNSc_polynomial": (None, [VP, VP, VP]),
+    "mlwe_sub_RNSc_polynomial": (None, [VP, VP, VP]),
+    "mlwe_RNS_add_polynomial": (None, [VP, VP, VP]),
+    "mlwe_RNS_sub_polynomial": (None, [VP, VP, VP]),
     "mlwe_scale_RNS_mlwe_RNS": (None, [VP, U64P]),
     "mlwe_round_division_RNSc": (None, [VP, U64]),
     "mlwe_extended_rank": (U64, [U64]),
@@ -155,6 +159,8 @@ _SIGNATURES = {
     "vfhe_hip_synchronize": (None, []),
     "vfhe_hip_last_error": (I32, []),
     "vfhe_hip_device_name": (ctypes.c_char_p, []),
+    "vfhe_hip_set_option": (I32, [ctypes.c_char_p, ctypes.c_long]),
+    "vfhe_hip_get_option": (ctypes.c_long, [ctypes.c_char_p]),
     "vfhe_hip_profile_start": (None, []),
     "vfhe_hip_profile_stop": (None, [ctypes.POINTER(ctypes.c_double), U64P]),
     "vfhe_hip_ntt_batch": (None, [VPP, U64, I32]),

@@ -46,6 +46,15 @@ struct vh_item {
     uint64_t scalar_sh;   // Shoup companion of `scalar` when used as a multiplier
 };
 
+// Global address space: pointers read from work lists are otherwise generic
+// (flat) pointers, whose loads wait on both the vector and the LDS counters.
+#define VH_GLOBAL __attribute__((address_space(1)))
+template <typename T> __device__ __forceinline__ const VH_GLOBAL T *vh_global(const T *p)
+{
+    return (const VH_GLOBAL T *)p;
+}
+template <typename T> __device__ __forceinline__ VH_GLOBAL T *vh_global(T *p) { return (VH_GLOBAL T *)p; }
+
 #define VH_MAGIC 6755399441055744.0   /* 1.5 * 2^52 */
 #define VH_TWO52 4503599627370496.0   /* 2^52 */
 

@@ -81,8 +81,22 @@ struct vh_ctx {
     hipEvent_t half_ev[2];
     bool half_used[2] = {false, false};
     char name[128] = {0};
+    int cus = 0;
+    unsigned *d_err = nullptr;   // device error word written by kernels with bounded waits
+    bool xcd_ok = false;         // the placement probe saw exactly the XCC ids 0..7
+    unsigned xcd_hist[8] = {0};
 };
 static vh_ctx C;
+
+// Tunables (vfhe_hip_set_option): NTT schedule for N = 2^13..2^16 on the FP64 path.
+struct vh_opts {
+    long ntt_mode = 0;        // 0: two launches per direction, 1: persistent per-XCD kernel
+    long ntt_chunk_mb = 0;    // two-launch mode: rows per sub-batch (MiB), 0 = whole batch
+    long xcd_lag = 3;         // persistent mode: queue slots between an item's two passes
+    long xcd_occ = 2;         // persistent mode: workgroups per CU (2 or 4)
+    long spin_limit = 1 << 22;
+};
+static vh_opts OPT;
 static std::recursive_mutex vh_mu;
 
 static void vh_ensure()
@@ -105,6 +119,29 @@ static void vh_ensure()
     C.ring_size = (size_t)64 << 20;
     VH_CHECK(hipHostMalloc((void **)&C.h_ring, C.ring_size, hipHostMallocDefault));
     VH_CHECK(hipMalloc((void **)&C.d_ring, C.ring_size));
+    C.cus = prop.multiProcessorCount;
+    if (const char *e = getenv("VFHE_HIP_NTT_MODE")) OPT.ntt_mode = atol(e);
+    if (const char *e = getenv("VFHE_HIP_NTT_CHUNK_MB")) OPT.ntt_chunk_mb = atol(e);
+    if (const char *e = getenv("VFHE_HIP_XCD_LAG")) OPT.xcd_lag = atol(e);
+    if (const char *e = getenv("VFHE_HIP_XCD_OCC")) OPT.xcd_occ = atol(e);
+    VH_CHECK(hipMalloc((void **)&C.d_err, 256));
+    VH_CHECK(hipMemset(C.d_err, 0, 256));
+    {
+        // Placement probe for the persistent per-XCD NTT: it relies on every XCC id
+        // 0..7 hosting workgroups (k_ntt_xcd_verify still guards each launch).
+        const unsigned nb = (unsigned)C.cus * 4;
+        unsigned *d = nullptr;
+        VH_CHECK(hipMalloc((void **)&d, nb * sizeof(unsigned)));
+        VH_CHECK(vh_launch_xcc_probe(C.stream, d, nb));
+        std::vector<unsigned> h(nb);
+        VH_CHECK(hipMemcpyAsync(h.data(), d, nb * sizeof(unsigned), hipMemcpyDeviceToHost, C.stream));
+        VH_CHECK(hipStreamSynchronize(C.stream));
+        VH_CHECK(hipFree(d));
+        bool ok = true;
+        for (unsigned x : h) { if (x < 8) C.xcd_hist[x]++; else ok = false; }
+        for (int k = 0; k < 8; k++) ok = ok && C.xcd_hist[k] > 0;
+        C.xcd_ok = ok;
+    }
     VH_CHECK(hipEventCreateWithFlags(&C.half_ev[0], hipEventDisableTiming));
     VH_CHECK(hipEventCreateWithFlags(&C.half_ev[1], hipEventDisableTiming));
     C.ready = true;
@@ -389,6 +426,15 @@ extern "C" NTT_proc *new_ntt_list(uint64_t *primes, uint64_t N, uint64_t l)
 // ---------------------------------------------------------------------------
 static const unsigned VH_MAX_GRID_Y = 65535;
 
+// Checks the device error word at a synchronisation point (the stream is idle).
+static void vh_check_device_error()
+{
+    unsigned e = 0;
+    VH_CHECK(hipMemcpy(&e, C.d_err, sizeof(e), hipMemcpyDeviceToHost));
+    if (e & 1) vh_die("a persistent NTT wait exceeded its spin limit (results are invalid)", __FILE__, __LINE__);
+    if (e) vh_die("a persistent NTT launch left transforms incomplete (results are invalid)", __FILE__, __LINE__);
+}
+
 // Forward (dir 0) or inverse (dir 1) transforms of n-point rows.
 static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const vh_limb *h_limbs, uint64_t n, int dir)
 {
@@ -396,13 +442,47 @@ static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const v
     const int logn = ilog2(n);
     std::vector<vh_item> grp[2];
     for (auto &it : items) grp[h_limbs[it.limb].fp64 ? 1 : 0].push_back(it);
+    // Limb-major order: workgroups running together share one limb's twiddle table in L2.
+    for (auto &g : grp)
+        std::stable_sort(g.begin(), g.end(), [](const vh_item &a, const vh_item &b) { return a.limb < b.limb; });
     for (int fp = 0; fp < 2; fp++) {
         auto &v = grp[fp];
         for (size_t off = 0; off < v.size(); off += VH_MAX_GRID_Y) {
             const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, v.size() - off);
             std::vector<vh_item> chunk(v.begin() + off, v.begin() + off + cnt);
             const vh_item *d_items = vh_stage_vec(chunk);
-            if (logn <= 12) {
+            if (fp && logn >= 13 && logn <= 16 && OPT.ntt_mode == 1 && C.xcd_ok) {
+                // Persistent kernel: both passes of every transform on one XCD, the
+                // intermediate exchanged through that XCD's L2.
+                unsigned *ctl = (unsigned *)vh_dmalloc((16 + 2 * cnt) * sizeof(unsigned));
+                VH_CHECK(vh_launch_zero_u32(C.stream, ctl, 16 + 2 * cnt));
+                const int occ = OPT.xcd_occ >= 4 ? 4 : 2;
+                {
+                    vh_prof_scope ps_(VH_PROF_NTT_FP64);
+                    VH_CHECK(vh_launch_ntt_xcd(C.stream, d_items, cnt, d_limbs, dir, logn, ctl, C.d_err,
+                                               (unsigned)OPT.xcd_lag, (unsigned)OPT.spin_limit,
+                                               (unsigned)(C.cus * occ), occ));
+                }
+                VH_CHECK(vh_launch_ntt_xcd_verify(C.stream, ctl, cnt, logn, C.d_err));
+                vh_dfree(ctl);
+            } else if (fp && logn >= 13 && logn <= 16) {
+                // Two register radix-16 passes (column pass of R1 = logn - 8 stages, row pass of 8),
+                // optionally over sub-batches of ntt_chunk_mb MiB of rows.
+                const int R1 = logn - 8;
+                const uint64_t chunk = OPT.ntt_chunk_mb > 0 ? (uint64_t)OPT.ntt_chunk_mb << 20 : UINT64_MAX;
+                const unsigned per = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(cnt, chunk / (n * 8)));
+                for (unsigned c0 = 0; c0 < cnt; c0 += per) {
+                    const unsigned cc = std::min(per, cnt - c0);
+                    const vh_item *ci = d_items + c0;
+                    if (dir == 0) {
+                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(C.stream, ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0)); }
+                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(C.stream, ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1)); }
+                    } else {
+                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(C.stream, ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0)); }
+                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(C.stream, ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1)); }
+                    }
+                }
+            } else if (logn <= 12) {
                 { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, dir, logn, 0, logn, 0, 1, 1, 1)); }
             } else {
                 const int R1 = logn - 8, R2 = 8;
@@ -701,6 +781,7 @@ extern "C" void polynomial_sync_to_host(RNS_Polynomial p)
     if (v->host_dirty) return;
     VH_CHECK(hipMemcpyAsync(v->hblock, v->d, p->allocated_l * p->ntt->N * 8, hipMemcpyDeviceToHost, C.stream));
     VH_CHECK(hipStreamSynchronize(C.stream));
+    vh_check_device_error();
 }
 
 extern "C" void polynomial_mark_host_dirty(RNS_Polynomial p)
@@ -721,6 +802,34 @@ extern "C" void vfhe_hip_synchronize(void)
 {
     VH_ENTER;
     VH_CHECK(hipStreamSynchronize(C.stream));
+    vh_check_device_error();
+}
+
+extern "C" int vfhe_hip_set_option(const char *key, long value)
+{
+    VH_ENTER;
+    if (!key) return -1;
+    if (!strcmp(key, "ntt.mode")) OPT.ntt_mode = value;
+    else if (!strcmp(key, "ntt.chunk_mb")) OPT.ntt_chunk_mb = value;
+    else if (!strcmp(key, "ntt.xcd_lag")) OPT.xcd_lag = value < 0 ? 0 : value;
+    else if (!strcmp(key, "ntt.xcd_occupancy")) OPT.xcd_occ = value;
+    else if (!strcmp(key, "ntt.spin_limit")) OPT.spin_limit = value < 1 ? 1 : value;
+    else return -1;
+    return 0;
+}
+
+extern "C" long vfhe_hip_get_option(const char *key)
+{
+    VH_ENTER;
+    if (!key) return -1;
+    if (!strcmp(key, "ntt.xcd_available")) return C.xcd_ok ? 1 : 0;
+    if (!strncmp(key, "probe.xcd", 9) && key[9] >= '0' && key[9] <= '7' && !key[10]) return C.xcd_hist[key[9] - '0'];
+    if (!strcmp(key, "ntt.mode")) return OPT.ntt_mode;
+    if (!strcmp(key, "ntt.chunk_mb")) return OPT.ntt_chunk_mb;
+    if (!strcmp(key, "ntt.xcd_lag")) return OPT.xcd_lag;
+    if (!strcmp(key, "ntt.xcd_occupancy")) return OPT.xcd_occ;
+    if (!strcmp(key, "ntt.spin_limit")) return OPT.spin_limit;
+    return -1;
 }
 
 extern "C" int vfhe_hip_last_error(void)
@@ -1137,6 +1246,7 @@ static void run_division(const std::vector<RNS_Polynomial> &polys, uint64_t divi
                 jb.nl = (int)l;
                 for (size_t k = 0; k < divs.size(); k++) jb.div_limb[k] = divs[k];
                 for (size_t k = 0; k < outs.size(); k++) jb.out_limb[k] = outs[k];
+                jb.nout = (int)outs.size();
                 jobs.push_back(jb);
             }
             const void *d_consts = vh_stage(consts.data(), consts.size() * sizeof(vh_divc_h));
@@ -1147,7 +1257,7 @@ static void run_division(const std::vector<RNS_Polynomial> &polys, uint64_t divi
                     {
                         vh_prof_scope ps_(VH_PROF_DIVISION);
                         VH_CHECK(vh_launch_division(C.stream, vh_stage_vec(chunkv), cnt, (unsigned)outs.size(),
-                                                VPOOL(t)->d_limbs, d_consts, t->N));
+                                                VPOOL(t)->d_limbs, d_consts, t->N, (int)divs.size()));
                     }
                 }
             }
@@ -1473,6 +1583,20 @@ static void mlwe_binop(RNS_MLWE out, RNS_MLWE a, RNS_MLWE b, int op)
 
 extern "C" void mlwe_add_RNSc_sample(RNSc_MLWE out, RNSc_MLWE in1, RNSc_MLWE in2) { mlwe_binop(out, in1, in2, VH_OP_ADD); }
 extern "C" void mlwe_sub_RNSc_sample(RNSc_MLWE out, RNSc_MLWE in1, RNSc_MLWE in2) { mlwe_binop(out, in1, in2, VH_OP_SUB); }
+
+// mlwe.c:464-482: only the b component changes; a is copied through when out != in1.
+static void mlwe_poly_op(RNS_MLWE out, RNS_MLWE in1, RNS_Polynomial in2, int op)
+{
+    VH_ENTER;
+    if (out != in1)
+        for (uint64_t i = 0; i < out->r; i++) polynomial_copy_RNS_polynomial(out->a[i], in1->a[i]);
+    binop(out->b, in1->b, in2, op);
+}
+
+extern "C" void mlwe_add_RNSc_polynomial(RNSc_MLWE out, RNSc_MLWE in1, RNSc_Polynomial in2) { mlwe_poly_op(out, in1, in2, VH_OP_ADD); }
+extern "C" void mlwe_sub_RNSc_polynomial(RNSc_MLWE out, RNSc_MLWE in1, RNSc_Polynomial in2) { mlwe_poly_op(out, in1, in2, VH_OP_SUB); }
+extern "C" void mlwe_RNS_add_polynomial(RNS_MLWE out, RNS_MLWE in1, RNS_Polynomial in2) { mlwe_poly_op(out, in1, in2, VH_OP_ADD); }
+extern "C" void mlwe_RNS_sub_polynomial(RNS_MLWE out, RNS_MLWE in1, RNS_Polynomial in2) { mlwe_poly_op(out, in1, in2, VH_OP_SUB); }
 
 extern "C" void mlwe_scale_RNS_mlwe_RNS(RNS_MLWE c, uint64_t *scale)
 {

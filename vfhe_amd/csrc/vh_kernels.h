@@ -26,6 +26,8 @@ struct vh_div_job {
     uint64_t stride;     // row length (N)
     int K;               // number of divide primes
     int nl;              // pool size (row count of the constant table)
+    int nout;            // number of output limbs
+    int pad;
     int div_limb[VH_MAX_DIV];
     int out_limb[VH_MAX_LIMBS];
 };
@@ -49,12 +51,22 @@ struct vh_mac_job {
 
 hipError_t vh_launch_ntt_pass(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                               int fp, int dir, int logn, int s0, int R, int col, int G, int first, int last);
+// Register radix-16 pass (FP64 path): 4 <= R <= 8, G = 4096 / 2^R sub-transforms per block.
+hipError_t vh_launch_ntt16(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
+                           int dir, int logn, int s0, int R, int col, int first, int last);
+// Persistent two-pass NTT with per-XCD queues; ctl holds 16 + nitems zeroed uint32 counters.
+hipError_t vh_launch_ntt_xcd(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,
+                             int logn, unsigned *ctl, unsigned *err, unsigned lag, unsigned spin_limit, unsigned nblocks,
+                             int occ);
+hipError_t vh_launch_ntt_xcd_verify(hipStream_t st, const unsigned *ctl, unsigned nitems, int logn, unsigned *err);
+hipError_t vh_launch_xcc_probe(hipStream_t st, unsigned *out, unsigned nblocks);
+hipError_t vh_launch_zero_u32(hipStream_t st, unsigned *p, unsigned n);
 hipError_t vh_launch_eltwise(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                              int op, uint64_t n);
 hipError_t vh_launch_split_mul(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                                int mode, uint64_t n, uint64_t S);
 hipError_t vh_launch_division(hipStream_t st, const vh_div_job *jobs, unsigned njobs, unsigned nout,
-                              const vh_limb *limbs, const void *consts, uint64_t n);
+                              const vh_limb *limbs, const void *consts, uint64_t n, int kmax);
 hipError_t vh_launch_base_conversion(hipStream_t st, const vh_bconv_job *jobs, unsigned njobs, unsigned nout,
                                      const vh_limb *limbs, const void *consts, uint64_t n);
 hipError_t vh_launch_permute(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, uint64_t N);

@@ -120,6 +120,261 @@ k_ntt_pass(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs,
 }
 
 // ---------------------------------------------------------------------------
+// Register radix-16 NTT pass (FP64 path), the hot kernel for N = 2^13 .. 2^16.
+//
+// One pass runs R (4 <= R <= 8) consecutive stages of S = 2^R-point
+// sub-transforms; TPS = S/16 threads own one sub-transform, 16 values each, and
+// a workgroup of 256 threads owns G = 4096/S sub-transforms.  The first four
+// stages (halves S/2 .. S/16) run in registers on the stride-TPS element set
+// k = t + TPS*m; one LDS exchange regroups the values into contiguous sets
+// k = 16t + m for the remaining R-4 stages (halves <= 8).  The inverse runs the
+// same schedule backwards.  Global stage s = s0 + s' uses the reference's ws
+// index 2^s + toff*2^s' + (k >> (R - s')), toff = 0 for column passes and the
+// row index for row passes (see k_ntt_pass for the derivation of the index).
+//
+// Column passes: lanes run over sub-transforms (adjacent columns), so every
+// global access is a contiguous run of G columns.  Row passes: lanes run over
+// the 16 threads of a row, loads in the stride ownership are contiguous; the
+// contiguous-ownership side goes through an LDS transpose.
+// ---------------------------------------------------------------------------
+// LDS doubles one task needs: column passes index [k][g] (lanes run over g), row
+// passes [g][k] with one pad slot per 16 values; both keep the exchange at the
+// 2-way minimum for 64-bit lanes.
+template <int R, bool COL> constexpr int vh_ntt16_lds()
+{
+    return COL ? (1 << R) * (256 / ((1 << R) / 16) + 1) : (256 / ((1 << R) / 16)) * ((1 << R) + (1 << R) / 16);
+}
+
+// One task: the G sub-transforms starting at sub-transform blk*G of one row item.
+// COHERENT: read the input with device-coherent loads (it was written by other
+// workgroups of the same launch and the L1 may hold stale lines of it).
+template <int DIR, int R, bool COL, bool FIRST, bool LAST, bool COHERENT>
+__device__ __forceinline__ void ntt16_task(const vh_item &it, const vh_limb &L, int logn, int s0, size_t blk,
+                                           double *__restrict__ lds)
+{
+    constexpr int S = 1 << R, TPS = S / 16, G = 256 / TPS;
+    const int tid = threadIdx.x;
+    const int g = COL ? (tid % G) : (tid / TPS);
+    const int t = COL ? (tid / G) : (tid % TPS);
+    const double q = L.qd, qinv = L.qinv;
+    const size_t nsub = (size_t)1 << (logn - R);
+    const size_t sub = blk * G + g;
+    const VH_GLOBAL uint64_t *in = vh_global(FIRST ? it.src : it.dst);
+    VH_GLOBAL uint64_t *out = vh_global(it.dst);
+    const VH_GLOBAL double *tw = vh_global(reinterpret_cast<const double *>(DIR == 0 ? L.tw_fwd : L.tw_inv));
+    const size_t toff = COL ? 0 : sub;
+
+    auto gidx = [&](int k) -> size_t { return COL ? sub + (size_t)k * nsub : sub * S + (size_t)k; };
+    auto lidx = [&](int k) -> int { return COL ? k * (G + 1) + g : g * (S + S / 16) + k + (k >> 4); };
+    auto load = [&](size_t j) -> double {
+        const uint64_t raw = COHERENT ? __scoped_atomic_load_n(in + j, __ATOMIC_RELAXED, __MEMORY_SCOPE_DEVICE) : in[j];
+        return FIRST ? vh_u2d(raw) : __longlong_as_double((long long)raw);
+    };
+    auto finish = [&](double x) -> uint64_t {
+        return LAST ? vh_d2u(vh_canon(x, q, qinv)) : (uint64_t)__double_as_longlong(vh_center(x, q, qinv));
+    };
+    auto twid = [&](int sp, size_t local) -> double2 {
+        const size_t i = ((size_t)1 << (s0 + sp)) + (toff << sp) + local;
+        return make_double2(tw[2 * i], tw[2 * i + 1]);
+    };
+
+    double v[16];
+    if (DIR == 0) {
+        // ---- stride ownership, stages s' = 0..3 ----
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = load(gidx(t + TPS * m));
+#pragma unroll
+        for (int sp = 0; sp < 4; sp++) {
+            const int hm = 8 >> sp;
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                if (m & hm) continue;
+                const double2 w = twid(sp, (size_t)(m >> (4 - sp)));
+                const double x = vh_mulmod_tw(v[m + hm], w.x, w.y, q);
+                const double u = v[m];
+                v[m] = u + x;
+                v[m + hm] = u - x;
+            }
+        }
+        if (R > 4) {
+#pragma unroll
+            for (int m = 0; m < 16; m++) lds[lidx(t + TPS * m)] = vh_center(v[m], q, qinv);
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < 16; m++) v[m] = lds[lidx(16 * t + m)];
+            // ---- contiguous ownership, stages s' = 4..R-1 ----
+#pragma unroll
+            for (int sp = 4; sp < R; sp++) {
+                const int hm = 1 << (R - sp - 1);
+#pragma unroll
+                for (int m = 0; m < 16; m++) {
+                    if (m & hm) continue;
+                    const double2 w = twid(sp, (size_t)((16 * t + m) >> (R - sp)));
+                    const double x = vh_mulmod_tw(v[m + hm], w.x, w.y, q);
+                    const double u = v[m];
+                    v[m] = u + x;
+                    v[m + hm] = u - x;
+                }
+            }
+        }
+        if (R == 4) {
+#pragma unroll
+            for (int m = 0; m < 16; m++) out[gidx(t + TPS * m)] = finish(v[m]);
+        } else if (COL) {
+#pragma unroll
+            for (int m = 0; m < 16; m++) out[gidx(16 * t + m)] = finish(v[m]);
+        } else {
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < 16; m++) lds[lidx(16 * t + m)] = __longlong_as_double((long long)finish(v[m]));
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < 16; m++)
+                out[gidx(t + TPS * m)] = (uint64_t)__double_as_longlong(lds[lidx(t + TPS * m)]);
+        }
+    } else {
+        // ---- load into contiguous ownership ----
+        if (R == 4) {
+#pragma unroll
+            for (int m = 0; m < 16; m++) v[m] = load(gidx(t + TPS * m));
+        } else if (COL) {
+#pragma unroll
+            for (int m = 0; m < 16; m++) v[m] = load(gidx(16 * t + m));
+        } else {
+#pragma unroll
+            for (int m = 0; m < 16; m++) lds[lidx(t + TPS * m)] = load(gidx(t + TPS * m));
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < 16; m++) v[m] = lds[lidx(16 * t + m)];
+        }
+        int c = 0;  // stages executed in this pass: GS sums are re-centred after every second one
+        if (R > 4) {
+#pragma unroll
+            for (int sp = R - 1; sp >= 4; sp--) {
+                const int hm = 1 << (R - sp - 1);
+#pragma unroll
+                for (int m = 0; m < 16; m++) {
+                    if (m & hm) continue;
+                    const double2 w = twid(sp, (size_t)((16 * t + m) >> (R - sp)));
+                    const double u = v[m], x = v[m + hm];
+                    double a = u + x;
+                    if (c & 1) a = vh_center(a, q, qinv);
+                    v[m] = a;
+                    v[m + hm] = vh_mulmod_tw(u - x, w.x, w.y, q);
+                }
+                c++;
+            }
+            if (!COL) __syncthreads();  // LDS is reused: every read of the load transpose is done
+#pragma unroll
+            for (int m = 0; m < 16; m++) lds[lidx(16 * t + m)] = v[m];
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < 16; m++) v[m] = lds[lidx(t + TPS * m)];
+        }
+#pragma unroll
+        for (int sp = 3; sp >= 0; sp--) {
+            const int hm = 8 >> sp;
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                if (m & hm) continue;
+                const double2 w = twid(sp, (size_t)(m >> (4 - sp)));
+                const double u = v[m], x = v[m + hm];
+                double a = u + x;
+                if (c & 1) a = vh_center(a, q, qinv);
+                v[m] = a;
+                v[m + hm] = vh_mulmod_tw(u - x, w.x, w.y, q);
+            }
+            c++;
+        }
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            const double x = LAST ? vh_mulmod_tw(v[m], L.ninv_d, L.ninv_q, q) : v[m];
+            out[gidx(t + TPS * m)] = finish(x);
+        }
+    }
+}
+
+template <int DIR, int R, bool COL, bool FIRST, bool LAST>
+__global__ void __launch_bounds__(256)
+k_ntt16(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, int logn, int s0)
+{
+    __shared__ double lds[vh_ntt16_lds<R, COL>()];
+    const vh_item &it = items[blockIdx.y];
+    ntt16_task<DIR, R, COL, FIRST, LAST, false>(it, limbs[it.limb], logn, s0, blockIdx.x, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent two-pass NTT with per-XCD work queues (FP64 path, N = 2^13..2^16).
+//
+// Item i belongs to XCD i mod 8; each XCD drains its own ticket counter.  Slot j
+// of an XCD's queue holds the T first-pass tasks of its j-th item followed by the
+// T second-pass tasks of its (j - lag)-th item (T = N/4096 tasks per pass).  A
+// second-pass task waits until the first-pass tasks of its item have signalled;
+// those hold smaller tickets of the same queue, were taken by running workgroups
+// of the same XCD and never wait themselves, so every wait ends.  The
+// intermediate therefore moves between the two passes through the XCD's own L2:
+// stores complete to L2 before the signal (s_waitcnt), and the second pass reads
+// with device-coherent loads that miss the L1.  A bounded spin sets an error flag
+// instead of waiting forever.
+//
+// ctl layout (uint32): [0..7] tickets per XCD, [16 + i] first-pass tasks done for
+// item i, [16 + nitems + i] second-pass tasks done; a failed wait ORs 1 into *err
+// and k_ntt_xcd_verify ORs 2 for any item not fully transformed (both are checked
+// by the host at its next synchronisation).
+// ---------------------------------------------------------------------------
+template <int DIR, int R1, int OCC>
+__global__ void __launch_bounds__(256, OCC)
+k_ntt_xcd(const vh_item *__restrict__ items, unsigned nitems, const vh_limb *__restrict__ limbs, int logn,
+          unsigned *__restrict__ ctl, unsigned *__restrict__ err, unsigned lag, unsigned spin_limit)
+{
+    constexpr int LDS_N = vh_ntt16_lds<R1, true>() > vh_ntt16_lds<8, false>() ? vh_ntt16_lds<R1, true>()
+                                                                             : vh_ntt16_lds<8, false>();
+    __shared__ double lds[LDS_N];
+    __shared__ unsigned s_ticket, s_ok;
+    const unsigned xcd = __builtin_amdgcn_s_getreg((3 << 11) | 20) & 7;  // HW_REG_XCC_ID
+    const unsigned mine = nitems > xcd ? (nitems - xcd + 7) / 8 : 0;
+    const unsigned T = 1u << (logn - 12);
+    const unsigned total = (mine + lag) * 2 * T;
+    unsigned *done = ctl + 16;
+    for (;;) {
+        if (threadIdx.x == 0) s_ticket = atomicAdd(ctl + xcd, 1u);
+        __syncthreads();
+        const unsigned tk = s_ticket;
+        __syncthreads();
+        if (tk >= total) break;
+        const unsigned j = tk / (2 * T), r = tk % (2 * T);
+        if (r < T) {
+            if (j >= mine) continue;
+            const unsigned i = xcd + 8 * j;
+            const vh_item &it = items[i];
+            if (DIR == 0) ntt16_task<0, R1, true, true, false, false>(it, limbs[it.limb], logn, 0, r, lds);
+            else ntt16_task<1, 8, false, true, false, false>(it, limbs[it.limb], logn, R1, r, lds);
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_fetch_add(done + i, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (j < lag || j - lag >= mine) continue;
+            const unsigned i = xcd + 8 * (j - lag);
+            if (threadIdx.x == 0) {
+                unsigned spins = 0;
+                while (__hip_atomic_load(done + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < T &&
+                       ++spins < spin_limit)
+                    __builtin_amdgcn_s_sleep(2);
+                s_ok = spins < spin_limit;
+                if (!s_ok) __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            if (!s_ok) break;
+            const vh_item &it = items[i];
+            if (DIR == 0) ntt16_task<0, 8, false, false, true, true>(it, limbs[it.limb], logn, R1, r - T, lds);
+            else ntt16_task<1, R1, true, false, true, true>(it, limbs[it.limb], logn, 0, r - T, lds);
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_fetch_add(done + nitems + i, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Elementwise ops (modules/arith/c/src/mod_portable.c:55-161 semantics, exact
 // for every uint64 input, canonical output).
 // ---------------------------------------------------------------------------
@@ -237,40 +492,54 @@ k_split_mul(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs
 // ---------------------------------------------------------------------------
 struct vh_divc { uint64_t h, inv, inv_sh; };
 
+// One thread per coefficient x of one polynomial: the K-prime chain is computed
+// once, then every output limb row is updated (coalesced row accesses).
+template <int KMAX>
 __global__ void __launch_bounds__(256)
 k_division(const vh_div_job *__restrict__ jobs, const vh_limb *__restrict__ limbs,
            const vh_divc *__restrict__ consts, uint64_t n)
 {
-    const vh_div_job jb = jobs[blockIdx.y];
+    const vh_div_job &jb = jobs[blockIdx.y];
     const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= n) return;
     const int K = jb.K, nl = jb.nl;
-    uint64_t t[VH_MAX_DIV];
-    uint64_t v[VH_MAX_DIV];
-    for (int k = 0; k < K; k++) v[k] = jb.base[(uint64_t)jb.div_limb[k] * jb.stride + x];
-    for (int k = 0; k < K; k++) {
-        const vh_limb Lk = limbs[jb.div_limb[k]];
-        const vh_divc ck = consts[k * nl + jb.div_limb[k]];
-        t[k] = vh_reduce64(vh_reduce64(v[k], Lk.q, Lk.mu64) + ck.h, Lk.q, Lk.mu64);
-        for (int k2 = k + 1; k2 < K; k2++) {
-            const vh_limb L2 = limbs[jb.div_limb[k2]];
-            const vh_divc c2 = consts[k * nl + jb.div_limb[k2]];
-            uint64_t a = vh_reduce64(vh_reduce64(v[k2], L2.q, L2.mu64) + c2.h, L2.q, L2.mu64);
-            a = vh_submod(a, vh_reduce64(t[k], L2.q, L2.mu64), L2.q);
-            v[k2] = vh_shoup(a, c2.inv, c2.inv_sh, L2.q);
+    uint64_t t[KMAX];
+    uint64_t v[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; k++)
+        if (k < K) v[k] = jb.base[(uint64_t)jb.div_limb[k] * jb.stride + x];
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        if (k >= K) break;
+        const int lk = jb.div_limb[k];
+        const uint64_t qk = limbs[lk].q, muk = limbs[lk].mu64;
+        t[k] = vh_reduce64(vh_reduce64(v[k], qk, muk) + consts[k * nl + lk].h, qk, muk);
+#pragma unroll
+        for (int k2 = k + 1; k2 < KMAX; k2++) {
+            if (k2 >= K) break;
+            const int l2 = jb.div_limb[k2];
+            const uint64_t q2 = limbs[l2].q, mu2 = limbs[l2].mu64;
+            const vh_divc c2 = consts[k * nl + l2];
+            uint64_t a = vh_reduce64(vh_reduce64(v[k2], q2, mu2) + c2.h, q2, mu2);
+            a = vh_submod(a, vh_reduce64(t[k], q2, mu2), q2);
+            v[k2] = vh_shoup(a, c2.inv, c2.inv_sh, q2);
         }
     }
-    const int li = jb.out_limb[blockIdx.z];
-    const vh_limb Li = limbs[li];
-    uint64_t *row = jb.base + (uint64_t)li * jb.stride;
-    uint64_t a = vh_reduce64(row[x], Li.q, Li.mu64);
-    for (int k = 0; k < K; k++) {
-        const vh_divc ck = consts[k * nl + li];
-        a = vh_addmod(a, ck.h, Li.q);
-        a = vh_submod(a, vh_reduce64(t[k], Li.q, Li.mu64), Li.q);
-        a = vh_shoup(a, ck.inv, ck.inv_sh, Li.q);
+    for (int o = 0; o < jb.nout; o++) {
+        const int li = jb.out_limb[o];
+        const uint64_t qi = limbs[li].q, mui = limbs[li].mu64;
+        uint64_t *row = jb.base + (uint64_t)li * jb.stride;
+        uint64_t a = vh_reduce64(row[x], qi, mui);
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            if (k >= K) break;
+            const vh_divc ck = consts[k * nl + li];
+            a = vh_addmod(a, ck.h, qi);
+            a = vh_submod(a, vh_reduce64(t[k], qi, mui), qi);
+            a = vh_shoup(a, ck.inv, ck.inv_sh, qi);
+        }
+        row[x] = a;
     }
-    row[x] = a;
 }
 
 // ---------------------------------------------------------------------------
@@ -333,24 +602,31 @@ __global__ void __launch_bounds__(256)
 k_keyswitch_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__ limbs,
                 const uint64_t *const *__restrict__ kptr, uint64_t n)
 {
-    const vh_mac_job jb = jobs[blockIdx.y];
-    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // blockIdx.x runs over the batch, so consecutive workgroups read the same key rows
+    // (same x range, same limb): the evaluation key is fetched once per batch.
+    const vh_mac_job &jb = jobs[blockIdx.x];
+    const uint64_t x = (uint64_t)blockIdx.y * blockDim.x + threadIdx.x;
     if (x >= n) return;
-    const int ti = blockIdx.z;                 // index into the extended-limb list
+    const int ti = blockIdx.z;
     const int limb = jb.limb[ti];
-    const vh_limb L = limbs[limb];
+    const vh_limb &L = limbs[limb];
+    const uint64_t q = L.q, mu = L.mu64;
     const int C = jb.comps, D = jb.digits;
     uint64_t acc[VH_MAX_COMPS];
-    for (int c = 0; c < C; c++) acc[c] = 0;
+#pragma unroll
+    for (int c = 0; c < VH_MAX_COMPS; c++) acc[c] = 0;
     for (int d = 0; d < D; d++) {
         const uint64_t m = jb.modup[((uint64_t)d * jb.next + ti) * n + x];
         const uint64_t *const *kp = kptr + jb.kptr_off + ((uint64_t)ti * D + d) * C;
-        for (int c = 0; c < C; c++)
-            acc[c] = vh_addmod(acc[c], vh_mulmod(vh_reduce64(kp[c][x], L.q, L.mu64), m, L), L.q);
+#pragma unroll
+        for (int c = 0; c < VH_MAX_COMPS; c++)
+            if (c < C) acc[c] = vh_addmod(acc[c], vh_mulmod(vh_reduce64(kp[c][x], q, mu), m, L), q);
     }
-    for (int c = 0; c < C; c++) {
-        uint64_t a = acc[c];
-        jb.out[c][(uint64_t)limb * n + x] = a ? L.q - a : 0;
+#pragma unroll
+    for (int c = 0; c < VH_MAX_COMPS; c++) {
+        if (c >= C) break;
+        const uint64_t a = acc[c];
+        jb.out[c][(uint64_t)limb * n + x] = a ? q - a : 0;
     }
 }
 
@@ -420,10 +696,15 @@ hipError_t vh_launch_split_mul(hipStream_t st, const vh_item *items, unsigned ni
 }
 
 hipError_t vh_launch_division(hipStream_t st, const vh_div_job *jobs, unsigned njobs, unsigned nout,
-                              const vh_limb *limbs, const void *consts, uint64_t n)
+                              const vh_limb *limbs, const void *consts, uint64_t n, int kmax)
 {
-    hipLaunchKernelGGL(k_division, dim3((unsigned)((n + 255) / 256), njobs, nout), dim3(256), 0, st, jobs, limbs,
-                       (const vh_divc *)consts, n);
+    (void)nout;
+    if (kmax <= 4)
+        hipLaunchKernelGGL(k_division<4>, dim3((unsigned)((n + 255) / 256), njobs), dim3(256), 0, st, jobs, limbs,
+                           (const vh_divc *)consts, n);
+    else
+        hipLaunchKernelGGL(k_division<VH_MAX_DIV>, dim3((unsigned)((n + 255) / 256), njobs), dim3(256), 0, st, jobs,
+                           limbs, (const vh_divc *)consts, n);
     return hipGetLastError();
 }
 
@@ -444,7 +725,7 @@ hipError_t vh_launch_permute(hipStream_t st, const vh_item *items, unsigned nite
 hipError_t vh_launch_keyswitch_mac(hipStream_t st, const vh_mac_job *jobs, unsigned njobs, unsigned next,
                                    const vh_limb *limbs, const uint64_t *const *kptr, uint64_t n)
 {
-    hipLaunchKernelGGL(k_keyswitch_mac, dim3((unsigned)((n + 255) / 256), njobs, next), dim3(256), 0, st, jobs,
+    hipLaunchKernelGGL(k_keyswitch_mac, dim3(njobs, (unsigned)((n + 255) / 256), next), dim3(256), 0, st, jobs,
                        limbs, kptr, n);
     return hipGetLastError();
 }
@@ -454,5 +735,81 @@ hipError_t vh_launch_scatter_split(hipStream_t st, const vh_item *items, unsigne
 {
     hipLaunchKernelGGL(k_scatter_split, dim3(vh_blocks(N, 256, 64), nitems), dim3(256), 0, st, items, limbs, N,
                        signed_in);
+    return hipGetLastError();
+}
+
+hipError_t vh_launch_ntt16(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
+                           int dir, int logn, int s0, int R, int col, int first, int last)
+{
+    if (R < 4 || R > 8 || logn < R) return hipErrorInvalidValue;
+    const unsigned G = 256u >> (R - 4);
+    const unsigned nsub = 1u << (logn - R);
+    if (nsub % G) return hipErrorInvalidValue;
+    dim3 grid(nsub / G, nitems);
+    // (dir, first, last, col) take the four combinations a two-pass transform uses.
+#define VH_NTT16(D, RR, CC, FF, LL) hipLaunchKernelGGL((k_ntt16<D, RR, CC, FF, LL>), grid, dim3(256), 0, st, items, limbs, logn, s0)
+#define VH_NTT16_R(D, CC, FF, LL) \
+    switch (R) { case 4: VH_NTT16(D, 4, CC, FF, LL); break; case 5: VH_NTT16(D, 5, CC, FF, LL); break; \
+                 case 6: VH_NTT16(D, 6, CC, FF, LL); break; case 7: VH_NTT16(D, 7, CC, FF, LL); break; \
+                 default: VH_NTT16(D, 8, CC, FF, LL); break; }
+    if (dir == 0 && col && first && !last) { VH_NTT16_R(0, true, true, false) }
+    else if (dir == 0 && !col && !first && last) { VH_NTT16_R(0, false, false, true) }
+    else if (dir == 1 && !col && first && !last) { VH_NTT16_R(1, false, true, false) }
+    else if (dir == 1 && col && !first && last) { VH_NTT16_R(1, true, false, true) }
+    else return hipErrorInvalidValue;
+#undef VH_NTT16_R
+#undef VH_NTT16
+    return hipGetLastError();
+}
+
+hipError_t vh_launch_ntt_xcd(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,
+                             int logn, unsigned *ctl, unsigned *err, unsigned lag, unsigned spin_limit, unsigned nblocks,
+                             int occ)
+{
+    if (logn < 13 || logn > 16) return hipErrorInvalidValue;
+#define VH_XCD(D, RR, OO) hipLaunchKernelGGL((k_ntt_xcd<D, RR, OO>), dim3(nblocks), dim3(256), 0, st, items, nitems, limbs, logn, ctl, err, lag, spin_limit)
+#define VH_XCD_R(D, OO) \
+    switch (logn - 8) { case 5: VH_XCD(D, 5, OO); break; case 6: VH_XCD(D, 6, OO); break; \
+                        case 7: VH_XCD(D, 7, OO); break; default: VH_XCD(D, 8, OO); }
+    if (occ >= 4) { if (dir == 0) { VH_XCD_R(0, 4) } else { VH_XCD_R(1, 4) } }
+    else          { if (dir == 0) { VH_XCD_R(0, 2) } else { VH_XCD_R(1, 2) } }
+#undef VH_XCD_R
+#undef VH_XCD
+    return hipGetLastError();
+}
+
+__global__ void k_ntt_xcd_verify(const unsigned *__restrict__ ctl, unsigned nitems, unsigned T, unsigned *err)
+{
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += gridDim.x * blockDim.x)
+        if (ctl[16 + i] != T || ctl[16 + nitems + i] != T) atomicOr(err, 2u);
+}
+
+hipError_t vh_launch_ntt_xcd_verify(hipStream_t st, const unsigned *ctl, unsigned nitems, int logn, unsigned *err)
+{
+    hipLaunchKernelGGL(k_ntt_xcd_verify, dim3((nitems + 255) / 256), dim3(256), 0, st, ctl, nitems,
+                       1u << (logn - 12), err);
+    return hipGetLastError();
+}
+
+// Records the XCC id of every workgroup (placement probe).
+__global__ void k_xcc_probe(unsigned *out)
+{
+    if (threadIdx.x == 0) out[blockIdx.x] = __builtin_amdgcn_s_getreg((3 << 11) | 20);
+}
+
+hipError_t vh_launch_xcc_probe(hipStream_t st, unsigned *out, unsigned nblocks)
+{
+    hipLaunchKernelGGL(k_xcc_probe, dim3(nblocks), dim3(64), 0, st, out);
+    return hipGetLastError();
+}
+
+__global__ void k_zero_u32(unsigned *p, unsigned n)
+{
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+hipError_t vh_launch_zero_u32(hipStream_t st, unsigned *p, unsigned n)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
