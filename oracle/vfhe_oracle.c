@@ -23,10 +23,15 @@
  *    CRT reconstruction (closed forms stated in tests/test_oracle.py);
  *  - permute: composition gen * gen^-1 = identity and big-int X -> X^gen;
  *  - tensor product: the reference's slot-layout identity (test_mlwe.py:183-207);
- *  - key-switch / multiply / rescale: exact decryption with valid keys
- *    (test_mlwe.py:73-119 style) and mask/shape rules; no golden vector exists
- *    for the bit pattern of a key-switched ciphertext (the reference's C for it
- *    includes blake3.h, absent from the image, so it is not buildable here).
+ *  - NTT and mod_eltwise_{mul,add,sub}: also the committed golden vectors
+ *    tests/golden/ref_ntt_mod.npz, written from the reference kernels;
+ *  - key-switch / multiply / automorphism: decryption with valid keys built by
+ *    the mlwe.py:156-176 key layout (phase preserved to |e| < 2^16 out of
+ *    Q ~ 2^150; reversed digits or a wrong source key fail), on top of the
+ *    pinned lift, NTT, product and division steps they are composed of.  No
+ *    reference-produced vector exists for the bit pattern of a key-switched
+ *    ciphertext: the reference's C for it (rns_polynomial.c, mlwe.c) includes
+ *    blake3.h, which the image lacks, so it is not buildable here.
  */
 #include <stdint.h>
 #include <stdlib.h>

@@ -119,6 +119,23 @@ def test_ntt_matches_reference_kernels(engine):
             assert np.array_equal(c, ref.reverse(y)), (engine, n, q)
 
 
+def test_ntt_and_eltwise_match_golden_vectors():
+    """tests/golden/ref_ntt_mod.npz holds the reference kernels' outputs (no reference needed here)."""
+    from conftest import golden_ntt_mod_cases
+    ops = {"mul": O.O_MUL, "add": O.O_ADD, "sub": O.O_SUB}
+    for n, q, x, y, exp in golden_ntt_mod_cases():
+        a = x.copy()
+        O.lib().o_ntt_forward(O.ptr(a), n, q)
+        assert np.array_equal(a, exp["fwd"]), (n, q)
+        b = y.copy()
+        O.lib().o_ntt_inverse(O.ptr(b), n, q)
+        assert np.array_equal(b, exp["rev"]), (n, q)
+        for name, op in ops.items():
+            o = np.zeros(n, dtype=np.uint64)
+            O.lib().o_eltwise(op, O.ptr(o), O.ptr(x), O.ptr(y), 0, n, q)
+            assert np.array_equal(o, exp[name]), (name, n, q)
+
+
 def test_negacyclic_convolution():
     """test_ntt.c:51-104 and test_arith.py:31-53: pointwise product of NTTs == schoolbook mod X^n + 1."""
     for n, bits in ((64, 20), (64, 62), (256, 50)):
@@ -407,3 +424,168 @@ def test_tensor_product_slot_identity():
         c.mul(tmp, out[k], m, keys[k], m, 0)
         c.addsub(acc, acc, m, tmp, m, True)
     assert np.array_equal(acc, exp)
+
+
+# --- key-switch, relinearization and rotation with valid keys ---------------------------------------
+# Pinned by decryption: for a key-switching key of s' under s (mlwe.py:156-176 layout, digit j =
+# an encryption of P * s' placed on limb j only), phase_s(KS(c)) == b - c_from * s' up to the
+# key-switch noise; a wrong sign, a dropped digit or a wrong digit index leaves noise of size Q.
+
+KS_N, KS_LVL, KS_SPECIAL = 32, 3, 1
+
+
+def _ks_setup(seed):
+    primes = O.gen_primes(KS_N, [50] * (KS_LVL + KS_SPECIAL))
+    c = O.Ctx(KS_N, primes)
+    lvl, ext = (1 << KS_LVL) - 1, (1 << (KS_LVL + KS_SPECIAL)) - 1
+    rng = np.random.default_rng(seed)
+    return c, primes, lvl, ext, rng
+
+
+def _small(c, primes, rng, bound):
+    v = rng.integers(-bound, bound + 1, KS_N)
+    return np.stack([np.array([int(x) % q for x in v], dtype=np.uint64) for q in primes]), v
+
+
+def _ntt(c, a):
+    out = a.copy()
+    c.to_ntt(out, a, c.mask_all())
+    return out
+
+
+def _gen_ksk(c, primes, rng, s_ntt, s_from_ntt, lvl):
+    """digit j (one per level limb): a uniform, b = a*s + e + (P mod q_j) * s' on limb j only."""
+    P = math.prod(primes[KS_LVL:])
+    m = c.mask_all()
+    digits = []
+    for j in range(KS_LVL):
+        a = np.stack([rng.integers(0, q, KS_N, dtype=np.uint64) for q in primes])
+        e, _ = _small(c, primes, rng, 3)
+        b = np.zeros_like(a)
+        c.mul(b, a, m, s_ntt, m, 0)
+        c.addsub(b, b, m, _ntt(c, e), m)
+        t = np.zeros_like(a)
+        t[j] = s_from_ntt[j]
+        c.scale(t, t, 1 << j, P % primes[j])
+        c.addsub(b, b, m, t, m)
+        digits.append(np.stack([a, b]))
+    return np.ascontiguousarray(np.stack(digits))
+
+
+def _phase_coeff(c, ct_ntt, s_ntt, mask):
+    """b - a*s in NTT form, returned in coefficient form."""
+    tmp = np.zeros_like(ct_ntt[0])
+    c.mul(tmp, ct_ntt[0], mask, s_ntt, mask, 0)
+    ph = ct_ntt[1].copy()
+    c.addsub(ph, ph, mask, tmp, mask, True)
+    out = np.zeros_like(ph)
+    c.to_coeff(out, ph, mask)
+    return out
+
+
+def _centered_max(c, primes, rows, mask):
+    mods = [q for i, q in enumerate(primes) if (mask >> i) & 1]
+    Q = math.prod(mods)
+    worst = 0
+    for k in range(KS_N):
+        v = crt([int(rows[i][k]) for i in range(len(primes)) if (mask >> i) & 1], mods)
+        worst = max(worst, min(v, Q - v))
+    return worst
+
+
+def _keyswitch_error(c, primes, lvl, ext, rng, s, s_from, ksk):
+    """|phase_s(KS(c_from, b)) - (b - c_from * s')| for a random coefficient-form input."""
+    cin = np.stack([np.stack([rng.integers(0, q, KS_N, dtype=np.uint64) if i < KS_LVL else np.zeros(KS_N, np.uint64)
+                              for i, q in enumerate(primes)]) for _ in range(2)])
+    out = np.zeros((2, len(primes), KS_N), dtype=np.uint64)
+    mo = c.ghs_keyswitch(out, 1, cin, 1, lvl, [ksk], ext)
+    assert mo == lvl
+    s_ntt = _ntt(c, s)
+    ph = _phase_coeff(c, np.stack([_ntt(c, out[0]), _ntt(c, out[1])]), s_ntt, lvl)
+    exp_ntt = np.zeros_like(s)
+    c.mul(exp_ntt, _ntt(c, cin[0]), lvl, _ntt(c, s_from), lvl, 0)
+    bb = _ntt(c, cin[1])
+    c.addsub(exp_ntt, bb, lvl, exp_ntt, lvl, True)
+    exp = np.zeros_like(exp_ntt)
+    c.to_coeff(exp, exp_ntt, lvl)
+    diff = np.zeros_like(exp)
+    c.addsub(diff, ph, lvl, exp, lvl, True)
+    return _centered_max(c, primes, diff, lvl)
+
+
+def test_keyswitch_decrypts_with_valid_key():
+    """mlwe.c:557-598 / gadget.c:22-37 restated: phase preserved up to |e| < 2^16 (Q ~ 2^150)."""
+    c, primes, lvl, ext, rng = _ks_setup(11)
+    s, _ = _small(c, primes, rng, 1)
+    s_from, _ = _small(c, primes, rng, 1)
+    ksk = _gen_ksk(c, primes, rng, _ntt(c, s), _ntt(c, s_from), lvl)
+    assert _keyswitch_error(c, primes, lvl, ext, rng, s, s_from, ksk) < 2**16
+    # planted mistakes: digits in the wrong order, and a key for the wrong source secret
+    assert _keyswitch_error(c, primes, lvl, ext, rng, s, s_from, np.ascontiguousarray(ksk[::-1])) > 2**100
+    other, _ = _small(c, primes, rng, 1)
+    assert _keyswitch_error(c, primes, lvl, ext, rng, s, other, ksk) > 2**100
+
+
+def test_multiply_relinearizes_with_valid_key():
+    """mlwe.c:711-760: phase_s(multiply(c1, c2)) == m1 * m2 (negacyclic) up to key-switch noise."""
+    c, primes, lvl, ext, rng = _ks_setup(12)
+    s, sv = _small(c, primes, rng, 1)
+    s_ntt = _ntt(c, s)
+    neg_s2 = np.zeros_like(s)
+    c.mul(neg_s2, s_ntt, ext, s_ntt, ext, 0)
+    c.negate(neg_s2, neg_s2, ext)          # quadratic slot key -s*s (tensor slot convention)
+    ksk = _gen_ksk(c, primes, rng, s_ntt, neg_s2, lvl)
+    msgs, cts = [], []
+    for _ in range(2):
+        m, mv = _small(c, primes, rng, 1000)
+        a = np.stack([rng.integers(0, q, KS_N, dtype=np.uint64) if i < KS_LVL else np.zeros(KS_N, np.uint64)
+                      for i, q in enumerate(primes)])
+        b = np.zeros_like(a)
+        c.mul(b, a, lvl, s_ntt, lvl, 0)
+        c.addsub(b, b, lvl, _ntt(c, m), lvl)
+        msgs.append(mv)
+        cts.append(np.stack([a, b]))
+    out = np.zeros((2, len(primes), KS_N), dtype=np.uint64)
+    mo = c.multiply(out, cts[0], cts[1], 1, lvl, [ksk, None], ext)
+    assert mo == lvl
+    ph = _phase_coeff(c, out, s_ntt, lvl)
+    prod = [0] * KS_N
+    for i in range(KS_N):
+        for j in range(KS_N):
+            k, v = i + j, int(msgs[0][i]) * int(msgs[1][j])
+            if k < KS_N:
+                prod[k] += v
+            else:
+                prod[k - KS_N] -= v
+    exp = np.stack([np.array([v % q for v in prod], dtype=np.uint64) for q in primes])
+    diff = np.zeros_like(exp)
+    c.addsub(diff, ph, lvl, exp, lvl, True)
+    assert _centered_max(c, primes, diff, lvl) < 2**16
+
+
+def test_automorphism_decrypts_with_galois_key():
+    """mlwe.c:314-326: phase_s(automorphism(c, g)) == sigma_g(phase_s(c)) with a key for sigma_g(s)."""
+    c, primes, lvl, ext, rng = _ks_setup(13)
+    g = 5
+    s, _ = _small(c, primes, rng, 1)
+    s_ntt = _ntt(c, s)
+    sg = np.zeros_like(s)
+    c.permute(sg, s, ext, g)
+    ksk = _gen_ksk(c, primes, rng, s_ntt, _ntt(c, sg), lvl)
+    m, _ = _small(c, primes, rng, 1000)
+    a = np.stack([rng.integers(0, q, KS_N, dtype=np.uint64) if i < KS_LVL else np.zeros(KS_N, np.uint64)
+                  for i, q in enumerate(primes)])
+    b = np.zeros_like(a)
+    c.mul(b, a, lvl, s_ntt, lvl, 0)
+    c.addsub(b, b, lvl, _ntt(c, m), lvl)
+    ct = np.stack([np.zeros_like(a), np.zeros_like(a)])
+    c.to_coeff(ct[0], a, lvl)
+    c.to_coeff(ct[1], b, lvl)
+    out = np.zeros((2, len(primes), KS_N), dtype=np.uint64)
+    assert c.automorphism(out, ct, 1, lvl, g, [ksk], ext) == lvl
+    ph = _phase_coeff(c, np.stack([_ntt(c, out[0]), _ntt(c, out[1])]), s_ntt, lvl)
+    exp = np.zeros_like(m)
+    c.permute(exp, m, lvl, g)
+    diff = np.zeros_like(exp)
+    c.addsub(diff, ph, lvl, exp, lvl, True)
+    assert _centered_max(c, primes, diff, lvl) < 2**16

@@ -846,9 +846,16 @@ extern "C" const char *vfhe_hip_device_name(void)
 
 extern "C" const char *vfhe_engine_active(void) { return "hip"; }
 
+// cpu_probe.c:14-33 semantics (empty = no requirement, unknown = no), plus "hip".
 extern "C" int vfhe_cpu_supports(const char *capability)
 {
-    if (!capability) return 0;
+    if (!capability || !capability[0]) return 1;
+#if defined(__x86_64__) && !defined(__HIP_DEVICE_COMPILE__)
+    __builtin_cpu_init();
+    if (!strcmp(capability, "avx512ifma")) return __builtin_cpu_supports("avx512ifma") ? 1 : 0;
+    if (!strcmp(capability, "avx512f")) return __builtin_cpu_supports("avx512f") ? 1 : 0;
+    if (!strcmp(capability, "avx2")) return __builtin_cpu_supports("avx2") ? 1 : 0;
+#endif
     if (strcmp(capability, "hip") != 0) return 0;
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { (void)hipGetLastError(); return 0; }
