@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--no-ckks", action="store_true")
     ap.add_argument("--ckks-batch", type=int, default=32)
     ap.add_argument("--ckks-steps", type=int, default=5)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=1.5, help="wall time per offered load")
     ap.add_argument("--only-ntt", action="store_true",
                     help="run unit A alone (the command profiled with rocprofv3)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -236,6 +238,10 @@ def main():
     if not args.no_ckks:
         ckks = bench_ckks(lib, _native, params, rt, args)
 
+    serve = None
+    if not args.no_serving:
+        serve = bench_serving(lib, args)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline:", args.cpu_seconds, "s")
@@ -270,6 +276,7 @@ def main():
                   "batched_over_device_plain": round(plain_dev_dt / dt, 2),
                   "batched_over_host_plain": round(plain_host_dt / dt, 2)},
         "ckks": ckks,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
 
@@ -327,6 +334,36 @@ def bench_ckks(lib, _native, params, rt, args):
     for c in in1 + in2 + outs + entries:
         lib.free_mlwe_RNS_sample(c)
     return res
+
+
+def bench_serving(lib, args):
+    """Serving loop: Poisson CKKS requests (multiply+relinearize+rescale or rotation), N in 2^13..2^16,
+    level uniform in 4..24, keys and ciphertexts resident in HBM, continuous batching on 4 streams."""
+    from vfhe_amd import serving
+    Ns = [1 << 13, 1 << 14, 1 << 15, 1 << 16]
+    log("serving: building resident keys and ciphertexts for N =", Ns)
+    t0 = time.perf_counter()
+    rings = {n: serving.Ring(lib, n, L=24, K=4, rot_steps=4, levels=(4, 24)) for n in Ns}
+    log(f"serving: setup {time.perf_counter() - t0:.1f} s")
+    budget, lanes, max_batch = 32 << 30, 4, 32
+    burst = serving.make_workload(40000.0, 0.2, Ns, seed=1)
+    cap = serving.run(lib, rings, burst, budget, lanes, max_batch)["requests_per_s"]
+    log(f"serving: saturated throughput {cap:.0f} requests/s ({len(burst)} requests)")
+    loads = []
+    for k, frac in enumerate((0.25, 0.5, 0.75, 0.95)):
+        wl = serving.make_workload(frac * cap, args.serving_seconds, Ns, seed=10 + k)
+        r = serving.run(lib, rings, wl, budget, lanes, max_batch)
+        r["offered_fraction"] = frac
+        log("serving:", r)
+        loads.append(r)
+    lib.vfhe_hip_synchronize()
+    for ring in rings.values():
+        ring.free()
+    return {"unit": "requests/s", "saturated_requests_per_s": cap, "loads": loads, "lanes": lanes,
+            "max_batch": max_batch, "memory_budget_bytes": budget,
+            "workload": "Poisson arrivals; 50% multiply+relinearize+rescale, 50% rotation by 5^(2^k), k<4; "
+                        "N uniform in {2^13..2^16}, level uniform in [4,24] limbs + 4 special limbs; "
+                        "random evaluation keys and ciphertexts resident in HBM"}
 
 
 if __name__ == "__main__":

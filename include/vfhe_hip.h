@@ -225,6 +225,49 @@ const char *vfhe_engine_active(void);        /* returns "hip" */
 int vfhe_cpu_supports(const char *capability); /* "hip": a gfx950 device is visible */
 
 /* ------------------------------------------------------------------------ */
+/* Sampling (on the device; ChaCha20 keystream, so the streams differ from     */
+/* the reference engines' AES-CTR / BLAKE3 streams, as those differ from each   */
+/* other; distributions and masks follow the reference)                         */
+/* ------------------------------------------------------------------------ */
+
+/* modules/misc/c/include/misc.h:46-47 (prng.c:132-143): deterministic key from
+   a seed (tests), or back to OS entropy. */
+void vfhe_prng_set_deterministic_seed(uint64_t seed);
+void vfhe_prng_clear_deterministic_seed(void);
+/* modules/arith/c/include/arith.h:200 (rns_polynomial.c:242-253): uniform residues
+   on every active limb. */
+void polynomial_gen_random_RNSc_polynomial(RNSc_Polynomial out);
+/* modules/arith/c/include/arith.h:267 (rns_polynomial.c:255-271): one rounded
+   N(0, sigma) vector reduced into every active limb. */
+void polynomial_gen_gaussian_RNSc_polynomial(RNSc_Polynomial out, double sigma);
+/* modules/mlwe/c/include/mlwe.h:74-91 (mlwe.c:70-83, 190-241, 281-300) */
+RNS_MLWE_Key mlwe_new_RNS_gaussian_key(uint64_t N, uint64_t r, uint64_t l, double key_sigma, incNTT ntt,
+                                       double sigma);
+void mlwe_RNS_sample_of_zero(RNS_MLWE out, RNS_MLWE_Key key);
+void mlwe_RNSc_sample_of_zero(RNSc_MLWE out, RNS_MLWE_Key key);
+RNS_MLWE mlwe_new_RNS_sample_of_zero(RNS_MLWE_Key key);
+RNSc_MLWE mlwe_new_RNSc_sample_of_zero(RNS_MLWE_Key key);
+RNS_MLWE mlwe_new_RNS_trivial_sample_of_zero(uint64_t N, uint64_t r, uint64_t mask, incNTT ntt);
+void mlwe_RNSc_sample(RNSc_MLWE out, RNS_MLWE_Key key, RNSc_Polynomial m);
+
+/* ------------------------------------------------------------------------ */
+/* CKKS encoder FFT: modules/arith/python/cdef/arith.cdef:83-93                */
+/* (modules/arith/c/src/complex.c).  Rows are 2n doubles, real then imaginary; */
+/* tables are the caller's roots, ws[m + i] for group i of the stage with m    */
+/* groups.  Results are bit-identical to the avx512ifma engine (option          */
+/* "fft.fma" = 1, default) or to the portable engine ("fft.fma" = 0).           */
+/* ------------------------------------------------------------------------ */
+void *load_rous_CT(double *rous_real, double *rous_imag, uint64_t size);
+void *load_rous_GS(double *rous_real, double *rous_imag, uint64_t size);
+void CT_NR(double *x, void *ws, uint64_t n);
+void GS_RN(double *x, void *ws, uint64_t n);
+void bit_reverse_array(double *v, uint64_t N, uint32_t prec);
+void complex_poly_scale_double(double *v, double scale, uint64_t N);
+void complex_poly_round_to_RNS(RNS_Polynomial out, double *in, uint64_t N);
+void complex_polys_ifft_scale_round_to_RNS_batch(void **rows_in, void **outs_rns, uint64_t count, uint64_t n_complex,
+                                                 uint32_t log_prec, void *gs_ws, double temp_delta);
+
+/* ------------------------------------------------------------------------ */
 /* Engine extensions (no counterpart in the reference)                       */
 /* ------------------------------------------------------------------------ */
 
@@ -238,6 +281,12 @@ uint64_t *polynomial_device_row(RNS_Polynomial p, uint64_t limb);
 /* Runtime: wait for all queued device work; return the last HIP error code
    (0 when none) and clear it; device name. */
 void vfhe_hip_synchronize(void);
+/* Select the calling thread's lane (0..7): a HIP stream with its own staging
+   ring; lane 0 is the default.  Work on different lanes may overlap on the
+   device, so a caller that uses several lanes orders dependent work itself
+   (vfhe_hip_synchronize waits for every lane).  Returns the previous lane, or
+   -1 for an invalid one. */
+int vfhe_hip_set_lane(int lane);
 int vfhe_hip_last_error(void);
 const char *vfhe_hip_device_name(void);
 
@@ -253,6 +302,11 @@ const char *vfhe_hip_device_name(void);
    set returns 0, or -1 for an unknown key; get returns -1 for an unknown key. */
 int vfhe_hip_set_option(const char *key, long value);
 long vfhe_hip_get_option(const char *key);
+
+/* Raw ChaCha20 keystream blocks (16 words each) for (key[8], nonce, counters
+   ctr0 .. ctr0+nblocks-1), computed on the device: the sampler's known-answer
+   check against RFC 8439. */
+void vfhe_hip_chacha20_blocks(const uint32_t *key, uint64_t nonce, uint64_t ctr0, uint64_t nblocks, uint32_t *out);
 
 /* Launch profiler: between start and stop every kernel launch is bracketed by
    HIP events on the engine stream.  stop() waits, then writes the summed
@@ -274,6 +328,41 @@ void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polyn
    when divide_mask == 0. */
 void vfhe_hip_mlwe_multiply_batch(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLWE *in2, uint64_t count,
                                   RNS_MLWE **rlk, uint64_t divide_mask);
+
+/* mlwe_automorphism_RNSc_GHS (mlwe.c:314-326) for `count` inputs sharing gen and ksk. */
+void vfhe_hip_mlwe_automorphism_batch(RNS_MLWE *out, RNS_MLWE *in, uint64_t count, uint64_t gen, RNS_MLWE **ksk);
+/* Views of `count` samples restricted to `mask`: new handles sharing the device
+   rows and host mirror of in[k] (no copy).  A key stored once for the top level
+   serves every lower level through a view with mask = level limbs | special
+   limbs.  Free with free_RNS_mlwe_array before the viewed samples. */
+RNS_MLWE *vfhe_hip_mlwe_view_array(RNS_MLWE *in, uint64_t count, uint64_t mask);
+/* The calling thread's current lane stream (a hipStream_t). */
+void *vfhe_hip_current_stream(void);
+
+/* ------------------------------------------------------------------------ */
+/* Serving runtime: continuous batching of homomorphic requests              */
+/* ------------------------------------------------------------------------ */
+#define VFHE_HIP_REQ_MULTIPLY 0      /* in1*in2, relinearize with ksk (rlk), rescale by divide_mask if != 0 */
+#define VFHE_HIP_REQ_AUTOMORPHISM 1  /* X -> X^gen on in1, key-switch with ksk (Galois key) */
+/* A server with a worker thread.  Requests are admitted while the device bytes
+   they hold in flight (output, temporaries, ModUp buffer) fit memory_budget
+   (the oldest request is admitted alone when nothing is in flight); requests
+   of one class (op, ring, masks, key, gen, divide_mask) are batched, up to
+   max_batch per launch, and batches run on `lanes` (1..7) engine lanes. */
+void *vfhe_hip_server_new(uint64_t memory_budget, uint32_t lanes, uint32_t max_batch);
+/* Thread-safe; returns the request id (>= 1), or 0 for an invalid request.
+   Inputs and keys must stay alive and unchanged until the request completes. */
+uint64_t vfhe_hip_server_submit(void *srv, int op, RNS_MLWE in1, RNS_MLWE in2, RNS_MLWE **ksk, uint64_t gen,
+                                uint64_t divide_mask);
+/* Up to `max` completed requests: id, output sample (the caller owns it) and
+   latency from submit to completion in seconds.  wait != 0 blocks until one
+   completes or none is pending.  Returns the number written. */
+uint64_t vfhe_hip_server_poll(void *srv, uint64_t *ids, RNS_MLWE *outs, double *latency_s, uint64_t max, int wait);
+/* stats[6]: batches, requests launched, mean batch size, peak in-flight bytes,
+   most lanes busy at once, requests pending. */
+void vfhe_hip_server_stats(void *srv, double *stats);
+/* Drains every submitted request, then stops the worker. */
+void vfhe_hip_server_free(void *srv);
 
 #ifdef __cplusplus
 }

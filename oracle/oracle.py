@@ -260,6 +260,39 @@ def aligned(n: int) -> np.ndarray:
     return raw[off:off + n]
 
 
+_cx = {}
+
+
+def ref_complex(engine: str):
+    """The reference's complex FFT (complex.c) built into oracle/_ref for `engine`."""
+    if engine not in _cx:
+        path = os.path.join(HERE, "_ref", f"libvfheref_complex_{engine}.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built (run `make -C oracle` where the reference is present)")
+        L = ctypes.CDLL(path)
+        DP = ctypes.POINTER(ctypes.c_double)
+        for name, res, args in (("load_rous_CT", VP, [DP, DP, U64]), ("load_rous_GS", VP, [DP, DP, U64]),
+                                ("CT_NR", None, [DP, VP, U64]), ("GS_RN", None, [DP, VP, U64]),
+                                ("bit_reverse_array", None, [DP, U64, ctypes.c_uint32]),
+                                ("complex_poly_scale_double", None, [DP, ctypes.c_double, U64])):
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+        _cx[engine] = L
+    return _cx[engine]
+
+
+def aligned_f64(n: int) -> np.ndarray:
+    """A 64-byte aligned float64 buffer (the AVX-512 FFT uses aligned loads)."""
+    raw = np.zeros(n + 8, dtype=np.float64)
+    off = (-raw.ctypes.data % 64) // 8
+    return raw[off:off + n]
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
 class RefProc:
     """One reference NTT_proc (ntt_new_proc(n, q)) for calling the reference kernels."""
 

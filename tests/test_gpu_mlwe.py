@@ -17,9 +17,9 @@ pytestmark = pytest.mark.gpu
 
 
 class Setup:
-    def __init__(self, N, L, K, seed, split=1):
+    def __init__(self, N, L, K, seed, split=1, bits=50):
         self.N, self.L, self.K = N, L, K
-        self.primes = O.gen_primes(N, [50] * (L + K), split)
+        self.primes = O.gen_primes(N, [bits] * (L + K), split)
         self.pool = rt.Pool(N, self.primes, split)
         self.octx = O.Ctx(N, self.primes, split)
         self.rng = np.random.default_rng(seed)
@@ -83,6 +83,34 @@ def test_ghs_keyswitch(lib, N, L, K, r, relin):
     mref = s.octx.ghs_keyswitch(ref, r, din, Rin, s.lvl_mask, [k[0] if k else None for k in keys], s.ext_mask)
     assert s.pool.sample_masks(out) == [mref] * (r + 1)
     s.check(s.pool.read_sample(out), ref, mref)
+
+
+@pytest.mark.parametrize("N,L,K,r,relin,fused,bits", [
+    (8192, 3, 1, 1, True, 1, 50),      # column pass + fused row-pass/inner-product kernel
+    (8192, 3, 1, 1, True, 0, 50),      # two-pass NTT + FP64 inner product
+    (16384, 2, 2, 2, True, 1, 50),     # three output components: FP64 inner product
+    (8192, 3, 1, 1, True, 1, 60),      # primes above 2^50: integer path throughout
+    (65536, 2, 1, 1, False, 1, 50),    # largest fused size
+])
+def test_keyswitch_schedules(lib, N, L, K, r, relin, fused, bits):
+    """Every key-switch schedule (mlwe.c:557-598) against the oracle, bit-exact."""
+    old = lib.vfhe_hip_get_option(b"ks.fused")
+    lib.vfhe_hip_set_option(b"ks.fused", fused)
+    try:
+        s = Setup(N, L, K, 100 + N + r + bits, bits=bits)
+        Rin = int(lib.mlwe_extended_rank(r)) if relin else r
+        nkeys = Rin - r if relin else Rin
+        keys = [s.key(L, r) for _ in range(nkeys)] + ([None] * r if relin else [])
+        din, cin = s.ct(Rin, s.lvl_mask)
+        out = s.pool.sample(r, s.ext_mask)
+        print(f"[keyswitch] N={N} L={L} K={K} r={r} fused={fused} bits={bits}", flush=True)
+        lib.mlwe_RNSc_GHS_hybrid_keyswitch(out, cin, s.outer(keys), 0)
+        ref = np.zeros((r + 1, s.L + s.K, s.N), dtype=np.uint64)
+        mref = s.octx.ghs_keyswitch(ref, r, din, Rin, s.lvl_mask, [k[0] if k else None for k in keys], s.ext_mask)
+        assert s.pool.sample_masks(out) == [mref] * (r + 1)
+        s.check(s.pool.read_sample(out), ref, mref)
+    finally:
+        lib.vfhe_hip_set_option(b"ks.fused", old)
 
 
 def test_multiply_and_batched_rescale(lib):

@@ -22,6 +22,7 @@ U64 = ctypes.c_uint64
 U64P = ctypes.POINTER(ctypes.c_uint64)
 VP = ctypes.c_void_p
 VPP = ctypes.POINTER(ctypes.c_void_p)
+DP = ctypes.POINTER(ctypes.c_double)
 I32 = ctypes.c_int
 
 
@@ -39,6 +40,17 @@ class RNSPolynomial(ctypes.Structure):
 class RNSMLWE(ctypes.Structure):
     """struct _RNS_MLWE (modules/mlwe/python/cdef/mlwe.cdef:27-31)."""
     _fields_ = [("a", VPP), ("b", VP), ("r", U64)]
+
+
+class IntPolynomial(ctypes.Structure):
+    """struct _IntPolynomial (modules/arith/c/include/arith.h:96-100)."""
+    _fields_ = [("coeffs", U64P), ("N", U64)]
+
+
+class RNSMLWEKey(ctypes.Structure):
+    """struct _RNS_MLWE_Key (modules/mlwe/python/cdef/mlwe.cdef:33-40)."""
+    _fields_ = [("s", ctypes.POINTER(ctypes.POINTER(IntPolynomial))), ("s_RNS", VPP), ("N", U64), ("l", U64),
+                ("r", U64), ("sigma", ctypes.c_double)]
 
 
 # name: (restype, argtypes)
@@ -157,6 +169,35 @@ _SIGNATURES = {
     "polynomial_mark_host_dirty": (None, [VP]),
     "polynomial_device_row": (U64P, [VP, U64]),
     "vfhe_hip_synchronize": (None, []),
+    "vfhe_hip_set_lane": (ctypes.c_int, [ctypes.c_int]),
+    "vfhe_prng_set_deterministic_seed": (None, [U64]),
+    "vfhe_prng_clear_deterministic_seed": (None, []),
+    "polynomial_gen_random_RNSc_polynomial": (None, [VP]),
+    "polynomial_gen_gaussian_RNSc_polynomial": (None, [VP, ctypes.c_double]),
+    "mlwe_new_RNS_gaussian_key": (VP, [U64, U64, U64, ctypes.c_double, VP, ctypes.c_double]),
+    "mlwe_RNS_sample_of_zero": (None, [VP, VP]),
+    "mlwe_RNSc_sample_of_zero": (None, [VP, VP]),
+    "mlwe_new_RNS_sample_of_zero": (VP, [VP]),
+    "mlwe_new_RNSc_sample_of_zero": (VP, [VP]),
+    "mlwe_new_RNS_trivial_sample_of_zero": (VP, [U64, U64, U64, VP]),
+    "mlwe_RNSc_sample": (None, [VP, VP, VP]),
+    "vfhe_hip_chacha20_blocks": (None, [VP, U64, U64, U64, VP]),
+    "vfhe_hip_mlwe_automorphism_batch": (None, [VPP, VPP, U64, U64, VPP]),
+    "vfhe_hip_mlwe_view_array": (VPP, [VPP, U64, U64]),
+    "vfhe_hip_current_stream": (VP, []),
+    "vfhe_hip_server_new": (VP, [U64, ctypes.c_uint32, ctypes.c_uint32]),
+    "vfhe_hip_server_submit": (U64, [VP, I32, VP, VP, VPP, U64, U64]),
+    "vfhe_hip_server_poll": (U64, [VP, U64P, VPP, ctypes.POINTER(ctypes.c_double), U64, I32]),
+    "vfhe_hip_server_stats": (None, [VP, ctypes.POINTER(ctypes.c_double)]),
+    "vfhe_hip_server_free": (None, [VP]),
+    "load_rous_CT": (VP, [DP, DP, U64]),
+    "load_rous_GS": (VP, [DP, DP, U64]),
+    "CT_NR": (None, [DP, VP, U64]),
+    "GS_RN": (None, [DP, VP, U64]),
+    "bit_reverse_array": (None, [DP, U64, ctypes.c_uint32]),
+    "complex_poly_scale_double": (None, [DP, ctypes.c_double, U64]),
+    "complex_poly_round_to_RNS": (None, [VP, DP, U64]),
+    "complex_polys_ifft_scale_round_to_RNS_batch": (None, [VPP, VPP, U64, U64, ctypes.c_uint32, VP, ctypes.c_double]),
     "vfhe_hip_last_error": (I32, []),
     "vfhe_hip_device_name": (ctypes.c_char_p, []),
     "vfhe_hip_set_option": (I32, [ctypes.c_char_p, ctypes.c_long]),
@@ -195,7 +236,7 @@ def voidp_array(items) -> ctypes.Array:
     """A `void *[]` of handles (ints, c_void_p, or nested ctypes arrays); None -> NULL."""
     arr = (VP * max(1, len(items)))()
     for i, x in enumerate(items):
-        if isinstance(x, ctypes.Array):
+        if isinstance(x, (ctypes.Array, ctypes._Pointer)):
             x = ctypes.cast(x, VP).value
         arr[i] = x
     return arr

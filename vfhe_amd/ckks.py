@@ -1,19 +1,18 @@
 """Host-side mirror of vfhe.fhe.ckks (CKKS_Scheme, CKKS_Ciphertext) on the hip engine.
 
 The homomorphic path (multiply, relinearize, rescale, rotate, key-switch)
-follows modules/fhe/python/src/vfhe/fhe/ckks.py and runs on the engine.  The
-encoder is a host FP64 canonical embedding (numpy FFT over the odd powers of
-zeta = exp(i pi / N), slot j at zeta^(5^j)); it is off the hot path, and it
-keeps the reference's rotation convention: rotate(ct, k) moves slot i + k to
-slot i (test_ckks.py:60-74).
+follows modules/fhe/python/src/vfhe/fhe/ckks.py and runs on the engine, and so
+does the encoder: the special-root complex FFT of complex.c on the device
+(vfhe_amd.complex), slot j at zeta^(5^j), zeta = exp(i pi / N), with the
+reference's rotation convention: rotate(ct, k) moves slot i + k to slot i
+(test_ckks.py:60-74).
 """
 from __future__ import annotations
 
 import math
 
-import numpy as np
-
 from .arith import Polynomial, Ring, crt, repr
+from .complex import ComplexPolynomial, ComplexRing
 from .mlwe import MLWE, MLWE_Scheme
 
 
@@ -22,31 +21,22 @@ class CKKS_Scheme(MLWE_Scheme):
                  special_rings=None):
         super().__init__(rings, special_primes=special_primes, special_rings=special_rings, module_rank=module_rank)
         self.scaling_factor = scaling_factor
-        N = self.ring.N
-        self._zeta_k = np.exp(1j * np.pi * np.arange(N) / N)
-        self._slot_t = np.array([(pow(5, j, 2 * N) - 1) // 2 for j in range(N // 2)])
-        self._conj_t = np.array([((2 * N - pow(5, j, 2 * N)) % (2 * N) - 1) // 2 for j in range(N // 2)])
-
-    # canonical embedding ------------------------------------------------------------
-    def _embed(self, coeffs):
-        N = self.ring.N
-        vals = N * np.fft.ifft(np.asarray(coeffs, dtype=np.float64) * self._zeta_k)
-        return vals[self._slot_t]
-
-    def _unembed(self, slots):
-        N = self.ring.N
-        vals = np.zeros(N, dtype=np.complex128)
-        slots = np.asarray(slots, dtype=np.complex128)
-        vals[self._slot_t] = slots
-        vals[self._conj_t] = np.conj(slots)
-        return (np.fft.fft(vals) / N * np.conj(self._zeta_k)).real
+        self.complex_ring = ComplexRing(self.ring.N // 2, True)
 
     def encode(self, values) -> Polynomial:
+        """ckks.py:37-56: slots -> IFFT (bit reversal, GS_RN, 1/N) -> scale -> round into RNS (NTT form)."""
         assert len(values) == self.ring.N // 2
-        coeffs = np.rint(self._unembed(values) * self.scaling_factor).astype(np.int64)
-        return Polynomial(self.ring).from_array([int(c) for c in coeffs])
+        c_poly = ComplexPolynomial(self.complex_ring)
+        c_poly.from_array(values)
+        c_poly.IFFT()
+        c_poly *= self.scaling_factor
+        poly = c_poly.round_to_RNS_cpp(self.ring)
+        poly.struct.rns_mask = self.ring.mask
+        return poly
 
     def decode(self, poly: Polynomial, scaling_factor: float | None = None):
+        """ckks.py:58-97: CRT-reconstruct the signed coefficients, pack (c_i, c_{i+N/2}) as
+        complex values, FFT (CT_NR, bit reversal), divide by the scaling factor."""
         scaling_factor = self.scaling_factor if scaling_factor is None else scaling_factor
         mask = poly.rns_mask
         rows = poly.get_coeff_matrix(repr.coeff)
@@ -58,7 +48,12 @@ class CKKS_Scheme(MLWE_Scheme):
         for j in range(poly.ring.N):
             v = crt([r[j] for r, _ in act], mods)
             coeffs.append(float(v - Q if v >= Q // 2 else v))
-        return list(self._embed(coeffs) / scaling_factor)
+        c_poly = ComplexPolynomial(self.complex_ring)
+        half = poly.ring.N // 2
+        for i in range(half):
+            c_poly[i] = complex(coeffs[i], coeffs[i + half])
+        c_poly.FFT()
+        return [val / scaling_factor for val in c_poly]
 
     def encrypt(self, message: Polynomial, key):
         out = CKKS_Ciphertext(self)

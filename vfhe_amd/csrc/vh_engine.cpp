@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <sys/random.h>
 #include <vector>
 
 #include "../../include/vfhe_hip.h"
@@ -63,6 +64,7 @@ struct vh_poly {
     uint64_t *d;        // device rows [allocated_l][N]
     uint64_t *hblock;   // host mirror block behind coeffs[]
     int host_dirty;
+    int borrowed;       // a view: d and hblock belong to another polynomial
 };
 
 static inline vh_poly *VP(RNS_Polynomial p) { return reinterpret_cast<vh_poly *>(p); }
@@ -71,15 +73,24 @@ static inline vh_pool *VPOOL(incNTT t) { return reinterpret_cast<vh_pool *>(t); 
 // ---------------------------------------------------------------------------
 // Runtime context: one device, one stream, a staging ring for work lists.
 // ---------------------------------------------------------------------------
-struct vh_ctx {
-    bool ready = false;
-    int device = 0;
+// A lane = one HIP stream (default priority) with its own staging ring for work
+// lists.  Lane 0 is the engine's; the serving runtime runs batches on others so
+// independent requests overlap on the device.  A thread selects its lane with
+// vfhe_hip_set_lane; every launch and copy it makes goes to that lane's stream.
+#define VH_MAX_LANES 8
+struct vh_lane {
     hipStream_t stream = nullptr;
     char *h_ring = nullptr, *d_ring = nullptr;
     size_t ring_size = 0, ring_off = 0;
     int cur_half = 0;
     hipEvent_t half_ev[2];
     bool half_used[2] = {false, false};
+};
+
+struct vh_ctx {
+    bool ready = false;
+    int device = 0;
+    vh_lane lanes[VH_MAX_LANES];
     char name[128] = {0};
     int cus = 0;
     unsigned *d_err = nullptr;   // device error word written by kernels with bounded waits
@@ -87,6 +98,25 @@ struct vh_ctx {
     unsigned xcd_hist[8] = {0};
 };
 static vh_ctx C;
+static thread_local int g_lane = 0;
+static inline vh_lane &LN() { return C.lanes[g_lane]; }
+static inline hipStream_t S() { return C.lanes[g_lane].stream; }
+
+static void lane_init(vh_lane &L, size_t ring)
+{
+    VH_CHECK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    L.ring_size = ring;
+    VH_CHECK(hipHostMalloc((void **)&L.h_ring, L.ring_size, hipHostMallocDefault));
+    VH_CHECK(hipMalloc((void **)&L.d_ring, L.ring_size));
+    VH_CHECK(hipEventCreateWithFlags(&L.half_ev[0], hipEventDisableTiming));
+    VH_CHECK(hipEventCreateWithFlags(&L.half_ev[1], hipEventDisableTiming));
+}
+
+static void sync_all_lanes()
+{
+    for (auto &L : C.lanes)
+        if (L.stream) VH_CHECK(hipStreamSynchronize(L.stream));
+}
 
 // Tunables (vfhe_hip_set_option): NTT schedule for N = 2^13..2^16 on the FP64 path.
 struct vh_opts {
@@ -95,6 +125,8 @@ struct vh_opts {
     long xcd_lag = 3;         // persistent mode: queue slots between an item's two passes
     long xcd_occ = 2;         // persistent mode: workgroups per CU (2 or 4)
     long spin_limit = 1 << 22;
+    long ks_fused = 1;        // key-switch: fuse the forward row pass with the key inner product
+    long fft_fma = 1;         // complex FFT: 1 = avx512ifma engine's fused complex product, 0 = portable's
 };
 static vh_opts OPT;
 static std::recursive_mutex vh_mu;
@@ -109,21 +141,20 @@ static void vh_ensure()
     hipDeviceProp_t prop;
     VH_CHECK(hipGetDeviceProperties(&prop, C.device));
     snprintf(C.name, sizeof(C.name), "%s", prop.gcnArchName);
-    VH_CHECK(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
     hipMemPool_t pool;
     if (hipDeviceGetDefaultMemPool(&pool, C.device) == hipSuccess) {
         uint64_t thr = UINT64_MAX;
-        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
     }
     (void)hipGetLastError();
-    C.ring_size = (size_t)64 << 20;
-    VH_CHECK(hipHostMalloc((void **)&C.h_ring, C.ring_size, hipHostMallocDefault));
-    VH_CHECK(hipMalloc((void **)&C.d_ring, C.ring_size));
+    lane_init(C.lanes[0], (size_t)64 << 20);
     C.cus = prop.multiProcessorCount;
     if (const char *e = getenv("VFHE_HIP_NTT_MODE")) OPT.ntt_mode = atol(e);
     if (const char *e = getenv("VFHE_HIP_NTT_CHUNK_MB")) OPT.ntt_chunk_mb = atol(e);
     if (const char *e = getenv("VFHE_HIP_XCD_LAG")) OPT.xcd_lag = atol(e);
     if (const char *e = getenv("VFHE_HIP_XCD_OCC")) OPT.xcd_occ = atol(e);
+    if (const char *e = getenv("VFHE_HIP_KS_FUSED")) OPT.ks_fused = atol(e);
+    if (const char *e = getenv("VFHE_HIP_FFT_FMA")) OPT.fft_fma = atol(e);
     VH_CHECK(hipMalloc((void **)&C.d_err, 256));
     VH_CHECK(hipMemset(C.d_err, 0, 256));
     {
@@ -132,18 +163,16 @@ static void vh_ensure()
         const unsigned nb = (unsigned)C.cus * 4;
         unsigned *d = nullptr;
         VH_CHECK(hipMalloc((void **)&d, nb * sizeof(unsigned)));
-        VH_CHECK(vh_launch_xcc_probe(C.stream, d, nb));
+        VH_CHECK(vh_launch_xcc_probe(S(), d, nb));
         std::vector<unsigned> h(nb);
-        VH_CHECK(hipMemcpyAsync(h.data(), d, nb * sizeof(unsigned), hipMemcpyDeviceToHost, C.stream));
-        VH_CHECK(hipStreamSynchronize(C.stream));
+        VH_CHECK(hipMemcpyAsync(h.data(), d, nb * sizeof(unsigned), hipMemcpyDeviceToHost, S()));
+        VH_CHECK(hipStreamSynchronize(S()));
         VH_CHECK(hipFree(d));
         bool ok = true;
         for (unsigned x : h) { if (x < 8) C.xcd_hist[x]++; else ok = false; }
         for (int k = 0; k < 8; k++) ok = ok && C.xcd_hist[k] > 0;
         C.xcd_ok = ok;
     }
-    VH_CHECK(hipEventCreateWithFlags(&C.half_ev[0], hipEventDisableTiming));
-    VH_CHECK(hipEventCreateWithFlags(&C.half_ev[1], hipEventDisableTiming));
     C.ready = true;
 }
 
@@ -152,21 +181,22 @@ static void vh_ensure()
 // Copy a small host array to device memory, ordered on the stream.
 static void *vh_stage(const void *src, size_t bytes)
 {
-    const size_t need = (bytes + 255) & ~(size_t)255, half = C.ring_size / 2;
+    vh_lane &L = LN();
+    const size_t need = (bytes + 255) & ~(size_t)255, half = L.ring_size / 2;
     VH_REQUIRE(need <= half);
-    if (C.ring_off + need > (size_t)(C.cur_half + 1) * half) {
-        VH_CHECK(hipEventRecord(C.half_ev[C.cur_half], C.stream));
-        C.half_used[C.cur_half] = true;
-        const int nh = 1 - C.cur_half;
-        if (C.half_used[nh]) VH_CHECK(hipEventSynchronize(C.half_ev[nh]));
-        C.cur_half = nh;
-        C.ring_off = (size_t)nh * half;
+    if (L.ring_off + need > (size_t)(L.cur_half + 1) * half) {
+        VH_CHECK(hipEventRecord(L.half_ev[L.cur_half], L.stream));
+        L.half_used[L.cur_half] = true;
+        const int nh = 1 - L.cur_half;
+        if (L.half_used[nh]) VH_CHECK(hipEventSynchronize(L.half_ev[nh]));
+        L.cur_half = nh;
+        L.ring_off = (size_t)nh * half;
     }
-    const size_t off = C.ring_off;
-    memcpy(C.h_ring + off, src, bytes);
-    VH_CHECK(hipMemcpyAsync(C.d_ring + off, C.h_ring + off, bytes, hipMemcpyHostToDevice, C.stream));
-    C.ring_off += need;
-    return C.d_ring + off;
+    const size_t off = L.ring_off;
+    memcpy(L.h_ring + off, src, bytes);
+    VH_CHECK(hipMemcpyAsync(L.d_ring + off, L.h_ring + off, bytes, hipMemcpyHostToDevice, L.stream));
+    L.ring_off += need;
+    return L.d_ring + off;
 }
 
 template <typename T> static T *vh_stage_vec(const std::vector<T> &v)
@@ -197,12 +227,12 @@ struct vh_prof_scope {
     {
         if (!g_prof) return;
         a = prof_event(); b = prof_event();
-        VH_CHECK(hipEventRecord(a, C.stream));
+        VH_CHECK(hipEventRecord(a, S()));
     }
     ~vh_prof_scope()
     {
         if (!a) return;
-        VH_CHECK(hipEventRecord(b, C.stream));
+        VH_CHECK(hipEventRecord(b, S()));
         g_prof_recs.push_back({tag, a, b});
     }
 };
@@ -210,12 +240,12 @@ struct vh_prof_scope {
 static void *vh_dmalloc(size_t bytes)
 {
     void *p = nullptr;
-    VH_CHECK(hipMallocAsync(&p, bytes ? bytes : 256, C.stream));
+    VH_CHECK(hipMallocAsync(&p, bytes ? bytes : 256, S()));
     return p;
 }
 static void vh_dfree(void *p)
 {
-    if (p) VH_CHECK(hipFreeAsync(p, C.stream));
+    if (p) VH_CHECK(hipFreeAsync(p, S()));
 }
 
 // ---------------------------------------------------------------------------
@@ -407,7 +437,7 @@ extern "C" void ntt_free_proc(NTT_proc proc)
 {
     if (!proc) return;
     VH_ENTER;
-    VH_CHECK(hipStreamSynchronize(C.stream));
+    VH_CHECK(hipStreamSynchronize(S()));
     VH_CHECK(hipFree(proc->d_tab));
     VH_CHECK(hipFree(proc->d_self));
     delete proc;
@@ -455,15 +485,15 @@ static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const v
                 // Persistent kernel: both passes of every transform on one XCD, the
                 // intermediate exchanged through that XCD's L2.
                 unsigned *ctl = (unsigned *)vh_dmalloc((16 + 2 * cnt) * sizeof(unsigned));
-                VH_CHECK(vh_launch_zero_u32(C.stream, ctl, 16 + 2 * cnt));
+                VH_CHECK(vh_launch_zero_u32(S(), ctl, 16 + 2 * cnt));
                 const int occ = OPT.xcd_occ >= 4 ? 4 : 2;
                 {
                     vh_prof_scope ps_(VH_PROF_NTT_FP64);
-                    VH_CHECK(vh_launch_ntt_xcd(C.stream, d_items, cnt, d_limbs, dir, logn, ctl, C.d_err,
+                    VH_CHECK(vh_launch_ntt_xcd(S(), d_items, cnt, d_limbs, dir, logn, ctl, C.d_err,
                                                (unsigned)OPT.xcd_lag, (unsigned)OPT.spin_limit,
                                                (unsigned)(C.cus * occ), occ));
                 }
-                VH_CHECK(vh_launch_ntt_xcd_verify(C.stream, ctl, cnt, logn, C.d_err));
+                VH_CHECK(vh_launch_ntt_xcd_verify(S(), ctl, cnt, logn, C.d_err));
                 vh_dfree(ctl);
             } else if (fp && logn >= 13 && logn <= 16) {
                 // Two register radix-16 passes (column pass of R1 = logn - 8 stages, row pass of 8),
@@ -475,25 +505,25 @@ static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const v
                     const unsigned cc = std::min(per, cnt - c0);
                     const vh_item *ci = d_items + c0;
                     if (dir == 0) {
-                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(C.stream, ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0)); }
-                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(C.stream, ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1)); }
+                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0)); }
+                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1)); }
                     } else {
-                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(C.stream, ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0)); }
-                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(C.stream, ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1)); }
+                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(S(), ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0)); }
+                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1)); }
                     }
                 }
             } else if (logn <= 12) {
-                { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, dir, logn, 0, logn, 0, 1, 1, 1)); }
+                { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, dir, logn, 0, logn, 0, 1, 1, 1)); }
             } else {
                 const int R1 = logn - 8, R2 = 8;
                 const int Gc = std::min(1 << (logn - R1), 4096 >> R1);
                 const int Gr = std::min(1 << R1, 16);
                 if (dir == 0) {
-                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 0, logn, 0, R1, 1, Gc, 1, 0)); }
-                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 0, logn, R1, R2, 0, Gr, 0, 1)); }
+                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 0, logn, 0, R1, 1, Gc, 1, 0)); }
+                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 0, logn, R1, R2, 0, Gr, 0, 1)); }
                 } else {
-                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 1, logn, R1, R2, 0, Gr, 1, 0)); }
-                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(C.stream, d_items, cnt, d_limbs, fp, 1, logn, 0, R1, 1, Gc, 0, 1)); }
+                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 1, logn, R1, R2, 0, Gr, 1, 0)); }
+                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 1, logn, 0, R1, 1, Gc, 0, 1)); }
                 }
             }
         }
@@ -507,7 +537,7 @@ static void run_eltwise(std::vector<vh_item> &items, const vh_limb *d_limbs, int
         std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
         {
             vh_prof_scope ps_(VH_PROF_ELTWISE);
-            VH_CHECK(vh_launch_eltwise(C.stream, vh_stage_vec(chunk), cnt, d_limbs, op, n));
+            VH_CHECK(vh_launch_eltwise(S(), vh_stage_vec(chunk), cnt, d_limbs, op, n));
         }
     }
 }
@@ -563,7 +593,7 @@ static vh_args stage_args(uint64_t n, uint64_t *out, const uint64_t *in1, const 
         a.host[i] = const_cast<uint64_t *>(ptrs[i]);
         a.dev[i] = (uint64_t *)vh_dmalloc(n * 8);
         if (i > 0 || out_is_input)
-            VH_CHECK(hipMemcpyAsync(a.dev[i], ptrs[i], n * 8, hipMemcpyHostToDevice, C.stream));
+            VH_CHECK(hipMemcpyAsync(a.dev[i], ptrs[i], n * 8, hipMemcpyHostToDevice, S()));
     }
     return a;
 }
@@ -571,8 +601,8 @@ static vh_args stage_args(uint64_t n, uint64_t *out, const uint64_t *in1, const 
 static void finish_args(vh_args &a)
 {
     if (!a.staged) return;
-    VH_CHECK(hipMemcpyAsync(a.host[0], a.dev[0], a.n * 8, hipMemcpyDeviceToHost, C.stream));
-    VH_CHECK(hipStreamSynchronize(C.stream));
+    VH_CHECK(hipMemcpyAsync(a.host[0], a.dev[0], a.n * 8, hipMemcpyDeviceToHost, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
     for (int i = 0; i < 3; i++) vh_dfree(a.dev[i]);
 }
 
@@ -639,8 +669,8 @@ static void pool_upload(vh_pool *P)
     vh_limb *old = P->d_limbs;
     P->d_limbs = (vh_limb *)vh_dmalloc(P->h_limbs.size() * sizeof(vh_limb));
     VH_CHECK(hipMemcpyAsync(P->d_limbs, P->h_limbs.data(), P->h_limbs.size() * sizeof(vh_limb),
-                            hipMemcpyHostToDevice, C.stream));
-    VH_CHECK(hipStreamSynchronize(C.stream));
+                            hipMemcpyHostToDevice, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
     vh_dfree(old);
 }
 
@@ -710,8 +740,8 @@ static void dev_ready(RNS_Polynomial p)
 {
     vh_poly *v = VP(p);
     if (!v->host_dirty) return;
-    VH_CHECK(hipMemcpyAsync(v->d, v->hblock, p->allocated_l * p->ntt->N * 8, hipMemcpyHostToDevice, C.stream));
-    VH_CHECK(hipStreamSynchronize(C.stream));
+    VH_CHECK(hipMemcpyAsync(v->d, v->hblock, p->allocated_l * p->ntt->N * 8, hipMemcpyHostToDevice, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
     v->host_dirty = 0;
 }
 
@@ -739,7 +769,7 @@ extern "C" RNS_Polynomial polynomial_new_RNS_polynomial(uint64_t N, uint64_t rns
     if (!v->pub.coeffs || !v->hblock) exit(EXIT_FAILURE);
     for (uint64_t i = 0; i < l; i++) v->pub.coeffs[i] = v->hblock + i * N;
     v->d = (uint64_t *)vh_dmalloc(l * N * 8);
-    VH_CHECK(hipMemsetAsync(v->d, 0, l * N * 8, C.stream));
+    VH_CHECK(hipMemsetAsync(v->d, 0, l * N * 8, S()));
     v->host_dirty = 0;
     return &v->pub;
 }
@@ -762,8 +792,10 @@ extern "C" void free_RNS_polynomial(void *p)
     if (!p) return;
     VH_ENTER;
     vh_poly *v = reinterpret_cast<vh_poly *>(p);
-    vh_dfree(v->d);
-    free(v->hblock);
+    if (!v->borrowed) {
+        vh_dfree(v->d);
+        free(v->hblock);
+    }
     free(v->pub.coeffs);
     delete v;
 }
@@ -779,8 +811,9 @@ extern "C" void polynomial_sync_to_host(RNS_Polynomial p)
     VH_ENTER;
     vh_poly *v = VP(p);
     if (v->host_dirty) return;
-    VH_CHECK(hipMemcpyAsync(v->hblock, v->d, p->allocated_l * p->ntt->N * 8, hipMemcpyDeviceToHost, C.stream));
-    VH_CHECK(hipStreamSynchronize(C.stream));
+    sync_all_lanes();   // the rows may have been written on any lane
+    VH_CHECK(hipMemcpyAsync(v->hblock, v->d, p->allocated_l * p->ntt->N * 8, hipMemcpyDeviceToHost, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
     vh_check_device_error();
 }
 
@@ -801,8 +834,18 @@ extern "C" uint64_t *polynomial_device_row(RNS_Polynomial p, uint64_t limb)
 extern "C" void vfhe_hip_synchronize(void)
 {
     VH_ENTER;
-    VH_CHECK(hipStreamSynchronize(C.stream));
+    sync_all_lanes();
     vh_check_device_error();
+}
+
+extern "C" int vfhe_hip_set_lane(int lane)
+{
+    VH_ENTER;
+    if (lane < 0 || lane >= VH_MAX_LANES) return -1;
+    if (!C.lanes[lane].stream) lane_init(C.lanes[lane], (size_t)16 << 20);
+    const int prev = g_lane;
+    g_lane = lane;
+    return prev;
 }
 
 extern "C" int vfhe_hip_set_option(const char *key, long value)
@@ -814,6 +857,8 @@ extern "C" int vfhe_hip_set_option(const char *key, long value)
     else if (!strcmp(key, "ntt.xcd_lag")) OPT.xcd_lag = value < 0 ? 0 : value;
     else if (!strcmp(key, "ntt.xcd_occupancy")) OPT.xcd_occ = value;
     else if (!strcmp(key, "ntt.spin_limit")) OPT.spin_limit = value < 1 ? 1 : value;
+    else if (!strcmp(key, "ks.fused")) OPT.ks_fused = value;
+    else if (!strcmp(key, "fft.fma")) OPT.fft_fma = value;
     else return -1;
     return 0;
 }
@@ -829,6 +874,8 @@ extern "C" long vfhe_hip_get_option(const char *key)
     if (!strcmp(key, "ntt.xcd_lag")) return OPT.xcd_lag;
     if (!strcmp(key, "ntt.xcd_occupancy")) return OPT.xcd_occ;
     if (!strcmp(key, "ntt.spin_limit")) return OPT.spin_limit;
+    if (!strcmp(key, "ks.fused")) return OPT.ks_fused;
+    if (!strcmp(key, "fft.fma")) return OPT.fft_fma;
     return -1;
 }
 
@@ -966,26 +1013,41 @@ extern "C" void polynomial_RNS_to_RNSc(RNSc_Polynomial out, RNS_Polynomial in)
     ntt_rows({{out, in}}, 1);
 }
 
+// Signed int64 rows already on the device -> every active limb (split placement),
+// then forward NTT: the device half of int_array_to_RNS for many polynomials.
+static void int_rows_to_RNS(const std::vector<std::pair<RNS_Polynomial, const uint64_t *>> &jobs)
+{
+    if (jobs.empty()) return;
+    std::vector<vh_item> items;
+    std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pairs;
+    for (auto &jb : jobs) {
+        RNS_Polynomial out = jb.first;
+        dev_ready(out);
+        for_active(out->rns_mask, out->ntt->l, [&](uint64_t i) {
+            items.push_back(mk_item(jb.second, drow(out, i), i, nullptr, out->ntt->split_degree));
+        });
+        pairs.push_back({out, out});
+    }
+    RNS_Polynomial p0 = jobs[0].first;
+    for (size_t off = 0; off < items.size(); off += VH_MAX_GRID_Y) {
+        const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, items.size() - off);
+        std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
+        vh_prof_scope ps_(VH_PROF_OTHER);
+        VH_CHECK(vh_launch_scatter_split(S(), vh_stage_vec(chunk), cnt, DL(p0), p0->ntt->N, 1));
+    }
+    ntt_rows(pairs, 0);
+}
+
 // int_array_to_RNS (rns_polynomial.c:204-223) and array_to_RNS (:225-240)
 extern "C" void int_array_to_RNS(RNS_Polynomial out, uint64_t *in)
 {
     VH_ENTER;
-    dev_ready(out);
     const uint64_t N = out->ntt->N;
     uint64_t *tmp = (uint64_t *)vh_dmalloc(N * 8);
-    VH_CHECK(hipMemcpyAsync(tmp, in, N * 8, hipMemcpyHostToDevice, C.stream));
-    VH_CHECK(hipStreamSynchronize(C.stream));
-    std::vector<vh_item> items;
-    for_active(out->rns_mask, out->ntt->l, [&](uint64_t i) {
-        items.push_back(mk_item(tmp, drow(out, i), i, nullptr, out->ntt->split_degree));
-    });
-    if (!items.empty())
-        {
-            vh_prof_scope ps_(VH_PROF_OTHER);
-            VH_CHECK(vh_launch_scatter_split(C.stream, vh_stage_vec(items), (unsigned)items.size(), DL(out), N, 1));
-        }
+    VH_CHECK(hipMemcpyAsync(tmp, in, N * 8, hipMemcpyHostToDevice, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
+    int_rows_to_RNS({{out, tmp}});
     vh_dfree(tmp);
-    ntt_rows({{out, out}}, 0);
 }
 
 extern "C" void array_to_RNS(RNS_Polynomial out, uint64_t **in)
@@ -1000,15 +1062,15 @@ extern "C" void array_to_RNS(RNS_Polynomial out, uint64_t **in)
     uint64_t *tmp = (uint64_t *)vh_dmalloc(cnt * N * 8);
     uint64_t k = 0;
     for_active(out->rns_mask, out->ntt->l, [&](uint64_t i) {
-        VH_CHECK(hipMemcpyAsync(tmp + k * N, in[i], N * 8, hipMemcpyHostToDevice, C.stream));
+        VH_CHECK(hipMemcpyAsync(tmp + k * N, in[i], N * 8, hipMemcpyHostToDevice, S()));
         items.push_back(mk_item(tmp + k * N, drow(out, i), i, nullptr, out->ntt->split_degree));
         k++;
     });
-    VH_CHECK(hipStreamSynchronize(C.stream));
+    VH_CHECK(hipStreamSynchronize(S()));
     if (!items.empty())
         {
             vh_prof_scope ps_(VH_PROF_OTHER);
-            VH_CHECK(vh_launch_scatter_split(C.stream, vh_stage_vec(items), (unsigned)items.size(), DL(out), N, 0));
+            VH_CHECK(vh_launch_scatter_split(S(), vh_stage_vec(items), (unsigned)items.size(), DL(out), N, 0));
         }
     vh_dfree(tmp);
     ntt_rows({{out, out}}, 0);
@@ -1046,7 +1108,7 @@ static void run_muls(const std::vector<vh_mulreq> &reqs, int mode)
             std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
             {
                 vh_prof_scope ps_(VH_PROF_ELTWISE);
-                VH_CHECK(vh_launch_split_mul(C.stream, vh_stage_vec(chunk), cnt, P->d_limbs, mode, P->n, t->split_degree));
+                VH_CHECK(vh_launch_split_mul(S(), vh_stage_vec(chunk), cnt, P->d_limbs, mode, P->n, t->split_degree));
             }
         }
     }
@@ -1263,7 +1325,7 @@ static void run_division(const std::vector<RNS_Polynomial> &polys, uint64_t divi
                     std::vector<vh_div_job> chunkv(jobs.begin() + off, jobs.begin() + off + cnt);
                     {
                         vh_prof_scope ps_(VH_PROF_DIVISION);
-                        VH_CHECK(vh_launch_division(C.stream, vh_stage_vec(chunkv), cnt, (unsigned)outs.size(),
+                        VH_CHECK(vh_launch_division(S(), vh_stage_vec(chunkv), cnt, (unsigned)outs.size(),
                                                 VPOOL(t)->d_limbs, d_consts, t->N, (int)divs.size()));
                     }
                 }
@@ -1394,7 +1456,7 @@ extern "C" void polynomial_base_conversion_RNSc(RNSc_Polynomial out, RNSc_Polyno
         const vh_bconv_job *d_job = (const vh_bconv_job *)vh_stage(&jb, sizeof(jb));
         {
             vh_prof_scope ps_(VH_PROF_OTHER);
-            VH_CHECK(vh_launch_base_conversion(C.stream, d_job, 1, p->v, DL(out), d_consts, out->ntt->N));
+            VH_CHECK(vh_launch_base_conversion(S(), d_job, 1, p->v, DL(out), d_consts, out->ntt->N));
         }
     }
     if (!params) free_base_conversion_params(p);
@@ -1454,7 +1516,7 @@ static void permute_into(RNS_Polynomial out, RNS_Polynomial in, uint64_t gen)
     if (!perm.empty())
         {
             vh_prof_scope ps_(VH_PROF_OTHER);
-            VH_CHECK(vh_launch_permute(C.stream, vh_stage_vec(perm), (unsigned)perm.size(), DL(out), out->ntt->N));
+            VH_CHECK(vh_launch_permute(S(), vh_stage_vec(perm), (unsigned)perm.size(), DL(out), out->ntt->N));
         }
     run_eltwise(zero, DL(out), VH_OP_ZERO, out->ntt->N);
 }
@@ -1724,15 +1786,39 @@ static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector
                 for (uint64_t ti = 0; ti < next; ti++)
                     lift.push_back(mk_item(src, modup + ((bi * ND + d) * next + ti) * N, (uint32_t)E[ti]));
             }
-        run_eltwise(lift, P->d_limbs, VH_OP_REDUCE, N);
+        // On the FP64 path (every prime < 2^50) the forward NTT mod q_t accepts any
+        // input below 2^50, so the digit row is transformed straight from the input
+        // (lift fused into the first pass); otherwise it is reduced first.
+        bool fuse = true;
+        for (int e : E) fuse = fuse && P->h_limbs[e].fp64;
+        for (auto &dg : D) fuse = fuse && P->h_limbs[dg.limb].fp64;
+        if (!fuse) run_eltwise(lift, P->d_limbs, VH_OP_REDUCE, N);
         std::vector<vh_item> nt;
         nt.reserve(lift.size() * t->split_degree);
         for (auto &it : lift)
             for (uint64_t k = 0; k < t->split_degree; k++)
-                nt.push_back(mk_item(it.dst + k * P->n, it.dst + k * P->n, it.limb));
-        run_ntt(nt, P->d_limbs, P->h_limbs.data(), P->n, 0);
+                nt.push_back(mk_item((fuse ? it.src : it.dst) + k * P->n, it.dst + k * P->n, it.limb));
+        const int logn = ilog2(P->n);
+        // Fused form: column pass here, then the row pass of every digit and the key
+        // inner product in one kernel (the transformed digits stay in registers).
+        const bool fused_mac = fuse && OPT.ks_fused && OPT.ntt_mode == 0 && t->split_degree == 1 && Cc <= 2 &&
+                               logn >= 13 && logn <= 16;
+        if (fused_mac) {
+            for (size_t off = 0; off < nt.size(); off += VH_MAX_GRID_Y) {
+                const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, nt.size() - off);
+                std::vector<vh_item> chunk(nt.begin() + off, nt.begin() + off + cnt);
+                vh_prof_scope ps_(VH_PROF_NTT_FP64);
+                VH_CHECK(vh_launch_ntt16(S(), vh_stage_vec(chunk), cnt, P->d_limbs, 0, logn, 0, logn - 8, 1, 1, 0));
+            }
+        } else {
+            run_ntt(nt, P->d_limbs, P->h_limbs.data(), P->n, 0);
+        }
 
         if (t->split_degree == 1) {
+            // FP64 accumulation needs canonical key rows: engine-made keys are; it is
+            // used only when every extended limb is on the FP64 path.
+            bool fp_mac = true;
+            for (int e : E) fp_mac = fp_mac && P->h_limbs[e].fp64;
             const uint64_t *const *d_kptr = vh_stage_vec(kptr);
             std::vector<vh_mac_job> jobs(B);
             for (size_t bi = 0; bi < B; bi++) {
@@ -1752,7 +1838,12 @@ static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector
                 std::vector<vh_mac_job> chunk(jobs.begin() + off, jobs.begin() + off + cnt);
                 {
                     vh_prof_scope ps_(VH_PROF_KSMAC);
-                    VH_CHECK(vh_launch_keyswitch_mac(C.stream, vh_stage_vec(chunk), cnt, (unsigned)next, P->d_limbs, d_kptr, N));
+                    if (fused_mac)
+                        VH_CHECK(vh_launch_ntt16_row_mac(S(), vh_stage_vec(chunk), cnt, (unsigned)next, P->d_limbs,
+                                                         d_kptr, logn, (int)Cc));
+                    else
+                        VH_CHECK(vh_launch_keyswitch_mac(S(), vh_stage_vec(chunk), cnt, (unsigned)next, P->d_limbs,
+                                                         d_kptr, N, fp_mac ? 1 : 0));
                 }
             }
         } else {
@@ -1782,7 +1873,7 @@ static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector
                     std::vector<vh_item> chunk(items.begin() + d * per, items.begin() + (d + 1) * per);
                     {
                         vh_prof_scope ps_(VH_PROF_ELTWISE);
-                        VH_CHECK(vh_launch_split_mul(C.stream, vh_stage_vec(chunk), (unsigned)chunk.size(), P->d_limbs, 2, P->n, t->split_degree));
+                        VH_CHECK(vh_launch_split_mul(S(), vh_stage_vec(chunk), (unsigned)chunk.size(), P->d_limbs, 2, P->n, t->split_degree));
                     }
                 }
                 items.clear();
@@ -1839,6 +1930,77 @@ extern "C" void mlwe_automorphism_RNSc_GHS(RNSc_MLWE out, RNSc_MLWE in, uint64_t
     permute_into(tmp->b, in->b, gen);
     keyswitch_batch({out}, {tmp}, ksk);
     free_mlwe_RNS_sample(tmp);
+}
+
+// mlwe_automorphism_RNSc_GHS (mlwe.c:314-326) for many inputs sharing gen and ksk:
+// permute every component, then one batched key-switch.
+static void automorphism_batch(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &ins, uint64_t gen,
+                               RNS_MLWE **ksk)
+{
+    const size_t B = outs.size();
+    VH_REQUIRE(B > 0 && ins.size() == B);
+    std::vector<RNS_MLWE> tmp(B);
+    for (size_t bi = 0; bi < B; bi++) {
+        RNS_MLWE o = outs[bi], x = ins[bi];
+        VH_REQUIRE(o != x && o->r == x->r);
+        tmp[bi] = mlwe_alloc_RNS_sample(o->a[0]->ntt->N, o->r, o->a[0]->rns_mask, o->a[0]->ntt);
+        for (uint64_t i = 0; i < o->r; i++) permute_into(tmp[bi]->a[i], x->a[i], gen);
+        permute_into(tmp[bi]->b, x->b, gen);
+    }
+    keyswitch_batch(outs, tmp, ksk);
+    for (auto t : tmp) free_mlwe_RNS_sample(t);
+}
+
+extern "C" void vfhe_hip_mlwe_automorphism_batch(RNS_MLWE *out, RNS_MLWE *in, uint64_t count, uint64_t gen,
+                                                 RNS_MLWE **ksk)
+{
+    VH_ENTER;
+    if (count == 0) return;
+    automorphism_batch(std::vector<RNS_MLWE>(out, out + count), std::vector<RNS_MLWE>(in, in + count), gen, ksk);
+}
+
+// A view of p restricted to `mask`: shares p's device rows and host mirror.
+static RNS_Polynomial view_poly(RNS_Polynomial p, uint64_t mask)
+{
+    dev_ready(p);
+    check_rows(p, mask);
+    vh_poly *src = VP(p);
+    vh_poly *v = new vh_poly();
+    v->pub = src->pub;
+    v->pub.rns_mask = mask;
+    const uint64_t l = p->allocated_l;
+    v->pub.coeffs = (uint64_t **)malloc(sizeof(uint64_t *) * (l ? l : 1));
+    if (!v->pub.coeffs) exit(EXIT_FAILURE);
+    for (uint64_t i = 0; i < l; i++) v->pub.coeffs[i] = p->coeffs[i];
+    v->d = src->d;
+    v->hblock = src->hblock;
+    v->host_dirty = 0;
+    v->borrowed = 1;
+    return &v->pub;
+}
+
+extern "C" RNS_MLWE *vfhe_hip_mlwe_view_array(RNS_MLWE *in, uint64_t count, uint64_t mask)
+{
+    VH_ENTER;
+    RNS_MLWE *out = (RNS_MLWE *)malloc(sizeof(RNS_MLWE) * (count ? count : 1));
+    if (!out) exit(EXIT_FAILURE);
+    for (uint64_t k = 0; k < count; k++) {
+        RNS_MLWE c = (RNS_MLWE)malloc(sizeof(*c));
+        if (!c) exit(EXIT_FAILURE);
+        c->r = in[k]->r;
+        c->a = (RNS_Polynomial *)malloc(sizeof(RNS_Polynomial) * (c->r ? c->r : 1));
+        if (!c->a) exit(EXIT_FAILURE);
+        for (uint64_t i = 0; i < c->r; i++) c->a[i] = view_poly(in[k]->a[i], mask);
+        c->b = view_poly(in[k]->b, mask);
+        out[k] = c;
+    }
+    return out;
+}
+
+extern "C" void *vfhe_hip_current_stream(void)
+{
+    VH_ENTER;
+    return (void *)S();
 }
 
 // mlwe_multiply (mlwe.c:711-760), batched: tensor, INTT, key-switch, then either
@@ -1996,12 +2158,400 @@ extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x
 
 
 // ---------------------------------------------------------------------------
+// CKKS encoder FFT (modules/arith/c/src/complex.c).  Twiddle tables live on the
+// device; the host-buffer entry points copy the row in, transform it on the
+// device and copy it back, keeping the reference's in-place contract.
+// ---------------------------------------------------------------------------
+struct vh_rous {
+    double2 *d;
+    uint64_t size;
+};
+
+static void *load_rous(const double *re, const double *im, uint64_t size)
+{
+    VH_REQUIRE(re && im && size >= 2);
+    std::vector<double2> h(size);
+    for (uint64_t i = 0; i < size; i++) h[i] = make_double2(re[i], im[i]);
+    vh_rous *r = new vh_rous();
+    r->size = size;
+    VH_CHECK(hipMalloc((void **)&r->d, size * sizeof(double2)));
+    VH_CHECK(hipMemcpy(r->d, h.data(), size * sizeof(double2), hipMemcpyHostToDevice));
+    return r;
+}
+
+extern "C" void *load_rous_CT(double *rous_real, double *rous_imag, uint64_t size)
+{
+    VH_ENTER;
+    return load_rous(rous_real, rous_imag, size);
+}
+
+extern "C" void *load_rous_GS(double *rous_real, double *rous_imag, uint64_t size)
+{
+    VH_ENTER;
+    return load_rous(rous_real, rous_imag, size);
+}
+
+static int log2_exact(uint64_t n)
+{
+    VH_REQUIRE(n >= 2 && (n & (n - 1)) == 0);
+    return ilog2(n);
+}
+
+// Transforms `count` device rows of 2n doubles in place.
+static void fft_rows(double *const *d_rows_dev, unsigned count, const vh_rous *ws, uint64_t n, int dir)
+{
+    VH_REQUIRE(ws && n <= ws->size);
+    vh_prof_scope ps_(VH_PROF_OTHER);
+    VH_CHECK(vh_launch_fft(S(), d_rows_dev, count, ws->d, log2_exact(n), dir, OPT.fft_fma ? 1 : 0));
+}
+
+static void fft_host_row(double *x, void *ws, uint64_t n, int dir)
+{
+    double *d = (double *)vh_dmalloc(2 * n * 8);
+    VH_CHECK(hipMemcpyAsync(d, x, 2 * n * 8, hipMemcpyHostToDevice, S()));
+    std::vector<double *> rows = {d};
+    fft_rows(vh_stage_vec(rows), 1, (const vh_rous *)ws, n, dir);
+    VH_CHECK(hipMemcpyAsync(x, d, 2 * n * 8, hipMemcpyDeviceToHost, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
+    vh_dfree(d);
+}
+
+extern "C" void CT_NR(double *x, void *ws, uint64_t n)
+{
+    VH_ENTER;
+    fft_host_row(x, ws, n, 0);
+}
+
+extern "C" void GS_RN(double *x, void *ws, uint64_t n)
+{
+    VH_ENTER;
+    fft_host_row(x, ws, n, 1);
+}
+
+static void bitrev_rows(double *const *d_rows, double *const *d_tmp, const std::vector<double *> &rows,
+                        const std::vector<double *> &tmp, uint64_t N, uint32_t prec)
+{
+    VH_REQUIRE(prec <= 20 && N == ((uint64_t)1 << prec));
+    {
+        vh_prof_scope ps_(VH_PROF_OTHER);
+        VH_CHECK(vh_launch_fft_bitrev(S(), d_rows, d_tmp, (unsigned)rows.size(), N, prec));
+    }
+    for (size_t k = 0; k < rows.size(); k++)
+        VH_CHECK(hipMemcpyAsync(rows[k], tmp[k], 2 * N * 8, hipMemcpyDeviceToDevice, S()));
+}
+
+extern "C" void bit_reverse_array(double *v, uint64_t N, uint32_t prec)
+{
+    VH_ENTER;
+    double *d = (double *)vh_dmalloc(4 * N * 8);
+    VH_CHECK(hipMemcpyAsync(d, v, 2 * N * 8, hipMemcpyHostToDevice, S()));
+    std::vector<double *> rows = {d}, tmp = {d + 2 * N};
+    bitrev_rows(vh_stage_vec(rows), vh_stage_vec(tmp), rows, tmp, N, prec);
+    VH_CHECK(hipMemcpyAsync(v, d, 2 * N * 8, hipMemcpyDeviceToHost, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
+    vh_dfree(d);
+}
+
+extern "C" void complex_poly_scale_double(double *v, double scale, uint64_t N)
+{
+    VH_ENTER;
+    double *d = (double *)vh_dmalloc(2 * N * 8);
+    VH_CHECK(hipMemcpyAsync(d, v, 2 * N * 8, hipMemcpyHostToDevice, S()));
+    std::vector<double *> rows = {d};
+    VH_CHECK(vh_launch_fft_scale(S(), vh_stage_vec(rows), 1, scale, 2 * N));
+    VH_CHECK(hipMemcpyAsync(v, d, 2 * N * 8, hipMemcpyDeviceToHost, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
+    vh_dfree(d);
+}
+
+// complex.c:39-50: the first out->ntt->N doubles of `in`, rounded half-to-even,
+// as signed coefficients of `out` (int_array_to_RNS: every active limb, NTT form).
+extern "C" void complex_poly_round_to_RNS(RNS_Polynomial out, double *in, uint64_t N)
+{
+    (void)N;
+    VH_ENTER;
+    const uint64_t n = out->ntt->N;
+    double *d = (double *)vh_dmalloc(n * 8);
+    uint64_t *di = (uint64_t *)vh_dmalloc(n * 8);
+    VH_CHECK(hipMemcpyAsync(d, in, n * 8, hipMemcpyHostToDevice, S()));
+    std::vector<double *> rows = {d};
+    std::vector<uint64_t *> outs = {di};
+    VH_CHECK(vh_launch_fft_round(S(), vh_stage_vec(rows), vh_stage_vec(outs), 1, n));
+    int_rows_to_RNS({{out, di}});
+    VH_CHECK(hipStreamSynchronize(S()));
+    vh_dfree(d);
+    vh_dfree(di);
+}
+
+// complex.c:455-500: per row, bit-reverse, GS_RN, scale by temp_delta / n_complex,
+// round into outs_rns[i] (NTT form).  Rows are host buffers of 2 n_complex doubles
+// and, as in the reference, hold the scaled transform afterwards.  All rows are
+// transformed in one launch sequence.
+extern "C" void complex_polys_ifft_scale_round_to_RNS_batch(void **rows_in, void **outs_rns, uint64_t count,
+                                                           uint64_t n_complex, uint32_t log_prec, void *gs_ws,
+                                                           double temp_delta)
+{
+    VH_ENTER;
+    if (count == 0) return;
+    const uint64_t n2 = 2 * n_complex;
+    for (uint64_t i = 0; i < count; i++)
+        VH_REQUIRE(((RNS_Polynomial)outs_rns[i])->ntt->N <= n2);
+    double *d = (double *)vh_dmalloc(2 * count * n2 * 8);
+    uint64_t *di = (uint64_t *)vh_dmalloc(count * n2 * 8);
+    std::vector<double *> rows(count), tmp(count);
+    std::vector<uint64_t *> ints(count);
+    for (uint64_t i = 0; i < count; i++) {
+        rows[i] = d + i * n2;
+        tmp[i] = d + (count + i) * n2;
+        ints[i] = di + i * n2;
+        VH_CHECK(hipMemcpyAsync(rows[i], rows_in[i], n2 * 8, hipMemcpyHostToDevice, S()));
+    }
+    for (uint64_t off = 0; off < count; off += VH_MAX_GRID_Y) {
+        const uint64_t cnt = std::min<uint64_t>(VH_MAX_GRID_Y, count - off);
+        std::vector<double *> r(rows.begin() + off, rows.begin() + off + cnt), t(tmp.begin() + off, tmp.begin() + off + cnt);
+        std::vector<uint64_t *> o(ints.begin() + off, ints.begin() + off + cnt);
+        double *const *d_rows = vh_stage_vec(r);
+        bitrev_rows(d_rows, vh_stage_vec(t), r, t, n_complex, log_prec);
+        fft_rows(d_rows, (unsigned)cnt, (const vh_rous *)gs_ws, n_complex, 1);
+        VH_CHECK(vh_launch_fft_scale(S(), d_rows, (unsigned)cnt, temp_delta / (double)n_complex, n2));
+        VH_CHECK(vh_launch_fft_round(S(), d_rows, vh_stage_vec(o), (unsigned)cnt, n2));
+    }
+    std::vector<std::pair<RNS_Polynomial, const uint64_t *>> jobs;
+    for (uint64_t i = 0; i < count; i++) jobs.push_back({(RNS_Polynomial)outs_rns[i], ints[i]});
+    int_rows_to_RNS(jobs);
+    for (uint64_t i = 0; i < count; i++)
+        VH_CHECK(hipMemcpyAsync(rows_in[i], rows[i], n2 * 8, hipMemcpyDeviceToHost, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
+    vh_dfree(d);
+    vh_dfree(di);
+}
+
+// ---------------------------------------------------------------------------
+// On-device sampling.  Keystream: ChaCha20 under a 256-bit engine key drawn
+// from the OS (getrandom) at first use, or derived from a deterministic seed
+// (prng.c:132-143 semantics: tests only); every sampling call takes a fresh
+// nonce, and rows/jobs inside a call take disjoint counter ranges.
+// ---------------------------------------------------------------------------
+struct vh_rng_state {
+    bool init = false;
+    uint32_t key[8];
+    uint64_t calls = 0;
+};
+static vh_rng_state RNG;
+
+static void rng_seed_os()
+{
+    size_t got = 0;
+    while (got < sizeof(RNG.key)) {
+        const ssize_t r = getrandom((char *)RNG.key + got, sizeof(RNG.key) - got, 0);
+        if (r <= 0) vh_die("getrandom failed", __FILE__, __LINE__);
+        got += (size_t)r;
+    }
+    RNG.calls = 0;
+    RNG.init = true;
+}
+
+static vh_rng_key rng_next()
+{
+    if (!RNG.init) rng_seed_os();
+    vh_rng_key k;
+    memcpy(k.k, RNG.key, sizeof(k.k));
+    k.nonce = RNG.calls++;
+    return k;
+}
+
+extern "C" void vfhe_prng_set_deterministic_seed(uint64_t seed)
+{
+    VH_ENTER;
+    uint64_t st = seed;
+    for (int i = 0; i < 4; i++) {   // splitmix64 expansion of the seed into the key
+        uint64_t z = (st += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        RNG.key[2 * i] = (uint32_t)z;
+        RNG.key[2 * i + 1] = (uint32_t)(z >> 32);
+    }
+    RNG.calls = 0;
+    RNG.init = true;
+}
+
+extern "C" void vfhe_prng_clear_deterministic_seed(void)
+{
+    VH_ENTER;
+    rng_seed_os();
+}
+
+extern "C" void vfhe_hip_chacha20_blocks(const uint32_t *key, uint64_t nonce, uint64_t ctr0, uint64_t nblocks,
+                                         uint32_t *out)
+{
+    VH_ENTER;
+    vh_rng_key k;
+    memcpy(k.k, key, sizeof(k.k));
+    k.nonce = nonce;
+    uint32_t *d = (uint32_t *)vh_dmalloc(nblocks * 64);
+    VH_CHECK(vh_launch_chacha_raw(S(), k, ctr0, nblocks, d));
+    VH_CHECK(hipMemcpyAsync(out, d, nblocks * 64, hipMemcpyDeviceToHost, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
+    vh_dfree(d);
+}
+
+static void sample_uniform_rows(const std::vector<std::pair<RNS_Polynomial, uint64_t>> &rows)
+{
+    if (rows.empty()) return;
+    RNS_Polynomial p0 = rows[0].first;
+    const uint64_t N = p0->ntt->N;
+    VH_REQUIRE(N % 8 == 0);
+    std::vector<vh_item> items;
+    uint32_t k = 0;
+    for (auto &pr : rows) {
+        vh_item it = mk_item(nullptr, drow(pr.first, pr.second), (uint32_t)pr.second);
+        it.aux = k++;
+        items.push_back(it);
+    }
+    const vh_rng_key key = rng_next();
+    for (size_t off = 0; off < items.size(); off += VH_MAX_GRID_Y) {
+        const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, items.size() - off);
+        std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
+        vh_prof_scope ps_(VH_PROF_OTHER);
+        VH_CHECK(vh_launch_sample_uniform(S(), vh_stage_vec(chunk), cnt, DL(p0), key, N));
+    }
+}
+
+// One noise vector per polynomial (shared by its active limbs); mode 1 writes
+// truncated raw int64 values into limb row 0 instead.
+static void sample_gaussian_polys(const std::vector<RNS_Polynomial> &polys, double sigma, uint32_t mode)
+{
+    if (polys.empty()) return;
+    const uint64_t N = polys[0]->ntt->N;
+    VH_REQUIRE(N % 4 == 0);
+    std::vector<vh_item> items;
+    std::vector<vh_gauss_job> jobs;
+    for (auto p : polys) {
+        VH_REQUIRE(p->ntt == polys[0]->ntt);
+        vh_gauss_job jb = {(uint32_t)items.size(), 0, (uint32_t)jobs.size(), mode};
+        if (mode) {
+            items.push_back(mk_item(nullptr, drow(p, 0), 0));
+        } else {
+            for_active(p->rns_mask, std::min(p->ntt->l, p->allocated_l),
+                       [&](uint64_t i) { items.push_back(mk_item(nullptr, drow(p, i), (uint32_t)i)); });
+        }
+        jb.count = (uint32_t)items.size() - jb.first;
+        jobs.push_back(jb);
+    }
+    const vh_rng_key key = rng_next();
+    const vh_item *d_items = vh_stage_vec(items);
+    for (size_t off = 0; off < jobs.size(); off += VH_MAX_GRID_Y) {
+        const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, jobs.size() - off);
+        std::vector<vh_gauss_job> chunk(jobs.begin() + off, jobs.begin() + off + cnt);
+        vh_prof_scope ps_(VH_PROF_OTHER);
+        VH_CHECK(vh_launch_sample_gaussian(S(), vh_stage_vec(chunk), cnt, d_items, DL(polys[0]), key, sigma, N));
+    }
+}
+
+extern "C" void polynomial_gen_random_RNSc_polynomial(RNSc_Polynomial out)
+{
+    VH_ENTER;
+    dev_ready(out);
+    std::vector<std::pair<RNS_Polynomial, uint64_t>> rows;
+    for_active(out->rns_mask, std::min(out->ntt->l, out->allocated_l), [&](uint64_t i) { rows.push_back({out, i}); });
+    sample_uniform_rows(rows);
+}
+
+extern "C" void polynomial_gen_gaussian_RNSc_polynomial(RNSc_Polynomial out, double sigma)
+{
+    VH_ENTER;
+    dev_ready(out);
+    sample_gaussian_polys({out}, sigma, 0);
+}
+
+// mlwe.c:190-200: b = e + sum_i s_i * a_i with a_i uniform, all in NTT form.
+static void sample_of_zero(RNS_MLWE out, RNS_MLWE_Key key)
+{
+    for (auto p : comps(out)) dev_ready(p);
+    sample_gaussian_polys({out->b}, key->sigma, 0);
+    std::vector<std::pair<RNS_Polynomial, uint64_t>> rows;
+    for (uint64_t i = 0; i < out->r; i++)
+        for_active(out->a[i]->rns_mask, std::min(out->a[i]->ntt->l, out->a[i]->allocated_l),
+                   [&](uint64_t j) { rows.push_back({out->a[i], j}); });
+    sample_uniform_rows(rows);
+    std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pairs;
+    for (auto p : comps(out)) pairs.push_back({p, p});
+    ntt_rows(pairs, 0);
+    for (uint64_t i = 0; i < out->r; i++) polynomial_mul_addto_RNS_polynomial(out->b, key->s_RNS[i], out->a[i]);
+}
+
+extern "C" void mlwe_RNS_sample_of_zero(RNS_MLWE out, RNS_MLWE_Key key)
+{
+    VH_ENTER;
+    sample_of_zero(out, key);
+}
+
+extern "C" void mlwe_RNSc_sample_of_zero(RNSc_MLWE out, RNS_MLWE_Key key)
+{
+    VH_ENTER;
+    sample_of_zero(out, key);
+    mlwe_RNS_to_RNSc(out, out);
+}
+
+// mlwe.c:281-293 allocate with mask = key->l (the limb count used as a mask),
+// kept as the reference has it so masks match.
+extern "C" RNS_MLWE mlwe_new_RNS_sample_of_zero(RNS_MLWE_Key key)
+{
+    VH_ENTER;
+    RNS_MLWE res = mlwe_alloc_RNS_sample(key->N, key->r, key->l, key->s_RNS[0]->ntt);
+    sample_of_zero(res, key);
+    return res;
+}
+
+extern "C" RNSc_MLWE mlwe_new_RNSc_sample_of_zero(RNS_MLWE_Key key)
+{
+    VH_ENTER;
+    RNSc_MLWE res = mlwe_alloc_RNS_sample(key->N, key->r, key->l, key->s_RNS[0]->ntt);
+    mlwe_RNSc_sample_of_zero(res, key);
+    return res;
+}
+
+extern "C" RNS_MLWE mlwe_new_RNS_trivial_sample_of_zero(uint64_t N, uint64_t r, uint64_t mask, incNTT ntt)
+{
+    VH_ENTER;
+    RNS_MLWE res = mlwe_alloc_RNS_sample(N, r, mask, ntt);
+    mlwe_RNS_trivial_sample_of_zero(res);
+    return res;
+}
+
+// mlwe.c:236-241
+extern "C" void mlwe_RNSc_sample(RNSc_MLWE out, RNS_MLWE_Key key, RNSc_Polynomial m)
+{
+    VH_ENTER;
+    VH_REQUIRE(m->rns_mask == out->b->rns_mask);
+    mlwe_RNSc_sample_of_zero(out, key);
+    polynomial_add_RNSc_polynomial(out->b, out->b, m);
+}
+
+// mlwe.c:70-83: secret coefficients (int64)N(0, key_sigma) (truncated), then RNS form.
+extern "C" RNS_MLWE_Key mlwe_new_RNS_gaussian_key(uint64_t N, uint64_t r, uint64_t l, double key_sigma, incNTT ntt,
+                                                  double sigma)
+{
+    VH_ENTER;
+    std::vector<uint64_t> arr(r * N);
+    RNS_Polynomial tmp = polynomial_new_RNS_polynomial(N, 1, ntt);
+    for (uint64_t i = 0; i < r; i++) {
+        sample_gaussian_polys({tmp}, key_sigma, 1);
+        VH_CHECK(hipMemcpyAsync(arr.data() + i * N, VP(tmp)->d, N * 8, hipMemcpyDeviceToHost, S()));
+    }
+    VH_CHECK(hipStreamSynchronize(S()));
+    free_RNS_polynomial(tmp);
+    return mlwe_new_RNS_key_from_array(arr.data(), N, r, l, ntt, sigma);
+}
+
+// ---------------------------------------------------------------------------
 // Profiler API
 // ---------------------------------------------------------------------------
 extern "C" void vfhe_hip_profile_start(void)
 {
     VH_ENTER;
-    VH_CHECK(hipStreamSynchronize(C.stream));
+    sync_all_lanes();
     for (auto &r : g_prof_recs) { g_prof_free.push_back(r.a); g_prof_free.push_back(r.b); }
     g_prof_recs.clear();
     g_prof = true;
@@ -2010,7 +2560,7 @@ extern "C" void vfhe_hip_profile_start(void)
 extern "C" void vfhe_hip_profile_stop(double *ms, uint64_t *launches)
 {
     VH_ENTER;
-    VH_CHECK(hipStreamSynchronize(C.stream));
+    sync_all_lanes();
     for (int t = 0; t < VH_PROF_TAGS; t++) { ms[t] = 0; launches[t] = 0; }
     for (auto &r : g_prof_recs) {
         float e = 0;

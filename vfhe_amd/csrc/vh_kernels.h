@@ -70,7 +70,40 @@ hipError_t vh_launch_division(hipStream_t st, const vh_div_job *jobs, unsigned n
 hipError_t vh_launch_base_conversion(hipStream_t st, const vh_bconv_job *jobs, unsigned njobs, unsigned nout,
                                      const vh_limb *limbs, const void *consts, uint64_t n);
 hipError_t vh_launch_permute(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, uint64_t N);
+// Forward row pass (last 8 stages) of every digit fused with the inner product
+// out_c = -sum_d digit_d * key[d][c]; modup holds the column-pass output.
+// FP64 path, 13 <= logn <= 16, comps 1 or 2.
+hipError_t vh_launch_ntt16_row_mac(hipStream_t st, const vh_mac_job *jobs, unsigned njobs, unsigned next,
+                                   const vh_limb *limbs, const uint64_t *const *kptr, int logn, int comps);
+// fp != 0: every extended limb has q < 2^50 and every operand is canonical (FP64 products).
 hipError_t vh_launch_keyswitch_mac(hipStream_t st, const vh_mac_job *jobs, unsigned njobs, unsigned next,
-                                   const vh_limb *limbs, const uint64_t *const *kptr, uint64_t n);
+                                   const vh_limb *limbs, const uint64_t *const *kptr, uint64_t n, int fp);
+// On-device sampling: ChaCha20 (RFC 8439 block function, 64-bit counter and
+// 64-bit nonce words) keyed by the engine's 256-bit key; one nonce per call.
+struct vh_rng_key {
+    uint32_t k[8];
+    uint64_t nonce;
+};
+// Gaussian job: items[first .. first+count) are the rows of one polynomial that
+// receive the same noise vector; stream numbers the job inside the call.
+// mode 0: round() and reduce into each row's limb; mode 1: truncate, raw int64
+// into the single row (secret-key sampling).
+struct vh_gauss_job {
+    uint32_t first, count, stream, mode;
+};
+hipError_t vh_launch_sample_uniform(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
+                                    vh_rng_key key, uint64_t n);
+hipError_t vh_launch_sample_gaussian(hipStream_t st, const vh_gauss_job *jobs, unsigned njobs, const vh_item *items,
+                                     const vh_limb *limbs, vh_rng_key key, double sigma, uint64_t n);
+hipError_t vh_launch_chacha_raw(hipStream_t st, vh_rng_key key, uint64_t ctr0, uint64_t nblocks, uint32_t *out);
+// Complex FP64 FFT (CKKS encoder): rows of 2n doubles (real | imaginary).
+// dir 0 = CT_NR, 1 = GS_RN; fma selects the reference engine whose complex
+// product is reproduced (1 avx512ifma, 0 portable).
+hipError_t vh_launch_fft(hipStream_t st, double *const *rows, unsigned nrows, const double2 *ws, int logn, int dir,
+                         int fma);
+hipError_t vh_launch_fft_bitrev(hipStream_t st, double *const *rows, double *const *tmp, unsigned nrows, uint64_t n,
+                                uint32_t prec);
+hipError_t vh_launch_fft_scale(hipStream_t st, double *const *rows, unsigned nrows, double scale, uint64_t count);
+hipError_t vh_launch_fft_round(hipStream_t st, double *const *rows, uint64_t *const *outs, unsigned nrows, uint64_t n);
 hipError_t vh_launch_scatter_split(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                                    uint64_t N, int signed_in);

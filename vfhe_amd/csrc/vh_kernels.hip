@@ -167,7 +167,7 @@ __device__ __forceinline__ void ntt16_task(const vh_item &it, const vh_limb &L, 
     auto gidx = [&](int k) -> size_t { return COL ? sub + (size_t)k * nsub : sub * S + (size_t)k; };
     auto lidx = [&](int k) -> int { return COL ? k * (G + 1) + g : g * (S + S / 16) + k + (k >> 4); };
     auto load = [&](size_t j) -> double {
-        const uint64_t raw = COHERENT ? __scoped_atomic_load_n(in + j, __ATOMIC_RELAXED, __MEMORY_SCOPE_DEVICE) : in[j];
+        const uint64_t raw = COHERENT ? __scoped_atomic_load_n(in + j, __ATOMIC_RELAXED, __MEMORY_SCOPE_WRKGRP) : in[j];
         return FIRST ? vh_u2d(raw) : __longlong_as_double((long long)raw);
     };
     auto finish = [&](double x) -> uint64_t {
@@ -301,6 +301,123 @@ k_ntt16(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, in
     __shared__ double lds[vh_ntt16_lds<R, COL>()];
     const vh_item &it = items[blockIdx.y];
     ntt16_task<DIR, R, COL, FIRST, LAST, false>(it, limbs[it.limb], logn, s0, blockIdx.x, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Key-switch inner product fused into the forward row pass (FP64 path,
+// N = 2^13..2^16, at most 2 output components).  The column pass has left the
+// lifted digit rows, transformed over their first logn-8 stages, in jb.modup;
+// this kernel finishes the last 8 stages of every digit for one 4096-element
+// block and accumulates   out_c = - sum_d NTT(digit_d) * key[d][c]   (the
+// mul_subto of gadget.c:22-37) in registers, so the transformed digits never
+// return to memory.  Grid: x = ciphertext (fastest: the batch shares each key
+// block in L2), y = 4096-element block, z = extended limb.
+// ---------------------------------------------------------------------------
+template <int C>
+__global__ void __launch_bounds__(256)
+k_ntt16_row_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__ limbs,
+                const uint64_t *const *__restrict__ kptr, int logn)
+{
+    constexpr int R = 8, S = 256, TPS = 16, G = 16;
+    __shared__ double lds[vh_ntt16_lds<R, false>()];
+    const vh_mac_job &jb = jobs[blockIdx.x];
+    const int ti = blockIdx.z, D = jb.digits;
+    const int limb = jb.limb[ti];
+    const vh_limb &L = limbs[limb];
+    const double q = L.qd, qinv = L.qinv;
+    const size_t n = (size_t)1 << logn;
+    const int s0 = logn - R;
+    const int tid = threadIdx.x, g = tid / TPS, t = tid % TPS;
+    const size_t sub = (size_t)blockIdx.y * G + g;
+    const VH_GLOBAL double *tw = vh_global(reinterpret_cast<const double *>(L.tw_fwd));
+    auto lidx = [&](int k) -> int { return g * (S + S / 16) + k + (k >> 4); };
+    auto twid = [&](int sp, size_t local) -> double2 {
+        const size_t i = ((size_t)1 << (s0 + sp)) + (sub << sp) + local;
+        return make_double2(tw[2 * i], tw[2 * i + 1]);
+    };
+
+    double acc[C][16];
+#pragma unroll
+    for (int c = 0; c < C; c++)
+#pragma unroll
+        for (int m = 0; m < 16; m++) acc[c][m] = 0.0;
+
+    for (int d = 0; d < D; d++) {
+        const VH_GLOBAL uint64_t *in = vh_global(jb.modup + ((size_t)d * jb.next + ti) * n + sub * S);
+        double v[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = __longlong_as_double((long long)in[t + TPS * m]);
+        // stages s' = 0..3, stride ownership
+#pragma unroll
+        for (int sp = 0; sp < 4; sp++) {
+            const int hm = 8 >> sp;
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                if (m & hm) continue;
+                const double2 w = twid(sp, (size_t)(m >> (4 - sp)));
+                const double x = vh_mulmod_tw(v[m + hm], w.x, w.y, q);
+                const double u = v[m];
+                v[m] = u + x;
+                v[m + hm] = u - x;
+            }
+        }
+        __syncthreads();  // the previous digit's reads of lds are done
+#pragma unroll
+        for (int m = 0; m < 16; m++) lds[lidx(t + TPS * m)] = vh_center(v[m], q, qinv);
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = lds[lidx(16 * t + m)];
+        // stages s' = 4..7, contiguous ownership
+#pragma unroll
+        for (int sp = 4; sp < R; sp++) {
+            const int hm = 1 << (R - sp - 1);
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                if (m & hm) continue;
+                const double2 w = twid(sp, (size_t)((16 * t + m) >> (R - sp)));
+                const double x = vh_mulmod_tw(v[m + hm], w.x, w.y, q);
+                const double u = v[m];
+                v[m] = u + x;
+                v[m + hm] = u - x;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < 16; m++) lds[lidx(16 * t + m)] = vh_center(v[m], q, qinv);
+        __syncthreads();
+        // back to stride ownership: element t + 16 m of the row, coalesced key rows
+        const size_t x0 = sub * S + t;
+        const uint64_t *const *kp = kptr + jb.kptr_off + ((size_t)ti * D + d) * C;
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            const double a = lds[lidx(t + TPS * m)];   // |a| <= q/2 + 1
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                const double k = vh_u2d(vh_global(kp[c])[x0 + TPS * m]);
+                const double hi = a * k;
+                const double lo = __fma_rn(a, k, -hi);
+                const double qt = __fma_rn(a, k * qinv, VH_MAGIC) - VH_MAGIC;
+                const double r = __fma_rn(-qt, q, hi) + lo;
+                acc[c][m] = vh_center(acc[c][m] + r, q, qinv);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        VH_GLOBAL uint64_t *out = vh_global(jb.out[c] + (size_t)limb * n + sub * S + t);
+#pragma unroll
+        for (int m = 0; m < 16; m++) out[TPS * m] = vh_d2u(vh_canon(-acc[c][m], q, qinv));
+    }
+}
+
+hipError_t vh_launch_ntt16_row_mac(hipStream_t st, const vh_mac_job *jobs, unsigned njobs, unsigned next,
+                                   const vh_limb *limbs, const uint64_t *const *kptr, int logn, int comps)
+{
+    if (logn < 13 || logn > 16 || comps < 1 || comps > 2) return hipErrorInvalidValue;
+    const dim3 grid(njobs, 1u << (logn - 12), next);
+    if (comps == 1) hipLaunchKernelGGL(k_ntt16_row_mac<1>, grid, dim3(256), 0, st, jobs, limbs, kptr, logn);
+    else hipLaunchKernelGGL(k_ntt16_row_mac<2>, grid, dim3(256), 0, st, jobs, limbs, kptr, logn);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -598,6 +715,7 @@ k_permute(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, 
 // every non-NULL key at once.  One thread per (x, extended limb t); it reads the
 // lifted-and-transformed digit once and every component's key row.
 // ---------------------------------------------------------------------------
+template <bool FP>
 __global__ void __launch_bounds__(256)
 k_keyswitch_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__ limbs,
                 const uint64_t *const *__restrict__ kptr, uint64_t n)
@@ -610,23 +728,52 @@ k_keyswitch_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__
     const int ti = blockIdx.z;
     const int limb = jb.limb[ti];
     const vh_limb &L = limbs[limb];
-    const uint64_t q = L.q, mu = L.mu64;
     const int C = jb.comps, D = jb.digits;
-    uint64_t acc[VH_MAX_COMPS];
+    if constexpr (FP) {
+        // Canonical operands below 2^50: exact product (hi + lo), quotient
+        // rint(m * (k/q)) within 1 of m*k/q, so r = m*k - qt*q is exact with
+        // |r| <= q + 2^47; the running sum is re-centred after every term.
+        const double q = L.qd, qinv = L.qinv;
+        double acc[VH_MAX_COMPS];
 #pragma unroll
-    for (int c = 0; c < VH_MAX_COMPS; c++) acc[c] = 0;
-    for (int d = 0; d < D; d++) {
-        const uint64_t m = jb.modup[((uint64_t)d * jb.next + ti) * n + x];
-        const uint64_t *const *kp = kptr + jb.kptr_off + ((uint64_t)ti * D + d) * C;
+        for (int c = 0; c < VH_MAX_COMPS; c++) acc[c] = 0.0;
+        for (int d = 0; d < D; d++) {
+            const double m = vh_u2d(jb.modup[((uint64_t)d * jb.next + ti) * n + x]);
+            const uint64_t *const *kp = kptr + jb.kptr_off + ((uint64_t)ti * D + d) * C;
 #pragma unroll
-        for (int c = 0; c < VH_MAX_COMPS; c++)
-            if (c < C) acc[c] = vh_addmod(acc[c], vh_mulmod(vh_reduce64(kp[c][x], q, mu), m, L), q);
-    }
+            for (int c = 0; c < VH_MAX_COMPS; c++) {
+                if (c >= C) break;
+                const double k = vh_u2d(kp[c][x]);
+                const double hi = m * k;
+                const double lo = __fma_rn(m, k, -hi);
+                const double qt = __fma_rn(m, k * qinv, VH_MAGIC) - VH_MAGIC;
+                const double r = __fma_rn(-qt, q, hi) + lo;
+                acc[c] = vh_center(acc[c] + r, q, qinv);
+            }
+        }
 #pragma unroll
-    for (int c = 0; c < VH_MAX_COMPS; c++) {
-        if (c >= C) break;
-        const uint64_t a = acc[c];
-        jb.out[c][(uint64_t)limb * n + x] = a ? q - a : 0;
+        for (int c = 0; c < VH_MAX_COMPS; c++) {
+            if (c >= C) break;
+            jb.out[c][(uint64_t)limb * n + x] = vh_d2u(vh_canon(-acc[c], q, qinv));
+        }
+    } else {
+        const uint64_t q = L.q, mu = L.mu64;
+        uint64_t acc[VH_MAX_COMPS];
+#pragma unroll
+        for (int c = 0; c < VH_MAX_COMPS; c++) acc[c] = 0;
+        for (int d = 0; d < D; d++) {
+            const uint64_t m = jb.modup[((uint64_t)d * jb.next + ti) * n + x];
+            const uint64_t *const *kp = kptr + jb.kptr_off + ((uint64_t)ti * D + d) * C;
+#pragma unroll
+            for (int c = 0; c < VH_MAX_COMPS; c++)
+                if (c < C) acc[c] = vh_addmod(acc[c], vh_mulmod(vh_reduce64(kp[c][x], q, mu), m, L), q);
+        }
+#pragma unroll
+        for (int c = 0; c < VH_MAX_COMPS; c++) {
+            if (c >= C) break;
+            const uint64_t a = acc[c];
+            jb.out[c][(uint64_t)limb * n + x] = a ? q - a : 0;
+        }
     }
 }
 
@@ -723,10 +870,11 @@ hipError_t vh_launch_permute(hipStream_t st, const vh_item *items, unsigned nite
 }
 
 hipError_t vh_launch_keyswitch_mac(hipStream_t st, const vh_mac_job *jobs, unsigned njobs, unsigned next,
-                                   const vh_limb *limbs, const uint64_t *const *kptr, uint64_t n)
+                                   const vh_limb *limbs, const uint64_t *const *kptr, uint64_t n, int fp)
 {
-    hipLaunchKernelGGL(k_keyswitch_mac, dim3(njobs, (unsigned)((n + 255) / 256), next), dim3(256), 0, st, jobs,
-                       limbs, kptr, n);
+    const dim3 grid(njobs, (unsigned)((n + 255) / 256), next);
+    if (fp) hipLaunchKernelGGL(k_keyswitch_mac<true>, grid, dim3(256), 0, st, jobs, limbs, kptr, n);
+    else hipLaunchKernelGGL(k_keyswitch_mac<false>, grid, dim3(256), 0, st, jobs, limbs, kptr, n);
     return hipGetLastError();
 }
 
@@ -788,6 +936,128 @@ hipError_t vh_launch_ntt_xcd_verify(hipStream_t st, const unsigned *ctl, unsigne
 {
     hipLaunchKernelGGL(k_ntt_xcd_verify, dim3((nitems + 255) / 256), dim3(256), 0, st, ctl, nitems,
                        1u << (logn - 12), err);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// On-device sampling (rns_polynomial.c:242-271 semantics; the reference draws
+// from AES-CTR or BLAKE3, this engine from ChaCha20, so streams differ by
+// engine exactly as the reference's two engines differ from each other).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t vh_rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+#define VH_QR(a, b, c, d)                                  \
+    a += b; d ^= a; d = vh_rotl32(d, 16);                  \
+    c += d; b ^= c; b = vh_rotl32(b, 12);                  \
+    a += b; d ^= a; d = vh_rotl32(d, 8);                   \
+    c += d; b ^= c; b = vh_rotl32(b, 7);
+
+// ChaCha20 block: 16 output words for (key, counter, nonce).
+__device__ __forceinline__ void vh_chacha20(const vh_rng_key &key, uint64_t ctr, uint32_t o[16])
+{
+    uint32_t x[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u,
+                      key.k[0], key.k[1], key.k[2], key.k[3], key.k[4], key.k[5], key.k[6], key.k[7],
+                      (uint32_t)ctr, (uint32_t)(ctr >> 32), (uint32_t)key.nonce, (uint32_t)(key.nonce >> 32)};
+#pragma unroll
+    for (int i = 0; i < 16; i++) o[i] = x[i];
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        VH_QR(o[0], o[4], o[8], o[12]) VH_QR(o[1], o[5], o[9], o[13])
+        VH_QR(o[2], o[6], o[10], o[14]) VH_QR(o[3], o[7], o[11], o[15])
+        VH_QR(o[0], o[5], o[10], o[15]) VH_QR(o[1], o[6], o[11], o[12])
+        VH_QR(o[2], o[7], o[8], o[13]) VH_QR(o[3], o[4], o[9], o[14])
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) o[i] += x[i];
+}
+#undef VH_QR
+
+// Uniform residues: r * q / 2^64 of a uniform 64-bit r (the reference's
+// mod-switch from 2^64, rns_polynomial.c:250).  Row k of the call uses
+// counters [k * n/8, (k+1) * n/8); item.aux holds k.
+__global__ void __launch_bounds__(256)
+k_sample_uniform(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, vh_rng_key key, uint64_t n)
+{
+    const vh_item it = items[blockIdx.y];
+    const uint64_t q = limbs[it.limb].q, nb = n / 8;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t o[16];
+        vh_chacha20(key, (uint64_t)it.aux * nb + b, o);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint64_t r = (uint64_t)o[2 * k] | ((uint64_t)o[2 * k + 1] << 32);
+            it.dst[8 * b + k] = __umul64hi(r, q);
+        }
+    }
+}
+
+// Rounded Box-Muller normal samples (misc_tp.c:26-31 semantics), one noise
+// vector per job written to every row of the job.
+__global__ void __launch_bounds__(256)
+k_sample_gaussian(const vh_gauss_job *__restrict__ jobs, const vh_item *__restrict__ items,
+                  const vh_limb *__restrict__ limbs, vh_rng_key key, double sigma, uint64_t n)
+{
+    const vh_gauss_job jb = jobs[blockIdx.y];
+    const uint64_t nb = n / 4;
+    const double two_pi = 6.283185307179586476925286766559;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t o[16];
+        vh_chacha20(key, (uint64_t)jb.stream * nb + b, o);
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const uint64_t r0 = (uint64_t)o[4 * s] | ((uint64_t)o[4 * s + 1] << 32);
+            const uint64_t r1 = (uint64_t)o[4 * s + 2] | ((uint64_t)o[4 * s + 3] << 32);
+            const double u0 = ((double)(r0 >> 11) + 0.5) * 0x1p-53, u1 = ((double)(r1 >> 11) + 0.5) * 0x1p-53;
+            const double z = cos(two_pi * u0) * sqrt(-2.0 * log(u1)) * sigma;
+            const int64_t v = jb.mode ? (int64_t)z : (int64_t)round(z);
+            const uint64_t j = 4 * b + s;
+            if (jb.mode) {
+                items[jb.first].dst[j] = (uint64_t)v;
+                continue;
+            }
+            const uint64_t mag = v < 0 ? (uint64_t)0 - (uint64_t)v : (uint64_t)v;
+            for (uint32_t c = 0; c < jb.count; c++) {
+                const vh_item &it = items[jb.first + c];
+                const uint64_t q = limbs[it.limb].q;
+                const uint64_t a = mag < q ? mag : mag % q;
+                it.dst[j] = (v < 0 && a) ? q - a : a;
+            }
+        }
+    }
+}
+
+// Raw keystream blocks (diagnostics: the RFC 8439 known answer).
+__global__ void __launch_bounds__(256)
+k_chacha_raw(vh_rng_key key, uint64_t ctr0, uint64_t nblocks, uint32_t *__restrict__ out)
+{
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nblocks; b += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t o[16];
+        vh_chacha20(key, ctr0 + b, o);
+#pragma unroll
+        for (int i = 0; i < 16; i++) out[16 * b + i] = o[i];
+    }
+}
+
+hipError_t vh_launch_chacha_raw(hipStream_t st, vh_rng_key key, uint64_t ctr0, uint64_t nblocks, uint32_t *out)
+{
+    hipLaunchKernelGGL(k_chacha_raw, dim3(vh_blocks(nblocks, 256, 64)), dim3(256), 0, st, key, ctr0, nblocks, out);
+    return hipGetLastError();
+}
+
+hipError_t vh_launch_sample_uniform(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
+                                    vh_rng_key key, uint64_t n)
+{
+    if (n % 8) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_sample_uniform, dim3(vh_blocks(n / 8, 256, 64), nitems), dim3(256), 0, st, items, limbs, key, n);
+    return hipGetLastError();
+}
+
+hipError_t vh_launch_sample_gaussian(hipStream_t st, const vh_gauss_job *jobs, unsigned njobs, const vh_item *items,
+                                     const vh_limb *limbs, vh_rng_key key, double sigma, uint64_t n)
+{
+    if (n % 4) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_sample_gaussian, dim3(vh_blocks(n / 4, 256, 64), njobs), dim3(256), 0, st, jobs, items, limbs,
+                       key, sigma, n);
     return hipGetLastError();
 }
 
