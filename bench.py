@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--serving-seconds", type=float, default=1.5, help="wall time per offered load")
     ap.add_argument("--only-ntt", action="store_true",
                     help="run unit A alone (the command profiled with rocprofv3)")
+    ap.add_argument("--only-ckks", action="store_true", help="run the CKKS leg alone (profiling)")
+    ap.add_argument("--only-serving", action="store_true", help="run the serving leg alone (profiling)")
+    ap.add_argument("--only-fused", action="store_true", help="run unit B alone (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also copy the 64-polynomial batch once per step (known-size copy for counter scaling)")
     return ap.parse_args()
@@ -145,6 +148,12 @@ def main():
     from vfhe_amd import _native, params, runtime as rt
 
     lib = _native.load()
+    if args.only_ckks:
+        print(json.dumps(bench_ckks(lib, _native, params, rt, args)), flush=True)
+        return
+    if args.only_serving:
+        print(json.dumps(bench_serving(lib, args)), flush=True)
+        return
     primes = params.gen_primes(N, [50] * L)
     log("device", lib.vfhe_hip_device_name().decode(), "primes", primes[0], "...", primes[-1])
     sched = {k: int(lib.vfhe_hip_get_option(k.encode())) for k in
@@ -157,6 +166,16 @@ def main():
         polys.append(pool.poly(rt.random_rows(rng, primes, N)))
     arr = _native.voidp_array(polys)
     ref0 = pool.read(polys[0])
+
+    if args.only_fused:
+        ys = [pool.poly(rt.random_rows(rng, primes, N)) for _ in range(POLYS)]
+        zs = [pool.poly() for _ in range(POLYS)]
+        ax, ay, az = _native.voidp_array(polys), _native.voidp_array(ys), _native.voidp_array(zs)
+        with Prof(lib) as fprof:
+            fused_dt = timed(lib, lambda: lib.vfhe_hip_ntt_mul_intt_batch(az, ax, ay, POLYS), args.steps, args.warmup)
+        print(json.dumps({"unit_b_ms_per_batch": fused_dt * 1e3, "products_per_s": POLYS / fused_dt,
+                          "ntt_ms": fprof.ms[0], "launches": fprof.launches[0]}), flush=True)
+        return
 
     def round_trip():
         lib.vfhe_hip_ntt_batch(arr, POLYS, 0)
@@ -347,8 +366,9 @@ def bench_serving(lib, args):
     log(f"serving: setup {time.perf_counter() - t0:.1f} s")
     budget, lanes, max_batch = 32 << 30, 4, 32
     burst = serving.make_workload(40000.0, 0.2, Ns, seed=1)
-    cap = serving.run(lib, rings, burst, budget, lanes, max_batch)["requests_per_s"]
-    log(f"serving: saturated throughput {cap:.0f} requests/s ({len(burst)} requests)")
+    sat = serving.run(lib, rings, burst, budget, lanes, max_batch)
+    cap = sat["requests_per_s"]
+    log(f"serving: saturated throughput {cap:.0f} requests/s ({len(burst)} requests)", sat)
     loads = []
     for k, frac in enumerate((0.25, 0.5, 0.75, 0.95)):
         wl = serving.make_workload(frac * cap, args.serving_seconds, Ns, seed=10 + k)
@@ -359,7 +379,7 @@ def bench_serving(lib, args):
     lib.vfhe_hip_synchronize()
     for ring in rings.values():
         ring.free()
-    return {"unit": "requests/s", "saturated_requests_per_s": cap, "loads": loads, "lanes": lanes,
+    return {"unit": "requests/s", "saturated_requests_per_s": cap, "saturated": sat, "loads": loads, "lanes": lanes,
             "max_batch": max_batch, "memory_budget_bytes": budget,
             "workload": "Poisson arrivals; 50% multiply+relinearize+rescale, 50% rotation by 5^(2^k), k<4; "
                         "N uniform in {2^13..2^16}, level uniform in [4,24] limbs + 4 special limbs; "

@@ -268,6 +268,27 @@ void complex_polys_ifft_scale_round_to_RNS_batch(void **rows_in, void **outs_rns
                                                  uint32_t log_prec, void *gs_ws, double temp_delta);
 
 /* ------------------------------------------------------------------------ */
+/* Monomial products and MGSW (CGGI building blocks)                          */
+/* ------------------------------------------------------------------------ */
+
+/* modules/arith/c/include/arith.h:240-241 (rns_polynomial.c:1048-1168):
+   out = X^a * in and out = (X^a - 1) * in mod X^N + 1, coefficient form, out != in. */
+void polynomial_RNSc_mul_by_xai(RNSc_Polynomial out, RNSc_Polynomial in, uint64_t a);
+void polynomial_RNSc_mul_by_xai_minus1(RNSc_Polynomial out, RNSc_Polynomial in, uint64_t a);
+/* modules/mlwe/c/include/mlwe.h:96-97 (mlwe.c:493-509): the same on every component. */
+void mlwe_RNSc_mul_by_xai(RNSc_MLWE out, RNSc_MLWE in, uint64_t a);
+void mlwe_RNSc_mul_by_xai_minus1(RNSc_MLWE out, RNSc_MLWE in, uint64_t a);
+/* modules/mlwe/c/include/mlwe.h:143-148 (mgsw.c:6-88): external product of an
+   MGSW sample (ell gadget rows per input component, NTT form) with a
+   coefficient-form MLWE sample, output in NTT form after the round division by
+   the top `special_primes` limbs; CMUX = ExtProd(in2 - in1) + in1; NCMUX =
+   CMUX(in1, automorphism_{2N-1}(in2)). */
+void mgsw_external_product(RNS_MLWE out, RNS_MLWE *mgsw, RNSc_MLWE in, uint64_t ell, uint64_t special_primes);
+void mgsw_CMUX(RNS_MLWE out, RNSc_MLWE in1, RNSc_MLWE in2, RNS_MLWE *mgsw, uint64_t ell, uint64_t special_primes);
+void mgsw_NCMUX(RNS_MLWE out, RNSc_MLWE in1, RNSc_MLWE in2, RNS_MLWE *mgsw, RNS_MLWE **ksk, uint64_t ell,
+                uint64_t special_primes);
+
+/* ------------------------------------------------------------------------ */
 /* Engine extensions (no counterpart in the reference)                       */
 /* ------------------------------------------------------------------------ */
 
@@ -329,13 +350,27 @@ void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polyn
 void vfhe_hip_mlwe_multiply_batch(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLWE *in2, uint64_t count,
                                   RNS_MLWE **rlk, uint64_t divide_mask);
 
+/* mgsw_CMUX for `count` independent (out, in1, in2, mgsw) quadruples in one
+   launch sequence (a batch of blind rotations stepping through one key index). */
+void vfhe_hip_mgsw_cmux_batch(RNS_MLWE *out, RNSc_MLWE *in1, RNSc_MLWE *in2, RNS_MLWE **mgsw, uint64_t count,
+                              uint64_t ell, uint64_t special_primes);
 /* mlwe_automorphism_RNSc_GHS (mlwe.c:314-326) for `count` inputs sharing gen and ksk. */
 void vfhe_hip_mlwe_automorphism_batch(RNS_MLWE *out, RNS_MLWE *in, uint64_t count, uint64_t gen, RNS_MLWE **ksk);
+/* The same two batches with a Galois element / key (and rescale mask) per job:
+   jobs may differ in level and key, and share the ring and the rank. */
+void vfhe_hip_mlwe_multiply_batch_multi(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLWE *in2, uint64_t count,
+                                        RNS_MLWE ***rlks, const uint64_t *divide_masks);
+void vfhe_hip_mlwe_automorphism_batch_multi(RNS_MLWE *out, RNS_MLWE *in, uint64_t count, const uint64_t *gens,
+                                            RNS_MLWE ***ksks);
 /* Views of `count` samples restricted to `mask`: new handles sharing the device
    rows and host mirror of in[k] (no copy).  A key stored once for the top level
    serves every lower level through a view with mask = level limbs | special
    limbs.  Free with free_RNS_mlwe_array before the viewed samples. */
 RNS_MLWE *vfhe_hip_mlwe_view_array(RNS_MLWE *in, uint64_t count, uint64_t mask);
+/* mlwe_alloc_RNS_sample without clearing the device rows (the reference's own
+   allocation leaves them uninitialised too): for outputs that are written
+   before they are read. */
+RNS_MLWE vfhe_hip_mlwe_alloc_RNS_sample_uninit(uint64_t N, uint64_t r, uint64_t mask, incNTT ntt);
 /* The calling thread's current lane stream (a hipStream_t). */
 void *vfhe_hip_current_stream(void);
 
@@ -347,8 +382,9 @@ void *vfhe_hip_current_stream(void);
 /* A server with a worker thread.  Requests are admitted while the device bytes
    they hold in flight (output, temporaries, ModUp buffer) fit memory_budget
    (the oldest request is admitted alone when nothing is in flight); requests
-   of one class (op, ring, masks, key, gen, divide_mask) are batched, up to
-   max_batch per launch, and batches run on `lanes` (1..7) engine lanes. */
+   of one class (op, ring, rank; levels, keys, gens and rescale masks may
+   differ) are batched, up to max_batch per launch, and batches run on `lanes`
+   (1..7) engine lanes. */
 void *vfhe_hip_server_new(uint64_t memory_budget, uint32_t lanes, uint32_t max_batch);
 /* Thread-safe; returns the request id (>= 1), or 0 for an invalid request.
    Inputs and keys must stay alive and unchanged until the request completes. */
@@ -358,8 +394,10 @@ uint64_t vfhe_hip_server_submit(void *srv, int op, RNS_MLWE in1, RNS_MLWE in2, R
    latency from submit to completion in seconds.  wait != 0 blocks until one
    completes or none is pending.  Returns the number written. */
 uint64_t vfhe_hip_server_poll(void *srv, uint64_t *ids, RNS_MLWE *outs, double *latency_s, uint64_t max, int wait);
-/* stats[6]: batches, requests launched, mean batch size, peak in-flight bytes,
-   most lanes busy at once, requests pending. */
+/* stats[9]: batches, requests launched, mean batch size, peak in-flight bytes,
+   most lanes busy at once, requests pending, seconds the worker spent
+   enqueueing batches, summed device seconds of the batches (each from its
+   lane's events), seconds since the server started. */
 void vfhe_hip_server_stats(void *srv, double *stats);
 /* Drains every submitted request, then stops the worker. */
 void vfhe_hip_server_free(void *srv);

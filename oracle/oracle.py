@@ -81,6 +81,9 @@ def lib():
             "o_multiply": (U64, [VP, U64P, U64P, U64P, U64, U64, ctypes.POINTER(U64P), U64]),
             "o_automorphism": (U64, [VP, U64P, U64P, U64, U64, U64, ctypes.POINTER(U64P), U64]),
             "o_mlwe_round_division": (U64, [VP, U64P, U64, U64, U64]),
+            "o_mul_by_xai": (U64, [VP, U64P, U64P, U64, U64, ctypes.c_int]),
+            "o_external_product": (U64, [VP, U64P, U64, U64P, U64, U64P, U64, U64, U64]),
+            "o_cmux": (U64, [VP, U64P, U64, U64P, U64P, U64, U64P, U64, U64, U64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -195,6 +198,17 @@ class Ctx:
 
     def automorphism(self, out, inp, r, mask, gen, ksk, ext_mask):
         return int(lib().o_automorphism(self.h, ptr(out), ptr(inp), r, mask, gen, ksk_array(ksk), ext_mask))
+
+    def mul_by_xai(self, out, inp, m, a, minus1=False):
+        return int(lib().o_mul_by_xai(self.h, ptr(out), ptr(inp), m, a, int(minus1)))
+
+    def external_product(self, out, r, inp, in_mask, mgsw, ell, special_primes, ext_mask):
+        return int(lib().o_external_product(self.h, ptr(out), r, ptr(inp), in_mask, ptr(mgsw), ell, special_primes,
+                                            ext_mask))
+
+    def cmux(self, out, r, in1, in2, mask, mgsw, ell, special_primes, ext_mask):
+        return int(lib().o_cmux(self.h, ptr(out), r, ptr(in1), ptr(in2), mask, ptr(mgsw), ell, special_primes,
+                                ext_mask))
 
     def mlwe_round_division(self, ct, r, mask, divide_mask):
         return int(lib().o_mlwe_round_division(self.h, ptr(ct), r, mask, divide_mask))

@@ -619,3 +619,87 @@ OAPI uint64_t o_mlwe_round_division(const o_ctx *c, uint64_t *ct, uint64_t r, ui
     for (uint64_t k = 0; k <= r; k++) m = o_division(c, POLY(ct, k), mask, divide_mask, 1);
     return m;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Monomial products (rns_polynomial.c:1048-1168) and MGSW (mgsw.c:6-88).      */
+/* ------------------------------------------------------------------------- */
+
+/* polynomial_RNSc_mul_by_xai / _minus1: out = X^a in (minus1: (X^a - 1) in) mod X^N + 1,
+   coefficient form, a taken mod 2N; a == 0 copies (minus1: zeroes) the rows. */
+OAPI uint64_t o_mul_by_xai(const o_ctx *c, uint64_t *out, const uint64_t *in, uint64_t m, uint64_t a, int minus1)
+{
+    const uint64_t N = c->N;
+    a &= 2 * N - 1;
+    for (uint64_t j = 0; j < c->l; j++) {
+        if (!ACTIVE(m, j)) continue;
+        const uint64_t q = c->q[j];
+        const uint64_t *x = ROW(in, j);
+        uint64_t *y = ROW(out, j);
+        for (uint64_t i = 0; i < N; i++) {
+            /* coefficient i of X^a * x: sum over e = k + a with e = i or e = i + N (mod 2N) */
+            uint64_t v;
+            if (a < N) v = i >= a ? x[i - a] % q : (q - x[i + N - a] % q) % q;
+            else {
+                const uint64_t b = a - N; /* X^a = -X^b */
+                v = i >= b ? (q - x[i - b] % q) % q : x[i + N - b] % q;
+            }
+            y[i] = minus1 ? submod(v, x[i] % q, q) : v;
+        }
+    }
+    return m;
+}
+
+/* mgsw_external_product (mgsw.c:6-49), gadget_mul_addto_polynomial (gadget.c:5-20).
+   in: r+1 coefficient-form polys on in_mask; mgsw: (r+1)*ell entries of [r+1][l][N]
+   (NTT form, extended mask ext_mask); out: r+1 polys, NTT form.  Returns out's mask. */
+OAPI uint64_t o_external_product(const o_ctx *c, uint64_t *out, uint64_t r, const uint64_t *in, uint64_t in_mask,
+                                 const uint64_t *mgsw, uint64_t ell, uint64_t special_primes, uint64_t ext_mask)
+{
+    const uint64_t PL = c->l * c->N;
+    memset(out, 0, (r + 1) * PL * sizeof(uint64_t));
+    uint64_t *tmp = malloc(PL * sizeof(uint64_t));
+    for (uint64_t comp = 0; comp <= r; comp++) {
+        uint64_t kidx = 0;
+        for (uint64_t j = 0; j < c->l; j++) {
+            if (!ACTIVE(in_mask, j)) continue;
+            o_mod_reduce_lifted(c, tmp, ext_mask, POLY(in, comp), j);
+            o_to_ntt(c, tmp, tmp, ext_mask);
+            const uint64_t *entry = mgsw + (comp * ell + kidx) * (r + 1) * PL;
+            for (uint64_t oc = 0; oc <= r; oc++) o_mul(c, POLY(out, oc), POLY(entry, oc), ext_mask, tmp, ext_mask, 1);
+            kidx++;
+        }
+    }
+    free(tmp);
+    uint64_t m = ext_mask;
+    if (special_primes > 0) {
+        uint64_t dm = 0, cnt = 0;
+        for (int idx = 63; idx >= 0 && cnt < special_primes; idx--)
+            if (ACTIVE(m, (uint64_t)idx)) { dm |= 1ULL << idx; cnt++; }
+        uint64_t mo = m;
+        for (uint64_t oc = 0; oc <= r; oc++) {
+            o_to_coeff(c, POLY(out, oc), POLY(out, oc), m);
+            if (dm) mo = o_division(c, POLY(out, oc), m, dm, 1);
+            o_to_ntt(c, POLY(out, oc), POLY(out, oc), mo);
+        }
+        m = mo;
+    }
+    return m;
+}
+
+/* mgsw_CMUX (mgsw.c:51-71): out = ExtProd(mgsw, in2 - in1) + NTT(in1); in1, in2 coefficient form. */
+OAPI uint64_t o_cmux(const o_ctx *c, uint64_t *out, uint64_t r, const uint64_t *in1, const uint64_t *in2, uint64_t mask,
+                     const uint64_t *mgsw, uint64_t ell, uint64_t special_primes, uint64_t ext_mask)
+{
+    const uint64_t PL = c->l * c->N;
+    uint64_t *diff = calloc((r + 1) * PL, sizeof(uint64_t)), *a = calloc((r + 1) * PL, sizeof(uint64_t));
+    for (uint64_t k = 0; k <= r; k++) {
+        o_addsub(c, POLY(diff, k), POLY(in2, k), mask, POLY(in1, k), mask, 1);
+        o_to_ntt(c, POLY(a, k), POLY(in1, k), mask);
+    }
+    uint64_t m = o_external_product(c, out, r, diff, mask, mgsw, ell, special_primes, ext_mask);
+    uint64_t mo = m;
+    for (uint64_t k = 0; k <= r; k++) mo = o_addsub(c, POLY(out, k), POLY(out, k), m, POLY(a, k), mask, 0);
+    free(diff);
+    free(a);
+    return mo;
+}

@@ -107,14 +107,17 @@ def test_uniform_distribution(lib, N):
     primes = O.gen_primes(N, [50] * 4)
     pool = rt.Pool(N, primes)
     p = pool.poly()
+    lib.vfhe_prng_set_deterministic_seed(N)
     lib.polynomial_gen_random_RNSc_polynomial(p)
+    lib.vfhe_prng_clear_deterministic_seed()
     rows = pool.read(p)
     for q, row in zip(primes, rows):
         x = row.astype(np.float64) / q
         assert (row < np.uint64(q)).all()
         assert abs(x.mean() - 0.5) < 6 * math.sqrt(1 / 12 / N)
         hist = np.bincount((x * 16).astype(int), minlength=16)
-        assert hist.min() > N / 16 * 0.8
+        chi2 = float(((hist - N / 16) ** 2 / (N / 16)).sum())
+        assert chi2 < 60.0, chi2        # 15 degrees of freedom: P(chi2 > 60) < 1e-6
     pool.free(p)
 
 

@@ -589,3 +589,86 @@ def test_automorphism_decrypts_with_galois_key():
     diff = np.zeros_like(exp)
     c.addsub(diff, ph, lvl, exp, lvl, True)
     assert _centered_max(c, primes, diff, lvl) < 2**16
+
+
+# --- monomial products and MGSW ----------------------------------------------------------------
+
+def test_mul_by_xai_closed_form():
+    """rns_polynomial.c:1048-1168: X^a * x and (X^a - 1) * x in Z_q[X]/(X^N + 1), every a mod 2N."""
+    N = 16
+    primes = O.gen_primes(N, [50, 40])
+    c = O.Ctx(N, primes)
+    rng = np.random.default_rng(21)
+    x = np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in primes])
+    for a in (0, 1, 5, 8, N - 1, N, N + 3, 2 * N - 1, 2 * N + 2):
+        for minus1 in (False, True):
+            out = np.zeros_like(x)
+            c.mul_by_xai(out, x, 0b11, a, minus1)
+            for i, q in enumerate(primes):
+                exp = [0] * N
+                for k in range(N):
+                    e = (k + a) % (2 * N)
+                    if e < N:
+                        exp[e] += int(x[i][k])
+                    else:
+                        exp[e - N] -= int(x[i][k])
+                if minus1:
+                    exp = [v - int(x[i][k]) for k, v in enumerate(exp)]
+                assert [int(v) for v in out[i]] == [v % q for v in exp], (a, minus1)
+
+
+def _gen_mgsw(c, primes, rng, s_ntt, bit, lvl):
+    """MGSW of `bit` under s (rank 1, ell = level limbs, mgsw.c layout): entry (j, k) is a sample
+    whose phase is bit * P * [limb k] * (-s for j = 0 (the a component), 1 for j = 1 (b))."""
+    P = math.prod(primes[KS_LVL:])
+    m = c.mask_all()
+    neg_s = np.zeros_like(s_ntt)
+    c.negate(neg_s, s_ntt, m)
+    one = np.stack([np.ones(KS_N, dtype=np.uint64) for _ in primes])   # NTT of the constant 1
+    ents = []
+    for factor in (neg_s, one):
+        for k in range(KS_LVL):
+            a = np.stack([rng.integers(0, q, KS_N, dtype=np.uint64) for q in primes])
+            e, _ = _small(c, primes, rng, 3)
+            b = np.zeros_like(a)
+            c.mul(b, a, m, s_ntt, m, 0)
+            c.addsub(b, b, m, _ntt(c, e), m)
+            if bit:
+                t = np.zeros_like(a)
+                t[k] = factor[k]
+                c.scale(t, t, 1 << k, P % primes[k])
+                c.addsub(b, b, m, t, m)
+            ents.append(np.stack([a, b]))
+    return np.ascontiguousarray(np.stack(ents))
+
+
+def test_cmux_selects_with_valid_mgsw():
+    """mgsw.c:51-71 restated: phase_s(CMUX(c0, c1, MGSW(bit))) == phase_s(c_bit) up to noise."""
+    c, primes, lvl, ext, rng = _ks_setup(14)
+    s, _ = _small(c, primes, rng, 1)
+    s_ntt = _ntt(c, s)
+    cts, msgs = [], []
+    for _ in range(2):
+        m, _ = _small(c, primes, rng, 1000)
+        a = np.stack([rng.integers(0, q, KS_N, dtype=np.uint64) if i < KS_LVL else np.zeros(KS_N, np.uint64)
+                      for i, q in enumerate(primes)])
+        b = np.zeros_like(a)
+        c.mul(b, a, lvl, s_ntt, lvl, 0)
+        c.addsub(b, b, lvl, _ntt(c, m), lvl)
+        ct = np.stack([np.zeros_like(a), np.zeros_like(a)])
+        c.to_coeff(ct[0], a, lvl)
+        c.to_coeff(ct[1], b, lvl)
+        cts.append(ct)
+        msgs.append(m)
+    for bit in (0, 1):
+        mg = _gen_mgsw(c, primes, rng, s_ntt, bit, lvl)
+        out = np.zeros((2, len(primes), KS_N), dtype=np.uint64)
+        mo = c.cmux(out, 1, cts[0], cts[1], lvl, mg, KS_LVL, KS_SPECIAL, ext)
+        assert mo == lvl
+        ph = _phase_coeff(c, out, s_ntt, lvl)
+        diff = np.zeros_like(ph)
+        c.addsub(diff, ph, lvl, msgs[bit], lvl, True)
+        assert _centered_max(c, primes, diff, lvl) < 2**8, bit
+        # planted mistake: the other message is far away
+        c.addsub(diff, ph, lvl, msgs[1 - bit], lvl, True)
+        assert _centered_max(c, primes, diff, lvl) > 2**8

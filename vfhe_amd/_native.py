@@ -183,8 +183,19 @@ _SIGNATURES = {
     "mlwe_RNSc_sample": (None, [VP, VP, VP]),
     "vfhe_hip_chacha20_blocks": (None, [VP, U64, U64, U64, VP]),
     "vfhe_hip_mlwe_automorphism_batch": (None, [VPP, VPP, U64, U64, VPP]),
+    "vfhe_hip_mlwe_multiply_batch_multi": (None, [VPP, VPP, VPP, U64, VPP, U64P]),
+    "polynomial_RNSc_mul_by_xai": (None, [VP, VP, U64]),
+    "polynomial_RNSc_mul_by_xai_minus1": (None, [VP, VP, U64]),
+    "mlwe_RNSc_mul_by_xai": (None, [VP, VP, U64]),
+    "mlwe_RNSc_mul_by_xai_minus1": (None, [VP, VP, U64]),
+    "mgsw_external_product": (None, [VP, VPP, VP, U64, U64]),
+    "mgsw_CMUX": (None, [VP, VP, VP, VPP, U64, U64]),
+    "mgsw_NCMUX": (None, [VP, VP, VP, VPP, VPP, U64, U64]),
+    "vfhe_hip_mgsw_cmux_batch": (None, [VPP, VPP, VPP, VPP, U64, U64, U64]),
+    "vfhe_hip_mlwe_automorphism_batch_multi": (None, [VPP, VPP, U64, U64P, VPP]),
     "vfhe_hip_mlwe_view_array": (VPP, [VPP, U64, U64]),
     "vfhe_hip_current_stream": (VP, []),
+    "vfhe_hip_mlwe_alloc_RNS_sample_uninit": (VP, [U64, U64, U64, VP]),
     "vfhe_hip_server_new": (VP, [U64, ctypes.c_uint32, ctypes.c_uint32]),
     "vfhe_hip_server_submit": (U64, [VP, I32, VP, VP, VPP, U64, U64]),
     "vfhe_hip_server_poll": (U64, [VP, U64P, VPP, ctypes.POINTER(ctypes.c_double), U64, I32]),

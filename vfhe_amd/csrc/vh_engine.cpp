@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <sys/random.h>
 #include <vector>
@@ -57,6 +58,8 @@ struct vh_pool {
     std::vector<vh_limb> h_limbs;    // host copy
     std::vector<uint64_t *> d_w;     // device w rows (split_degree > 1 only)
     uint64_t n, logn;                // transform size N/split and its log
+    // division constants per (active mask, divide-prime chunk, rounding), built once
+    std::map<std::tuple<uint64_t, uint64_t, int>, void *> div_consts;
 };
 
 struct vh_poly {
@@ -81,6 +84,7 @@ static inline vh_pool *VPOOL(incNTT t) { return reinterpret_cast<vh_pool *>(t); 
 struct vh_lane {
     hipStream_t stream = nullptr;
     char *h_ring = nullptr, *d_ring = nullptr;
+    char *hd_ring = nullptr;   // device address of the pinned host ring (small arrays are read in place)
     size_t ring_size = 0, ring_off = 0;
     int cur_half = 0;
     hipEvent_t half_ev[2];
@@ -106,7 +110,8 @@ static void lane_init(vh_lane &L, size_t ring)
 {
     VH_CHECK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
     L.ring_size = ring;
-    VH_CHECK(hipHostMalloc((void **)&L.h_ring, L.ring_size, hipHostMallocDefault));
+    VH_CHECK(hipHostMalloc((void **)&L.h_ring, L.ring_size, hipHostMallocMapped));
+    VH_CHECK(hipHostGetDevicePointer((void **)&L.hd_ring, L.h_ring, 0));
     VH_CHECK(hipMalloc((void **)&L.d_ring, L.ring_size));
     VH_CHECK(hipEventCreateWithFlags(&L.half_ev[0], hipEventDisableTiming));
     VH_CHECK(hipEventCreateWithFlags(&L.half_ev[1], hipEventDisableTiming));
@@ -179,6 +184,7 @@ static void vh_ensure()
 #define VH_ENTER std::lock_guard<std::recursive_mutex> vh_lock_(vh_mu); vh_ensure()
 
 // Copy a small host array to device memory, ordered on the stream.
+static const size_t VH_STAGE_INPLACE = 65536;
 static void *vh_stage(const void *src, size_t bytes)
 {
     vh_lane &L = LN();
@@ -194,8 +200,11 @@ static void *vh_stage(const void *src, size_t bytes)
     }
     const size_t off = L.ring_off;
     memcpy(L.h_ring + off, src, bytes);
-    VH_CHECK(hipMemcpyAsync(L.d_ring + off, L.h_ring + off, bytes, hipMemcpyHostToDevice, L.stream));
     L.ring_off += need;
+    // Small work lists are read by the kernels straight from pinned host memory
+    // (one read per workgroup); larger ones are copied to HBM first.
+    if (bytes <= VH_STAGE_INPLACE) return L.hd_ring + off;
+    VH_CHECK(hipMemcpyAsync(L.d_ring + off, L.h_ring + off, bytes, hipMemcpyHostToDevice, L.stream));
     return L.d_ring + off;
 }
 
@@ -678,6 +687,12 @@ static void pool_add_primes(vh_pool *P, const uint64_t *primes, uint64_t count)
 {
     incNTT t = &P->pub;
     const uint64_t n = P->n, old_l = t->l, new_l = old_l + count;
+    if (!P->div_consts.empty()) {
+        // cached division constants are laid out for the old limb count
+        sync_all_lanes();
+        for (auto &kv : P->div_consts) VH_CHECK(hipFree(kv.second));
+        P->div_consts.clear();
+    }
     t->ntt = (NTT_proc *)realloc(t->ntt, sizeof(NTT_proc) * new_l);
     t->w = (uint64_t **)realloc(t->w, sizeof(uint64_t *) * new_l);
     if (!t->ntt || !t->w) exit(EXIT_FAILURE);
@@ -754,9 +769,10 @@ static void check_rows(RNS_Polynomial p, uint64_t mask)
     VH_REQUIRE(p->allocated_l >= 64 || (mask >> p->allocated_l) == 0);
 }
 
-extern "C" RNS_Polynomial polynomial_new_RNS_polynomial(uint64_t N, uint64_t rns_mask, incNTT ntt)
+// zero = false leaves the device rows uninitialised, as the reference's allocation
+// does (rns_polynomial.c:65-78); the public entry point clears them.
+static RNS_Polynomial new_poly(uint64_t N, uint64_t rns_mask, incNTT ntt, bool zero)
 {
-    VH_ENTER;
     VH_REQUIRE(N == ntt->N);
     vh_poly *v = new vh_poly();
     const uint64_t l = ntt->l;
@@ -769,9 +785,15 @@ extern "C" RNS_Polynomial polynomial_new_RNS_polynomial(uint64_t N, uint64_t rns
     if (!v->pub.coeffs || !v->hblock) exit(EXIT_FAILURE);
     for (uint64_t i = 0; i < l; i++) v->pub.coeffs[i] = v->hblock + i * N;
     v->d = (uint64_t *)vh_dmalloc(l * N * 8);
-    VH_CHECK(hipMemsetAsync(v->d, 0, l * N * 8, S()));
+    if (zero) VH_CHECK(hipMemsetAsync(v->d, 0, l * N * 8, S()));
     v->host_dirty = 0;
     return &v->pub;
+}
+
+extern "C" RNS_Polynomial polynomial_new_RNS_polynomial(uint64_t N, uint64_t rns_mask, incNTT ntt)
+{
+    VH_ENTER;
+    return new_poly(N, rns_mask, ntt, true);
 }
 
 extern "C" RNS_Polynomial *polynomial_new_RNS_polynomial_array(uint64_t size, uint64_t N, uint64_t rns_mask, incNTT ntt)
@@ -1293,17 +1315,22 @@ static void run_division(const std::vector<RNS_Polynomial> &polys, uint64_t divi
             for (int d : divs) chunk |= 1ULL << d;
             std::vector<int> outs;
             for (uint64_t i = 0; i < l; i++) if (ACTIVE(mask, i) && !ACTIVE(chunk, i)) outs.push_back((int)i);
-            std::vector<vh_divc_h> consts(divs.size() * l);
-            for (size_t k = 0; k < divs.size(); k++) {
-                const uint64_t p = t->ntt[divs[k]]->q, half = round ? p / 2 : 0;
-                for (uint64_t i = 0; i < l; i++) {
-                    if (!ACTIVE(mask, i)) continue;
-                    const uint64_t q = t->ntt[i]->q;
-                    vh_divc_h &c = consts[k * l + i];
-                    c.h = half % q;
-                    c.inv = (i == (uint64_t)divs[k]) ? 0 : inverse_mod(p, q);
-                    c.inv_sh = shoup_of(c.inv, q);
+            void *&d_consts = VPOOL(t)->div_consts[std::make_tuple(mask, chunk, round ? 1 : 0)];
+            if (!d_consts) {
+                std::vector<vh_divc_h> consts(divs.size() * l);
+                for (size_t k = 0; k < divs.size(); k++) {
+                    const uint64_t p = t->ntt[divs[k]]->q, half = round ? p / 2 : 0;
+                    for (uint64_t i = 0; i < l; i++) {
+                        if (!ACTIVE(mask, i)) continue;
+                        const uint64_t q = t->ntt[i]->q;
+                        vh_divc_h &c = consts[k * l + i];
+                        c.h = half % q;
+                        c.inv = (i == (uint64_t)divs[k]) ? 0 : inverse_mod(p, q);
+                        c.inv_sh = shoup_of(c.inv, q);
+                    }
                 }
+                VH_CHECK(hipMalloc(&d_consts, consts.size() * sizeof(vh_divc_h)));
+                VH_CHECK(hipMemcpy(d_consts, consts.data(), consts.size() * sizeof(vh_divc_h), hipMemcpyHostToDevice));
             }
             std::vector<vh_div_job> jobs;
             for (auto p : g.second) {
@@ -1318,7 +1345,6 @@ static void run_division(const std::vector<RNS_Polynomial> &polys, uint64_t divi
                 jb.nout = (int)outs.size();
                 jobs.push_back(jb);
             }
-            const void *d_consts = vh_stage(consts.data(), consts.size() * sizeof(vh_divc_h));
             if (!outs.empty()) {
                 for (size_t off = 0; off < jobs.size(); off += VH_MAX_GRID_Y) {
                     const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, jobs.size() - off);
@@ -1530,6 +1556,26 @@ extern "C" void polynomial_RNSc_permute(RNSc_Polynomial out, RNSc_Polynomial in,
 // ---------------------------------------------------------------------------
 // MLWE samples (mlwe.c:98-312)
 // ---------------------------------------------------------------------------
+// An MLWE sample whose rows are left uninitialised: engine temporaries and the
+// serving runtime's outputs, which are written before they are read.
+static RNS_MLWE alloc_sample_raw(uint64_t N, uint64_t r, uint64_t mask, incNTT ntt)
+{
+    RNS_MLWE c = (RNS_MLWE)malloc(sizeof(*c));
+    if (!c) exit(EXIT_FAILURE);
+    c->a = (RNS_Polynomial *)malloc(sizeof(RNS_Polynomial) * (r ? r : 1));
+    if (!c->a) exit(EXIT_FAILURE);
+    for (uint64_t i = 0; i < r; i++) c->a[i] = new_poly(N, mask, ntt, false);
+    c->b = new_poly(N, mask, ntt, false);
+    c->r = r;
+    return c;
+}
+
+extern "C" RNS_MLWE vfhe_hip_mlwe_alloc_RNS_sample_uninit(uint64_t N, uint64_t r, uint64_t mask, incNTT ntt)
+{
+    VH_ENTER;
+    return alloc_sample_raw(N, r, mask, ntt);
+}
+
 extern "C" RNS_MLWE mlwe_alloc_RNS_sample(uint64_t N, uint64_t r, uint64_t mask, incNTT ntt)
 {
     RNS_MLWE c = (RNS_MLWE)malloc(sizeof(*c));
@@ -1729,69 +1775,91 @@ extern "C" void mlwe_tensor_product(RNS_Polynomial *out, RNS_MLWE in1, RNS_MLWE 
 // output is inverse-transformed, divided by the special primes in ascending
 // order with rounding, and the pass-through components are added.
 // ---------------------------------------------------------------------------
-static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &ins, RNS_MLWE **ksk)
+// Gadget products (gadget.c:5-37) for a batch of jobs: job i sets
+//   outs[i] = (add ? +1 : -1) * sum over its sources (poly, keys) of
+//             sum over the active limbs j of poly: NTT(lift_j(poly)) * keys[k++]
+// in NTT form on the extended mask of its keys.  Jobs may differ in level and
+// key and share the ring and the output rank.  The key-switch and the MGSW
+// external product are this plus their own ModDown.
+struct vh_gsrc {
+    RNS_Polynomial poly;
+    RNS_MLWE *keys;
+};
+
+static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<std::vector<vh_gsrc>> &srcs, bool add)
 {
     const size_t B = outs.size();
-    VH_REQUIRE(B == ins.size() && B > 0);
-    const uint64_t Rin = ins[0]->r, r = outs[0]->r;
-    RNS_MLWE first = nullptr;
-    for (uint64_t i = 0; i < Rin && !first; i++) if (ksk[i]) first = ksk[i][0];
-    VH_REQUIRE(first != nullptr);
-    const uint64_t ext = first->b->rns_mask;
-    incNTT t = first->b->ntt;
-    vh_pool *P = VPOOL(t);
-    const uint64_t N = t->N, l = t->l, Cc = r + 1;
+    VH_REQUIRE(B == srcs.size() && B > 0);
+    const uint64_t r = outs[0]->r, Cc = r + 1;
     VH_REQUIRE(Cc <= VH_MAX_COMPS);
-    std::vector<int> E;
-    for (uint64_t i = 0; i < l; i++) if (ACTIVE(ext, i)) E.push_back((int)i);
-    const uint64_t next = E.size();
-    const uint64_t in_mask = ins[0]->b->rns_mask;
-
-    // Digits: (component i, limb j, key entry k)
-    struct digit { uint64_t comp, limb, entry; };
-    std::vector<digit> D;
-    for (uint64_t i = 0; i < Rin; i++) {
-        if (!ksk[i]) continue;
-        uint64_t k = 0;
-        for (uint64_t j = 0; j < l; j++) if (ACTIVE(ins[0]->a[i]->rns_mask, j)) D.push_back({i, j, k++});
-    }
-    const uint64_t ND = D.size();
-
+    incNTT t = nullptr;
+    struct digit { uint64_t src, limb, entry; };
+    struct job {
+        uint64_t ext, next, nd, kptr_off, modup_off;
+        std::vector<int> E;
+        std::vector<digit> D;
+    };
+    std::vector<job> J(B);
+    std::vector<const uint64_t *> kptr;
+    uint64_t modup_rows = 0;
     for (size_t bi = 0; bi < B; bi++) {
-        VH_REQUIRE(outs[bi] != ins[bi] && ins[bi]->r == Rin && outs[bi]->r == r);
-        for (uint64_t i = 0; i < Rin; i++) VH_REQUIRE(ins[bi]->a[i]->rns_mask == ins[0]->a[i]->rns_mask);
-        for (auto p : comps(ins[bi])) dev_ready(p);
-        for (auto p : comps(outs[bi])) { dev_ready(p); p->rns_mask = ext; check_rows(p, ext); }
-    }
-    // Key rows: kptr[(ti * ND + d) * Cc + c]
-    std::vector<const uint64_t *> kptr(next * ND * Cc);
-    for (uint64_t d = 0; d < ND; d++) {
-        RNS_MLWE e = ksk[D[d].comp][D[d].entry];
-        VH_REQUIRE(e->r == r);
-        for (uint64_t c = 0; c < Cc; c++) {
-            RNS_Polynomial kp = c < r ? e->a[c] : e->b;
-            dev_ready(kp);
-            for (uint64_t ti = 0; ti < next; ti++) kptr[(ti * ND + d) * Cc + c] = drow(kp, E[ti]);
+        VH_REQUIRE(!srcs[bi].empty() && outs[bi]->r == r);
+        job &jb = J[bi];
+        RNS_MLWE first = srcs[bi][0].keys[0];
+        jb.ext = first->b->rns_mask;
+        if (!t) t = first->b->ntt;
+        VH_REQUIRE(first->b->ntt == t);
+        for (uint64_t i = 0; i < t->l; i++) if (ACTIVE(jb.ext, i)) jb.E.push_back((int)i);
+        jb.next = jb.E.size();
+        for (size_t si = 0; si < srcs[bi].size(); si++) {
+            RNS_Polynomial poly = srcs[bi][si].poly;
+            VH_REQUIRE(poly->ntt == t);
+            dev_ready(poly);
+            uint64_t k = 0;
+            for (uint64_t j = 0; j < t->l; j++) if (ACTIVE(poly->rns_mask, j)) jb.D.push_back({si, j, k++});
         }
-    }
-
-    if (ND > 0) {
-        // ModUp: lift every digit to every extended limb, then forward NTT.
-        uint64_t *modup = (uint64_t *)vh_dmalloc(B * ND * next * N * 8);
-        std::vector<vh_item> lift;
-        lift.reserve(B * ND * next);
-        for (size_t bi = 0; bi < B; bi++)
-            for (uint64_t d = 0; d < ND; d++) {
-                const uint64_t *src = drow(ins[bi]->a[D[d].comp], D[d].limb);
-                for (uint64_t ti = 0; ti < next; ti++)
-                    lift.push_back(mk_item(src, modup + ((bi * ND + d) * next + ti) * N, (uint32_t)E[ti]));
+        jb.nd = jb.D.size();
+        for (auto p : comps(outs[bi])) { dev_ready(p); p->rns_mask = jb.ext; check_rows(p, jb.ext); }
+        // key rows: kptr[kptr_off + (ti * nd + d) * Cc + c]
+        jb.kptr_off = kptr.size();
+        kptr.resize(kptr.size() + jb.next * jb.nd * Cc);
+        for (uint64_t d = 0; d < jb.nd; d++) {
+            RNS_MLWE e = srcs[bi][jb.D[d].src].keys[jb.D[d].entry];
+            VH_REQUIRE(e->r == r && e->b->ntt == t);
+            for (uint64_t c = 0; c < Cc; c++) {
+                RNS_Polynomial kp = c < r ? e->a[c] : e->b;
+                dev_ready(kp);
+                for (uint64_t ti = 0; ti < jb.next; ti++)
+                    kptr[jb.kptr_off + (ti * jb.nd + d) * Cc + c] = drow(kp, jb.E[ti]);
             }
+        }
+        jb.modup_off = modup_rows;
+        modup_rows += jb.nd * jb.next;
+    }
+    vh_pool *P = VPOOL(t);
+    const uint64_t N = t->N, l = t->l;
+
+    if (modup_rows > 0) {
+        // ModUp: lift every digit to every extended limb, then forward NTT.
+        uint64_t *modup = (uint64_t *)vh_dmalloc(modup_rows * N * 8);
+        std::vector<vh_item> lift;
+        lift.reserve(modup_rows);
+        bool fuse = true;   // FP64 path everywhere: the lift folds into the NTT's first pass
+        uint64_t maxnext = 0;
+        for (size_t bi = 0; bi < B; bi++) {
+            const job &jb = J[bi];
+            maxnext = std::max(maxnext, jb.next);
+            for (int e : jb.E) fuse = fuse && P->h_limbs[e].fp64;
+            for (auto &dg : jb.D) fuse = fuse && P->h_limbs[dg.limb].fp64;
+            for (uint64_t d = 0; d < jb.nd; d++) {
+                const uint64_t *src = drow(srcs[bi][jb.D[d].src].poly, jb.D[d].limb);
+                for (uint64_t ti = 0; ti < jb.next; ti++)
+                    lift.push_back(mk_item(src, modup + (jb.modup_off + d * jb.next + ti) * N, (uint32_t)jb.E[ti]));
+            }
+        }
         // On the FP64 path (every prime < 2^50) the forward NTT mod q_t accepts any
-        // input below 2^50, so the digit row is transformed straight from the input
-        // (lift fused into the first pass); otherwise it is reduced first.
-        bool fuse = true;
-        for (int e : E) fuse = fuse && P->h_limbs[e].fp64;
-        for (auto &dg : D) fuse = fuse && P->h_limbs[dg.limb].fp64;
+        // input below 2^50, so the digit row is transformed straight from the input;
+        // otherwise it is reduced first.
         if (!fuse) run_eltwise(lift, P->d_limbs, VH_OP_REDUCE, N);
         std::vector<vh_item> nt;
         nt.reserve(lift.size() * t->split_degree);
@@ -1814,24 +1882,26 @@ static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector
             run_ntt(nt, P->d_limbs, P->h_limbs.data(), P->n, 0);
         }
 
+        const uint64_t *const *d_kptr = vh_stage_vec(kptr);
         if (t->split_degree == 1) {
             // FP64 accumulation needs canonical key rows: engine-made keys are; it is
             // used only when every extended limb is on the FP64 path.
             bool fp_mac = true;
-            for (int e : E) fp_mac = fp_mac && P->h_limbs[e].fp64;
-            const uint64_t *const *d_kptr = vh_stage_vec(kptr);
+            for (auto &jb : J) for (int e : jb.E) fp_mac = fp_mac && P->h_limbs[e].fp64;
             std::vector<vh_mac_job> jobs(B);
             for (size_t bi = 0; bi < B; bi++) {
+                const job &src = J[bi];
                 vh_mac_job &jb = jobs[bi];
                 memset(&jb, 0, sizeof(jb));
                 auto oc = comps(outs[bi]);
                 for (uint64_t c = 0; c < Cc; c++) jb.out[c] = VP(oc[c])->d;
-                jb.modup = modup + bi * ND * next * N;
-                jb.kptr_off = 0;
+                jb.modup = modup + src.modup_off * N;
+                jb.kptr_off = src.kptr_off;
                 jb.comps = (int)Cc;
-                jb.digits = (int)ND;
-                jb.next = (int)next;
-                for (uint64_t ti = 0; ti < next; ti++) jb.limb[ti] = E[ti];
+                jb.add = add ? 1 : 0;
+                jb.digits = (int)src.nd;
+                jb.next = (int)src.next;
+                for (uint64_t ti = 0; ti < src.next; ti++) jb.limb[ti] = src.E[ti];
             }
             for (size_t off = 0; off < B; off += VH_MAX_GRID_Y) {
                 const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, B - off);
@@ -1839,10 +1909,10 @@ static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector
                 {
                     vh_prof_scope ps_(VH_PROF_KSMAC);
                     if (fused_mac)
-                        VH_CHECK(vh_launch_ntt16_row_mac(S(), vh_stage_vec(chunk), cnt, (unsigned)next, P->d_limbs,
+                        VH_CHECK(vh_launch_ntt16_row_mac(S(), vh_stage_vec(chunk), cnt, (unsigned)maxnext, P->d_limbs,
                                                          d_kptr, logn, (int)Cc));
                     else
-                        VH_CHECK(vh_launch_keyswitch_mac(S(), vh_stage_vec(chunk), cnt, (unsigned)next, P->d_limbs,
+                        VH_CHECK(vh_launch_keyswitch_mac(S(), vh_stage_vec(chunk), cnt, (unsigned)maxnext, P->d_limbs,
                                                          d_kptr, N, fp_mac ? 1 : 0));
                 }
             }
@@ -1854,29 +1924,25 @@ static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector
                 for (auto p : comps(outs[bi]))
                     for (uint64_t i = 0; i < std::min(l, p->allocated_l); i++) z.push_back(mk_item(nullptr, drow(p, i), (uint32_t)i));
             run_eltwise(z, P->d_limbs, VH_OP_ZERO, N);
-            std::vector<vh_item> items;
             for (size_t bi = 0; bi < B; bi++) {
+                const job &jb = J[bi];
                 auto oc = comps(outs[bi]);
-                for (uint64_t d = 0; d < ND; d++)
-                    for (uint64_t c = 0; c < Cc; c++)
-                        for (uint64_t ti = 0; ti < next; ti++) {
-                            vh_item it = mk_item(kptr[(ti * ND + d) * Cc + c], drow(oc[c], E[ti]), (uint32_t)E[ti],
-                                                 modup + ((bi * ND + d) * next + ti) * N);
-                            it.aux = (uint32_t)t->split_degree;
-                            it.scalar = (uint64_t)(uintptr_t)P->d_w[E[ti]];
-                            items.push_back(it);
-                        }
                 // One digit's rows may not be accumulated concurrently into one output row:
                 // launch per digit so the read-modify-write of out is ordered.
-                const size_t per = Cc * next;
-                for (uint64_t d = 0; d < ND; d++) {
-                    std::vector<vh_item> chunk(items.begin() + d * per, items.begin() + (d + 1) * per);
-                    {
-                        vh_prof_scope ps_(VH_PROF_ELTWISE);
-                        VH_CHECK(vh_launch_split_mul(S(), vh_stage_vec(chunk), (unsigned)chunk.size(), P->d_limbs, 2, P->n, t->split_degree));
-                    }
+                for (uint64_t d = 0; d < jb.nd; d++) {
+                    std::vector<vh_item> chunk;
+                    for (uint64_t c = 0; c < Cc; c++)
+                        for (uint64_t ti = 0; ti < jb.next; ti++) {
+                            vh_item it = mk_item(kptr[jb.kptr_off + (ti * jb.nd + d) * Cc + c], drow(oc[c], jb.E[ti]),
+                                                 (uint32_t)jb.E[ti], modup + (jb.modup_off + d * jb.next + ti) * N);
+                            it.aux = (uint32_t)t->split_degree;
+                            it.scalar = (uint64_t)(uintptr_t)P->d_w[jb.E[ti]];
+                            chunk.push_back(it);
+                        }
+                    vh_prof_scope ps_(VH_PROF_ELTWISE);
+                    VH_CHECK(vh_launch_split_mul(S(), vh_stage_vec(chunk), (unsigned)chunk.size(), P->d_limbs,
+                                                 add ? 1 : 2, P->n, t->split_degree));
                 }
-                items.clear();
             }
         }
         vh_dfree(modup);
@@ -1888,21 +1954,46 @@ static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector
         run_eltwise(z, P->d_limbs, VH_OP_ZERO, N);
     }
 
-    // Back to coefficient form, then ModDown by the special primes.
+}
+
+// mlwe_RNSc_GHS_hybrid_keyswitch (mlwe.c:557-598) for a batch whose jobs may
+// differ in level (masks) and key: job i switches ins[i] with ksks[i].  All jobs
+// share the ring and the output rank.
+static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &ins,
+                            const std::vector<RNS_MLWE **> &ksks)
+{
+    const size_t B = outs.size();
+    VH_REQUIRE(B == ins.size() && B == ksks.size() && B > 0);
+    std::vector<std::vector<vh_gsrc>> srcs(B);
+    for (size_t bi = 0; bi < B; bi++) {
+        VH_REQUIRE(ksks[bi] != nullptr && outs[bi] != ins[bi] && outs[bi]->r == outs[0]->r);
+        for (uint64_t i = 0; i < ins[bi]->r; i++)
+            if (ksks[bi][i]) srcs[bi].push_back({ins[bi]->a[i], ksks[bi][i]});
+        VH_REQUIRE(!srcs[bi].empty());
+        for (auto p : comps(ins[bi])) dev_ready(p);
+    }
+    gadget_batch(outs, srcs, false);
+    const uint64_t l = outs[0]->b->ntt->l, N = outs[0]->b->ntt->N;
+    vh_pool *P = VPOOL(outs[0]->b->ntt);
+
+    // Back to coefficient form, then ModDown by the special primes of each job.
     std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pairs;
-    std::vector<RNS_Polynomial> all;
+    std::map<uint64_t, std::vector<RNS_Polynomial>> by_divide;
     for (size_t bi = 0; bi < B; bi++)
-        for (auto p : comps(outs[bi])) { pairs.push_back({p, p}); all.push_back(p); }
+        for (auto p : comps(outs[bi])) {
+            pairs.push_back({p, p});
+            by_divide[p->rns_mask & ~ins[bi]->b->rns_mask].push_back(p);
+        }
     ntt_rows(pairs, 1);
-    const uint64_t divide_mask = ext & ~in_mask;
-    if (divide_mask) run_division(all, divide_mask, true);
+    for (auto &g : by_divide)
+        if (g.first) run_division(g.second, g.first, true);
 
     // Pass-through components keep their key: added after the rescale (mlwe.c:588-597).
     std::vector<vh_item> adds;
     for (size_t bi = 0; bi < B; bi++) {
         uint64_t keep = 0;
         std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pt;
-        for (uint64_t i = 0; i < Rin; i++) if (!ksk[i]) pt.push_back({outs[bi]->a[keep++], ins[bi]->a[i]});
+        for (uint64_t i = 0; i < ins[bi]->r; i++) if (!ksks[bi][i]) pt.push_back({outs[bi]->a[keep++], ins[bi]->a[i]});
         pt.push_back({outs[bi]->b, ins[bi]->b});
         for (auto &pr : pt) {
             RNS_Polynomial o = pr.first, x = pr.second;
@@ -1918,36 +2009,36 @@ extern "C" void mlwe_RNSc_GHS_hybrid_keyswitch(RNSc_MLWE out, RNSc_MLWE in, RNS_
 {
     (void)lvl;
     VH_ENTER;
-    keyswitch_batch({out}, {in}, ksk);
+    keyswitch_batch({out}, {in}, {ksk});
 }
 
 extern "C" void mlwe_automorphism_RNSc_GHS(RNSc_MLWE out, RNSc_MLWE in, uint64_t gen, RNS_MLWE **ksk, uint64_t lvl)
 {
     (void)lvl;
     VH_ENTER;
-    RNSc_MLWE tmp = mlwe_alloc_RNS_sample(out->a[0]->ntt->N, out->r, out->a[0]->rns_mask, out->a[0]->ntt);
+    RNSc_MLWE tmp = alloc_sample_raw(out->a[0]->ntt->N, out->r, out->a[0]->rns_mask, out->a[0]->ntt);
     for (uint64_t i = 0; i < out->r; i++) permute_into(tmp->a[i], in->a[i], gen);
     permute_into(tmp->b, in->b, gen);
-    keyswitch_batch({out}, {tmp}, ksk);
+    keyswitch_batch({out}, {tmp}, {ksk});
     free_mlwe_RNS_sample(tmp);
 }
 
-// mlwe_automorphism_RNSc_GHS (mlwe.c:314-326) for many inputs sharing gen and ksk:
-// permute every component, then one batched key-switch.
-static void automorphism_batch(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &ins, uint64_t gen,
-                               RNS_MLWE **ksk)
+// mlwe_automorphism_RNSc_GHS (mlwe.c:314-326) for many inputs, each with its own
+// Galois element and key: permute every component, then one batched key-switch.
+static void automorphism_batch(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &ins,
+                               const std::vector<uint64_t> &gens, const std::vector<RNS_MLWE **> &ksks)
 {
     const size_t B = outs.size();
-    VH_REQUIRE(B > 0 && ins.size() == B);
+    VH_REQUIRE(B > 0 && ins.size() == B && gens.size() == B && ksks.size() == B);
     std::vector<RNS_MLWE> tmp(B);
     for (size_t bi = 0; bi < B; bi++) {
         RNS_MLWE o = outs[bi], x = ins[bi];
         VH_REQUIRE(o != x && o->r == x->r);
-        tmp[bi] = mlwe_alloc_RNS_sample(o->a[0]->ntt->N, o->r, o->a[0]->rns_mask, o->a[0]->ntt);
-        for (uint64_t i = 0; i < o->r; i++) permute_into(tmp[bi]->a[i], x->a[i], gen);
-        permute_into(tmp[bi]->b, x->b, gen);
+        tmp[bi] = alloc_sample_raw(o->a[0]->ntt->N, o->r, o->a[0]->rns_mask, o->a[0]->ntt);
+        for (uint64_t i = 0; i < o->r; i++) permute_into(tmp[bi]->a[i], x->a[i], gens[bi]);
+        permute_into(tmp[bi]->b, x->b, gens[bi]);
     }
-    keyswitch_batch(outs, tmp, ksk);
+    keyswitch_batch(outs, tmp, ksks);
     for (auto t : tmp) free_mlwe_RNS_sample(t);
 }
 
@@ -1956,7 +2047,17 @@ extern "C" void vfhe_hip_mlwe_automorphism_batch(RNS_MLWE *out, RNS_MLWE *in, ui
 {
     VH_ENTER;
     if (count == 0) return;
-    automorphism_batch(std::vector<RNS_MLWE>(out, out + count), std::vector<RNS_MLWE>(in, in + count), gen, ksk);
+    automorphism_batch(std::vector<RNS_MLWE>(out, out + count), std::vector<RNS_MLWE>(in, in + count),
+                       std::vector<uint64_t>(count, gen), std::vector<RNS_MLWE **>(count, ksk));
+}
+
+extern "C" void vfhe_hip_mlwe_automorphism_batch_multi(RNS_MLWE *out, RNS_MLWE *in, uint64_t count,
+                                                       const uint64_t *gens, RNS_MLWE ***ksks)
+{
+    VH_ENTER;
+    if (count == 0) return;
+    automorphism_batch(std::vector<RNS_MLWE>(out, out + count), std::vector<RNS_MLWE>(in, in + count),
+                       std::vector<uint64_t>(gens, gens + count), std::vector<RNS_MLWE **>(ksks, ksks + count));
 }
 
 // A view of p restricted to `mask`: shares p's device rows and host mirror.
@@ -2008,16 +2109,16 @@ extern "C" void *vfhe_hip_current_stream(void)
 // (INTT(NTT(x)) == x exactly, so the pair the reference runs between the two
 // calls is skipped without changing a bit).
 static void multiply_batch(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &in1,
-                           const std::vector<RNS_MLWE> &in2, RNS_MLWE **ksk, uint64_t divide_mask)
+                           const std::vector<RNS_MLWE> &in2, const std::vector<RNS_MLWE **> &ksks,
+                           const std::vector<uint64_t> &divide_masks)
 {
     const size_t B = outs.size();
-    VH_REQUIRE(B > 0 && in1.size() == B && in2.size() == B);
+    VH_REQUIRE(B > 0 && in1.size() == B && in2.size() == B && ksks.size() == B && divide_masks.size() == B);
     const uint64_t N = in1[0]->b->ntt->N, r = in1[0]->r, R = mlwe_extended_rank(r);
     incNTT t = in1[0]->b->ntt;
-    if (!ksk) {
-        VH_REQUIRE(divide_mask == 0);
+    if (!ksks[0]) {
         for (size_t bi = 0; bi < B; bi++) {
-            VH_REQUIRE(outs[bi]->r == R);
+            VH_REQUIRE(!ksks[bi] && divide_masks[bi] == 0 && outs[bi]->r == R);
             tensor_into(comps(outs[bi]), in1[bi], in2[bi]);
         }
         return;
@@ -2025,29 +2126,29 @@ static void multiply_batch(const std::vector<RNS_MLWE> &outs, const std::vector<
     std::vector<RNS_MLWE> ext(B);
     std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pairs;
     for (size_t bi = 0; bi < B; bi++) {
-        VH_REQUIRE(outs[bi]->r == r);
-        ext[bi] = mlwe_alloc_RNS_sample(N, R, in1[bi]->b->rns_mask, t);
+        VH_REQUIRE(ksks[bi] && outs[bi]->r == r && in1[bi]->r == r && in1[bi]->b->ntt == t);
+        ext[bi] = alloc_sample_raw(N, R, in1[bi]->b->rns_mask, t);
         tensor_into(comps(ext[bi]), in1[bi], in2[bi]);
         for (auto p : comps(ext[bi])) pairs.push_back({p, p});
     }
     ntt_rows(pairs, 1);
-    keyswitch_batch(outs, ext, ksk);
-    if (divide_mask == 0) {
-        std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> o;
-        for (size_t bi = 0; bi < B; bi++) for (auto p : comps(outs[bi])) o.push_back({p, p});
-        ntt_rows(o, 0);
-    } else {
-        std::vector<RNS_Polynomial> all;
-        for (size_t bi = 0; bi < B; bi++) for (auto p : comps(outs[bi])) all.push_back(p);
-        run_division(all, divide_mask, true);
-    }
+    keyswitch_batch(outs, ext, ksks);
+    std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> o;
+    std::map<uint64_t, std::vector<RNS_Polynomial>> by_divide;
+    for (size_t bi = 0; bi < B; bi++)
+        for (auto p : comps(outs[bi])) {
+            if (divide_masks[bi] == 0) o.push_back({p, p});
+            else by_divide[divide_masks[bi]].push_back(p);
+        }
+    ntt_rows(o, 0);
+    for (auto &g : by_divide) run_division(g.second, g.first, true);
     for (auto e : ext) free_mlwe_RNS_sample(e);
 }
 
 extern "C" void mlwe_multiply(RNS_MLWE out, RNS_MLWE in1, RNS_MLWE in2, RNS_MLWE **ksk)
 {
     VH_ENTER;
-    multiply_batch({out}, {in1}, {in2}, ksk, 0);
+    multiply_batch({out}, {in1}, {in2}, {ksk}, {0});
 }
 
 extern "C" void vfhe_hip_mlwe_multiply_batch(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLWE *in2, uint64_t count,
@@ -2056,7 +2157,167 @@ extern "C" void vfhe_hip_mlwe_multiply_batch(RNS_MLWE *out, RNS_MLWE *in1, RNS_M
     VH_ENTER;
     if (count == 0) return;
     multiply_batch(std::vector<RNS_MLWE>(out, out + count), std::vector<RNS_MLWE>(in1, in1 + count),
-                   std::vector<RNS_MLWE>(in2, in2 + count), rlk, divide_mask);
+                   std::vector<RNS_MLWE>(in2, in2 + count), std::vector<RNS_MLWE **>(count, rlk),
+                   std::vector<uint64_t>(count, divide_mask));
+}
+
+extern "C" void vfhe_hip_mlwe_multiply_batch_multi(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLWE *in2, uint64_t count,
+                                                   RNS_MLWE ***rlks, const uint64_t *divide_masks)
+{
+    VH_ENTER;
+    if (count == 0) return;
+    multiply_batch(std::vector<RNS_MLWE>(out, out + count), std::vector<RNS_MLWE>(in1, in1 + count),
+                   std::vector<RNS_MLWE>(in2, in2 + count), std::vector<RNS_MLWE **>(rlks, rlks + count),
+                   std::vector<uint64_t>(divide_masks, divide_masks + count));
+}
+
+// ---------------------------------------------------------------------------
+// MGSW (mlwe/c/src/mgsw.c) and monomial products (rns_polynomial.c:1048-1168,
+// mlwe.c:493-509)
+// ---------------------------------------------------------------------------
+static void mul_xai_rows(RNS_Polynomial out, RNS_Polynomial in, uint64_t a, bool minus1)
+{
+    VH_REQUIRE(out != in && out->ntt->split_degree == 1 && out->ntt == in->ntt);
+    const uint64_t N = out->ntt->N;
+    a &= (N << 1) - 1;
+    dev_ready(in); dev_ready(out);
+    if (a == 0 && !minus1) { copy_rows(out, in, in->rns_mask); out->rns_mask = in->rns_mask; return; }
+    out->rns_mask = in->rns_mask;
+    std::vector<vh_item> items;
+    if (a == 0) {   // (X^0 - 1) * in = 0 on every row (rns_polynomial.c:1114-1121)
+        for (uint64_t j = 0; j < std::min(out->ntt->l, out->allocated_l); j++)
+            items.push_back(mk_item(nullptr, drow(out, j), (uint32_t)j));
+        run_eltwise(items, DL(out), VH_OP_ZERO, N);
+        return;
+    }
+    for_active(out->rns_mask, out->ntt->l, [&](uint64_t j) {
+        items.push_back(mk_item(drow(in, j), drow(out, j), (uint32_t)j, nullptr, 0, 0, minus1 ? 1u : 0u));
+    });
+    if (items.empty()) return;
+    vh_prof_scope ps_(VH_PROF_OTHER);
+    VH_CHECK(vh_launch_mul_xai(S(), vh_stage_vec(items), (unsigned)items.size(), DL(out), N, a));
+}
+
+extern "C" void polynomial_RNSc_mul_by_xai(RNSc_Polynomial out, RNSc_Polynomial in, uint64_t a)
+{
+    VH_ENTER;
+    mul_xai_rows(out, in, a, false);
+}
+
+extern "C" void polynomial_RNSc_mul_by_xai_minus1(RNSc_Polynomial out, RNSc_Polynomial in, uint64_t a)
+{
+    VH_ENTER;
+    mul_xai_rows(out, in, a, true);
+}
+
+extern "C" void mlwe_RNSc_mul_by_xai(RNSc_MLWE out, RNSc_MLWE in, uint64_t a)
+{
+    VH_ENTER;
+    for (uint64_t i = 0; i < out->r; i++) mul_xai_rows(out->a[i], in->a[i], a, false);
+    mul_xai_rows(out->b, in->b, a, false);
+}
+
+extern "C" void mlwe_RNSc_mul_by_xai_minus1(RNSc_MLWE out, RNSc_MLWE in, uint64_t a)
+{
+    VH_ENTER;
+    for (uint64_t i = 0; i < out->r; i++) mul_xai_rows(out->a[i], in->a[i], a, true);
+    mul_xai_rows(out->b, in->b, a, true);
+}
+
+// mgsw.c:6-49: out = sum_j G^-1(in_j) * mgsw[j*ell ..] (NTT form, extended mask),
+// then, with special primes, round division by the top `special_primes` active
+// limbs (INTT, divide, NTT).
+static void external_product_batch(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &ins,
+                                   const std::vector<RNS_MLWE *> &mgsws, uint64_t ell, uint64_t special_primes)
+{
+    const size_t B = outs.size();
+    VH_REQUIRE(B > 0 && ins.size() == B && mgsws.size() == B);
+    std::vector<std::vector<vh_gsrc>> srcs(B);
+    for (size_t bi = 0; bi < B; bi++) {
+        const uint64_t r = ins[bi]->r;
+        VH_REQUIRE(outs[bi] != ins[bi] && outs[bi]->r == r);
+        for (uint64_t j = 0; j < r; j++) srcs[bi].push_back({ins[bi]->a[j], mgsws[bi] + j * ell});
+        srcs[bi].push_back({ins[bi]->b, mgsws[bi] + r * ell});
+    }
+    gadget_batch(outs, srcs, true);
+    if (special_primes == 0) return;
+    std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pairs;
+    std::map<uint64_t, std::vector<RNS_Polynomial>> by_divide;
+    for (size_t bi = 0; bi < B; bi++) {
+        uint64_t dm = 0, cnt = 0;
+        const uint64_t m = outs[bi]->b->rns_mask;
+        for (int idx = 63; idx >= 0 && cnt < special_primes; idx--)
+            if (ACTIVE(m, (uint64_t)idx)) { dm |= 1ULL << idx; cnt++; }
+        for (auto p : comps(outs[bi])) {
+            pairs.push_back({p, p});
+            if (dm) by_divide[dm].push_back(p);
+        }
+    }
+    ntt_rows(pairs, 1);
+    for (auto &g : by_divide) run_division(g.second, g.first, true);
+    ntt_rows(pairs, 0);
+}
+
+extern "C" void mgsw_external_product(RNS_MLWE out, RNS_MLWE *mgsw, RNSc_MLWE in, uint64_t ell, uint64_t special_primes)
+{
+    VH_ENTER;
+    external_product_batch({out}, {in}, {mgsw}, ell, special_primes);
+}
+
+// mgsw.c:51-71: out = ExtProd(mgsw, in2 - in1) + NTT(in1).
+static void cmux_batch(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &in1,
+                       const std::vector<RNS_MLWE> &in2, const std::vector<RNS_MLWE *> &mgsws, uint64_t ell,
+                       uint64_t special_primes)
+{
+    const size_t B = outs.size();
+    VH_REQUIRE(B > 0 && in1.size() == B && in2.size() == B && mgsws.size() == B);
+    std::vector<RNS_MLWE> diff(B), in1n(B);
+    std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> fw;
+    for (size_t bi = 0; bi < B; bi++) {
+        const uint64_t N = in1[bi]->b->ntt->N, r = in1[bi]->r, mask = in1[bi]->b->rns_mask;
+        incNTT t = in1[bi]->b->ntt;
+        diff[bi] = alloc_sample_raw(N, r, mask, t);
+        mlwe_sub_RNSc_sample(diff[bi], in2[bi], in1[bi]);
+        in1n[bi] = alloc_sample_raw(N, r, mask, t);
+        mlwe_copy_RNS_sample(in1n[bi], in1[bi]);
+        for (auto p : comps(in1n[bi])) fw.push_back({p, p});
+    }
+    external_product_batch(outs, diff, mgsws, ell, special_primes);
+    ntt_rows(fw, 0);
+    for (size_t bi = 0; bi < B; bi++) mlwe_add_RNSc_sample(outs[bi], outs[bi], in1n[bi]);
+    for (size_t bi = 0; bi < B; bi++) { free_mlwe_RNS_sample(diff[bi]); free_mlwe_RNS_sample(in1n[bi]); }
+}
+
+extern "C" void mgsw_CMUX(RNS_MLWE out, RNSc_MLWE in1, RNSc_MLWE in2, RNS_MLWE *mgsw, uint64_t ell,
+                          uint64_t special_primes)
+{
+    VH_ENTER;
+    cmux_batch({out}, {in1}, {in2}, {mgsw}, ell, special_primes);
+}
+
+// mgsw.c:73-88: CMUX(in1, automorphism_{2N-1}(in2)).
+extern "C" void mgsw_NCMUX(RNS_MLWE out, RNSc_MLWE in1, RNSc_MLWE in2, RNS_MLWE *mgsw, RNS_MLWE **ksk, uint64_t ell,
+                           uint64_t special_primes)
+{
+    VH_ENTER;
+    const uint64_t N = in1->b->ntt->N;
+    RNSc_MLWE tmp = alloc_sample_raw(N, in1->r, in1->b->rns_mask, in1->b->ntt);
+    mlwe_automorphism_RNSc_GHS(tmp, in2, 2 * N - 1, ksk, ell);
+    cmux_batch({out}, {in1}, {tmp}, {mgsw}, ell, special_primes);
+    free_mlwe_RNS_sample(tmp);
+}
+
+// CMUX for many (in1, in2, mgsw) triples in one launch sequence: the blind
+// rotation of a batch of LWE ciphertexts (cggi16.py:70-90) steps all of them
+// through the same bootstrapping-key index at once.
+extern "C" void vfhe_hip_mgsw_cmux_batch(RNS_MLWE *out, RNSc_MLWE *in1, RNSc_MLWE *in2, RNS_MLWE **mgsw, uint64_t count,
+                                         uint64_t ell, uint64_t special_primes)
+{
+    VH_ENTER;
+    if (count == 0) return;
+    cmux_batch(std::vector<RNS_MLWE>(out, out + count), std::vector<RNS_MLWE>(in1, in1 + count),
+               std::vector<RNS_MLWE>(in2, in2 + count), std::vector<RNS_MLWE *>(mgsw, mgsw + count), ell,
+               special_primes);
 }
 
 // ---------------------------------------------------------------------------
@@ -2120,13 +2381,46 @@ extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x
         dev_ready(x[i]); dev_ready(y[i]); dev_ready(z[i]);
         const uint64_t m = x[i]->rns_mask & y[i]->rns_mask;
         check_rows(z[i], m);
-        tmp[i] = polynomial_new_RNS_polynomial(N, m, t);
+        tmp[i] = new_poly(N, m, t, false);
         z[i]->rns_mask = m;
+    }
+    vh_pool *P = VPOOL(t);
+    const int logn = ilog2(P->n);
+    bool fused = t->split_degree == 1 && OPT.ntt_mode == 0 && logn >= 13 && logn <= 16;
+    for (uint64_t i = 0; i < count && fused; i++)
+        for_active(z[i]->rns_mask, t->l, [&](uint64_t li) { fused = fused && P->h_limbs[li].fp64; });
+    if (fused) {
+        // Three launches: forward column passes of x (into z) and y (into tmp); forward
+        // row stages, product and inverse row stages in one kernel; inverse column pass.
+        std::vector<vh_item> col, mid, inv;
+        for (uint64_t i = 0; i < count; i++)
+            for_active(z[i]->rns_mask, t->l, [&](uint64_t li) {
+                col.push_back(mk_item(drow(x[i], li), drow(z[i], li), (uint32_t)li));
+                col.push_back(mk_item(drow(y[i], li), drow(tmp[i], li), (uint32_t)li));
+                mid.push_back(mk_item(drow(z[i], li), drow(z[i], li), (uint32_t)li, drow(tmp[i], li)));
+                inv.push_back(mk_item(drow(z[i], li), drow(z[i], li), (uint32_t)li));
+            });
+        const int R1 = logn - 8;
+        auto launch = [&](std::vector<vh_item> &v, int which) {
+            for (size_t off = 0; off < v.size(); off += VH_MAX_GRID_Y) {
+                const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, v.size() - off);
+                std::vector<vh_item> chunk(v.begin() + off, v.begin() + off + cnt);
+                const vh_item *d = vh_stage_vec(chunk);
+                vh_prof_scope ps_(VH_PROF_NTT_FP64);
+                if (which == 0) VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0));
+                else if (which == 1) VH_CHECK(vh_launch_ntt16_row_mul_inv(S(), d, cnt, P->d_limbs, logn));
+                else VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1));
+            }
+        };
+        launch(col, 0);
+        launch(mid, 1);
+        launch(inv, 2);
+        for (auto p : tmp) free_RNS_polynomial(p);
+        return;
     }
     {
         // forward transforms of x into z and of y into a temporary, common limbs only
         std::vector<vh_item> items;
-        vh_pool *P = VPOOL(t);
         for (uint64_t i = 0; i < count; i++) {
             const uint64_t m = z[i]->rns_mask;
             for_active(m, t->l, [&](uint64_t li) {

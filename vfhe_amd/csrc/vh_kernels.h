@@ -46,6 +46,7 @@ struct vh_mac_job {
     const uint64_t *modup;        // [digits][next][n] lifted, forward-transformed digits
     uint64_t kptr_off;            // offset of this job's key-row table in kptr
     int comps, digits, next;
+    int add;                      // 1: out = +sum (external product), 0: out = -sum (key-switch)
     int limb[VH_MAX_LIMBS];       // extended limb list
 };
 
@@ -75,6 +76,10 @@ hipError_t vh_launch_permute(hipStream_t st, const vh_item *items, unsigned nite
 // FP64 path, 13 <= logn <= 16, comps 1 or 2.
 hipError_t vh_launch_ntt16_row_mac(hipStream_t st, const vh_mac_job *jobs, unsigned njobs, unsigned next,
                                    const vh_limb *limbs, const uint64_t *const *kptr, int logn, int comps);
+// Forward row stages of it.src and it.src2 (column-pass outputs), their pointwise
+// product and the inverse row stages, into it.dst (the inverse column pass input).
+hipError_t vh_launch_ntt16_row_mul_inv(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
+                                       int logn);
 // fp != 0: every extended limb has q < 2^50 and every operand is canonical (FP64 products).
 hipError_t vh_launch_keyswitch_mac(hipStream_t st, const vh_mac_job *jobs, unsigned njobs, unsigned next,
                                    const vh_limb *limbs, const uint64_t *const *kptr, uint64_t n, int fp);
@@ -105,5 +110,8 @@ hipError_t vh_launch_fft_bitrev(hipStream_t st, double *const *rows, double *con
                                 uint32_t prec);
 hipError_t vh_launch_fft_scale(hipStream_t st, double *const *rows, unsigned nrows, double scale, uint64_t count);
 hipError_t vh_launch_fft_round(hipStream_t st, double *const *rows, uint64_t *const *outs, unsigned nrows, uint64_t n);
+// out = X^a * in (item.aux = 0) or (X^a - 1) * in (aux = 1) mod X^n + 1, a < 2n.
+hipError_t vh_launch_mul_xai(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, uint64_t n,
+                             uint64_t a);
 hipError_t vh_launch_scatter_split(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                                    uint64_t N, int signed_in);

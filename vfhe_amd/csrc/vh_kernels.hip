@@ -140,6 +140,17 @@ k_ntt_pass(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs,
 // LDS doubles one task needs: column passes index [k][g] (lanes run over g), row
 // passes [g][k] with one pad slot per 16 values; both keep the exchange at the
 // 2-way minimum for 64-bit lanes.
+// Row passes keep each row's 16 lanes inside one wave and each row's LDS slice
+// private to it, so their exchanges only need the wave's own LDS operations to
+// stay in program order (they do: a wave's LDS instructions execute in order);
+// the fences stop the compiler from moving loads and stores across the exchange.
+__device__ __forceinline__ void vh_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int R, bool COL> constexpr int vh_ntt16_lds()
 {
     return COL ? (1 << R) * (256 / ((1 << R) / 16) + 1) : (256 / ((1 << R) / 16)) * ((1 << R) + (1 << R) / 16);
@@ -199,7 +210,7 @@ __device__ __forceinline__ void ntt16_task(const vh_item &it, const vh_limb &L, 
         if (R > 4) {
 #pragma unroll
             for (int m = 0; m < 16; m++) lds[lidx(t + TPS * m)] = vh_center(v[m], q, qinv);
-            __syncthreads();
+            if (COL) __syncthreads(); else vh_wave_sync();
 #pragma unroll
             for (int m = 0; m < 16; m++) v[m] = lds[lidx(16 * t + m)];
             // ---- contiguous ownership, stages s' = 4..R-1 ----
@@ -224,10 +235,12 @@ __device__ __forceinline__ void ntt16_task(const vh_item &it, const vh_limb &L, 
 #pragma unroll
             for (int m = 0; m < 16; m++) out[gidx(16 * t + m)] = finish(v[m]);
         } else {
-            __syncthreads();
+            // row pass: back to the stride ownership through LDS, so each store
+            // instruction writes whole 128-B runs
+            vh_wave_sync();
 #pragma unroll
             for (int m = 0; m < 16; m++) lds[lidx(16 * t + m)] = __longlong_as_double((long long)finish(v[m]));
-            __syncthreads();
+            vh_wave_sync();
 #pragma unroll
             for (int m = 0; m < 16; m++)
                 out[gidx(t + TPS * m)] = (uint64_t)__double_as_longlong(lds[lidx(t + TPS * m)]);
@@ -241,9 +254,10 @@ __device__ __forceinline__ void ntt16_task(const vh_item &it, const vh_limb &L, 
 #pragma unroll
             for (int m = 0; m < 16; m++) v[m] = load(gidx(16 * t + m));
         } else {
+            // row pass: whole 128-B runs per load instruction, regrouped through LDS
 #pragma unroll
             for (int m = 0; m < 16; m++) lds[lidx(t + TPS * m)] = load(gidx(t + TPS * m));
-            __syncthreads();
+            vh_wave_sync();
 #pragma unroll
             for (int m = 0; m < 16; m++) v[m] = lds[lidx(16 * t + m)];
         }
@@ -264,10 +278,10 @@ __device__ __forceinline__ void ntt16_task(const vh_item &it, const vh_limb &L, 
                 }
                 c++;
             }
-            if (!COL) __syncthreads();  // LDS is reused: every read of the load transpose is done
+            if (!COL) vh_wave_sync();  // LDS is reused: every read of the load transpose is done
 #pragma unroll
             for (int m = 0; m < 16; m++) lds[lidx(16 * t + m)] = v[m];
-            __syncthreads();
+            if (COL) __syncthreads(); else vh_wave_sync();
 #pragma unroll
             for (int m = 0; m < 16; m++) v[m] = lds[lidx(t + TPS * m)];
         }
@@ -322,6 +336,7 @@ k_ntt16_row_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__
     __shared__ double lds[vh_ntt16_lds<R, false>()];
     const vh_mac_job &jb = jobs[blockIdx.x];
     const int ti = blockIdx.z, D = jb.digits;
+    if (ti >= jb.next) return;   // jobs of a batch may have fewer extended limbs (whole block)
     const int limb = jb.limb[ti];
     const vh_limb &L = limbs[limb];
     const double q = L.qd, qinv = L.qinv;
@@ -335,6 +350,12 @@ k_ntt16_row_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__
         const size_t i = ((size_t)1 << (s0 + sp)) + (sub << sp) + local;
         return make_double2(tw[2 * i], tw[2 * i + 1]);
     };
+    auto digit_row = [&](int d) -> const VH_GLOBAL uint64_t * {
+        return vh_global(jb.modup + ((size_t)d * jb.next + ti) * n + sub * S);
+    };
+    // After the last stage a thread owns the contiguous elements x0 .. x0+15 of its
+    // row: key rows are read and outputs written as 16 consecutive words per lane.
+    const size_t x0 = sub * S + 16 * t;
 
     double acc[C][16];
 #pragma unroll
@@ -342,11 +363,21 @@ k_ntt16_row_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__
 #pragma unroll
         for (int m = 0; m < 16; m++) acc[c][m] = 0.0;
 
+    uint64_t raw[16];
+    if (D > 0) {
+        const VH_GLOBAL uint64_t *in = digit_row(0);
+#pragma unroll
+        for (int m = 0; m < 16; m++) raw[m] = in[t + TPS * m];
+    }
     for (int d = 0; d < D; d++) {
-        const VH_GLOBAL uint64_t *in = vh_global(jb.modup + ((size_t)d * jb.next + ti) * n + sub * S);
         double v[16];
 #pragma unroll
-        for (int m = 0; m < 16; m++) v[m] = __longlong_as_double((long long)in[t + TPS * m]);
+        for (int m = 0; m < 16; m++) v[m] = __longlong_as_double((long long)raw[m]);
+        if (d + 1 < D) {   // next digit's loads overlap this digit's butterflies
+            const VH_GLOBAL uint64_t *in = digit_row(d + 1);
+#pragma unroll
+            for (int m = 0; m < 16; m++) raw[m] = in[t + TPS * m];
+        }
         // stages s' = 0..3, stride ownership
 #pragma unroll
         for (int sp = 0; sp < 4; sp++) {
@@ -361,10 +392,10 @@ k_ntt16_row_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__
                 v[m + hm] = u - x;
             }
         }
-        __syncthreads();  // the previous digit's reads of lds are done
+        vh_wave_sync();  // every read of the previous digit's exchange is done
 #pragma unroll
         for (int m = 0; m < 16; m++) lds[lidx(t + TPS * m)] = vh_center(v[m], q, qinv);
-        __syncthreads();
+        vh_wave_sync();
 #pragma unroll
         for (int m = 0; m < 16; m++) v[m] = lds[lidx(16 * t + m)];
         // stages s' = 4..7, contiguous ownership
@@ -381,32 +412,31 @@ k_ntt16_row_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__
                 v[m + hm] = u - x;
             }
         }
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < 16; m++) lds[lidx(16 * t + m)] = vh_center(v[m], q, qinv);
-        __syncthreads();
-        // back to stride ownership: element t + 16 m of the row, coalesced key rows
-        const size_t x0 = sub * S + t;
         const uint64_t *const *kp = kptr + jb.kptr_off + ((size_t)ti * D + d) * C;
 #pragma unroll
-        for (int m = 0; m < 16; m++) {
-            const double a = lds[lidx(t + TPS * m)];   // |a| <= q/2 + 1
+        for (int c = 0; c < C; c++) {
+            const VH_GLOBAL uint64_t *kr = vh_global(kp[c]) + x0;
 #pragma unroll
-            for (int c = 0; c < C; c++) {
-                const double k = vh_u2d(vh_global(kp[c])[x0 + TPS * m]);
-                const double hi = a * k;
-                const double lo = __fma_rn(a, k, -hi);
-                const double qt = __fma_rn(a, k * qinv, VH_MAGIC) - VH_MAGIC;
-                const double r = __fma_rn(-qt, q, hi) + lo;
-                acc[c][m] = vh_center(acc[c][m] + r, q, qinv);
+            for (int h = 0; h < 8; h++) {
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const int m = 2 * h + e;
+                    const double a = vh_center(v[m], q, qinv);          // |a| <= q/2 + 1
+                    const double k = vh_u2d(kr[m]);
+                    const double hi = a * k;
+                    const double lo = __fma_rn(a, k, -hi);
+                    const double qt = __fma_rn(a, k * qinv, VH_MAGIC) - VH_MAGIC;
+                    const double r = __fma_rn(-qt, q, hi) + lo;
+                    acc[c][m] = vh_center(acc[c][m] + r, q, qinv);
+                }
             }
         }
     }
 #pragma unroll
     for (int c = 0; c < C; c++) {
-        VH_GLOBAL uint64_t *out = vh_global(jb.out[c] + (size_t)limb * n + sub * S + t);
+        VH_GLOBAL uint64_t *out = vh_global(jb.out[c] + (size_t)limb * n + x0);
 #pragma unroll
-        for (int m = 0; m < 16; m++) out[TPS * m] = vh_d2u(vh_canon(-acc[c][m], q, qinv));
+        for (int m = 0; m < 16; m++) out[m] = vh_d2u(vh_canon(jb.add ? acc[c][m] : -acc[c][m], q, qinv));
     }
 }
 
@@ -417,6 +447,142 @@ hipError_t vh_launch_ntt16_row_mac(hipStream_t st, const vh_mac_job *jobs, unsig
     const dim3 grid(njobs, 1u << (logn - 12), next);
     if (comps == 1) hipLaunchKernelGGL(k_ntt16_row_mac<1>, grid, dim3(256), 0, st, jobs, limbs, kptr, logn);
     else hipLaunchKernelGGL(k_ntt16_row_mac<2>, grid, dim3(256), 0, st, jobs, limbs, kptr, logn);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// z = INTT(NTT(x) * NTT(y)) middle step (FP64 path, N = 2^13..2^16): after the
+// forward column passes of x (in it.src) and y (in it.src2), one kernel runs the
+// forward row stages of both, the pointwise product and the inverse row stages,
+// and writes the input of the inverse column pass to it.dst.  The transformed
+// operands never return to memory.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_ntt16_row_mul_inv(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, int logn)
+{
+    constexpr int R = 8, S = 256, TPS = 16, G = 16;
+    __shared__ double lds[vh_ntt16_lds<R, false>()];
+    const vh_item &it = items[blockIdx.y];
+    const vh_limb &L = limbs[it.limb];
+    const double q = L.qd, qinv = L.qinv;
+    const int s0 = logn - R;
+    const int tid = threadIdx.x, g = tid / TPS, t = tid % TPS;
+    const size_t sub = (size_t)blockIdx.x * G + g;
+    const VH_GLOBAL double *twf = vh_global(reinterpret_cast<const double *>(L.tw_fwd));
+    const VH_GLOBAL double *twi = vh_global(reinterpret_cast<const double *>(L.tw_inv));
+    auto lidx = [&](int k) -> int { return g * (S + S / 16) + k + (k >> 4); };
+    auto tw = [&](const VH_GLOBAL double *tab, int sp, size_t local) -> double2 {
+        const size_t i = ((size_t)1 << (s0 + sp)) + (sub << sp) + local;
+        return make_double2(tab[2 * i], tab[2 * i + 1]);
+    };
+    const VH_GLOBAL uint64_t *xa = vh_global(it.src) + sub * S;
+    const VH_GLOBAL uint64_t *xb = vh_global(it.src2) + sub * S;
+    VH_GLOBAL uint64_t *out = vh_global(it.dst) + sub * S;
+
+    double a[16], b[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        a[m] = __longlong_as_double((long long)xa[t + TPS * m]);
+        b[m] = __longlong_as_double((long long)xb[t + TPS * m]);
+    }
+    // forward stages s' = 0..3 (stride ownership) on both operands
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++) {
+        const int hm = 8 >> sp;
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            if (m & hm) continue;
+            const double2 w = tw(twf, sp, (size_t)(m >> (4 - sp)));
+            const double xa_ = vh_mulmod_tw(a[m + hm], w.x, w.y, q), xb_ = vh_mulmod_tw(b[m + hm], w.x, w.y, q);
+            const double ua = a[m], ub = b[m];
+            a[m] = ua + xa_; a[m + hm] = ua - xa_;
+            b[m] = ub + xb_; b[m + hm] = ub - xb_;
+        }
+    }
+    // regroup both into contiguous ownership through LDS
+#pragma unroll
+    for (int m = 0; m < 16; m++) lds[lidx(t + TPS * m)] = vh_center(a[m], q, qinv);
+    vh_wave_sync();
+#pragma unroll
+    for (int m = 0; m < 16; m++) a[m] = lds[lidx(16 * t + m)];
+    vh_wave_sync();
+#pragma unroll
+    for (int m = 0; m < 16; m++) lds[lidx(t + TPS * m)] = vh_center(b[m], q, qinv);
+    vh_wave_sync();
+#pragma unroll
+    for (int m = 0; m < 16; m++) b[m] = lds[lidx(16 * t + m)];
+    // forward stages s' = 4..7
+#pragma unroll
+    for (int sp = 4; sp < R; sp++) {
+        const int hm = 1 << (R - sp - 1);
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            if (m & hm) continue;
+            const double2 w = tw(twf, sp, (size_t)((16 * t + m) >> (R - sp)));
+            const double xa_ = vh_mulmod_tw(a[m + hm], w.x, w.y, q), xb_ = vh_mulmod_tw(b[m + hm], w.x, w.y, q);
+            const double ua = a[m], ub = b[m];
+            a[m] = ua + xa_; a[m + hm] = ua - xa_;
+            b[m] = ub + xb_; b[m + hm] = ub - xb_;
+        }
+    }
+    // pointwise product of centred operands (|.| <= q/2 + 1): exact product,
+    // quotient within 1, |r| <= q + 2^47, then centred
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        const double u = vh_center(a[m], q, qinv), v = vh_center(b[m], q, qinv);
+        const double hi = u * v;
+        const double lo = __fma_rn(u, v, -hi);
+        const double qt = __fma_rn(u, v * qinv, VH_MAGIC) - VH_MAGIC;
+        a[m] = vh_center(__fma_rn(-qt, q, hi) + lo, q, qinv);
+    }
+    // inverse stages s' = 7..4 (contiguous ownership), GS sums re-centred every second stage
+    int c = 0;
+#pragma unroll
+    for (int sp = R - 1; sp >= 4; sp--) {
+        const int hm = 1 << (R - sp - 1);
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            if (m & hm) continue;
+            const double2 w = tw(twi, sp, (size_t)((16 * t + m) >> (R - sp)));
+            const double u = a[m], x = a[m + hm];
+            double s_ = u + x;
+            if (c & 1) s_ = vh_center(s_, q, qinv);
+            a[m] = s_;
+            a[m + hm] = vh_mulmod_tw(u - x, w.x, w.y, q);
+        }
+        c++;
+    }
+    vh_wave_sync();   // every read of b's regrouping is done
+#pragma unroll
+    for (int m = 0; m < 16; m++) lds[lidx(16 * t + m)] = a[m];
+    vh_wave_sync();
+#pragma unroll
+    for (int m = 0; m < 16; m++) a[m] = lds[lidx(t + TPS * m)];
+    // inverse stages s' = 3..0 (stride ownership)
+#pragma unroll
+    for (int sp = 3; sp >= 0; sp--) {
+        const int hm = 8 >> sp;
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            if (m & hm) continue;
+            const double2 w = tw(twi, sp, (size_t)(m >> (4 - sp)));
+            const double u = a[m], x = a[m + hm];
+            double s_ = u + x;
+            if (c & 1) s_ = vh_center(s_, q, qinv);
+            a[m] = s_;
+            a[m + hm] = vh_mulmod_tw(u - x, w.x, w.y, q);
+        }
+        c++;
+    }
+#pragma unroll
+    for (int m = 0; m < 16; m++) out[t + TPS * m] = (uint64_t)__double_as_longlong(vh_center(a[m], q, qinv));
+}
+
+hipError_t vh_launch_ntt16_row_mul_inv(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
+                                       int logn)
+{
+    if (logn < 13 || logn > 16) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_ntt16_row_mul_inv, dim3(1u << (logn - 12), nitems), dim3(256), 0, st, items, limbs, logn);
     return hipGetLastError();
 }
 
@@ -724,8 +890,8 @@ k_keyswitch_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__
     // (same x range, same limb): the evaluation key is fetched once per batch.
     const vh_mac_job &jb = jobs[blockIdx.x];
     const uint64_t x = (uint64_t)blockIdx.y * blockDim.x + threadIdx.x;
-    if (x >= n) return;
     const int ti = blockIdx.z;
+    if (x >= n || ti >= jb.next) return;
     const int limb = jb.limb[ti];
     const vh_limb &L = limbs[limb];
     const int C = jb.comps, D = jb.digits;
@@ -754,7 +920,7 @@ k_keyswitch_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__
 #pragma unroll
         for (int c = 0; c < VH_MAX_COMPS; c++) {
             if (c >= C) break;
-            jb.out[c][(uint64_t)limb * n + x] = vh_d2u(vh_canon(-acc[c], q, qinv));
+            jb.out[c][(uint64_t)limb * n + x] = vh_d2u(vh_canon(jb.add ? acc[c] : -acc[c], q, qinv));
         }
     } else {
         const uint64_t q = L.q, mu = L.mu64;
@@ -772,7 +938,7 @@ k_keyswitch_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__
         for (int c = 0; c < VH_MAX_COMPS; c++) {
             if (c >= C) break;
             const uint64_t a = acc[c];
-            jb.out[c][(uint64_t)limb * n + x] = a ? q - a : 0;
+            jb.out[c][(uint64_t)limb * n + x] = jb.add ? a : (a ? q - a : 0);
         }
     }
 }
@@ -1058,6 +1224,35 @@ hipError_t vh_launch_sample_gaussian(hipStream_t st, const vh_gauss_job *jobs, u
     if (n % 4) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_sample_gaussian, dim3(vh_blocks(n / 4, 256, 64), njobs), dim3(256), 0, st, jobs, items, limbs,
                        key, sigma, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Negacyclic monomial products (rns_polynomial.c:1048-1168):
+//   out = X^a * in, or (X^a - 1) * in when item.aux == 1, mod X^n + 1, a < 2n.
+// Output coefficient i takes u = (i - a) mod 2n: +in[u] if u < n, else
+// -in[u - n].  One thread per output coefficient; out != in.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_mul_xai(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, uint64_t n, uint64_t a)
+{
+    const vh_item it = items[blockIdx.y];
+    const vh_limb L = limbs[it.limb];
+    const uint64_t q = L.q, mu = L.mu64, n2 = 2 * n;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t u = (i + n2 - a) % n2;
+        const uint64_t x = vh_reduce64(it.src[u < n ? u : u - n], q, mu);
+        uint64_t r = u < n ? x : (x ? q - x : 0);
+        if (it.aux) r = vh_submod(r, vh_reduce64(it.src[i], q, mu), q);
+        it.dst[i] = r;
+    }
+}
+
+hipError_t vh_launch_mul_xai(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, uint64_t n,
+                             uint64_t a)
+{
+    if (a >= 2 * n) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mul_xai, dim3(vh_blocks(n, 256, 64), nitems), dim3(256), 0, st, items, limbs, n, a);
     return hipGetLastError();
 }
 

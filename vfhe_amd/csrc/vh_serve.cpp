@@ -2,8 +2,9 @@
 //
 // Callers submit homomorphic requests (multiply + relinearize [+ rescale], or
 // automorphism/rotation + key-switch) from any thread; a worker thread admits
-// them by memory footprint, groups requests of one class (operation, ring,
-// level, key, Galois element, rescale mask) into one batched engine call, and
+// them by memory footprint, groups requests of one class (operation, ring and
+// rank; level, key, Galois element and rescale mask may differ) into one
+// batched engine call, and
 // runs batches on several engine lanes (HIP streams of default priority) so
 // that work of different classes overlaps on the device.  A batch launched on
 // a free lane does not wait for batches on other lanes; requests that arrive
@@ -58,17 +59,16 @@ struct Req {
 
 struct Batch {
     std::vector<Req *> reqs;
-    hipEvent_t ev;
+    hipEvent_t ev0, ev;   // timing events around the batch on its lane
     int lane;
     uint64_t bytes;
 };
 
+// Requests batch together when they share the operation, the ring and the rank;
+// level, key, Galois element and rescale mask may differ per request.
 bool same_class(const Req *a, const Req *b)
 {
-    if (a->op != b->op || a->ksk != b->ksk || a->gen != b->gen || a->divide_mask != b->divide_mask ||
-        a->in1->b->ntt != b->in1->b->ntt || a->in1->b->rns_mask != b->in1->b->rns_mask || a->in1->r != b->in1->r)
-        return false;
-    return a->op != VFHE_HIP_REQ_MULTIPLY || a->in2->b->rns_mask == b->in2->b->rns_mask;
+    return a->op == b->op && a->in1->b->ntt == b->in1->b->ntt && a->in1->r == b->in1->r;
 }
 
 uint64_t popcount(uint64_t x) { return (uint64_t)__builtin_popcountll(x); }
@@ -103,37 +103,48 @@ struct Server {
     bool stop = false;
     std::thread worker;
     uint64_t n_batches = 0, n_reqs = 0, max_busy = 0;
+    double host_s = 0, device_s = 0, t_start = now_s();
     std::vector<hipEvent_t> ev_free;
 
     hipEvent_t event()
     {
         if (!ev_free.empty()) { hipEvent_t e = ev_free.back(); ev_free.pop_back(); return e; }
         hipEvent_t e;
-        srv_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        srv_check(hipEventCreate(&e), "hipEventCreate");
         return e;
     }
 
     void launch(std::vector<Req *> &reqs, int lane)
     {
+        const double th0 = now_s();
         const int prev = vfhe_hip_set_lane(lane + 1);   // lanes 1..lanes; lane 0 stays the caller's
+        hipEvent_t ev0 = event();
+        srv_check(hipEventRecord(ev0, (hipStream_t)vfhe_hip_current_stream()), "hipEventRecord");
         const size_t B = reqs.size();
         std::vector<RNS_MLWE> out(B), in1(B), in2(B);
         uint64_t bytes = 0;
         for (size_t i = 0; i < B; i++) {
             Req *r = reqs[i];
             const incNTT t = r->in1->b->ntt;
-            r->out = mlwe_alloc_RNS_sample(t->N, r->in1->r, r->in1->b->rns_mask, t);
+            r->out = vfhe_hip_mlwe_alloc_RNS_sample_uninit(t->N, r->in1->r, r->in1->b->rns_mask, t);
             out[i] = r->out;
             in1[i] = r->in1;
             in2[i] = r->in2;
             bytes += r->bytes;
         }
+        std::vector<RNS_MLWE **> ksks(B);
+        std::vector<uint64_t> arg(B);
+        for (size_t i = 0; i < B; i++) {
+            ksks[i] = reqs[i]->ksk;
+            arg[i] = reqs[i]->op == VFHE_HIP_REQ_MULTIPLY ? reqs[i]->divide_mask : reqs[i]->gen;
+        }
         if (reqs[0]->op == VFHE_HIP_REQ_MULTIPLY)
-            vfhe_hip_mlwe_multiply_batch(out.data(), in1.data(), in2.data(), B, reqs[0]->ksk, reqs[0]->divide_mask);
+            vfhe_hip_mlwe_multiply_batch_multi(out.data(), in1.data(), in2.data(), B, ksks.data(), arg.data());
         else
-            vfhe_hip_mlwe_automorphism_batch(out.data(), in1.data(), B, reqs[0]->gen, reqs[0]->ksk);
+            vfhe_hip_mlwe_automorphism_batch_multi(out.data(), in1.data(), B, arg.data(), ksks.data());
         Batch b;
         b.reqs = reqs;
+        b.ev0 = ev0;
         b.ev = event();
         b.lane = lane;
         b.bytes = bytes;
@@ -143,6 +154,7 @@ struct Server {
         inflight.push_back(b);
         n_batches++;
         n_reqs += B;
+        host_s += now_s() - th0;
     }
 
     // One worker step; returns true when it made progress.
@@ -157,6 +169,9 @@ struct Server {
             const double t = now_s();
             Batch b = inflight[k];
             inflight.erase(inflight.begin() + (long)k);
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, b.ev0, b.ev) == hipSuccess) device_s += ms * 1e-3;
+            ev_free.push_back(b.ev0);
             ev_free.push_back(b.ev);
             lane_busy[b.lane] = 0;
             {
@@ -244,7 +259,7 @@ extern "C" uint64_t vfhe_hip_server_submit(void *srv, int op, RNS_MLWE in1, RNS_
 {
     Server *s = (Server *)srv;
     if (!s || !in1 || !ksk || (op != VFHE_HIP_REQ_MULTIPLY && op != VFHE_HIP_REQ_AUTOMORPHISM)) return 0;
-    if (op == VFHE_HIP_REQ_MULTIPLY && (!in2 || in2->r != in1->r)) return 0;
+    if (op == VFHE_HIP_REQ_MULTIPLY && (!in2 || in2->r != in1->r || in2->b->ntt != in1->b->ntt)) return 0;
     Req *r = new Req();
     r->op = op;
     r->in1 = in1;
@@ -292,6 +307,9 @@ extern "C" void vfhe_hip_server_stats(void *srv, double *stats)
     stats[3] = (double)s->peak;
     stats[4] = (double)s->max_busy;
     stats[5] = (double)s->pending;
+    stats[6] = s->host_s;
+    stats[7] = s->device_s;
+    stats[8] = now_s() - s->t_start;
 }
 
 extern "C" void vfhe_hip_server_free(void *srv)

@@ -140,10 +140,11 @@ class Server:
         return [(int(ids[k]), outs[k], float(lat[k])) for k in range(n)]
 
     def stats(self) -> dict:
-        s = (ctypes.c_double * 6)()
+        s = (ctypes.c_double * 9)()
         self.lib.vfhe_hip_server_stats(self.h, s)
         return {"batches": int(s[0]), "requests": int(s[1]), "mean_batch": round(s[2], 2),
-                "peak_inflight_bytes": int(s[3]), "max_lanes_busy": int(s[4]), "pending": int(s[5])}
+                "peak_inflight_bytes": int(s[3]), "max_lanes_busy": int(s[4]), "pending": int(s[5]),
+                "host_enqueue_s": round(s[6], 4), "device_batch_s": round(s[7], 4), "server_s": round(s[8], 4)}
 
     def close(self):
         if self.h:
