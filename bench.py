@@ -61,6 +61,7 @@ def parse():
     ap.add_argument("--only-ckks", action="store_true", help="run the CKKS leg alone (profiling)")
     ap.add_argument("--only-serving", action="store_true", help="run the serving leg alone (profiling)")
     ap.add_argument("--only-fused", action="store_true", help="run unit B alone (profiling)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default NTT schedule")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also copy the 64-polynomial batch once per step (known-size copy for counter scaling)")
     return ap.parse_args()
@@ -160,6 +161,13 @@ def main():
              ("ntt.mode", "ntt.xcd_available", "ntt.xcd_lag", "ntt.xcd_occupancy", "ntt.chunk_mb")}
     log("schedule", sched, "probe", [int(lib.vfhe_hip_get_option(f"probe.xcd{k}".encode())) for k in range(8)])
     pool = rt.Pool(N, primes)
+    tuned = None
+    if not (args.no_autotune or args.only_ntt or args.only_fused or args.only_ckks or args.only_serving):
+        us = (ctypes.c_double * 6)()
+        best = lib.vfhe_hip_autotune_ntt(pool.h, POLYS * L, 2, us)
+        tuned = {"chosen": int(best), "us_per_round_trip": [round(x, 1) for x in us]}
+        sched = {k: int(lib.vfhe_hip_get_option(k.encode())) for k in sched}
+        log("autotune:", tuned, "->", sched)
     rng = np.random.default_rng(20261202)
     polys = []
     for _ in range(POLYS):
@@ -281,7 +289,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "batched NTT round trip, 64 polynomials x 24 limbs, N=2^16, ~50-bit primes "
                                "q=1 mod 2^17 (Ring.gen_prime), uniform residues, seed 20261202",
-                   "N": N, "L": L, "polys": POLYS, "schedule": sched},
+                   "N": N, "L": L, "polys": POLYS, "schedule": sched, "autotune": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_ntt16 (FP64 radix-16 column and row passes)", "launches": ntt_launches,

@@ -45,7 +45,8 @@ def _native_b(c):
 
 
 def test_results_same_alone_and_in_mix(lib, rings):
-    wl = serving.make_workload(3000.0, 0.04, NS, levels=LEVELS, rot_steps=STEPS, seed=3)
+    # a burst (about 100 requests within 2 ms) so that requests queue behind running batches
+    wl = serving.make_workload(50000.0, 0.002, NS, levels=LEVELS, rot_steps=STEPS, seed=3)
     assert len(wl) > 60
     print(f"[serving] {len(wl)} requests", flush=True)
     res = serving.run(lib, rings, wl, memory_budget=1 << 30, lanes=4, max_batch=8, keep_outputs=True)

@@ -170,6 +170,7 @@ _SIGNATURES = {
     "polynomial_device_row": (U64P, [VP, U64]),
     "vfhe_hip_synchronize": (None, []),
     "vfhe_hip_set_lane": (ctypes.c_int, [ctypes.c_int]),
+    "vfhe_hip_autotune_ntt": (ctypes.c_long, [VP, U64, I32, ctypes.POINTER(ctypes.c_double)]),
     "vfhe_prng_set_deterministic_seed": (None, [U64]),
     "vfhe_prng_clear_deterministic_seed": (None, []),
     "polynomial_gen_random_RNSc_polynomial": (None, [VP]),

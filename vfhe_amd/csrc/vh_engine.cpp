@@ -246,15 +246,104 @@ struct vh_prof_scope {
     }
 };
 
+// Device blocks: a cache in front of the stream-ordered allocator.  A freed
+// block is parked, by size class, on the list of the lane that frees it, with
+// an event recorded on that lane's stream at the free.  An allocation takes a
+// parked block that fits within twice its class (best fit): from its own lane
+// at once (the reuse is ordered after the free by the stream, as with
+// hipFreeAsync + hipMallocAsync on one stream), from another lane once the
+// free's event has completed.  Size classes: powers of two up to 1 MiB, then
+// multiples of 1 MiB.  At most VH_CACHE_MAX bytes stay parked; beyond that a
+// freed block goes back to the allocator.  (hipMallocAsync / hipFreeAsync cost
+// tens to hundreds of microseconds per call in a serving loop.)
+static const size_t VH_CACHE_MAX = (size_t)32 << 30;
+struct vh_parked {
+    void *p;
+    hipEvent_t ev;
+};
+struct vh_block_cache {
+    std::map<size_t, std::vector<vh_parked>> free_[VH_MAX_LANES];
+    std::map<void *, size_t> size_of;   // every live or parked block -> its class size
+    std::vector<hipEvent_t> events;     // idle events
+    size_t cached = 0;
+};
+static vh_block_cache BC;
+
+static size_t vh_size_class(size_t bytes)
+{
+    if (bytes <= 256) return 256;
+    if (bytes <= ((size_t)1 << 20)) {
+        size_t c = 256;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    const size_t mib = (size_t)1 << 20;
+    return (bytes + mib - 1) / mib * mib;
+}
+
+// Takes a block of class in [c, 2c] from lane `ln`; other lanes' blocks only when
+// their free has completed on the device.
+static void *vh_take_parked(int ln, size_t c)
+{
+    auto &lane = BC.free_[ln];
+    for (auto it = lane.lower_bound(c); it != lane.end() && it->first <= 2 * c; ++it) {
+        auto &v = it->second;
+        for (size_t k = v.size(); k-- > 0;) {
+            if (ln != g_lane) {
+                const hipError_t q = hipEventQuery(v[k].ev);
+                if (q == hipErrorNotReady) continue;
+                VH_CHECK(q);
+            }
+            const vh_parked b = v[k];
+            v.erase(v.begin() + (long)k);
+            BC.events.push_back(b.ev);
+            BC.cached -= it->first;
+            return b.p;
+        }
+    }
+    return nullptr;
+}
+
 static void *vh_dmalloc(size_t bytes)
 {
+    const size_t c = vh_size_class(bytes);
+    if (void *p = vh_take_parked(g_lane, c)) return p;
+    for (int ln = 0; ln < VH_MAX_LANES; ln++)
+        if (ln != g_lane && !BC.free_[ln].empty())
+            if (void *p = vh_take_parked(ln, c)) return p;
     void *p = nullptr;
-    VH_CHECK(hipMallocAsync(&p, bytes ? bytes : 256, S()));
+    if (hipMallocAsync(&p, c, S()) != hipSuccess) {
+        // out of memory with blocks parked in the cache: give them back and retry once
+        (void)hipGetLastError();
+        sync_all_lanes();
+        for (auto &lane : BC.free_)
+            for (auto &kv : lane)
+                for (auto &b : kv.second) { VH_CHECK(hipFree(b.p)); BC.size_of.erase(b.p); BC.events.push_back(b.ev); }
+        for (auto &lane : BC.free_) lane.clear();
+        BC.cached = 0;
+        VH_CHECK(hipMallocAsync(&p, c, S()));
+    }
+    BC.size_of[p] = c;
     return p;
 }
+
 static void vh_dfree(void *p)
 {
-    if (p) VH_CHECK(hipFreeAsync(p, S()));
+    if (!p) return;
+    auto it = BC.size_of.find(p);
+    VH_REQUIRE(it != BC.size_of.end());
+    const size_t c = it->second;
+    if (BC.cached + c <= VH_CACHE_MAX) {
+        hipEvent_t ev;
+        if (!BC.events.empty()) { ev = BC.events.back(); BC.events.pop_back(); }
+        else VH_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        VH_CHECK(hipEventRecord(ev, S()));
+        BC.free_[g_lane][c].push_back({p, ev});
+        BC.cached += c;
+    } else {
+        BC.size_of.erase(it);
+        VH_CHECK(hipFreeAsync(p, S()));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2837,6 +2926,84 @@ extern "C" RNS_MLWE_Key mlwe_new_RNS_gaussian_key(uint64_t N, uint64_t r, uint64
     VH_CHECK(hipStreamSynchronize(S()));
     free_RNS_polynomial(tmp);
     return mlwe_new_RNS_key_from_array(arr.data(), N, r, l, ntt, sigma);
+}
+
+// ---------------------------------------------------------------------------
+// NTT schedule autotuner: times forward+inverse round trips of `rows` scratch
+// rows (limbs of `ntt` in turn) under each candidate schedule and keeps the
+// fastest in the engine options.  Candidates: two launches per transform over
+// the whole batch or sub-batches of 256 / 128 MiB, and (when the XCC placement
+// probe passed and 2^13 <= N <= 2^16) the persistent per-XCD kernel with queue
+// lags 3, 5 and 8.  us_out[k] receives candidate k's microseconds per round
+// trip (-1 if skipped); returns the index of the chosen candidate.
+// ---------------------------------------------------------------------------
+struct vh_sched { long mode, chunk_mb, lag, occ; };
+static const vh_sched VH_SCHEDS[] = {{0, 0, 3, 2}, {0, 256, 3, 2}, {0, 128, 3, 2}, {1, 0, 3, 2}, {1, 0, 5, 2}, {1, 0, 8, 2}};
+
+extern "C" long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, double *us_out)
+{
+    VH_ENTER;
+    const long nc = (long)(sizeof(VH_SCHEDS) / sizeof(VH_SCHEDS[0]));
+    VH_REQUIRE(ntt && rows > 0 && reps > 0);
+    vh_pool *P = VPOOL(ntt);
+    const uint64_t n = P->n;
+    const int logn = ilog2(n);
+    uint64_t *d = (uint64_t *)vh_dmalloc(rows * n * 8);
+    std::vector<vh_item> items;
+    for (uint64_t i = 0; i < rows; i++) {
+        vh_item it = mk_item(nullptr, d + i * n, (uint32_t)(i % ntt->l));
+        it.aux = (uint32_t)i;
+        items.push_back(it);
+    }
+    {
+        const vh_rng_key key = rng_next();
+        for (size_t off = 0; off < items.size(); off += VH_MAX_GRID_Y) {
+            const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, items.size() - off);
+            std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
+            VH_CHECK(vh_launch_sample_uniform(S(), vh_stage_vec(chunk), cnt, P->d_limbs, key, n));
+        }
+    }
+    for (auto &it : items) it.src = it.dst;
+    const vh_opts saved = OPT;
+    hipEvent_t e0, e1;
+    VH_CHECK(hipEventCreate(&e0));
+    VH_CHECK(hipEventCreate(&e1));
+    long best = 0;
+    double best_us = 1e300;
+    std::vector<double> us(nc, -1.0);
+    for (int round = 0; round < 2; round++)   // two interleaved rounds, the faster of each kept
+        for (long k = 0; k < nc; k++) {
+            const vh_sched &sc = VH_SCHEDS[k];
+            if (sc.mode == 1 && !(C.xcd_ok && logn >= 13 && logn <= 16)) continue;
+            OPT.ntt_mode = sc.mode; OPT.ntt_chunk_mb = sc.chunk_mb; OPT.xcd_lag = sc.lag; OPT.xcd_occ = sc.occ;
+            for (int w = 0; w < (round == 0 && k == 0 ? 3 : 1); w++) {   // warm-up
+                run_ntt(items, P->d_limbs, P->h_limbs.data(), n, 0);
+                run_ntt(items, P->d_limbs, P->h_limbs.data(), n, 1);
+            }
+            VH_CHECK(hipEventRecord(e0, S()));
+            for (int r = 0; r < reps; r++) {
+                run_ntt(items, P->d_limbs, P->h_limbs.data(), n, 0);
+                run_ntt(items, P->d_limbs, P->h_limbs.data(), n, 1);
+            }
+            VH_CHECK(hipEventRecord(e1, S()));
+            VH_CHECK(hipEventSynchronize(e1));
+            vh_check_device_error();
+            float ms = 0;
+            VH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            const double t = ms * 1e3 / reps;
+            if (us[k] < 0 || t < us[k]) us[k] = t;
+        }
+    for (long k = 0; k < nc; k++) {
+        if (us_out) us_out[k] = us[k];
+        if (us[k] > 0 && us[k] < best_us) { best_us = us[k]; best = k; }
+    }
+    VH_CHECK(hipEventDestroy(e0));
+    VH_CHECK(hipEventDestroy(e1));
+    OPT = saved;
+    OPT.ntt_mode = VH_SCHEDS[best].mode; OPT.ntt_chunk_mb = VH_SCHEDS[best].chunk_mb;
+    OPT.xcd_lag = VH_SCHEDS[best].lag; OPT.xcd_occ = VH_SCHEDS[best].occ;
+    vh_dfree(d);
+    return best;
 }
 
 // ---------------------------------------------------------------------------
