@@ -194,6 +194,9 @@ RNS_MLWE *mlwe_alloc_RNS_sample_array(uint64_t size, uint64_t N, uint64_t r, uin
 RNS_MLWE *mlwe_create_copy_array(RNS_MLWE *in, uint64_t size);                                        /* mlwe.c:121-127 */
 void free_mlwe_RNS_sample(void *p);                                                                   /* mlwe.c:182-188 */
 void free_RNS_mlwe_array(uint64_t size, RNS_MLWE *v);                                                 /* mlwe.c:152-159 */
+/* Engine extension: free_mlwe_RNS_sample (mlwe.c:182-188) over an array, one lock acquisition;
+   the array itself is not freed. */
+void vfhe_hip_free_mlwe_samples(RNS_MLWE *samples, uint64_t count);
 void mlwe_copy_RNS_sample(RNS_MLWE out, RNS_MLWE in);                                                 /* mlwe.c:168-175 */
 void mlwe_RNS_trivial_sample_of_zero(RNS_MLWE out);                                                   /* mlwe.c:302-312 */
 void mlwe_RNSc_to_RNS(RNS_MLWE out, RNSc_MLWE in);                                                    /* mlwe.c:513-520 */

@@ -141,6 +141,7 @@ _SIGNATURES = {
     "mlwe_create_copy_array": (VPP, [VPP, U64]),
     "free_mlwe_RNS_sample": (None, [VP]),
     "free_RNS_mlwe_array": (None, [U64, VPP]),
+    "vfhe_hip_free_mlwe_samples": (None, [VPP, U64]),
     "mlwe_copy_RNS_sample": (None, [VP, VP]),
     "mlwe_RNS_trivial_sample_of_zero": (None, [VP]),
     "mlwe_RNSc_to_RNS": (None, [VP, VP]),

@@ -1716,6 +1716,15 @@ extern "C" void free_RNS_mlwe_array(uint64_t size, RNS_MLWE *v)
     free(v);
 }
 
+// Frees `count` samples under one acquisition of the engine lock (a serving
+// front end returns a poll's worth of outputs at once).  The array itself stays
+// the caller's.
+extern "C" void vfhe_hip_free_mlwe_samples(RNS_MLWE *samples, uint64_t count)
+{
+    VH_ENTER;
+    for (uint64_t i = 0; i < count; i++) free_mlwe_RNS_sample(samples[i]);
+}
+
 static std::vector<RNS_Polynomial> comps(RNS_MLWE c)
 {
     std::vector<RNS_Polynomial> v(c->a, c->a + c->r);

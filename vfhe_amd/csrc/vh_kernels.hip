@@ -426,8 +426,9 @@ k_ntt16_row_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__
                     const double hi = a * k;
                     const double lo = __fma_rn(a, k, -hi);
                     const double qt = __fma_rn(a, k * qinv, VH_MAGIC) - VH_MAGIC;
-                    const double r = __fma_rn(-qt, q, hi) + lo;
-                    acc[c][m] = vh_center(acc[c][m] + r, q, qinv);
+                    const double r = __fma_rn(-qt, q, hi) + lo;   // |r| <= q + 2^47
+                    // centred every 4th digit: |acc| <= q/2 + 4 (q + 2^47) < 2^53 stays exact
+                    acc[c][m] = (d & 3) == 3 ? vh_center(acc[c][m] + r, q, qinv) : acc[c][m] + r;
                 }
             }
         }

@@ -19,6 +19,7 @@ follow the level primes in the ring).
 from __future__ import annotations
 
 import ctypes
+import sys
 import threading
 import time
 from dataclasses import dataclass
@@ -139,6 +140,11 @@ class Server:
         n = self.lib.vfhe_hip_server_poll(self.h, ids, outs, lat, max_n, int(wait))
         return [(int(ids[k]), outs[k], float(lat[k])) for k in range(n)]
 
+    def free_outputs(self, samples):
+        """Frees a list of output samples with one engine call."""
+        arr = (ctypes.c_void_p * len(samples))(*samples)
+        self.lib.vfhe_hip_free_mlwe_samples(arr, len(samples))
+
     def stats(self) -> dict:
         s = (ctypes.c_double * 9)()
         self.lib.vfhe_hip_server_stats(self.h, s)
@@ -154,16 +160,24 @@ class Server:
 
 def run(lib, rings: dict, workload: list[Request], memory_budget: int, lanes: int = 4, max_batch: int = 32,
         keep_outputs: bool = False) -> dict:
-    """Replay `workload` in real time through a fresh server; return throughput, latencies, stats.
+    """Replay `workload` open-loop in real time through a fresh server; return throughput, latencies, stats.
 
-    With keep_outputs the output samples are returned by request index (the
-    caller frees them); otherwise they are freed as they complete."""
+    A request's latency runs from its scheduled arrival to its completion: the
+    runtime's submit-to-completion time plus how late the replay submitted it,
+    so a client that falls behind shows up in the latencies (`service_p99_ms`
+    is the runtime's part alone).  With keep_outputs the output samples are
+    returned by request index (the caller frees them); otherwise each poll's
+    outputs are freed together as they complete."""
     srv = Server(lib, memory_budget, lanes, max_batch)
     n = len(workload)
     lat = [None] * n
+    svc = [None] * n
+    late = [0.0] * n
     outs = [None] * n
     index = {}
     done_t = [0.0]
+    switch = sys.getswitchinterval()
+    sys.setswitchinterval(2e-4)
     t0 = time.perf_counter()
 
     def collector():
@@ -172,36 +186,54 @@ def run(lib, rings: dict, workload: list[Request], memory_budget: int, lanes: in
             batch = srv.poll(wait=True)
             if not batch:                 # nothing pending yet: the submitter is between arrivals
                 time.sleep(0.0002)
+                continue
+            frees = []
             for rid, out, l in batch:
-                k = index.pop(rid) if rid in index else None
+                k = index.pop(rid, None)
                 while k is None:          # submit() returns after the runtime may already have finished it
                     time.sleep(0)
-                    k = index.pop(rid) if rid in index else None
-                lat[k] = l
+                    k = index.pop(rid, None)
+                svc[k] = l
+                lat[k] = l + late[k]
                 if keep_outputs:
                     outs[k] = out
                 else:
-                    lib.free_mlwe_RNS_sample(out)
-                got += 1
+                    frees.append(out)
+            if frees:
+                srv.free_outputs(frees)
+            got += len(batch)
         done_t[0] = time.perf_counter()
 
     th = threading.Thread(target=collector, daemon=True)
     th.start()
-    for k, req in enumerate(workload):
-        delay = t0 + req.t - time.perf_counter()
-        if delay > 0.002:
-            time.sleep(delay - 0.001)
-        while time.perf_counter() < t0 + req.t:
-            pass
-        rid = srv.submit(*rings[req.N].args(req))
-        index[rid] = k
-    th.join()
+    try:
+        k = 0
+        while k < n:
+            now = time.perf_counter() - t0
+            wait = workload[k].t - now
+            if wait > 0:
+                time.sleep(wait - 2e-4 if wait > 4e-4 else 0)
+                continue
+            while k < n and workload[k].t <= now:     # every request that is due
+                req = workload[k]
+                ts = time.perf_counter() - t0
+                rid = srv.submit(*rings[req.N].args(req))
+                late[k] = max(ts - req.t, 0.0)
+                index[rid] = k
+                k += 1
+        th.join()
+    finally:
+        sys.setswitchinterval(switch)
     stats = srv.stats()
     srv.close()
     span = max(done_t[0] - t0, 1e-9)
     res = {"requests": n, "seconds": round(span, 4), "requests_per_s": round(n / span, 2),
            "offered_per_s": round(n / max(workload[-1].t, 1e-9), 2) if n else 0.0}
     res.update(percentiles([x for x in lat if x is not None]))
+    sv = [x for x in svc if x is not None]
+    if sv:
+        res["service_p99_ms"] = round(float(np.percentile(np.asarray(sv) * 1e3, 99)), 3)
+        res["max_lateness_ms"] = round(max(late) * 1e3, 3)
     res.update(stats)
     if keep_outputs:
         res["outputs"] = outs
