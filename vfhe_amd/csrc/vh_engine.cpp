@@ -1974,7 +1974,7 @@ static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<st
                 const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, nt.size() - off);
                 std::vector<vh_item> chunk(nt.begin() + off, nt.begin() + off + cnt);
                 vh_prof_scope ps_(VH_PROF_NTT_FP64);
-                VH_CHECK(vh_launch_ntt16(S(), vh_stage_vec(chunk), cnt, P->d_limbs, 0, logn, 0, logn - 8, 1, 1, 0));
+                VH_CHECK(vh_launch_ntt16(S(), vh_stage_vec(chunk), cnt, P->d_limbs, 0, logn, 0, logn - 8, 1, 2, 0));
             }
         } else {
             run_ntt(nt, P->d_limbs, P->h_limbs.data(), P->n, 0);

@@ -53,6 +53,8 @@ struct vh_mac_job {
 hipError_t vh_launch_ntt_pass(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                               int fp, int dir, int logn, int s0, int R, int col, int G, int first, int last);
 // Register radix-16 pass (FP64 path): 4 <= R <= 8, G = 4096 / 2^R sub-transforms per block.
+// first: 1 = the pass reads it.src; 2 = also, and many items share one source row
+// (read through the caches instead of streaming).
 hipError_t vh_launch_ntt16(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                            int dir, int logn, int s0, int R, int col, int first, int last);
 // Persistent two-pass NTT with per-XCD queues; ctl holds 16 + nitems zeroed uint32 counters.
