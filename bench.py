@@ -57,10 +57,10 @@ def parse():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=1.5, help="wall time per offered load")
     ap.add_argument("--only-ntt", action="store_true",
-                    help="run unit A alone (the command profiled with rocprofv3)")
+                    help="run the batched NTT round trip alone (the command profiled with rocprofv3)")
     ap.add_argument("--only-ckks", action="store_true", help="run the CKKS leg alone (profiling)")
     ap.add_argument("--only-serving", action="store_true", help="run the serving leg alone (profiling)")
-    ap.add_argument("--only-fused", action="store_true", help="run unit B alone (profiling)")
+    ap.add_argument("--only-fused", action="store_true", help="run the fused product z = INTT(NTT(x) NTT(y)) alone (profiling)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default NTT schedule")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also copy the 64-polynomial batch once per step (known-size copy for counter scaling)")
@@ -142,8 +142,8 @@ def cpu_baseline(primes, seconds):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only; run with --gpus 1 "
-              "(the multi-GPU sharding of the path is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the engine runs on a single device and has no "
+              "multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from vfhe_amd import _native, params, runtime as rt
@@ -181,7 +181,7 @@ def main():
         ax, ay, az = _native.voidp_array(polys), _native.voidp_array(ys), _native.voidp_array(zs)
         with Prof(lib) as fprof:
             fused_dt = timed(lib, lambda: lib.vfhe_hip_ntt_mul_intt_batch(az, ax, ay, POLYS), args.steps, args.warmup)
-        print(json.dumps({"unit_b_ms_per_batch": fused_dt * 1e3, "products_per_s": POLYS / fused_dt,
+        print(json.dumps({"fused_ms_per_batch": fused_dt * 1e3, "products_per_s": POLYS / fused_dt,
                           "ntt_ms": fprof.ms[0], "launches": fprof.launches[0]}), flush=True)
         return
 
@@ -189,7 +189,7 @@ def main():
         lib.vfhe_hip_ntt_batch(arr, POLYS, 0)
         lib.vfhe_hip_ntt_batch(arr, POLYS, 1)
 
-    log("unit A: warmup", args.warmup, "steps", args.steps)
+    log("round trip: warmup", args.warmup, "steps", args.steps)
     for _ in range(args.warmup):
         round_trip()
     lib.vfhe_hip_synchronize()
@@ -213,7 +213,7 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-    log(f"unit A: {dt * 1e3:.3f} ms/step, {value:.1f} round-trips/s, NTT launches {ntt_launches}, "
+    log(f"round trip: {dt * 1e3:.3f} ms/step, {value:.1f} round-trips/s, NTT launches {ntt_launches}, "
         f"avg {avg_launch_s * 1e3:.3f} ms, {achieved:.0f} GB/s algorithmic")
 
     if args.calibrate_copy:
@@ -224,17 +224,17 @@ def main():
         lib.vfhe_hip_synchronize()
         log(f"copy calibration: {args.steps} x {POLYS} copies of {L * N * 8} bytes")
     if args.only_ntt:
-        print(json.dumps({"schedule": sched, "unit_a_ms_per_step": dt * 1e3, "value": value, "ntt_launches": ntt_launches,
+        print(json.dumps({"schedule": sched, "round_trip_ms_per_step": dt * 1e3, "value": value, "ntt_launches": ntt_launches,
                           "avg_launch_ms": avg_launch_s * 1e3, "achieved_gbs": achieved}), flush=True)
         return
 
-    # fused product (unit B)
+    # fused product z = INTT(NTT(x) * NTT(y))
     xs = polys[:POLYS]
     ys = [pool.poly(rt.random_rows(rng, primes, N)) for _ in range(POLYS)]
     zs = [pool.poly() for _ in range(POLYS)]
     ax, ay, az = _native.voidp_array(xs), _native.voidp_array(ys), _native.voidp_array(zs)
     fused_dt = timed(lib, lambda: lib.vfhe_hip_ntt_mul_intt_batch(az, ax, ay, POLYS), args.steps, 2)
-    log(f"unit B: {fused_dt * 1e3:.3f} ms per 64 products")
+    log(f"fused product: {fused_dt * 1e3:.3f} ms per 64 products")
 
     # plain one-call-per-polynomial forms (device resident, and host resident)
     def plain_dev():

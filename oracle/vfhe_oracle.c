@@ -13,7 +13,7 @@
  *
  * How each part is pinned (tests/test_oracle.py):
  *  - number theory: the known answers of modules/arith/c/test/test_number_theory.c
- *    and the survey probe's psi values; Python big-int primality cross-check;
+ *    and the psi values of the reference's ntt_new_proc at n = 4096; Python big-int primality cross-check;
  *  - NTT forward: brute-force closed form NTT(x)[i] = sum_j x_j psi^{j(2 brv(i)+1)}
  *    and bit-exact agreement with the reference's own ntt_forward (oracle/_ref,
  *    both the portable and the avx512ifma builds) on every size/prime tested;

@@ -272,7 +272,7 @@ def test_ciphertext_multiplication(special_primes):
 
 
 def test_ciphertext_multiplication_dense_slots():
-    """All slots populated, uniform in the unit disk (BASELINE config 3's message distribution)."""
+    """All slots populated, uniform in the unit disk (the benchmarked CKKS message distribution)."""
     scheme = CKKS_Scheme(Ring(N, 300, split_degree=1), scaling_factor=2**40, special_primes=1)
     key = scheme.key_gen_sparse(N // 8, 3.2)
     s0 = key.poly[0]

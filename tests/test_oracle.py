@@ -68,7 +68,7 @@ def test_gen_prime_rule():
 
 
 def test_psi_known_answers():
-    """psi values the survey probe printed from the reference's ntt_new_proc at n = 4096."""
+    """psi values printed by the reference's own ntt_new_proc (built from its sources) at n = 4096."""
     assert O.lib().o_find_psi(4096, 562949953216513) == 169658170966992
     assert O.lib().o_find_psi(4096, 1125899906424833) == 371347710277771
 

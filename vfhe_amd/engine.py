@@ -39,7 +39,7 @@ def select(pin: str | None = None) -> str:
                 raise RuntimeError(f"VFHE_ENGINE={pin}, but this host lacks {requires}")
             return name
         known = ", ".join(n for n, _ in ENGINES)
-        raise RuntimeError(f"unknown VFHE_ENGINE '{pin}'; this build has {known}")
+        raise RuntimeError(f"unknown VFHE_ENGINE '{pin}'; known engines: {known}")
     for name, requires in ENGINES:
         if _installed(name) and _runnable(requires):
             return name
