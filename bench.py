@@ -163,7 +163,7 @@ def main():
     pool = rt.Pool(N, primes)
     tuned = None
     if not (args.no_autotune or args.only_ntt or args.only_fused or args.only_ckks or args.only_serving):
-        us = (ctypes.c_double * 6)()
+        us = (ctypes.c_double * 9)()
         best = lib.vfhe_hip_autotune_ntt(pool.h, POLYS * L, 2, us)
         tuned = {"chosen": int(best), "us_per_round_trip": [round(x, 1) for x in us]}
         sched = {k: int(lib.vfhe_hip_get_option(k.encode())) for k in sched}

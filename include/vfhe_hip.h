@@ -333,11 +333,12 @@ long vfhe_hip_get_option(const char *key);
 void vfhe_hip_chacha20_blocks(const uint32_t *key, uint64_t nonce, uint64_t ctr0, uint64_t nblocks, uint32_t *out);
 
 /* NTT schedule autotuner: times round trips of `rows` scratch rows of `ntt`'s
-   transform size under each candidate schedule (two launches over the whole
-   batch or 256 / 128 MiB sub-batches; the persistent per-XCD kernel with queue
-   lags 3, 5, 8 where available), keeps the fastest in the options above and
-   returns its index; us_out (6 entries, may be NULL) receives each
-   candidate's microseconds per round trip, -1 when not applicable. */
+   transform size under each candidate schedule (two launches of the pipelined
+   pass kernel, then of the plain one, each over the whole batch or 256 / 128 MiB
+   sub-batches; the persistent per-XCD kernel with queue lags 3, 5, 8 where
+   available), keeps the fastest in the options above and returns its index;
+   us_out (9 entries, may be NULL) receives each candidate's microseconds per
+   round trip, -1 when not applicable. */
 long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, double *us_out);
 
 /* Launch profiler: between start and stop every kernel launch is bracketed by

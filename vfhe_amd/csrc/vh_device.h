@@ -33,6 +33,11 @@ struct vh_limb {
     const double2 *tw_inv;    // FP64 path: {w, RN(w/q)} with w = psi^-brv(i) centred
     const ulonglong2 *twi_fwd;  // integer path: {w, floor(w 2^64 / q)}
     const ulonglong2 *twi_inv;
+    // FP64 path, n = 2^13 .. 2^16, s0 = log2(n) - 8: row factors {b, RN(b/q)} (b centred) with
+    //   tw[2^s + (sub << s') + local] == tw[2^s' + local] * b[sub][s']  (mod q),  s = s0 + s',
+    // for sub < 2^s0, s' < 8, local < 2^s'; [sub][s'] at index 8 sub + s'.  Null otherwise.
+    const double2 *rowf_fwd;
+    const double2 *rowf_inv;
 };
 
 // One unit of work: a row of n coefficients of one limb.

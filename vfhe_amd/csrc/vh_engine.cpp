@@ -100,6 +100,7 @@ struct vh_ctx {
     unsigned *d_err = nullptr;   // device error word written by kernels with bounded waits
     bool xcd_ok = false;         // the placement probe saw exactly the XCC ids 0..7
     unsigned xcd_hist[8] = {0};
+    unsigned pipe_blocks = 0;    // persistent workgroups of the pipelined NTT pass (CUs x occupancy)
 };
 static vh_ctx C;
 static thread_local int g_lane = 0;
@@ -125,7 +126,8 @@ static void sync_all_lanes()
 
 // Tunables (vfhe_hip_set_option): NTT schedule for N = 2^13..2^16 on the FP64 path.
 struct vh_opts {
-    long ntt_mode = 0;        // 0: two launches per direction, 1: persistent per-XCD kernel
+    long ntt_mode = 2;        // 0: two launches per direction, 1: persistent per-XCD kernel,
+                              // 2: two launches of the pipelined pass kernel
     long ntt_chunk_mb = 0;    // two-launch mode: rows per sub-batch (MiB), 0 = whole batch
     long xcd_lag = 3;         // persistent mode: queue slots between an item's two passes
     long xcd_occ = 2;         // persistent mode: workgroups per CU (2 or 4)
@@ -154,6 +156,7 @@ static void vh_ensure()
     (void)hipGetLastError();
     lane_init(C.lanes[0], (size_t)64 << 20);
     C.cus = prop.multiProcessorCount;
+    C.pipe_blocks = (unsigned)(C.cus * vh_ntt16p_occupancy());
     if (const char *e = getenv("VFHE_HIP_NTT_MODE")) OPT.ntt_mode = atol(e);
     if (const char *e = getenv("VFHE_HIP_NTT_CHUNK_MB")) OPT.ntt_chunk_mb = atol(e);
     if (const char *e = getenv("VFHE_HIP_XCD_LAG")) OPT.xcd_lag = atol(e);
@@ -504,18 +507,41 @@ extern "C" NTT_proc ntt_new_proc(uint64_t n, uint64_t q)
         a = mulmod_h(a, root, q);
         b = mulmod_h(b, p->inv_root_of_unity, q);
     }
-    const size_t bytes = 2 * n * 16;
+    // Row factors of the pipelined row pass (FP64 path, n = 2^13 .. 2^16).  With
+    // tw[i] = psi^brv(i), brv(2^s + k) = 2^(logn-1-s) + 2^(logn-s) brv_s(k) for k < 2^s, so
+    // for k = (sub << s') + local (s = s0 + s'):
+    //   tw[2^s + k] / tw[2^s' + local] = psi^(2^(logn-1-s) - 2^(logn-1-s') + 2^(logn-s) brv_s0(sub)).
+    const bool rowf = p->h.fp64 && logn >= 13 && logn <= 16;
+    const uint64_t s0 = rowf ? (uint64_t)logn - 8 : 0, nrow = rowf ? ((uint64_t)8 << s0) : 0;
+    const size_t bytes = 2 * n * 16 + 2 * nrow * 16;
     VH_CHECK(hipMalloc(&p->d_tab, bytes));
     if (p->h.fp64) {
-        std::vector<double2> t(2 * n);
+        std::vector<double2> t(2 * n + 2 * nrow);
         for (uint64_t i = 0; i < n; i++) {
             double wf = centred(fw[i], q), wi = centred(iw[i], q);
             t[i] = make_double2(wf, wf / (double)q);
             t[n + i] = make_double2(wi, wi / (double)q);
         }
+        for (uint64_t sub = 0; sub < ((uint64_t)1 << s0) && rowf; sub++) {
+            for (uint64_t sp = 0; sp < 8; sp++) {
+                const uint64_t s = s0 + sp, two_n = 2 * n;
+                const uint64_t e = ((n >> (s + 1)) + two_n - (n >> (sp + 1)) + (brv(sub, (int)s0) << (logn - s))) % two_n;
+                const uint64_t bf = power_mod(root, e, q), bi = power_mod(p->inv_root_of_unity, e, q);
+                for (uint64_t local = 0; local < ((uint64_t)1 << sp); local++) {   // the identity, checked
+                    const uint64_t k = (sub << sp) + local;
+                    VH_REQUIRE(fw[((uint64_t)1 << s) + k] == mulmod_h(fw[((uint64_t)1 << sp) + local], bf, q));
+                    VH_REQUIRE(iw[((uint64_t)1 << s) + k] == mulmod_h(iw[((uint64_t)1 << sp) + local], bi, q));
+                }
+                const double cf = centred(bf, q), ci = centred(bi, q);
+                t[2 * n + 8 * sub + sp] = make_double2(cf, cf / (double)q);
+                t[2 * n + nrow + 8 * sub + sp] = make_double2(ci, ci / (double)q);
+            }
+        }
         VH_CHECK(hipMemcpy(p->d_tab, t.data(), bytes, hipMemcpyHostToDevice));
         p->h.tw_fwd = reinterpret_cast<const double2 *>(p->d_tab);
         p->h.tw_inv = p->h.tw_fwd + n;
+        p->h.rowf_fwd = rowf ? p->h.tw_fwd + 2 * n : nullptr;
+        p->h.rowf_inv = rowf ? p->h.tw_fwd + 2 * n + nrow : nullptr;
     } else {
         std::vector<ulonglong2> t(2 * n);
         for (uint64_t i = 0; i < n; i++) {
@@ -599,10 +625,20 @@ static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const v
                 const int R1 = logn - 8;
                 const uint64_t chunk = OPT.ntt_chunk_mb > 0 ? (uint64_t)OPT.ntt_chunk_mb << 20 : UINT64_MAX;
                 const unsigned per = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(cnt, chunk / (n * 8)));
+                const bool pipe = OPT.ntt_mode == 2 && C.pipe_blocks > 0;
                 for (unsigned c0 = 0; c0 < cnt; c0 += per) {
                     const unsigned cc = std::min(per, cnt - c0);
                     const vh_item *ci = d_items + c0;
-                    if (dir == 0) {
+                    if (pipe) {
+                        const unsigned nb = C.pipe_blocks;
+                        if (dir == 0) {
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0, nb)); }
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1, nb)); }
+                        } else {
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0, nb)); }
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1, nb)); }
+                        }
+                    } else if (dir == 0) {
                         { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0)); }
                         { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1)); }
                     } else {
@@ -1967,7 +2003,7 @@ static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<st
         const int logn = ilog2(P->n);
         // Fused form: column pass here, then the row pass of every digit and the key
         // inner product in one kernel (the transformed digits stay in registers).
-        const bool fused_mac = fuse && OPT.ks_fused && OPT.ntt_mode == 0 && t->split_degree == 1 && Cc <= 2 &&
+        const bool fused_mac = fuse && OPT.ks_fused && OPT.ntt_mode != 1 && t->split_degree == 1 && Cc <= 2 &&
                                logn >= 13 && logn <= 16;
         if (fused_mac) {
             for (size_t off = 0; off < nt.size(); off += VH_MAX_GRID_Y) {
@@ -2484,7 +2520,7 @@ extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x
     }
     vh_pool *P = VPOOL(t);
     const int logn = ilog2(P->n);
-    bool fused = t->split_degree == 1 && OPT.ntt_mode == 0 && logn >= 13 && logn <= 16;
+    bool fused = t->split_degree == 1 && OPT.ntt_mode != 1 && logn >= 13 && logn <= 16;
     for (uint64_t i = 0; i < count && fused; i++)
         for_active(z[i]->rns_mask, t->l, [&](uint64_t li) { fused = fused && P->h_limbs[li].fp64; });
     if (fused) {
@@ -2505,8 +2541,11 @@ extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x
                 std::vector<vh_item> chunk(v.begin() + off, v.begin() + off + cnt);
                 const vh_item *d = vh_stage_vec(chunk);
                 vh_prof_scope ps_(VH_PROF_NTT_FP64);
-                if (which == 0) VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0));
+                const bool pipe = OPT.ntt_mode == 2 && C.pipe_blocks > 0;
+                if (which == 0 && pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0, C.pipe_blocks));
+                else if (which == 0) VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0));
                 else if (which == 1) VH_CHECK(vh_launch_ntt16_row_mul_inv(S(), d, cnt, P->d_limbs, logn));
+                else if (pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1, C.pipe_blocks));
                 else VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1));
             }
         };
@@ -2940,14 +2979,16 @@ extern "C" RNS_MLWE_Key mlwe_new_RNS_gaussian_key(uint64_t N, uint64_t r, uint64
 // ---------------------------------------------------------------------------
 // NTT schedule autotuner: times forward+inverse round trips of `rows` scratch
 // rows (limbs of `ntt` in turn) under each candidate schedule and keeps the
-// fastest in the engine options.  Candidates: two launches per transform over
-// the whole batch or sub-batches of 256 / 128 MiB, and (when the XCC placement
-// probe passed and 2^13 <= N <= 2^16) the persistent per-XCD kernel with queue
-// lags 3, 5 and 8.  us_out[k] receives candidate k's microseconds per round
-// trip (-1 if skipped); returns the index of the chosen candidate.
+// fastest in the engine options.  Candidates: two launches per transform of the
+// pipelined pass kernel, then of the plain pass kernel, each over the whole batch
+// or sub-batches of 256 / 128 MiB, and (when the XCC placement probe passed and
+// 2^13 <= N <= 2^16) the persistent per-XCD kernel with queue lags 3, 5 and 8.
+// us_out[k] receives candidate k's microseconds per round trip (-1 if skipped);
+// returns the index of the chosen candidate.
 // ---------------------------------------------------------------------------
 struct vh_sched { long mode, chunk_mb, lag, occ; };
-static const vh_sched VH_SCHEDS[] = {{0, 0, 3, 2}, {0, 256, 3, 2}, {0, 128, 3, 2}, {1, 0, 3, 2}, {1, 0, 5, 2}, {1, 0, 8, 2}};
+static const vh_sched VH_SCHEDS[] = {{2, 0, 3, 2}, {2, 256, 3, 2}, {2, 128, 3, 2}, {0, 0, 3, 2}, {0, 256, 3, 2},
+                                     {0, 128, 3, 2}, {1, 0, 3, 2}, {1, 0, 5, 2}, {1, 0, 8, 2}};
 
 extern "C" long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, double *us_out)
 {

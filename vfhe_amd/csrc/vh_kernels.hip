@@ -341,6 +341,189 @@ k_ntt16(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, in
 }
 
 // ---------------------------------------------------------------------------
+// Pipelined pass (FP64 path, N = 2^13 .. 2^16): the same stages, ownership and
+// arithmetic as ntt16_task, but a persistent workgroup walks the tiles of the
+// launch (tile u = G sub-transforms of row item u / tiles_per_item), issuing the
+// loads of its next tile before transforming the current one, and takes no
+// twiddle from global memory inside the transform, so those loads stay in
+// flight through it.  Twiddles come from the table head tw[0..255] staged in
+// LDS (column passes: s0 = 0 and sub-independent indices) and, in row passes,
+// from the product of a head entry and one of the row's 8 factors
+// (vh_limb::rowf_*): tw[2^s + (sub << s') + local] = tw[2^s' + local] * b[sub][s'].
+// A product is centred and paired with RN(w * RN(1/q)), whose relative error
+// (<= 2^-52) moves the quotient estimate by at most |b| / 2^53; the row passes
+// keep every operand below 3q (inverse sums are re-centred after every stage),
+// so each product stays within q of zero, as in ntt16_task.
+// ---------------------------------------------------------------------------
+template <int DIR, int R, bool COL, bool FIRST, bool LAST>
+__global__ void __launch_bounds__(256, 4)
+k_ntt16p(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, int logn, int s0,
+         unsigned tiles_per_item, unsigned ntiles)
+{
+    constexpr int S = 1 << R, TPS = S / 16, G = 256 / TPS;
+    __shared__ double lds[vh_ntt16_lds<R, COL>()];
+    __shared__ double2 head[256];
+    const int tid = threadIdx.x;
+    const int g = COL ? (tid % G) : (tid / TPS);
+    const int t = COL ? (tid / G) : (tid % TPS);
+    const size_t nsub = (size_t)1 << (logn - R);
+    // Element k of sub-transform sub lives at k * nsub + sub (column) or sub * S + k (row).  An
+    // access is split into a uniform row pointer (k's multiple of the ownership stride) and a
+    // 32-bit per-lane byte offset, so the 16 accesses of a thread share one offset register.
+    const unsigned kstep = COL ? (unsigned)nsub : 1u;
+    auto at = [&](auto *base, size_t sub, int kl, int ku) {
+        const uint32_t off = (uint32_t)((COL ? sub + (size_t)kl * nsub : sub * S + (size_t)kl) * 8);
+        using P = decltype(base);
+        return (P)((typename std::conditional<std::is_const<typename std::remove_pointer<P>::type>::value,
+                                              const VH_GLOBAL char, VH_GLOBAL char>::type *)(base + (size_t)ku * kstep) + off);
+    };
+    auto lidx = [&](int k) -> int { return COL ? k * (G + 1) + g : g * (S + S / 16) + k + (k >> 4); };
+    // raw[m] holds element 16t + m (inverse column pass: contiguous set) or t + TPS m (stride set)
+    constexpr bool CLOAD = DIR == 1 && COL;
+
+    uint64_t raw[16];
+    auto fetch = [&](unsigned uu) {
+        const vh_item &iu = items[uu / tiles_per_item];
+        const size_t su = (size_t)(uu % tiles_per_item) * G + g;
+        const VH_GLOBAL uint64_t *in = vh_global(FIRST ? iu.src : iu.dst);
+#pragma unroll
+        for (int m = 0; m < 16; m++) raw[m] = vh_ld(CLOAD ? at(in, su, 16 * t, m) : at(in, su, t, TPS * m));
+    };
+    unsigned u = blockIdx.x;
+    if (u < ntiles) fetch(u);
+    for (; u < ntiles; u += gridDim.x) {
+        const vh_item &it = items[u / tiles_per_item];
+        const vh_limb &L = limbs[it.limb];
+        const double q = L.qd, qinv = L.qinv;
+        const size_t sub = (size_t)(u % tiles_per_item) * G + g;
+        const VH_GLOBAL double *tw = vh_global(reinterpret_cast<const double *>(DIR == 0 ? L.tw_fwd : L.tw_inv));
+        VH_GLOBAL uint64_t *out = vh_global(it.dst);
+
+        __syncthreads();   // the previous tile's LDS reads are done
+        head[tid] = make_double2(tw[2 * tid], tw[2 * tid + 1]);
+        double2 bf[8];
+        if (!COL) {
+            const VH_GLOBAL double *rf =
+                vh_global(reinterpret_cast<const double *>(DIR == 0 ? L.rowf_fwd : L.rowf_inv)) + 16 * sub;
+#pragma unroll
+            for (int sp = 0; sp < 8; sp++) bf[sp] = make_double2(rf[2 * sp], rf[2 * sp + 1]);
+        }
+        __syncthreads();
+        double v[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = FIRST ? vh_u2d(raw[m]) : __longlong_as_double((long long)raw[m]);
+        __builtin_amdgcn_sched_barrier(0);
+        if (u + gridDim.x < ntiles) fetch(u + gridDim.x);
+        __builtin_amdgcn_sched_barrier(0);
+
+        auto twid = [&](int sp, int local) -> double2 {
+            const double2 h = head[(1 << sp) + local];
+            if (COL) return h;
+            const double w = vh_center(vh_mulmod_tw(h.x, bf[sp].x, bf[sp].y, q), q, qinv);
+            return make_double2(w, w * qinv);
+        };
+        auto finish = [&](double x) -> uint64_t {
+            return LAST ? vh_d2u(vh_canon(x, q, qinv)) : (uint64_t)__double_as_longlong(vh_center(x, q, qinv));
+        };
+        if (DIR == 0) {
+#pragma unroll
+            for (int sp = 0; sp < 4; sp++) {
+                const int hm = 8 >> sp;
+#pragma unroll
+                for (int m = 0; m < 16; m++) {
+                    if (m & hm) continue;
+                    const double2 w = twid(sp, m >> (4 - sp));
+                    const double x = vh_mulmod_tw(v[m + hm], w.x, w.y, q);
+                    const double u0 = v[m];
+                    v[m] = u0 + x;
+                    v[m + hm] = u0 - x;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 16; m++) lds[lidx(t + TPS * m)] = vh_center(v[m], q, qinv);
+            if (COL) __syncthreads(); else vh_wave_sync();
+#pragma unroll
+            for (int m = 0; m < 16; m++) v[m] = lds[lidx(16 * t + m)];
+#pragma unroll
+            for (int sp = 4; sp < R; sp++) {
+                const int hm = 1 << (R - sp - 1);
+#pragma unroll
+                for (int m = 0; m < 16; m++) {
+                    if (m & hm) continue;
+                    const double2 w = twid(sp, (16 * t + m) >> (R - sp));
+                    const double x = vh_mulmod_tw(v[m + hm], w.x, w.y, q);
+                    const double u0 = v[m];
+                    v[m] = u0 + x;
+                    v[m + hm] = u0 - x;
+                }
+            }
+            if (COL) {
+#pragma unroll
+                for (int m = 0; m < 16; m++) vh_st(at(out, sub, 16 * t, m), finish(v[m]));
+            } else {
+                vh_wave_sync();
+#pragma unroll
+                for (int m = 0; m < 16; m++) lds[lidx(16 * t + m)] = __longlong_as_double((long long)finish(v[m]));
+                vh_wave_sync();
+#pragma unroll
+                for (int m = 0; m < 16; m++)
+                    vh_st(at(out, sub, t, TPS * m), (uint64_t)__double_as_longlong(lds[lidx(t + TPS * m)]));
+            }
+        } else {
+            if (!COL) {   // row pass: regroup the stride loads into contiguous ownership
+#pragma unroll
+                for (int m = 0; m < 16; m++) lds[lidx(t + TPS * m)] = v[m];
+                vh_wave_sync();
+#pragma unroll
+                for (int m = 0; m < 16; m++) v[m] = lds[lidx(16 * t + m)];
+            }
+            int c = 0;
+#pragma unroll
+            for (int sp = R - 1; sp >= 4; sp--) {
+                const int hm = 1 << (R - sp - 1);
+#pragma unroll
+                for (int m = 0; m < 16; m++) {
+                    if (m & hm) continue;
+                    const double2 w = twid(sp, (16 * t + m) >> (R - sp));
+                    const double u0 = v[m], x = v[m + hm];
+                    double a = u0 + x;
+                    if (COL ? (c & 1) : 1) a = vh_center(a, q, qinv);
+                    v[m] = a;
+                    v[m + hm] = vh_mulmod_tw(u0 - x, w.x, w.y, q);
+                }
+                c++;
+            }
+            if (!COL) vh_wave_sync();
+#pragma unroll
+            for (int m = 0; m < 16; m++) lds[lidx(16 * t + m)] = v[m];
+            if (COL) __syncthreads(); else vh_wave_sync();
+#pragma unroll
+            for (int m = 0; m < 16; m++) v[m] = lds[lidx(t + TPS * m)];
+#pragma unroll
+            for (int sp = 3; sp >= 0; sp--) {
+                const int hm = 8 >> sp;
+#pragma unroll
+                for (int m = 0; m < 16; m++) {
+                    if (m & hm) continue;
+                    const double2 w = twid(sp, m >> (4 - sp));
+                    const double u0 = v[m], x = v[m + hm];
+                    double a = u0 + x;
+                    if (COL ? (c & 1) : 1) a = vh_center(a, q, qinv);
+                    v[m] = a;
+                    v[m + hm] = vh_mulmod_tw(u0 - x, w.x, w.y, q);
+                }
+                c++;
+            }
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                const double x = LAST ? vh_mulmod_tw(v[m], L.ninv_d, L.ninv_q, q) : v[m];
+                vh_st(at(out, sub, t, TPS * m), finish(x));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Key-switch inner product fused into the forward row pass (FP64 path,
 // N = 2^13..2^16, at most 2 output components).  The column pass has left the
 // lifted digit rows, transformed over their first logn-8 stages, in jb.modup;
@@ -1112,6 +1295,44 @@ hipError_t vh_launch_ntt16(hipStream_t st, const vh_item *items, unsigned nitems
 #undef VH_NTT16_R
 #undef VH_NTT16
     return hipGetLastError();
+}
+
+hipError_t vh_launch_ntt16p(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,
+                            int logn, int s0, int R, int col, int first, int last, unsigned nblocks)
+{
+    if (logn < 13 || logn > 16 || R < 5 || R > 8 || (!col && R != 8) || nblocks == 0) return hipErrorInvalidValue;
+    if (col ? (s0 != 0 || R != logn - 8) : (s0 != logn - 8)) return hipErrorInvalidValue;
+    const unsigned G = 256u >> (R - 4);
+    const unsigned nsub = 1u << (logn - R);
+    if (nsub % G) return hipErrorInvalidValue;
+    const unsigned tpi = nsub / G, ntiles = tpi * nitems;
+    const dim3 grid(nblocks < ntiles ? nblocks : ntiles);
+#define VH_P(D, RR, CC, FF, LL) hipLaunchKernelGGL((k_ntt16p<D, RR, CC, FF, LL>), grid, dim3(256), 0, st, items, limbs, logn, s0, tpi, ntiles)
+#define VH_P_R(D, CC, FF, LL) \
+    switch (R) { case 5: VH_P(D, 5, CC, FF, LL); break; case 6: VH_P(D, 6, CC, FF, LL); break; \
+                 case 7: VH_P(D, 7, CC, FF, LL); break; default: VH_P(D, 8, CC, FF, LL); break; }
+    if (dir == 0 && col && first && !last) { VH_P_R(0, true, true, false) }
+    else if (dir == 0 && !col && !first && last) { VH_P(0, 8, false, false, true); }
+    else if (dir == 1 && !col && first && !last) { VH_P(1, 8, false, true, false); }
+    else if (dir == 1 && col && !first && last) { VH_P_R(1, true, false, true) }
+    else return hipErrorInvalidValue;
+#undef VH_P_R
+#undef VH_P
+    return hipGetLastError();
+}
+
+// Workgroups of the pipelined pass one CU holds at once (the smallest over its instances).
+int vh_ntt16p_occupancy()
+{
+    int a = 0, b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_ntt16p<0, 8, false, false, true>, 256, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_ntt16p<1, 8, false, true, false>, 256, 0) != hipSuccess) return 0;
+    int m = a < b ? a : b;
+    int c = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_ntt16p<0, 8, true, true, false>, 256, 0) != hipSuccess) return 0;
+    m = m < c ? m : c;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_ntt16p<1, 8, true, false, true>, 256, 0) != hipSuccess) return 0;
+    return m < c ? m : c;
 }
 
 hipError_t vh_launch_ntt_xcd(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,
