@@ -208,11 +208,16 @@ def main():
     step_bytes = POLYS * L * 2 * 16 * N
     bytes_per_launch = step_bytes * args.steps / max(1, ntt_launches)
     achieved = bytes_per_launch / avg_launch_s / 1e9
+    # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE measurement: its ratio to
+    # the algorithmic bytes of the measured launch, applied to this run's launch size (the
+    # schedule may split the batch into sub-batches).
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "ntt_pass_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            t = json.load(f)
+        if t.get("hbm_bytes_per_launch") and t.get("algorithmic_bytes_per_launch"):
+            traffic = int(round(t["hbm_bytes_per_launch"] / t["algorithmic_bytes_per_launch"] * bytes_per_launch))
     log(f"round trip: {dt * 1e3:.3f} ms/step, {value:.1f} round-trips/s, NTT launches {ntt_launches}, "
         f"avg {avg_launch_s * 1e3:.3f} ms, {achieved:.0f} GB/s algorithmic")
 
