@@ -563,9 +563,33 @@ k_ntt16_row_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__
     const size_t sub = (size_t)by * G + g;
     const VH_GLOBAL double *tw = vh_global(reinterpret_cast<const double *>(L.tw_fwd));
     auto lidx = [&](int k) -> int { return g * (S + S / 16) + k + (k >> 4); };
+    // Twiddles do not depend on the digit: they are staged in LDS once, so the digit
+    // loop issues no twiddle load behind the next digit's prefetch.  rowt[g]: the 15
+    // stride-stage twiddles of row g; head: tw[0..255]; rowb[g]: row g's factors of
+    // stages 4..7 (contiguous-stage twiddle = head entry * factor, as in k_ntt16p).
+    __shared__ double2 head[256], rowt[G][15], rowb[G][4];
+    {
+        const int tid0 = threadIdx.x;
+        head[tid0] = make_double2(tw[2 * tid0], tw[2 * tid0 + 1]);
+        if (tid0 < G * 15) {
+            const int r = tid0 / 15, e = tid0 % 15;           // e = 2^sp - 1 + local
+            const int sp = 31 - __builtin_clz(e + 1), local = e + 1 - (1 << sp);
+            const size_t i = ((size_t)1 << (s0 + sp)) + (((size_t)by * G + r) << sp) + local;
+            rowt[r][e] = make_double2(tw[2 * i], tw[2 * i + 1]);
+        }
+        if (tid0 < G * 4) {
+            const int r = tid0 / 4, k = tid0 % 4;
+            const VH_GLOBAL double *rf = vh_global(reinterpret_cast<const double *>(L.rowf_fwd));
+            const size_t i = ((size_t)by * G + r) * 8 + 4 + k;
+            rowb[r][k] = make_double2(rf[2 * i], rf[2 * i + 1]);
+        }
+        __syncthreads();
+    }
     auto twid = [&](int sp, size_t local) -> double2 {
-        const size_t i = ((size_t)1 << (s0 + sp)) + (sub << sp) + local;
-        return make_double2(tw[2 * i], tw[2 * i + 1]);
+        if (sp < 4) return rowt[g][(1 << sp) - 1 + (int)local];
+        const double2 h = head[(1 << sp) + (int)local], b = rowb[g][sp - 4];
+        const double w = vh_center(vh_mulmod_tw(h.x, b.x, b.y, q), q, qinv);
+        return make_double2(w, w * qinv);
     };
     auto digit_row = [&](int d) -> const VH_GLOBAL uint64_t * {
         return vh_global(jb.modup + ((size_t)d * jb.next + ti) * n + sub * S);
