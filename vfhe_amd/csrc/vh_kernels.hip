@@ -619,6 +619,17 @@ k_ntt16_row_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__
 #pragma unroll
             for (int m = 0; m < 16; m++) raw[m] = vh_ld(in + t + TPS * m);
         }
+        // this digit's key words, issued with the prefetch so they too land during the butterflies
+        uint64_t kw[C][16];
+        {
+            const uint64_t *const *kp = kptr + jb.kptr_off + ((size_t)ti * D + d) * C;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                const VH_GLOBAL uint64_t *kr = vh_global(kp[c]) + x0;
+#pragma unroll
+                for (int m = 0; m < 16; m++) kw[c][m] = kr[m];
+            }
+        }
         // stages s' = 0..3, stride ownership
 #pragma unroll
         for (int sp = 0; sp < 4; sp++) {
@@ -653,17 +664,15 @@ k_ntt16_row_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__
                 v[m + hm] = u - x;
             }
         }
-        const uint64_t *const *kp = kptr + jb.kptr_off + ((size_t)ti * D + d) * C;
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            const VH_GLOBAL uint64_t *kr = vh_global(kp[c]) + x0;
 #pragma unroll
             for (int h = 0; h < 8; h++) {
 #pragma unroll
                 for (int e = 0; e < 2; e++) {
                     const int m = 2 * h + e;
                     const double a = vh_center(v[m], q, qinv);          // |a| <= q/2 + 1
-                    const double k = vh_u2d(kr[m]);
+                    const double k = vh_u2d(kw[c][m]);
                     const double hi = a * k;
                     const double lo = __fma_rn(a, k, -hi);
                     const double qt = __fma_rn(a, k * qinv, VH_MAGIC) - VH_MAGIC;
