@@ -382,38 +382,55 @@ k_ntt16p(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, i
     constexpr bool CLOAD = DIR == 1 && COL;
 
     uint64_t raw[16];
+    // Row passes: the 8 factors of each of the tile's 16 rows; thread 16 r + k (k < 8) loads
+    // factor k of row r, which lies in the wave that consumes it.
+    __shared__ double2 rowb[2][G][8];
+    double2 fnext = make_double2(0.0, 0.0);
     auto fetch = [&](unsigned uu) {
         const vh_item &iu = items[uu / tiles_per_item];
         const size_t su = (size_t)(uu % tiles_per_item) * G + g;
+        if (!COL && t < 8) {   // issued first: landed whenever the data has
+            const vh_limb &Lu = limbs[iu.limb];
+            const VH_GLOBAL double *rf = vh_global(reinterpret_cast<const double *>(DIR == 0 ? Lu.rowf_fwd : Lu.rowf_inv));
+            fnext = make_double2(rf[16 * su + 2 * t], rf[16 * su + 2 * t + 1]);
+        }
         const VH_GLOBAL uint64_t *in = vh_global(FIRST ? iu.src : iu.dst);
 #pragma unroll
         for (int m = 0; m < 16; m++) raw[m] = vh_ld(CLOAD ? at(in, su, 16 * t, m) : at(in, su, t, TPS * m));
     };
-    unsigned u = blockIdx.x;
-    if (u < ntiles) fetch(u);
-    for (; u < ntiles; u += gridDim.x) {
+    // A workgroup walks a contiguous range of tiles (items are sorted limb-major, so the
+    // limb, and with it the table head in LDS, changes rarely).  Nothing in the loop waits
+    // on a global load issued after the prefetch except at a change of limb.
+    const unsigned per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const unsigned u0 = blockIdx.x * per, u1 = u0 + per < ntiles ? u0 + per : ntiles;
+    unsigned cur_limb = ~0u;
+    if (u0 < u1) fetch(u0);
+    for (unsigned u = u0; u < u1; u++) {
         const vh_item &it = items[u / tiles_per_item];
         const vh_limb &L = limbs[it.limb];
         const double q = L.qd, qinv = L.qinv;
         const size_t sub = (size_t)(u % tiles_per_item) * G + g;
-        const VH_GLOBAL double *tw = vh_global(reinterpret_cast<const double *>(DIR == 0 ? L.tw_fwd : L.tw_inv));
         VH_GLOBAL uint64_t *out = vh_global(it.dst);
+        const int buf = (u - u0) & 1;
 
-        __syncthreads();   // the previous tile's LDS reads are done
-        head[tid] = make_double2(tw[2 * tid], tw[2 * tid + 1]);
-        double2 bf[8];
-        if (!COL) {
-            const VH_GLOBAL double *rf =
-                vh_global(reinterpret_cast<const double *>(DIR == 0 ? L.rowf_fwd : L.rowf_inv)) + 16 * sub;
-#pragma unroll
-            for (int sp = 0; sp < 8; sp++) bf[sp] = make_double2(rf[2 * sp], rf[2 * sp + 1]);
-        }
-        __syncthreads();
         double v[16];
 #pragma unroll
         for (int m = 0; m < 16; m++) v[m] = FIRST ? vh_u2d(raw[m]) : __longlong_as_double((long long)raw[m]);
+        if (!COL && t < 8) rowb[buf][g][t] = fnext;
+        __syncthreads();   // the previous tile's LDS reads are done; rowb[buf] is visible
+        if (it.limb != cur_limb) {   // uniform: reload the table head
+            cur_limb = it.limb;
+            const VH_GLOBAL double *tw = vh_global(reinterpret_cast<const double *>(DIR == 0 ? L.tw_fwd : L.tw_inv));
+            head[tid] = make_double2(tw[2 * tid], tw[2 * tid + 1]);
+            __syncthreads();
+        }
+        double2 bf[8];
+        if (!COL) {
+#pragma unroll
+            for (int sp = 0; sp < 8; sp++) bf[sp] = rowb[buf][g][sp];
+        }
         __builtin_amdgcn_sched_barrier(0);
-        if (u + gridDim.x < ntiles) fetch(u + gridDim.x);
+        if (u + 1 < u1) fetch(u + 1);
         __builtin_amdgcn_sched_barrier(0);
 
         auto twid = [&](int sp, int local) -> double2 {
