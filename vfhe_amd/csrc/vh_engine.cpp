@@ -1865,11 +1865,14 @@ extern "C" void mlwe_round_division_RNSc(RNSc_MLWE out, uint64_t divide_mask)
 extern "C" uint64_t mlwe_extended_rank(uint64_t r) { return r * (r + 3) / 2; }
 
 // mlwe_tensor_product (mlwe.c:665-709): quadratic slots q_ij (i <= j), linear slots, constant.
-static void tensor_into(const std::vector<RNS_Polynomial> &out, RNS_MLWE in1, RNS_MLWE in2)
+// Appends the products of one tensor product (mlwe.c:665-708) to `first` (plain
+// products) and `second` (products added to a slot of `first`), so a batch of
+// tensor products runs as two launches.
+static void tensor_reqs(const std::vector<RNS_Polynomial> &out, RNS_MLWE in1, RNS_MLWE in2,
+                        std::vector<vh_mulreq> &first, std::vector<vh_mulreq> &second)
 {
     const uint64_t r = in1->r, R = mlwe_extended_rank(r);
     VH_REQUIRE(in2->r == r && out.size() == R + 1);
-    std::vector<vh_mulreq> first, second;
     auto m = [](RNS_Polynomial o, RNS_Polynomial a, RNS_Polynomial b) {
         dev_ready(a); dev_ready(b); dev_ready(o);
         o->rns_mask = a->rns_mask & b->rns_mask;
@@ -1888,6 +1891,12 @@ static void tensor_into(const std::vector<RNS_Polynomial> &out, RNS_MLWE in1, RN
         k++;
     }
     first.push_back(m(out[R], in1->b, in2->b));
+}
+
+static void tensor_into(const std::vector<RNS_Polynomial> &out, RNS_MLWE in1, RNS_MLWE in2)
+{
+    std::vector<vh_mulreq> first, second;
+    tensor_reqs(out, in1, in2, first, second);
     run_muls(first, 0);
     run_muls(second, 1);
 }
@@ -2250,11 +2259,14 @@ static void multiply_batch(const std::vector<RNS_MLWE> &outs, const std::vector<
     VH_REQUIRE(B > 0 && in1.size() == B && in2.size() == B && ksks.size() == B && divide_masks.size() == B);
     const uint64_t N = in1[0]->b->ntt->N, r = in1[0]->r, R = mlwe_extended_rank(r);
     incNTT t = in1[0]->b->ntt;
+    std::vector<vh_mulreq> first, second;
     if (!ksks[0]) {
         for (size_t bi = 0; bi < B; bi++) {
             VH_REQUIRE(!ksks[bi] && divide_masks[bi] == 0 && outs[bi]->r == R);
-            tensor_into(comps(outs[bi]), in1[bi], in2[bi]);
+            tensor_reqs(comps(outs[bi]), in1[bi], in2[bi], first, second);
         }
+        run_muls(first, 0);
+        run_muls(second, 1);
         return;
     }
     std::vector<RNS_MLWE> ext(B);
@@ -2262,9 +2274,11 @@ static void multiply_batch(const std::vector<RNS_MLWE> &outs, const std::vector<
     for (size_t bi = 0; bi < B; bi++) {
         VH_REQUIRE(ksks[bi] && outs[bi]->r == r && in1[bi]->r == r && in1[bi]->b->ntt == t);
         ext[bi] = alloc_sample_raw(N, R, in1[bi]->b->rns_mask, t);
-        tensor_into(comps(ext[bi]), in1[bi], in2[bi]);
+        tensor_reqs(comps(ext[bi]), in1[bi], in2[bi], first, second);
         for (auto p : comps(ext[bi])) pairs.push_back({p, p});
     }
+    run_muls(first, 0);
+    run_muls(second, 1);
     ntt_rows(pairs, 1);
     keyswitch_batch(outs, ext, ksks);
     std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> o;
@@ -2405,20 +2419,38 @@ static void cmux_batch(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_
 {
     const size_t B = outs.size();
     VH_REQUIRE(B > 0 && in1.size() == B && in2.size() == B && mgsws.size() == B);
+    incNTT t = in1[0]->b->ntt;
+    const uint64_t N = t->N;
     std::vector<RNS_MLWE> diff(B), in1n(B);
     std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> fw;
+    std::vector<vh_item> subs, copies, adds;   // one launch each over the whole batch
     for (size_t bi = 0; bi < B; bi++) {
-        const uint64_t N = in1[bi]->b->ntt->N, r = in1[bi]->r, mask = in1[bi]->b->rns_mask;
-        incNTT t = in1[bi]->b->ntt;
+        VH_REQUIRE(in1[bi]->b->ntt == t && in2[bi]->b->ntt == t && in2[bi]->r == in1[bi]->r);
+        const uint64_t r = in1[bi]->r, mask = in1[bi]->b->rns_mask;
         diff[bi] = alloc_sample_raw(N, r, mask, t);
-        mlwe_sub_RNSc_sample(diff[bi], in2[bi], in1[bi]);
         in1n[bi] = alloc_sample_raw(N, r, mask, t);
-        mlwe_copy_RNS_sample(in1n[bi], in1[bi]);
-        for (auto p : comps(in1n[bi])) fw.push_back({p, p});
+        const auto d = comps(diff[bi]), x = comps(in2[bi]), y = comps(in1[bi]), c = comps(in1n[bi]);
+        for (size_t k = 0; k < d.size(); k++) {
+            dev_ready(x[k]); dev_ready(y[k]);
+            d[k]->rns_mask = x[k]->rns_mask & y[k]->rns_mask;   // diff = in2 - in1
+            for_active(d[k]->rns_mask, t->l, [&](uint64_t i) { subs.push_back(mk_item(drow(x[k], i), drow(d[k], i), (uint32_t)i, drow(y[k], i))); });
+            c[k]->rns_mask = y[k]->rns_mask;                    // in1n = in1, to be transformed
+            for_active(c[k]->rns_mask, t->l, [&](uint64_t i) { copies.push_back(mk_item(drow(y[k], i), drow(c[k], i), (uint32_t)i)); });
+            fw.push_back({c[k], c[k]});
+        }
     }
+    run_eltwise(subs, DL(diff[0]->b), VH_OP_SUB, N);
+    run_eltwise(copies, DL(diff[0]->b), VH_OP_COPY, N);
     external_product_batch(outs, diff, mgsws, ell, special_primes);
     ntt_rows(fw, 0);
-    for (size_t bi = 0; bi < B; bi++) mlwe_add_RNSc_sample(outs[bi], outs[bi], in1n[bi]);
+    for (size_t bi = 0; bi < B; bi++) {   // out += NTT(in1)
+        const auto o = comps(outs[bi]), c = comps(in1n[bi]);
+        for (size_t k = 0; k < o.size(); k++) {
+            o[k]->rns_mask &= c[k]->rns_mask;
+            for_active(o[k]->rns_mask, t->l, [&](uint64_t i) { adds.push_back(mk_item(drow(o[k], i), drow(o[k], i), (uint32_t)i, drow(c[k], i))); });
+        }
+    }
+    run_eltwise(adds, DL(diff[0]->b), VH_OP_ADD, N);
     for (size_t bi = 0; bi < B; bi++) { free_mlwe_RNS_sample(diff[bi]); free_mlwe_RNS_sample(in1n[bi]); }
 }
 
