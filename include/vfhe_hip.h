@@ -315,12 +315,17 @@ int vfhe_hip_last_error(void);
 const char *vfhe_hip_device_name(void);
 
 /* Tunables, also readable from the environment at first use (VFHE_HIP_NTT_MODE,
-   VFHE_HIP_NTT_CHUNK_MB, VFHE_HIP_XCD_LAG, VFHE_HIP_XCD_OCC):
-     "ntt.mode"          0: two launches per direction (default); 1: persistent per-XCD kernel
-     "ntt.chunk_mb"      two-launch mode: MiB of rows per sub-batch, 0 = whole batch
+   VFHE_HIP_NTT_CHUNK_MB, VFHE_HIP_XCD_LAG, VFHE_HIP_XCD_OCC, VFHE_HIP_KS_FUSED,
+   VFHE_HIP_FFT_FMA):
+     "ntt.mode"          2: two launches per direction of the pipelined pass kernel (default);
+                         0: two launches of the plain pass kernel; 1: persistent per-XCD kernel
+     "ntt.chunk_mb"      two-launch modes: MiB of rows per sub-batch, 0 = whole batch
      "ntt.xcd_lag"       persistent mode: queue slots between the two passes of a transform
      "ntt.xcd_occupancy" persistent mode: workgroups per CU, 2 or 4
      "ntt.spin_limit"    persistent mode: bound on a wait; exceeding it aborts at the next sync
+     "ks.fused"          1: key-switch forward row pass fused with the key inner product (default)
+     "fft.fma"           1: CKKS encoder FFT with the avx512ifma engine's fused complex product
+                         (default), 0: the portable engine's
    get also answers "ntt.xcd_available" (1 when the placement probe found XCC ids 0..7)
    and "probe.xcd0".."probe.xcd7" (workgroups of the probe placed on each XCC).
    set returns 0, or -1 for an unknown key; get returns -1 for an unknown key. */
