@@ -1893,6 +1893,42 @@ static void tensor_reqs(const std::vector<RNS_Polynomial> &out, RNS_MLWE in1, RN
     first.push_back(m(out[R], in1->b, in2->b));
 }
 
+// Rank-1 tensor products whose four inputs share one limb mask, in one launch
+// (k_tensor1); returns false, doing nothing, when some job does not qualify.
+static bool tensor1_batch(const std::vector<std::vector<RNS_Polynomial>> &outs, const std::vector<RNS_MLWE> &in1,
+                          const std::vector<RNS_MLWE> &in2)
+{
+    for (size_t bi = 0; bi < in1.size(); bi++) {
+        const uint64_t m = in1[bi]->b->rns_mask;
+        if (in1[bi]->r != 1 || in2[bi]->r != 1 || in1[bi]->a[0]->rns_mask != m || in2[bi]->b->rns_mask != m ||
+            in2[bi]->a[0]->rns_mask != m || in1[bi]->b->ntt != in1[0]->b->ntt || in2[bi]->b->ntt != in1[0]->b->ntt)
+            return false;
+    }
+    incNTT t = in1[0]->b->ntt;
+    std::vector<vh_tensor_job> jobs;
+    for (size_t bi = 0; bi < in1.size(); bi++) {
+        RNS_Polynomial a1 = in1[bi]->a[0], b1 = in1[bi]->b, a2 = in2[bi]->a[0], b2 = in2[bi]->b;
+        for (RNS_Polynomial p : {a1, b1, a2, b2}) dev_ready(p);
+        const uint64_t m = b1->rns_mask;
+        for (int k = 0; k < 3; k++) {
+            dev_ready(outs[bi][k]);
+            outs[bi][k]->rns_mask = m;
+            check_rows(outs[bi][k], m);
+        }
+        for_active(m, t->l, [&](uint64_t i) {
+            jobs.push_back({drow(a1, i), drow(b1, i), drow(a2, i), drow(b2, i), drow(outs[bi][0], i),
+                            drow(outs[bi][1], i), drow(outs[bi][2], i), (uint32_t)i});
+        });
+    }
+    for (size_t off = 0; off < jobs.size(); off += VH_MAX_GRID_Y) {
+        const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, jobs.size() - off);
+        std::vector<vh_tensor_job> chunk(jobs.begin() + off, jobs.begin() + off + cnt);
+        vh_prof_scope ps_(VH_PROF_ELTWISE);
+        VH_CHECK(vh_launch_tensor1(S(), vh_stage_vec(chunk), cnt, DL(in1[0]->b), t->N));
+    }
+    return true;
+}
+
 static void tensor_into(const std::vector<RNS_Polynomial> &out, RNS_MLWE in1, RNS_MLWE in2)
 {
     std::vector<vh_mulreq> first, second;
@@ -2260,13 +2296,19 @@ static void multiply_batch(const std::vector<RNS_MLWE> &outs, const std::vector<
     const uint64_t N = in1[0]->b->ntt->N, r = in1[0]->r, R = mlwe_extended_rank(r);
     incNTT t = in1[0]->b->ntt;
     std::vector<vh_mulreq> first, second;
+    std::vector<std::vector<RNS_Polynomial>> slots(B);
+    auto tensors = [&]() {
+        if (tensor1_batch(slots, in1, in2)) return;
+        for (size_t bi = 0; bi < B; bi++) tensor_reqs(slots[bi], in1[bi], in2[bi], first, second);
+        run_muls(first, 0);
+        run_muls(second, 1);
+    };
     if (!ksks[0]) {
         for (size_t bi = 0; bi < B; bi++) {
             VH_REQUIRE(!ksks[bi] && divide_masks[bi] == 0 && outs[bi]->r == R);
-            tensor_reqs(comps(outs[bi]), in1[bi], in2[bi], first, second);
+            slots[bi] = comps(outs[bi]);
         }
-        run_muls(first, 0);
-        run_muls(second, 1);
+        tensors();
         return;
     }
     std::vector<RNS_MLWE> ext(B);
@@ -2274,11 +2316,10 @@ static void multiply_batch(const std::vector<RNS_MLWE> &outs, const std::vector<
     for (size_t bi = 0; bi < B; bi++) {
         VH_REQUIRE(ksks[bi] && outs[bi]->r == r && in1[bi]->r == r && in1[bi]->b->ntt == t);
         ext[bi] = alloc_sample_raw(N, R, in1[bi]->b->rns_mask, t);
-        tensor_reqs(comps(ext[bi]), in1[bi], in2[bi], first, second);
-        for (auto p : comps(ext[bi])) pairs.push_back({p, p});
+        slots[bi] = comps(ext[bi]);
+        for (auto p : slots[bi]) pairs.push_back({p, p});
     }
-    run_muls(first, 0);
-    run_muls(second, 1);
+    tensors();
     ntt_rows(pairs, 1);
     keyswitch_batch(outs, ext, ksks);
     std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> o;

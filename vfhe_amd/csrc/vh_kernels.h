@@ -20,6 +20,13 @@ enum {
     VH_OP_COPY, VH_OP_ZERO, VH_OP_REDUCE_MP, VH_OP_ADD_AT0
 };
 
+// One limb row of a rank-1 tensor product: d0 = a1 a2, d1 = a1 b2 + b1 a2, d2 = b1 b2.
+struct vh_tensor_job {
+    const uint64_t *a1, *b1, *a2, *b2;
+    uint64_t *d0, *d1, *d2;
+    uint32_t limb;
+};
+
 // One polynomial's sequential division job.
 struct vh_div_job {
     uint64_t *base;      // device rows of the polynomial, limb-major
@@ -69,6 +76,8 @@ hipError_t vh_launch_ntt_xcd(hipStream_t st, const vh_item *items, unsigned nite
 hipError_t vh_launch_ntt_xcd_verify(hipStream_t st, const unsigned *ctl, unsigned nitems, int logn, unsigned *err);
 hipError_t vh_launch_xcc_probe(hipStream_t st, unsigned *out, unsigned nblocks);
 hipError_t vh_launch_zero_u32(hipStream_t st, unsigned *p, unsigned n);
+hipError_t vh_launch_tensor1(hipStream_t st, const vh_tensor_job *jobs, unsigned njobs, const vh_limb *limbs,
+                             uint64_t n);
 hipError_t vh_launch_eltwise(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                              int op, uint64_t n);
 hipError_t vh_launch_split_mul(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,

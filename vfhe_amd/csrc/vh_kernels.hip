@@ -1263,6 +1263,31 @@ hipError_t vh_launch_ntt_pass(hipStream_t st, const vh_item *items, unsigned nit
     return hipGetLastError();
 }
 
+// Rank-1 tensor product of one limb row (mlwe.c:665-708): d0 = a1 a2,
+// d1 = a1 b2 + b1 a2, d2 = b1 b2, in one pass over the four inputs (the
+// reference's slot order and arithmetic; canonical results).
+__global__ void __launch_bounds__(256)
+k_tensor1(const vh_tensor_job *__restrict__ jobs, const vh_limb *__restrict__ limbs, uint64_t n)
+{
+    const vh_tensor_job jb = jobs[blockIdx.y];
+    const vh_limb L = limbs[jb.limb];
+    const uint64_t q = L.q, mu = L.mu64;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t a1 = vh_reduce64(jb.a1[i], q, mu), b1 = vh_reduce64(jb.b1[i], q, mu);
+        const uint64_t a2 = vh_reduce64(jb.a2[i], q, mu), b2 = vh_reduce64(jb.b2[i], q, mu);
+        jb.d0[i] = vh_mulmod(a1, a2, L);
+        jb.d1[i] = vh_addmod(vh_mulmod(a1, b2, L), vh_mulmod(b1, a2, L), q);
+        jb.d2[i] = vh_mulmod(b1, b2, L);
+    }
+}
+
+hipError_t vh_launch_tensor1(hipStream_t st, const vh_tensor_job *jobs, unsigned njobs, const vh_limb *limbs,
+                             uint64_t n)
+{
+    hipLaunchKernelGGL(k_tensor1, dim3(vh_blocks(n, 256, 64), njobs), dim3(256), 0, st, jobs, limbs, n);
+    return hipGetLastError();
+}
+
 hipError_t vh_launch_eltwise(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                              int op, uint64_t n)
 {
