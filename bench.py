@@ -297,7 +297,10 @@ def main():
                    "N": N, "L": L, "polys": POLYS, "schedule": sched, "autotune": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_ntt16 (FP64 radix-16 column and row passes)", "launches": ntt_launches,
+                     "kernel": {2: "k_ntt16p (pipelined FP64 radix-16 column and row passes)",
+                                0: "k_ntt16 (FP64 radix-16 column and row passes)",
+                                1: "k_ntt_xcd (persistent per-XCD column and row passes)"}.get(sched["ntt.mode"], "?"),
+                     "launches": ntt_launches,
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                      "algorithmic_bytes_per_launch": int(bytes_per_launch)},
         "cpu_baseline": cpu,
