@@ -1418,6 +1418,33 @@ extern "C" void polynomial_RNSc_mod_reduce(RNSc_Polynomial out, RNSc_Polynomial 
 // ---------------------------------------------------------------------------
 struct vh_divc_h { uint64_t h, inv, inv_sh; };
 
+// Device table of division constants for dividing the limbs of `mask` by the
+// primes `divs` (= the bits of `chunk`, ascending): entry [k][i] = {half of p_k
+// mod q_i (0 for floor division), p_k^-1 mod q_i, its Shoup factor}.  Cached
+// per pool for the lifetime of the pool's prime list.
+static const void *div_consts(incNTT t, uint64_t mask, uint64_t chunk, const std::vector<int> &divs, bool round)
+{
+    const uint64_t l = t->l;
+    void *&d_consts = VPOOL(t)->div_consts[std::make_tuple(mask, chunk, round ? 1 : 0)];
+    if (!d_consts) {
+        std::vector<vh_divc_h> consts(divs.size() * l);
+        for (size_t k = 0; k < divs.size(); k++) {
+            const uint64_t p = t->ntt[divs[k]]->q, half = round ? p / 2 : 0;
+            for (uint64_t i = 0; i < l; i++) {
+                if (!ACTIVE(mask, i)) continue;
+                const uint64_t q = t->ntt[i]->q;
+                vh_divc_h &c = consts[k * l + i];
+                c.h = half % q;
+                c.inv = (i == (uint64_t)divs[k]) ? 0 : inverse_mod(p, q);
+                c.inv_sh = shoup_of(c.inv, q);
+            }
+        }
+        VH_CHECK(hipMalloc(&d_consts, consts.size() * sizeof(vh_divc_h)));
+        VH_CHECK(hipMemcpy(d_consts, consts.data(), consts.size() * sizeof(vh_divc_h), hipMemcpyHostToDevice));
+    }
+    return d_consts;
+}
+
 static void run_division(const std::vector<RNS_Polynomial> &polys, uint64_t divide_mask, bool round)
 {
     if (polys.empty()) return;
@@ -1440,23 +1467,7 @@ static void run_division(const std::vector<RNS_Polynomial> &polys, uint64_t divi
             for (int d : divs) chunk |= 1ULL << d;
             std::vector<int> outs;
             for (uint64_t i = 0; i < l; i++) if (ACTIVE(mask, i) && !ACTIVE(chunk, i)) outs.push_back((int)i);
-            void *&d_consts = VPOOL(t)->div_consts[std::make_tuple(mask, chunk, round ? 1 : 0)];
-            if (!d_consts) {
-                std::vector<vh_divc_h> consts(divs.size() * l);
-                for (size_t k = 0; k < divs.size(); k++) {
-                    const uint64_t p = t->ntt[divs[k]]->q, half = round ? p / 2 : 0;
-                    for (uint64_t i = 0; i < l; i++) {
-                        if (!ACTIVE(mask, i)) continue;
-                        const uint64_t q = t->ntt[i]->q;
-                        vh_divc_h &c = consts[k * l + i];
-                        c.h = half % q;
-                        c.inv = (i == (uint64_t)divs[k]) ? 0 : inverse_mod(p, q);
-                        c.inv_sh = shoup_of(c.inv, q);
-                    }
-                }
-                VH_CHECK(hipMalloc(&d_consts, consts.size() * sizeof(vh_divc_h)));
-                VH_CHECK(hipMemcpy(d_consts, consts.data(), consts.size() * sizeof(vh_divc_h), hipMemcpyHostToDevice));
-            }
+            const void *d_consts = div_consts(t, mask, chunk, divs, round);
             std::vector<vh_div_job> jobs;
             for (auto p : g.second) {
                 vh_div_job jb;
@@ -2138,8 +2149,12 @@ static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<st
 // mlwe_RNSc_GHS_hybrid_keyswitch (mlwe.c:557-598) for a batch whose jobs may
 // differ in level (masks) and key: job i switches ins[i] with ksks[i].  All jobs
 // share the ring and the output rank.
+// rescale (optional, per job): after the key-switch, also round-divide by these
+// primes (the CKKS rescale of mlwe_multiply's caller, ckks.py:128-148).  A job
+// whose rescale is one prime is finished by k_moddown_rescale: ModDown, the
+// pass-through add and the rescale in one pass over its rows.
 static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &ins,
-                            const std::vector<RNS_MLWE **> &ksks)
+                            const std::vector<RNS_MLWE **> &ksks, const std::vector<uint64_t> &rescale = {})
 {
     const size_t B = outs.size();
     VH_REQUIRE(B == ins.size() && B == ksks.size() && B > 0);
@@ -2164,24 +2179,98 @@ static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector
             by_divide[p->rns_mask & ~ins[bi]->b->rns_mask].push_back(p);
         }
     ntt_rows(pairs, 1);
-    for (auto &g : by_divide)
-        if (g.first) run_division(g.second, g.first, true);
 
-    // Pass-through components keep their key: added after the rescale (mlwe.c:588-597).
-    std::vector<vh_item> adds;
+    // Pass-through components keep their key: added after the ModDown (mlwe.c:588-597).
+    // pass[bi][c]: the input added to output component c (null: none).
+    std::vector<std::vector<RNS_Polynomial>> pass(B);
     for (size_t bi = 0; bi < B; bi++) {
+        const auto oc = comps(outs[bi]);
+        pass[bi].assign(oc.size(), nullptr);
         uint64_t keep = 0;
-        std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pt;
-        for (uint64_t i = 0; i < ins[bi]->r; i++) if (!ksks[bi][i]) pt.push_back({outs[bi]->a[keep++], ins[bi]->a[i]});
-        pt.push_back({outs[bi]->b, ins[bi]->b});
-        for (auto &pr : pt) {
-            RNS_Polynomial o = pr.first, x = pr.second;
+        for (uint64_t i = 0; i < ins[bi]->r; i++) if (!ksks[bi][i]) pass[bi][keep++] = ins[bi]->a[i];
+        pass[bi][oc.size() - 1] = ins[bi]->b;
+    }
+    // Jobs finished in one pass: a one-prime rescale, at most VH_MAX_DIV special primes,
+    // one mask for every output component, pass-through inputs covering it.
+    std::vector<char> fused(B, 0);
+    for (size_t bi = 0; bi < B && !rescale.empty(); bi++) {
+        const auto oc = comps(outs[bi]);
+        const uint64_t m = oc[0]->rns_mask, down = m & ~ins[bi]->b->rns_mask, keep = m & ~down;
+        bool ok = __builtin_popcountll(rescale[bi]) == 1 && (rescale[bi] & keep) && down &&
+                  __builtin_popcountll(down) <= VH_MAX_DIV;
+        for (size_t c = 0; c < oc.size() && ok; c++)
+            ok = oc[c]->rns_mask == m && (!pass[bi][c] || (pass[bi][c]->rns_mask & keep) == keep);
+        fused[bi] = ok;
+    }
+    for (auto &g : by_divide) {
+        std::vector<RNS_Polynomial> rest;
+        for (auto p : g.second) {
+            bool f = false;
+            for (size_t bi = 0; bi < B && !f; bi++)
+                if (fused[bi]) for (auto o : comps(outs[bi])) f = f || o == p;
+            if (!f) rest.push_back(p);
+        }
+        if (g.first) run_division(rest, g.first, true);
+    }
+    std::vector<vh_item> adds;
+    std::vector<vh_mdr_job> mdr;
+    std::vector<vh_item> zero;
+    uint64_t mdr_mask = 0, mdr_down = 0, mdr_resc = 0;
+    for (size_t bi = 0; bi < B; bi++) {
+        const auto oc = comps(outs[bi]);
+        if (fused[bi]) {
+            const uint64_t m = oc[0]->rns_mask, down = m & ~ins[bi]->b->rns_mask, keep = m & ~down;
+            if (mdr.empty()) { mdr_mask = m; mdr_down = down; mdr_resc = rescale[bi]; }
+            if (m != mdr_mask || down != mdr_down || rescale[bi] != mdr_resc) { fused[bi] = 0; }
+            else {
+                for (size_t c = 0; c < oc.size(); c++) {
+                    vh_mdr_job jb;
+                    memset(&jb, 0, sizeof(jb));
+                    jb.base = VP(oc[c])->d;
+                    jb.add = pass[bi][c] ? VP(pass[bi][c])->d : nullptr;
+                    jb.stride = N;
+                    jb.nl = (int)l;
+                    for (uint64_t i = 0; i < l; i++) if (ACTIVE(down, i)) jb.div_limb[jb.K++] = (int)i;
+                    jb.resc = __builtin_ctzll(rescale[bi]);
+                    for (uint64_t i = 0; i < l; i++)
+                        if (ACTIVE(keep, i) && !ACTIVE(rescale[bi], i)) jb.out_limb[jb.nout++] = (int)i;
+                    mdr.push_back(jb);
+                    for_active(down | rescale[bi], l, [&](uint64_t i) { zero.push_back(mk_item(nullptr, drow(oc[c], i), (uint32_t)i)); });
+                    oc[c]->rns_mask = keep & ~rescale[bi];
+                }
+                continue;
+            }
+            // a job of another mask or prime set: the sequential path (rare)
+            run_division(oc, down, true);
+        }
+        for (size_t c = 0; c < oc.size(); c++) {
+            RNS_Polynomial o = oc[c], x = pass[bi][c];
+            if (!x) continue;
             const uint64_t m = o->rns_mask & x->rns_mask;
             o->rns_mask = m;
             for_active(m, l, [&](uint64_t i) { adds.push_back(mk_item(drow(o, i), drow(o, i), (uint32_t)i, drow(x, i))); });
         }
     }
     run_eltwise(adds, P->d_limbs, VH_OP_ADD, N);
+    if (!mdr.empty()) {
+        std::vector<int> dv, rv = {__builtin_ctzll(mdr_resc)};
+        for (uint64_t i = 0; i < l; i++) if (ACTIVE(mdr_down, i)) dv.push_back((int)i);
+        incNTT t = outs[0]->b->ntt;
+        const void *cd = div_consts(t, mdr_mask, mdr_down, dv, true);
+        const void *cr = div_consts(t, mdr_mask & ~mdr_down, mdr_resc, rv, true);
+        for (size_t off = 0; off < mdr.size(); off += VH_MAX_GRID_Y) {
+            const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, mdr.size() - off);
+            std::vector<vh_mdr_job> chunk(mdr.begin() + off, mdr.begin() + off + cnt);
+            vh_prof_scope ps_(VH_PROF_DIVISION);
+            VH_CHECK(vh_launch_moddown_rescale(S(), vh_stage_vec(chunk), cnt, P->d_limbs, cd, cr, N, (int)dv.size()));
+        }
+        run_eltwise(zero, P->d_limbs, VH_OP_ZERO, N);
+    }
+    // jobs with a rescale not done above
+    std::map<uint64_t, std::vector<RNS_Polynomial>> resc_groups;
+    for (size_t bi = 0; bi < B && !rescale.empty(); bi++)
+        if (!fused[bi] && rescale[bi]) for (auto o : comps(outs[bi])) resc_groups[rescale[bi]].push_back(o);
+    for (auto &g : resc_groups) run_division(g.second, g.first, true);
 }
 
 extern "C" void mlwe_RNSc_GHS_hybrid_keyswitch(RNSc_MLWE out, RNSc_MLWE in, RNS_MLWE **ksk, uint64_t lvl)
@@ -2321,16 +2410,12 @@ static void multiply_batch(const std::vector<RNS_MLWE> &outs, const std::vector<
     }
     tensors();
     ntt_rows(pairs, 1);
-    keyswitch_batch(outs, ext, ksks);
+    keyswitch_batch(outs, ext, ksks, divide_masks);   // also rescales the jobs that ask for it
     std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> o;
-    std::map<uint64_t, std::vector<RNS_Polynomial>> by_divide;
     for (size_t bi = 0; bi < B; bi++)
-        for (auto p : comps(outs[bi])) {
-            if (divide_masks[bi] == 0) o.push_back({p, p});
-            else by_divide[divide_masks[bi]].push_back(p);
-        }
+        if (divide_masks[bi] == 0)
+            for (auto p : comps(outs[bi])) o.push_back({p, p});
     ntt_rows(o, 0);
-    for (auto &g : by_divide) run_division(g.second, g.first, true);
     for (auto e : ext) free_mlwe_RNS_sample(e);
 }
 

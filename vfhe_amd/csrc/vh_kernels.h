@@ -39,6 +39,20 @@ struct vh_div_job {
     int out_limb[VH_MAX_LIMBS];
 };
 
+// ModDown by K special primes, pass-through add and rescale by one prime of a
+// key-switch output component (k_moddown_rescale).
+struct vh_mdr_job {
+    uint64_t *base;          // device rows of the component, limb-major
+    const uint64_t *add;     // rows of the pass-through polynomial (same layout), or null
+    uint64_t stride;         // row length (N)
+    int K;                   // special primes
+    int nl;                  // pool size (row count of the constant tables)
+    int nout;                // output limbs (the rescale limb excluded)
+    int resc;                // the rescale limb
+    int div_limb[VH_MAX_DIV];
+    int out_limb[VH_MAX_LIMBS];
+};
+
 struct vh_bconv_job {
     const uint64_t *in;
     uint64_t *out;
@@ -82,6 +96,9 @@ hipError_t vh_launch_eltwise(hipStream_t st, const vh_item *items, unsigned nite
                              int op, uint64_t n);
 hipError_t vh_launch_split_mul(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                                int mode, uint64_t n, uint64_t S);
+// cdown: K x nl division constants of the special primes; cresc: 1 x nl of the rescale prime.
+hipError_t vh_launch_moddown_rescale(hipStream_t st, const vh_mdr_job *jobs, unsigned njobs, const vh_limb *limbs,
+                                     const void *cdown, const void *cresc, uint64_t n, int kmax);
 hipError_t vh_launch_division(hipStream_t st, const vh_div_job *jobs, unsigned njobs, unsigned nout,
                               const vh_limb *limbs, const void *consts, uint64_t n, int kmax);
 hipError_t vh_launch_base_conversion(hipStream_t st, const vh_bconv_job *jobs, unsigned njobs, unsigned nout,

@@ -1094,6 +1094,87 @@ k_division(const vh_div_job *__restrict__ jobs, const vh_limb *__restrict__ limb
     }
 }
 
+// ModDown, pass-through add and CKKS rescale of one key-switch output component
+// in one pass (the sequence mlwe.c:583-597 then rns_polynomial.c:948-973 for the
+// top prime): per coefficient, the K special limbs' chain is formed as in
+// k_division; limb `resc` is divided, gets the pass-through value added and
+// becomes the one-prime chain of the rescale; every other output limb is
+// divided, gets its pass-through value, and is divided by the rescale prime.
+// Every step is the sequential kernels' arithmetic in their order, so results
+// are identical.  The divided rows (specials and `resc`) are not written; the
+// caller clears them, as the sequential divisions do.
+template <int KMAX>
+__global__ void __launch_bounds__(256)
+k_moddown_rescale(const vh_mdr_job *__restrict__ jobs, const vh_limb *__restrict__ limbs,
+                  const vh_divc *__restrict__ cdown, const vh_divc *__restrict__ cresc, uint64_t n)
+{
+    const vh_mdr_job &jb = jobs[blockIdx.y];
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const int K = jb.K, nl = jb.nl;
+    uint64_t t[KMAX], v[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; k++)
+        if (k < K) v[k] = jb.base[(uint64_t)jb.div_limb[k] * jb.stride + x];
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        if (k >= K) break;
+        const int lk = jb.div_limb[k];
+        const uint64_t qk = limbs[lk].q, muk = limbs[lk].mu64;
+        t[k] = vh_reduce64(vh_reduce64(v[k], qk, muk) + cdown[k * nl + lk].h, qk, muk);
+#pragma unroll
+        for (int k2 = k + 1; k2 < KMAX; k2++) {
+            if (k2 >= K) break;
+            const int l2 = jb.div_limb[k2];
+            const uint64_t q2 = limbs[l2].q, mu2 = limbs[l2].mu64;
+            const vh_divc c2 = cdown[k * nl + l2];
+            uint64_t a = vh_reduce64(vh_reduce64(v[k2], q2, mu2) + c2.h, q2, mu2);
+            a = vh_submod(a, vh_reduce64(t[k], q2, mu2), q2);
+            v[k2] = vh_shoup(a, c2.inv, c2.inv_sh, q2);
+        }
+    }
+    auto down_add = [&](int li) -> uint64_t {
+        const uint64_t qi = limbs[li].q, mui = limbs[li].mu64;
+        uint64_t a = vh_reduce64(jb.base[(uint64_t)li * jb.stride + x], qi, mui);
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            if (k >= K) break;
+            const vh_divc ck = cdown[k * nl + li];
+            a = vh_addmod(a, ck.h, qi);
+            a = vh_submod(a, vh_reduce64(t[k], qi, mui), qi);
+            a = vh_shoup(a, ck.inv, ck.inv_sh, qi);
+        }
+        if (jb.add) a = vh_reduce64(a + jb.add[(uint64_t)li * jb.stride + x], qi, mui);
+        return a;
+    };
+    const int lr = jb.resc;
+    const uint64_t ar = down_add(lr);
+    const uint64_t qr = limbs[lr].q, mur = limbs[lr].mu64;
+    const uint64_t tr = vh_reduce64(vh_reduce64(ar, qr, mur) + cresc[lr].h, qr, mur);
+    for (int o = 0; o < jb.nout; o++) {
+        const int li = jb.out_limb[o];
+        const uint64_t qi = limbs[li].q, mui = limbs[li].mu64;
+        uint64_t a = down_add(li);
+        const vh_divc c = cresc[li];
+        a = vh_addmod(a, c.h, qi);
+        a = vh_submod(a, vh_reduce64(tr, qi, mui), qi);
+        jb.base[(uint64_t)li * jb.stride + x] = vh_shoup(a, c.inv, c.inv_sh, qi);
+    }
+}
+
+hipError_t vh_launch_moddown_rescale(hipStream_t st, const vh_mdr_job *jobs, unsigned njobs, const vh_limb *limbs,
+                                     const void *cdown, const void *cresc, uint64_t n, int kmax)
+{
+    const dim3 grid((unsigned)((n + 255) / 256), njobs);
+    if (kmax <= 4)
+        hipLaunchKernelGGL(k_moddown_rescale<4>, grid, dim3(256), 0, st, jobs, limbs, (const vh_divc *)cdown,
+                           (const vh_divc *)cresc, n);
+    else
+        hipLaunchKernelGGL(k_moddown_rescale<VH_MAX_DIV>, grid, dim3(256), 0, st, jobs, limbs, (const vh_divc *)cdown,
+                           (const vh_divc *)cresc, n);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Fast base extension (rns_polynomial.c:738-822): for output limb p,
 // out_p = sum_j [x_j * Dhat_j]_{q_j} * D_mod_p[p][j] mod p.  One thread per
