@@ -386,7 +386,10 @@ void vfhe_hip_mlwe_automorphism_batch_multi(RNS_MLWE *out, RNS_MLWE *in, uint64_
 RNS_MLWE *vfhe_hip_mlwe_view_array(RNS_MLWE *in, uint64_t count, uint64_t mask);
 /* mlwe_alloc_RNS_sample without clearing the device rows (the reference's own
    allocation leaves them uninitialised too): for outputs that are written
-   before they are read. */
+   before they are read.  The host rows behind coeffs[] are created by the
+   first polynomial_sync_to_host of a component (until then coeffs[i] is
+   NULL), so an output that is only passed on to other engine calls never
+   costs a host block. */
 RNS_MLWE vfhe_hip_mlwe_alloc_RNS_sample_uninit(uint64_t N, uint64_t r, uint64_t mask, incNTT ntt);
 /* The calling thread's current lane stream (a hipStream_t). */
 void *vfhe_hip_current_stream(void);

@@ -876,6 +876,18 @@ extern "C" uint64_t **incNTT_get_rou_matrix(incNTT ntt) { return ntt->w; }
 // ---------------------------------------------------------------------------
 // Polynomials (rns_polynomial.c:65-181)
 // ---------------------------------------------------------------------------
+// Allocates the host mirror of a polynomial created without one.
+static void ensure_mirror(vh_poly *v)
+{
+    if (v->hblock) return;
+    VH_REQUIRE(!v->borrowed);
+    const uint64_t l = v->pub.allocated_l, N = v->pub.ntt->N;
+    const size_t hb = ((l * N * 8 + 63) / 64) * 64;
+    v->hblock = (uint64_t *)aligned_alloc(64, hb ? hb : 64);
+    if (!v->hblock) exit(EXIT_FAILURE);
+    for (uint64_t i = 0; i < l; i++) v->pub.coeffs[i] = v->hblock + i * N;
+}
+
 static void dev_ready(RNS_Polynomial p)
 {
     vh_poly *v = VP(p);
@@ -895,8 +907,11 @@ static void check_rows(RNS_Polynomial p, uint64_t mask)
 }
 
 // zero = false leaves the device rows uninitialised, as the reference's allocation
-// does (rns_polynomial.c:65-78); the public entry point clears them.
-static RNS_Polynomial new_poly(uint64_t N, uint64_t rns_mask, incNTT ntt, bool zero)
+// does (rns_polynomial.c:65-78); the public entry point clears them.  mirror =
+// false defers the host mirror (and the coeffs[] row pointers) to the first
+// polynomial_sync_to_host: engine temporaries never need it, and allocating a
+// host block per temporary cost a third of a serving worker's host time.
+static RNS_Polynomial new_poly(uint64_t N, uint64_t rns_mask, incNTT ntt, bool zero, bool mirror)
 {
     VH_REQUIRE(N == ntt->N);
     vh_poly *v = new vh_poly();
@@ -905,10 +920,10 @@ static RNS_Polynomial new_poly(uint64_t N, uint64_t rns_mask, incNTT ntt, bool z
     v->pub.rns_mask = rns_mask;
     v->pub.allocated_l = l;
     v->pub.coeffs = (uint64_t **)malloc(sizeof(uint64_t *) * (l ? l : 1));
-    size_t hb = ((l * N * 8 + 63) / 64) * 64;
-    v->hblock = (uint64_t *)aligned_alloc(64, hb ? hb : 64);
-    if (!v->pub.coeffs || !v->hblock) exit(EXIT_FAILURE);
-    for (uint64_t i = 0; i < l; i++) v->pub.coeffs[i] = v->hblock + i * N;
+    if (!v->pub.coeffs) exit(EXIT_FAILURE);
+    v->hblock = nullptr;
+    for (uint64_t i = 0; i < l; i++) v->pub.coeffs[i] = nullptr;
+    if (mirror) ensure_mirror(v);
     v->d = (uint64_t *)vh_dmalloc(l * N * 8);
     if (zero) VH_CHECK(hipMemsetAsync(v->d, 0, l * N * 8, S()));
     v->host_dirty = 0;
@@ -918,7 +933,7 @@ static RNS_Polynomial new_poly(uint64_t N, uint64_t rns_mask, incNTT ntt, bool z
 extern "C" RNS_Polynomial polynomial_new_RNS_polynomial(uint64_t N, uint64_t rns_mask, incNTT ntt)
 {
     VH_ENTER;
-    return new_poly(N, rns_mask, ntt, true);
+    return new_poly(N, rns_mask, ntt, true, true);
 }
 
 extern "C" RNS_Polynomial *polynomial_new_RNS_polynomial_array(uint64_t size, uint64_t N, uint64_t rns_mask, incNTT ntt)
@@ -958,6 +973,7 @@ extern "C" void polynomial_sync_to_host(RNS_Polynomial p)
     VH_ENTER;
     vh_poly *v = VP(p);
     if (v->host_dirty) return;
+    ensure_mirror(v);
     sync_all_lanes();   // the rows may have been written on any lane
     VH_CHECK(hipMemcpyAsync(v->hblock, v->d, p->allocated_l * p->ntt->N * 8, hipMemcpyDeviceToHost, S()));
     VH_CHECK(hipStreamSynchronize(S()));
@@ -967,6 +983,7 @@ extern "C" void polynomial_sync_to_host(RNS_Polynomial p)
 extern "C" void polynomial_mark_host_dirty(RNS_Polynomial p)
 {
     VH_ENTER;
+    VH_REQUIRE(VP(p)->hblock);   // rows written through coeffs[] exist only once the mirror does
     VP(p)->host_dirty = 1;
 }
 
@@ -1700,8 +1717,8 @@ static RNS_MLWE alloc_sample_raw(uint64_t N, uint64_t r, uint64_t mask, incNTT n
     if (!c) exit(EXIT_FAILURE);
     c->a = (RNS_Polynomial *)malloc(sizeof(RNS_Polynomial) * (r ? r : 1));
     if (!c->a) exit(EXIT_FAILURE);
-    for (uint64_t i = 0; i < r; i++) c->a[i] = new_poly(N, mask, ntt, false);
-    c->b = new_poly(N, mask, ntt, false);
+    for (uint64_t i = 0; i < r; i++) c->a[i] = new_poly(N, mask, ntt, false, false);
+    c->b = new_poly(N, mask, ntt, false, false);
     c->r = r;
     return c;
 }
@@ -2673,7 +2690,7 @@ extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x
         dev_ready(x[i]); dev_ready(y[i]); dev_ready(z[i]);
         const uint64_t m = x[i]->rns_mask & y[i]->rns_mask;
         check_rows(z[i], m);
-        tmp[i] = new_poly(N, m, t, false);
+        tmp[i] = new_poly(N, m, t, false, false);
         z[i]->rns_mask = m;
     }
     vh_pool *P = VPOOL(t);
