@@ -159,7 +159,7 @@ def test_automorphism(lib):
 
 @pytest.mark.slow
 def test_ckks_config3_shape_multiply_rescale(lib):
-    """BASELINE config 3 shape: N = 2^16, L = 24 level limbs + K = 4 special limbs, rank 1."""
+    """The benchmarked CKKS shape: N = 2^16, L = 24 level limbs + K = 4 special limbs, rank 1."""
     s = Setup(65536, 24, 4, 41)
     r, m = 1, s.lvl_mask
     rlk = [s.key(s.L, r), None]

@@ -4,8 +4,8 @@
 // Two exact arithmetics are used, both producing the canonical residue in
 // [0, q) that the reference produces (modules/arith/c/src/mod_portable.c:7-161):
 //
-//  * FP64 exact-integer arithmetic for q < 2^50 (every prime of the BASELINE
-//    configurations).  Residues are held as integer-valued doubles.  A product
+//  * FP64 exact-integer arithmetic for q < 2^50 (every prime Ring.gen_prime makes
+//    at 50 bits, the benchmarked rings').  Residues are held as integer-valued doubles.  A product
 //    b*w (|b| < 2^52, |w| <= q/2) is split exactly into hi + lo with one FMA
 //    (TwoProduct); the quotient estimate qt = round(b * (w/q)) comes from one
 //    FMA against the 1.5*2^52 rounding constant, so |b*w/q - qt| <= 3/4 and
