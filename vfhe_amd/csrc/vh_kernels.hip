@@ -362,7 +362,10 @@ k_ntt16p(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, i
 {
     constexpr int S = 1 << R, TPS = S / 16, G = 256 / TPS;
     __shared__ double lds[vh_ntt16_lds<R, COL>()];
-    __shared__ double2 head[256];
+    // Table head tw[0..255]: column passes use (w, w/q) pairs; row passes multiply the
+    // entry by a row factor and use w alone, which keeps their LDS at 40 KiB (four
+    // workgroups per CU).
+    __shared__ __align__(16) double headw[COL ? 512 : 256];
     const int tid = threadIdx.x;
     const int g = COL ? (tid % G) : (tid / TPS);
     const int t = COL ? (tid / G) : (tid % TPS);
@@ -421,22 +424,20 @@ k_ntt16p(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, i
         if (it.limb != cur_limb) {   // uniform: reload the table head
             cur_limb = it.limb;
             const VH_GLOBAL double *tw = vh_global(reinterpret_cast<const double *>(DIR == 0 ? L.tw_fwd : L.tw_inv));
-            head[tid] = make_double2(tw[2 * tid], tw[2 * tid + 1]);
+            if (COL) reinterpret_cast<double2 *>(headw)[tid] = make_double2(tw[2 * tid], tw[2 * tid + 1]);
+            else headw[tid] = tw[2 * tid];
             __syncthreads();
-        }
-        double2 bf[8];
-        if (!COL) {
-#pragma unroll
-            for (int sp = 0; sp < 8; sp++) bf[sp] = rowb[buf][g][sp];
         }
         __builtin_amdgcn_sched_barrier(0);
         if (u + 1 < u1) fetch(u + 1);
         __builtin_amdgcn_sched_barrier(0);
 
         auto twid = [&](int sp, int local) -> double2 {
-            const double2 h = head[(1 << sp) + local];
-            if (COL) return h;
-            const double w = vh_center(vh_mulmod_tw(h.x, bf[sp].x, bf[sp].y, q), q, qinv);
+            if (COL) return reinterpret_cast<const double2 *>(headw)[(1 << sp) + local];
+            // the factor is read at its point of use: held in registers across the tile,
+            // the 8 factors would take 32 VGPRs
+            const double2 f = rowb[buf][g][sp];
+            const double w = vh_center(vh_mulmod_tw(headw[(1 << sp) + local], f.x, f.y, q), q, qinv);
             return make_double2(w, w * qinv);
         };
         auto finish = [&](double x) -> uint64_t {
