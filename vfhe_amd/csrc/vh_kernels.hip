@@ -1200,15 +1200,17 @@ __global__ void __launch_bounds__(256)
 k_base_conversion(const vh_bconv_job *__restrict__ jobs, const vh_limb *__restrict__ limbs,
                   const ulonglong2 *__restrict__ consts, uint64_t n)
 {
-    const vh_bconv_job jb = jobs[blockIdx.y];
+    // by reference: the job's limb lists are indexed with loop variables, which a private
+    // copy would place in scratch memory
+    const vh_bconv_job &jb = jobs[blockIdx.y];
     const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= n) return;
     const int t = blockIdx.z;
     const int P = jb.P[t];
-    const vh_limb Lp = limbs[P];
+    const vh_limb &Lp = limbs[P];
     uint64_t acc = 0;
     for (int j = 0; j < jb.w; j++) {
-        const vh_limb Lj = limbs[jb.D[j]];
+        const vh_limb &Lj = limbs[jb.D[j]];
         const ulonglong2 dh = consts[j];
         uint64_t y = vh_shoup(jb.in[(uint64_t)jb.D[j] * jb.stride + x], dh.x, dh.y, Lj.q);
         const ulonglong2 dm = consts[jb.w + t * jb.w + j];
