@@ -351,9 +351,13 @@ k_ntt16(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, in
 // from the product of a head entry and one of the row's 8 factors
 // (vh_limb::rowf_*): tw[2^s + (sub << s') + local] = tw[2^s' + local] * b[sub][s'].
 // A product is centred and paired with RN(w * RN(1/q)), whose relative error
-// (<= 2^-52) moves the quotient estimate by at most |b| / 2^53; the row passes
-// keep every operand below 3q (inverse sums are re-centred after every stage),
-// so each product stays within q of zero, as in ntt16_task.
+// (<= 2^-52) moves the quotient estimate by at most |b| / 2^53 for a multiplied
+// operand b.  Every multiplied operand stays below 4q <= 2^52 (forward sums grow
+// by less than q per stage from q/2; inverse sums are re-centred every other
+// stage, as in the column passes), so each quotient estimate is below 2^51 and
+// each product within q of zero.  The 15 twiddles of a row's stages s' = 0..3
+// are the same for its 16 threads: thread e < 15 forms twiddle e once per tile
+// into the row's padding slots of the exchange buffer, which no exchange writes.
 // ---------------------------------------------------------------------------
 template <int DIR, int R, bool COL, bool FIRST, bool LAST>
 __global__ void __launch_bounds__(256, 4)
@@ -361,6 +365,7 @@ k_ntt16p(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, i
          unsigned tiles_per_item, unsigned ntiles)
 {
     constexpr int S = 1 << R, TPS = S / 16, G = 256 / TPS;
+    static_assert(COL || R == 8, "row passes: 256-point rows (15 padding slots per row)");
     __shared__ double lds[vh_ntt16_lds<R, COL>()];
     // Table head tw[0..255]: column passes use (w, w/q) pairs; row passes multiply the
     // entry by a row factor and use w alone, which keeps their LDS at 40 KiB (four
@@ -431,9 +436,23 @@ k_ntt16p(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, i
         __builtin_amdgcn_sched_barrier(0);
         if (u + 1 < u1) fetch(u + 1);
         __builtin_amdgcn_sched_barrier(0);
+        // padding slot of row-twiddle e (e < 15): 17 e + 16 (lidx skips 16, 33, ..., 271)
+        auto rslot = [&](int e) -> int { return g * (S + S / 16) + 17 * e + 16; };
+        if (!COL) {
+            if (t < 15) {
+                const int sp = 31 - __builtin_clz(t + 1), local = t + 1 - (1 << sp);
+                const double2 f = rowb[buf][g][sp];
+                lds[rslot(t)] = vh_center(vh_mulmod_tw(headw[(1 << sp) + local], f.x, f.y, q), q, qinv);
+            }
+            vh_wave_sync();   // a row's 16 threads are in one wave
+        }
 
         auto twid = [&](int sp, int local) -> double2 {
             if (COL) return reinterpret_cast<const double2 *>(headw)[(1 << sp) + local];
+            if (sp < 4) {
+                const double w = lds[rslot((1 << sp) - 1 + local)];
+                return make_double2(w, w * qinv);
+            }
             // the factor is read at its point of use: held in registers across the tile,
             // the 8 factors would take 32 VGPRs
             const double2 f = rowb[buf][g][sp];
@@ -505,7 +524,7 @@ k_ntt16p(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, i
                     const double2 w = twid(sp, (16 * t + m) >> (R - sp));
                     const double u0 = v[m], x = v[m + hm];
                     double a = u0 + x;
-                    if (COL ? (c & 1) : 1) a = vh_center(a, q, qinv);
+                    if (c & 1) a = vh_center(a, q, qinv);
                     v[m] = a;
                     v[m + hm] = vh_mulmod_tw(u0 - x, w.x, w.y, q);
                 }
@@ -526,7 +545,7 @@ k_ntt16p(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, i
                     const double2 w = twid(sp, m >> (4 - sp));
                     const double u0 = v[m], x = v[m + hm];
                     double a = u0 + x;
-                    if (COL ? (c & 1) : 1) a = vh_center(a, q, qinv);
+                    if (c & 1) a = vh_center(a, q, qinv);
                     v[m] = a;
                     v[m + hm] = vh_mulmod_tw(u0 - x, w.x, w.y, q);
                 }
