@@ -2701,14 +2701,17 @@ extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x
     if (fused) {
         // Three launches: forward column passes of x (into z) and y (into tmp); forward
         // row stages, product and inverse row stages in one kernel; inverse column pass.
+        // Items limb-major: the pipelined passes reload their twiddle table head when the
+        // limb changes.
         std::vector<vh_item> col, mid, inv;
-        for (uint64_t i = 0; i < count; i++)
-            for_active(z[i]->rns_mask, t->l, [&](uint64_t li) {
+        for (uint64_t li = 0; li < t->l; li++)
+            for (uint64_t i = 0; i < count; i++) {
+                if (!((z[i]->rns_mask >> li) & 1)) continue;
                 col.push_back(mk_item(drow(x[i], li), drow(z[i], li), (uint32_t)li));
                 col.push_back(mk_item(drow(y[i], li), drow(tmp[i], li), (uint32_t)li));
                 mid.push_back(mk_item(drow(z[i], li), drow(z[i], li), (uint32_t)li, drow(tmp[i], li)));
                 inv.push_back(mk_item(drow(z[i], li), drow(z[i], li), (uint32_t)li));
-            });
+            }
         const int R1 = logn - 8;
         auto launch = [&](std::vector<vh_item> &v, int which) {
             for (size_t off = 0; off < v.size(); off += VH_MAX_GRID_Y) {
@@ -2719,6 +2722,7 @@ extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x
                 const bool pipe = OPT.ntt_mode == 2 && C.pipe_blocks > 0;
                 if (which == 0 && pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0, C.pipe_blocks));
                 else if (which == 0) VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0));
+                else if (which == 1 && pipe) VH_CHECK(vh_launch_ntt16p_row_mul(S(), d, cnt, P->d_limbs, logn, C.cus));
                 else if (which == 1) VH_CHECK(vh_launch_ntt16_row_mul_inv(S(), d, cnt, P->d_limbs, logn));
                 else if (pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1, C.pipe_blocks));
                 else VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1));

@@ -882,6 +882,226 @@ k_ntt16_row_mul_inv(const vh_item *__restrict__ items, const vh_limb *__restrict
     for (int m = 0; m < 16; m++) vh_st(out + t + TPS * m, (uint64_t)__double_as_longlong(vh_center(a[m], q, qinv)));
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined form of k_ntt16_row_mul_inv (FP64 path, N = 2^13..2^16): persistent
+// workgroups walk contiguous ranges of 4096-element tiles (16 rows of x and y),
+// loading the current tile's y and then the next tile's x before the transform
+// (y lands while x's first stages run).  Twiddles are formed as in k_ntt16p's row passes: head entry (LDS) times
+// the row's factor, the 15 stage-0..3 twiddles of a row once per tile into the
+// padding slots of the exchange buffer (forward ones first, then the inverse
+// ones after the forward stages have used them).  Every multiplied operand stays
+// below 4q (inverse sums re-centred every other stage), as in k_ntt16p.  234 VGPRs,
+// two workgroups per CU: 5% faster than the same kernel without the prefetch at
+// three (which spills), and than k_ntt16_row_mul_inv.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256, 2)
+k_ntt16p_row_mul(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, int logn,
+                 unsigned tiles_per_item, unsigned ntiles)
+{
+    constexpr int R = 8, S = 256, TPS = 16, G = 16;
+    __shared__ double lds[vh_ntt16_lds<R, false>()];
+    __shared__ double headf[256], headi[256];
+    __shared__ double2 rowbf[2][G][8], rowbi[2][G][8];
+    const int tid = threadIdx.x, g = tid / TPS, t = tid % TPS;
+    auto lidx = [&](int k) -> int { return g * (S + S / 16) + k + (k >> 4); };
+    // padding slot of row-twiddle e (e < 15): 17 e + 16 (lidx skips 16, 33, ..., 271)
+    auto rslot = [&](int e) -> int { return g * (S + S / 16) + 17 * e + 16; };
+
+    // x of the next tile is prefetched; y of the current tile is loaded at its start, ahead
+    // of that prefetch, and lands while x is transformed.
+    uint64_t rx[16];
+    double2 fnf = make_double2(0.0, 0.0), fni = make_double2(0.0, 0.0);
+    auto fetch = [&](unsigned uu) {
+        const vh_item &iu = items[uu / tiles_per_item];
+        const size_t su = (size_t)(uu % tiles_per_item) * G + g;
+        if (t < 8) {   // issued first: landed whenever the data has
+            const vh_limb &Lu = limbs[iu.limb];
+            const VH_GLOBAL double *rf = vh_global(reinterpret_cast<const double *>(Lu.rowf_fwd));
+            const VH_GLOBAL double *ri = vh_global(reinterpret_cast<const double *>(Lu.rowf_inv));
+            fnf = make_double2(rf[16 * su + 2 * t], rf[16 * su + 2 * t + 1]);
+            fni = make_double2(ri[16 * su + 2 * t], ri[16 * su + 2 * t + 1]);
+        }
+        const VH_GLOBAL uint64_t *xa = vh_global(iu.src) + su * S;
+#pragma unroll
+        for (int m = 0; m < 16; m++) rx[m] = vh_ld(xa + t + TPS * m);
+    };
+    const unsigned per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const unsigned u0 = blockIdx.x * per, u1 = u0 + per < ntiles ? u0 + per : ntiles;
+    unsigned cur_limb = ~0u;
+    if (u0 < u1) fetch(u0);
+    for (unsigned u = u0; u < u1; u++) {
+        const vh_item &it = items[u / tiles_per_item];
+        const vh_limb &L = limbs[it.limb];
+        const double q = L.qd, qinv = L.qinv;
+        const size_t sub = (size_t)(u % tiles_per_item) * G + g;
+        VH_GLOBAL uint64_t *out = vh_global(it.dst) + sub * S;
+        const int buf = (u - u0) & 1;
+
+        double a[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++) a[m] = __longlong_as_double((long long)rx[m]);
+        if (t < 8) {
+            rowbf[buf][g][t] = fnf;
+            rowbi[buf][g][t] = fni;
+        }
+        __syncthreads();   // the previous tile's LDS reads are done; rowb*[buf] are visible
+        if (it.limb != cur_limb) {   // uniform: reload the table heads
+            cur_limb = it.limb;
+            headf[tid] = vh_global(reinterpret_cast<const double *>(L.tw_fwd))[2 * tid];
+            headi[tid] = vh_global(reinterpret_cast<const double *>(L.tw_inv))[2 * tid];
+            __syncthreads();
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        uint64_t ry[16];
+        {
+            const VH_GLOBAL uint64_t *xb = vh_global(it.src2) + sub * S;
+#pragma unroll
+            for (int m = 0; m < 16; m++) ry[m] = vh_ld(xb + t + TPS * m);
+        }
+        if (u + 1 < u1) fetch(u + 1);
+        __builtin_amdgcn_sched_barrier(0);
+
+        // INV selects the inverse table; the LDS arrays are named directly (a pointer
+        // parameter would be a generic address, read with flat loads)
+        auto slot_twiddles = [&](auto inv) {
+            constexpr bool INV = decltype(inv)::value;
+            if (t < 15) {
+                const int sp = 31 - __builtin_clz(t + 1), local = t + 1 - (1 << sp);
+                const double2 f = INV ? rowbi[buf][g][sp] : rowbf[buf][g][sp];
+                const double h = INV ? headi[(1 << sp) + local] : headf[(1 << sp) + local];
+                lds[rslot(t)] = vh_center(vh_mulmod_tw(h, f.x, f.y, q), q, qinv);
+            }
+        };
+        auto twid = [&](auto inv, int sp, int local) -> double2 {
+            constexpr bool INV = decltype(inv)::value;
+            if (sp < 4) {
+                const double w = lds[rslot((1 << sp) - 1 + local)];
+                return make_double2(w, w * qinv);
+            }
+            const double2 f = INV ? rowbi[buf][g][sp] : rowbf[buf][g][sp];
+            const double h = INV ? headi[(1 << sp) + local] : headf[(1 << sp) + local];
+            const double w = vh_center(vh_mulmod_tw(h, f.x, f.y, q), q, qinv);
+            return make_double2(w, w * qinv);
+        };
+        const std::false_type FWD{};
+        const std::true_type INVT{};
+        // forward row stages: 0..3 in stride ownership on each operand (x first, while y
+        // lands), both regrouped through the data slots of LDS, then 4..7 on both at once
+        // (each twiddle formed once and used for both)
+        auto stages03 = [&](double (&v)[16]) {
+#pragma unroll
+            for (int sp = 0; sp < 4; sp++) {
+                const int hm = 8 >> sp;
+#pragma unroll
+                for (int m = 0; m < 16; m++) {
+                    if (m & hm) continue;
+                    const double2 w = twid(FWD, sp, m >> (4 - sp));
+                    const double x = vh_mulmod_tw(v[m + hm], w.x, w.y, q);
+                    const double uu = v[m];
+                    v[m] = uu + x;
+                    v[m + hm] = uu - x;
+                }
+            }
+        };
+        auto regroup = [&](double (&v)[16]) {
+            vh_wave_sync();   // the previous reads of the data slots are done
+#pragma unroll
+            for (int m = 0; m < 16; m++) lds[lidx(t + TPS * m)] = vh_center(v[m], q, qinv);
+            vh_wave_sync();
+#pragma unroll
+            for (int m = 0; m < 16; m++) v[m] = lds[lidx(16 * t + m)];
+        };
+        slot_twiddles(FWD);
+        vh_wave_sync();   // a row's 16 threads are in one wave
+        stages03(a);
+        // compiler barrier: the twiddles are re-read for y rather than held in registers
+        asm volatile("" ::: "memory");
+        double b[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++) b[m] = __longlong_as_double((long long)ry[m]);
+        stages03(b);
+        regroup(a);
+        regroup(b);
+#pragma unroll
+        for (int sp = 4; sp < R; sp++) {
+            const int hm = 1 << (R - sp - 1);
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                if (m & hm) continue;
+                const double2 w = twid(FWD, sp, (16 * t + m) >> (R - sp));
+                const double xa_ = vh_mulmod_tw(a[m + hm], w.x, w.y, q), xb_ = vh_mulmod_tw(b[m + hm], w.x, w.y, q);
+                const double ua = a[m], ub = b[m];
+                a[m] = ua + xa_; a[m + hm] = ua - xa_;
+                b[m] = ub + xb_; b[m + hm] = ub - xb_;
+            }
+        }
+        // pointwise product of centred operands (|.| <= q/2 + 1), centred
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            const double x = vh_center(a[m], q, qinv), y = vh_center(b[m], q, qinv);
+            const double hi = x * y;
+            const double lo = __fma_rn(x, y, -hi);
+            const double qt = __fma_rn(x, y * qinv, VH_MAGIC) - VH_MAGIC;
+            a[m] = vh_center(__fma_rn(-qt, q, hi) + lo, q, qinv);
+        }
+        // inverse stages s' = 7..4 (contiguous ownership), sums re-centred every other stage
+        int c = 0;
+#pragma unroll
+        for (int sp = R - 1; sp >= 4; sp--) {
+            const int hm = 1 << (R - sp - 1);
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                if (m & hm) continue;
+                const double2 w = twid(INVT, sp, (16 * t + m) >> (R - sp));
+                const double uu = a[m], x = a[m + hm];
+                double s_ = uu + x;
+                if (c & 1) s_ = vh_center(s_, q, qinv);
+                a[m] = s_;
+                a[m + hm] = vh_mulmod_tw(uu - x, w.x, w.y, q);
+            }
+            c++;
+        }
+        vh_wave_sync();   // every read of the forward twiddle slots and of b's regrouping is done
+        slot_twiddles(INVT);   // the inverse stage-0..3 twiddles replace the forward ones
+#pragma unroll
+        for (int m = 0; m < 16; m++) lds[lidx(16 * t + m)] = a[m];
+        vh_wave_sync();
+#pragma unroll
+        for (int m = 0; m < 16; m++) a[m] = lds[lidx(t + TPS * m)];
+        // inverse stages s' = 3..0 (stride ownership)
+#pragma unroll
+        for (int sp = 3; sp >= 0; sp--) {
+            const int hm = 8 >> sp;
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                if (m & hm) continue;
+                const double2 w = twid(INVT, sp, m >> (4 - sp));
+                const double uu = a[m], x = a[m + hm];
+                double s_ = uu + x;
+                if (c & 1) s_ = vh_center(s_, q, qinv);
+                a[m] = s_;
+                a[m + hm] = vh_mulmod_tw(uu - x, w.x, w.y, q);
+            }
+            c++;
+        }
+#pragma unroll
+        for (int m = 0; m < 16; m++) vh_st(out + t + TPS * m, (uint64_t)__double_as_longlong(vh_center(a[m], q, qinv)));
+    }
+}
+
+hipError_t vh_launch_ntt16p_row_mul(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
+                                    int logn, unsigned cus)
+{
+    if (logn < 13 || logn > 16 || cus == 0) return hipErrorInvalidValue;
+    static int occ = -1;
+    if (occ < 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ntt16p_row_mul, 256, 0) != hipSuccess) occ = -1;
+    if (occ <= 0) return hipErrorInvalidValue;
+    const unsigned tpi = 1u << (logn - 12), ntiles = tpi * nitems, nb = cus * (unsigned)occ;
+    hipLaunchKernelGGL(k_ntt16p_row_mul, dim3(nb < ntiles ? nb : ntiles), dim3(256), 0, st, items, limbs, logn, tpi,
+                       ntiles);
+    return hipGetLastError();
+}
+
 hipError_t vh_launch_ntt16_row_mul_inv(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                                        int logn)
 {
