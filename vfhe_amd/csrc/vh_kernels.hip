@@ -1317,13 +1317,13 @@ k_division(const vh_div_job *__restrict__ jobs, const vh_limb *__restrict__ limb
         if (k < K) v[k] = jb.base[(uint64_t)jb.div_limb[k] * jb.stride + x];
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
-        if (k >= K) break;
+        if (k >= K) continue;   // (continue, not break: the loop unrolls fully)
         const int lk = jb.div_limb[k];
         const uint64_t qk = limbs[lk].q, muk = limbs[lk].mu64;
         t[k] = vh_reduce64(vh_reduce64(v[k], qk, muk) + consts[k * nl + lk].h, qk, muk);
 #pragma unroll
         for (int k2 = k + 1; k2 < KMAX; k2++) {
-            if (k2 >= K) break;
+            if (k2 >= K) continue;
             const int l2 = jb.div_limb[k2];
             const uint64_t q2 = limbs[l2].q, mu2 = limbs[l2].mu64;
             const vh_divc c2 = consts[k * nl + l2];
@@ -1339,7 +1339,7 @@ k_division(const vh_div_job *__restrict__ jobs, const vh_limb *__restrict__ limb
         uint64_t a = vh_reduce64(row[x], qi, mui);
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
-            if (k >= K) break;
+            if (k >= K) continue;   // (continue, not break: the loop unrolls fully)
             const vh_divc ck = consts[k * nl + li];
             a = vh_addmod(a, ck.h, qi);
             a = vh_submod(a, vh_reduce64(t[k], qi, mui), qi);
@@ -1373,13 +1373,13 @@ k_moddown_rescale(const vh_mdr_job *__restrict__ jobs, const vh_limb *__restrict
         if (k < K) v[k] = jb.base[(uint64_t)jb.div_limb[k] * jb.stride + x];
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
-        if (k >= K) break;
+        if (k >= K) continue;   // (continue, not break: the loop unrolls fully)
         const int lk = jb.div_limb[k];
         const uint64_t qk = limbs[lk].q, muk = limbs[lk].mu64;
         t[k] = vh_reduce64(vh_reduce64(v[k], qk, muk) + cdown[k * nl + lk].h, qk, muk);
 #pragma unroll
         for (int k2 = k + 1; k2 < KMAX; k2++) {
-            if (k2 >= K) break;
+            if (k2 >= K) continue;
             const int l2 = jb.div_limb[k2];
             const uint64_t q2 = limbs[l2].q, mu2 = limbs[l2].mu64;
             const vh_divc c2 = cdown[k * nl + l2];
@@ -1393,7 +1393,7 @@ k_moddown_rescale(const vh_mdr_job *__restrict__ jobs, const vh_limb *__restrict
         uint64_t a = vh_reduce64(jb.base[(uint64_t)li * jb.stride + x], qi, mui);
 #pragma unroll
         for (int k = 0; k < KMAX; k++) {
-            if (k >= K) break;
+            if (k >= K) continue;   // (continue, not break: the loop unrolls fully)
             const vh_divc ck = cdown[k * nl + li];
             a = vh_addmod(a, ck.h, qi);
             a = vh_submod(a, vh_reduce64(t[k], qi, mui), qi);
