@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -2083,7 +2084,13 @@ static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<st
                 const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, nt.size() - off);
                 std::vector<vh_item> chunk(nt.begin() + off, nt.begin() + off + cnt);
                 vh_prof_scope ps_(VH_PROF_NTT_FP64);
-                VH_CHECK(vh_launch_ntt16(S(), vh_stage_vec(chunk), cnt, P->d_limbs, 0, logn, 0, logn - 8, 1, 2, 0));
+                // One tile per workgroup (the grid runs tile-major, so the workgroups in
+                // flight share two or three digit rows, which stay in the L2).
+                if (OPT.ntt_mode == 2 && C.pipe_blocks > 0)
+                    VH_CHECK(vh_launch_ntt16p(S(), vh_stage_vec(chunk), cnt, P->d_limbs, 0, logn, 0, logn - 8, 1, 2, 0,
+                                              UINT_MAX));
+                else
+                    VH_CHECK(vh_launch_ntt16(S(), vh_stage_vec(chunk), cnt, P->d_limbs, 0, logn, 0, logn - 8, 1, 2, 0));
             }
         } else {
             run_ntt(nt, P->d_limbs, P->h_limbs.data(), P->n, 0);

@@ -78,8 +78,9 @@ hipError_t vh_launch_ntt_pass(hipStream_t st, const vh_item *items, unsigned nit
 // (read through the caches instead of streaming).
 hipError_t vh_launch_ntt16(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                            int dir, int logn, int s0, int R, int col, int first, int last);
-// Pipelined pass (FP64, N = 2^13..2^16): same arguments as vh_launch_ntt16 (first = 1 only),
-// nblocks persistent workgroups; needs vh_limb::rowf_* for row passes.
+// Pipelined pass (FP64, N = 2^13..2^16): same arguments as vh_launch_ntt16 (first = 2 for the
+// forward column pass only), nblocks persistent workgroups (at most one per tile); needs
+// vh_limb::rowf_* for row passes.
 hipError_t vh_launch_ntt16p(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,
                             int logn, int s0, int R, int col, int first, int last, unsigned nblocks);
 int vh_ntt16p_occupancy();

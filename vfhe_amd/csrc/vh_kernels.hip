@@ -359,7 +359,7 @@ k_ntt16(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, in
 // are the same for its 16 threads: thread e < 15 forms twiddle e once per tile
 // into the row's padding slots of the exchange buffer, which no exchange writes.
 // ---------------------------------------------------------------------------
-template <int DIR, int R, bool COL, bool FIRST, bool LAST>
+template <int DIR, int R, bool COL, bool FIRST, bool LAST, bool SHARED = false>
 __global__ void __launch_bounds__(256, 4)
 k_ntt16p(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, int logn, int s0,
          unsigned tiles_per_item, unsigned ntiles)
@@ -404,7 +404,10 @@ k_ntt16p(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, i
         }
         const VH_GLOBAL uint64_t *in = vh_global(FIRST ? iu.src : iu.dst);
 #pragma unroll
-        for (int m = 0; m < 16; m++) raw[m] = vh_ld(CLOAD ? at(in, su, 16 * t, m) : at(in, su, t, TPS * m));
+        for (int m = 0; m < 16; m++) {
+            const VH_GLOBAL uint64_t *pm = CLOAD ? at(in, su, 16 * t, m) : at(in, su, t, TPS * m);
+            raw[m] = SHARED ? *pm : vh_ld(pm);   // a shared source row stays in the caches
+        }
     };
     // A workgroup walks a contiguous range of tiles (items are sorted limb-major, so the
     // limb, and with it the table head in LDS, changes rarely).  Nothing in the loop waits
@@ -1720,14 +1723,16 @@ hipError_t vh_launch_ntt16p(hipStream_t st, const vh_item *items, unsigned nitem
     if (nsub % G) return hipErrorInvalidValue;
     const unsigned tpi = nsub / G, ntiles = tpi * nitems;
     const dim3 grid(nblocks < ntiles ? nblocks : ntiles);
-#define VH_P(D, RR, CC, FF, LL) hipLaunchKernelGGL((k_ntt16p<D, RR, CC, FF, LL>), grid, dim3(256), 0, st, items, limbs, logn, s0, tpi, ntiles)
-#define VH_P_R(D, CC, FF, LL) \
-    switch (R) { case 5: VH_P(D, 5, CC, FF, LL); break; case 6: VH_P(D, 6, CC, FF, LL); break; \
-                 case 7: VH_P(D, 7, CC, FF, LL); break; default: VH_P(D, 8, CC, FF, LL); break; }
-    if (dir == 0 && col && first && !last) { VH_P_R(0, true, true, false) }
-    else if (dir == 0 && !col && !first && last) { VH_P(0, 8, false, false, true); }
-    else if (dir == 1 && !col && first && !last) { VH_P(1, 8, false, true, false); }
-    else if (dir == 1 && col && !first && last) { VH_P_R(1, true, false, true) }
+#define VH_P(D, RR, CC, FF, LL, SS) hipLaunchKernelGGL((k_ntt16p<D, RR, CC, FF, LL, SS>), grid, dim3(256), 0, st, items, limbs, logn, s0, tpi, ntiles)
+#define VH_P_R(D, CC, FF, LL, SS) \
+    switch (R) { case 5: VH_P(D, 5, CC, FF, LL, SS); break; case 6: VH_P(D, 6, CC, FF, LL, SS); break; \
+                 case 7: VH_P(D, 7, CC, FF, LL, SS); break; default: VH_P(D, 8, CC, FF, LL, SS); break; }
+    // first == 2: forward first pass whose source rows are shared by items (ModUp lift)
+    if (dir == 0 && col && first == 2 && !last) { VH_P_R(0, true, true, false, true) }
+    else if (dir == 0 && col && first && !last) { VH_P_R(0, true, true, false, false) }
+    else if (dir == 0 && !col && !first && last) { VH_P(0, 8, false, false, true, false); }
+    else if (dir == 1 && !col && first && !last) { VH_P(1, 8, false, true, false, false); }
+    else if (dir == 1 && col && !first && last) { VH_P_R(1, true, false, true, false) }
     else return hipErrorInvalidValue;
 #undef VH_P_R
 #undef VH_P
