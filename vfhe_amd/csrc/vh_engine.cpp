@@ -130,6 +130,9 @@ struct vh_opts {
     long ntt_mode = 2;        // 0: two launches per direction, 1: persistent per-XCD kernel,
                               // 2: two launches of the pipelined pass kernel
     long ntt_chunk_mb = 0;    // two-launch mode: rows per sub-batch (MiB), 0 = whole batch
+    long ntt_tile_grid = 6;   // pipelined passes launched one tile per workgroup instead of
+                              // persistent: bit 0 forward column, 1 forward row, 2 inverse row,
+                              // 3 inverse column, 4 the fused product's row kernel
     long xcd_lag = 3;         // persistent mode: queue slots between an item's two passes
     long xcd_occ = 2;         // persistent mode: workgroups per CU (2 or 4)
     long spin_limit = 1 << 22;
@@ -160,6 +163,7 @@ static void vh_ensure()
     C.pipe_blocks = (unsigned)(C.cus * vh_ntt16p_occupancy());
     if (const char *e = getenv("VFHE_HIP_NTT_MODE")) OPT.ntt_mode = atol(e);
     if (const char *e = getenv("VFHE_HIP_NTT_CHUNK_MB")) OPT.ntt_chunk_mb = atol(e);
+    if (const char *e = getenv("VFHE_HIP_NTT_TILE_GRID")) OPT.ntt_tile_grid = atol(e);
     if (const char *e = getenv("VFHE_HIP_XCD_LAG")) OPT.xcd_lag = atol(e);
     if (const char *e = getenv("VFHE_HIP_XCD_OCC")) OPT.xcd_occ = atol(e);
     if (const char *e = getenv("VFHE_HIP_KS_FUSED")) OPT.ks_fused = atol(e);
@@ -631,13 +635,13 @@ static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const v
                     const unsigned cc = std::min(per, cnt - c0);
                     const vh_item *ci = d_items + c0;
                     if (pipe) {
-                        const unsigned nb = C.pipe_blocks;
+                        auto nb = [&](int bit) -> unsigned { return (OPT.ntt_tile_grid >> bit) & 1 ? UINT_MAX : C.pipe_blocks; };
                         if (dir == 0) {
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0, nb)); }
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1, nb)); }
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0, nb(0))); }
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1, nb(1))); }
                         } else {
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0, nb)); }
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1, nb)); }
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0, nb(2))); }
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1, nb(3))); }
                         }
                     } else if (dir == 0) {
                         { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0)); }
@@ -1019,6 +1023,7 @@ extern "C" int vfhe_hip_set_option(const char *key, long value)
     if (!key) return -1;
     if (!strcmp(key, "ntt.mode")) OPT.ntt_mode = value;
     else if (!strcmp(key, "ntt.chunk_mb")) OPT.ntt_chunk_mb = value;
+    else if (!strcmp(key, "ntt.tile_grid")) OPT.ntt_tile_grid = value;
     else if (!strcmp(key, "ntt.xcd_lag")) OPT.xcd_lag = value < 0 ? 0 : value;
     else if (!strcmp(key, "ntt.xcd_occupancy")) OPT.xcd_occ = value;
     else if (!strcmp(key, "ntt.spin_limit")) OPT.spin_limit = value < 1 ? 1 : value;
@@ -1036,6 +1041,7 @@ extern "C" long vfhe_hip_get_option(const char *key)
     if (!strncmp(key, "probe.xcd", 9) && key[9] >= '0' && key[9] <= '7' && !key[10]) return C.xcd_hist[key[9] - '0'];
     if (!strcmp(key, "ntt.mode")) return OPT.ntt_mode;
     if (!strcmp(key, "ntt.chunk_mb")) return OPT.ntt_chunk_mb;
+    if (!strcmp(key, "ntt.tile_grid")) return OPT.ntt_tile_grid;
     if (!strcmp(key, "ntt.xcd_lag")) return OPT.xcd_lag;
     if (!strcmp(key, "ntt.xcd_occupancy")) return OPT.xcd_occ;
     if (!strcmp(key, "ntt.spin_limit")) return OPT.spin_limit;
@@ -2727,11 +2733,14 @@ extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x
                 const vh_item *d = vh_stage_vec(chunk);
                 vh_prof_scope ps_(VH_PROF_NTT_FP64);
                 const bool pipe = OPT.ntt_mode == 2 && C.pipe_blocks > 0;
-                if (which == 0 && pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0, C.pipe_blocks));
+                const unsigned nb0 = (OPT.ntt_tile_grid & 1) ? UINT_MAX : C.pipe_blocks;
+                const unsigned nb3 = ((OPT.ntt_tile_grid >> 3) & 1) ? UINT_MAX : C.pipe_blocks;
+                if (which == 0 && pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0, nb0));
                 else if (which == 0) VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0));
-                else if (which == 1 && pipe) VH_CHECK(vh_launch_ntt16p_row_mul(S(), d, cnt, P->d_limbs, logn, C.cus));
+                else if (which == 1 && pipe)
+                    VH_CHECK(vh_launch_ntt16p_row_mul(S(), d, cnt, P->d_limbs, logn, C.cus, (int)((OPT.ntt_tile_grid >> 4) & 1)));
                 else if (which == 1) VH_CHECK(vh_launch_ntt16_row_mul_inv(S(), d, cnt, P->d_limbs, logn));
-                else if (pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1, C.pipe_blocks));
+                else if (pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1, nb3));
                 else VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1));
             }
         };

@@ -112,9 +112,10 @@ hipError_t vh_launch_ntt16_row_mac(hipStream_t st, const vh_mac_job *jobs, unsig
                                    const vh_limb *limbs, const uint64_t *const *kptr, int logn, int comps);
 // Forward row stages of it.src and it.src2 (column-pass outputs), their pointwise
 // product and the inverse row stages, into it.dst (the inverse column pass input).
-// Pipelined form of the same step (persistent workgroups, cus x its occupancy); needs vh_limb::rowf_*.
+// Pipelined form of the same step (persistent workgroups, cus x its occupancy, or one tile per
+// workgroup with one_tile); needs vh_limb::rowf_*.
 hipError_t vh_launch_ntt16p_row_mul(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
-                                    int logn, unsigned cus);
+                                    int logn, unsigned cus, int one_tile);
 hipError_t vh_launch_ntt16_row_mul_inv(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                                        int logn);
 // fp != 0: every extended limb has q < 2^50 and every operand is canonical (FP64 products).

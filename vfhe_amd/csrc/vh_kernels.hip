@@ -1093,13 +1093,13 @@ k_ntt16p_row_mul(const vh_item *__restrict__ items, const vh_limb *__restrict__ 
 }
 
 hipError_t vh_launch_ntt16p_row_mul(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
-                                    int logn, unsigned cus)
+                                    int logn, unsigned cus, int one_tile)
 {
     if (logn < 13 || logn > 16 || cus == 0) return hipErrorInvalidValue;
     static int occ = -1;
     if (occ < 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ntt16p_row_mul, 256, 0) != hipSuccess) occ = -1;
     if (occ <= 0) return hipErrorInvalidValue;
-    const unsigned tpi = 1u << (logn - 12), ntiles = tpi * nitems, nb = cus * (unsigned)occ;
+    const unsigned tpi = 1u << (logn - 12), ntiles = tpi * nitems, nb = one_tile ? ntiles : cus * (unsigned)occ;
     hipLaunchKernelGGL(k_ntt16p_row_mul, dim3(nb < ntiles ? nb : ntiles), dim3(256), 0, st, items, limbs, logn, tpi,
                        ntiles);
     return hipGetLastError();
