@@ -3175,15 +3175,18 @@ extern "C" RNS_MLWE_Key mlwe_new_RNS_gaussian_key(uint64_t N, uint64_t r, uint64
 // NTT schedule autotuner: times forward+inverse round trips of `rows` scratch
 // rows (limbs of `ntt` in turn) under each candidate schedule and keeps the
 // fastest in the engine options.  Candidates: two launches per transform of the
-// pipelined pass kernel, then of the plain pass kernel, each over the whole batch
-// or sub-batches of 256 / 128 MiB, and (when the XCC placement probe passed and
-// 2^13 <= N <= 2^16) the persistent per-XCD kernel with queue lags 3, 5 and 8.
+// pipelined pass kernel (row passes one tile per workgroup, or every pass
+// persistent; whole batch or 256 MiB sub-batches), of the plain pass kernel
+// (whole batch or 256 / 128 MiB sub-batches), and (when the XCC placement probe
+// passed and 2^13 <= N <= 2^16) the persistent per-XCD kernel with queue lags
+// 3, 5 and 8.
 // us_out[k] receives candidate k's microseconds per round trip (-1 if skipped);
 // returns the index of the chosen candidate.
 // ---------------------------------------------------------------------------
-struct vh_sched { long mode, chunk_mb, lag, occ; };
-static const vh_sched VH_SCHEDS[] = {{2, 0, 3, 2}, {2, 256, 3, 2}, {2, 128, 3, 2}, {0, 0, 3, 2}, {0, 256, 3, 2},
-                                     {0, 128, 3, 2}, {1, 0, 3, 2}, {1, 0, 5, 2}, {1, 0, 8, 2}};
+struct vh_sched { long mode, chunk_mb, lag, occ, tile_grid; };
+static const vh_sched VH_SCHEDS[] = {{2, 0, 3, 2, 6},   {2, 0, 3, 2, 0},   {2, 256, 3, 2, 6},
+                                     {0, 0, 3, 2, 6},   {0, 256, 3, 2, 6}, {0, 128, 3, 2, 6},
+                                     {1, 0, 3, 2, 6},   {1, 0, 5, 2, 6},   {1, 0, 8, 2, 6}};
 
 extern "C" long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, double *us_out)
 {
@@ -3221,6 +3224,7 @@ extern "C" long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, doubl
             const vh_sched &sc = VH_SCHEDS[k];
             if (sc.mode == 1 && !(C.xcd_ok && logn >= 13 && logn <= 16)) continue;
             OPT.ntt_mode = sc.mode; OPT.ntt_chunk_mb = sc.chunk_mb; OPT.xcd_lag = sc.lag; OPT.xcd_occ = sc.occ;
+            OPT.ntt_tile_grid = sc.tile_grid;
             for (int w = 0; w < (round == 0 && k == 0 ? 3 : 1); w++) {   // warm-up
                 run_ntt(items, P->d_limbs, P->h_limbs.data(), n, 0);
                 run_ntt(items, P->d_limbs, P->h_limbs.data(), n, 1);
@@ -3247,6 +3251,7 @@ extern "C" long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, doubl
     OPT = saved;
     OPT.ntt_mode = VH_SCHEDS[best].mode; OPT.ntt_chunk_mb = VH_SCHEDS[best].chunk_mb;
     OPT.xcd_lag = VH_SCHEDS[best].lag; OPT.xcd_occ = VH_SCHEDS[best].occ;
+    OPT.ntt_tile_grid = VH_SCHEDS[best].tile_grid;
     vh_dfree(d);
     return best;
 }
