@@ -158,7 +158,8 @@ def main():
     primes = params.gen_primes(N, [50] * L)
     log("device", lib.vfhe_hip_device_name().decode(), "primes", primes[0], "...", primes[-1])
     sched = {k: int(lib.vfhe_hip_get_option(k.encode())) for k in
-             ("ntt.mode", "ntt.xcd_available", "ntt.xcd_lag", "ntt.xcd_occupancy", "ntt.chunk_mb", "ntt.tile_grid")}
+             ("ntt.mode", "ntt.xcd_available", "ntt.xcd_lag", "ntt.xcd_occupancy", "ntt.chunk_mb", "ntt.tile_grid",
+              "ntt.col_wide")}
     log("schedule", sched, "probe", [int(lib.vfhe_hip_get_option(f"probe.xcd{k}".encode())) for k in range(8)])
     pool = rt.Pool(N, primes)
     tuned = None

@@ -315,14 +315,16 @@ int vfhe_hip_last_error(void);
 const char *vfhe_hip_device_name(void);
 
 /* Tunables, also readable from the environment at first use (VFHE_HIP_NTT_MODE,
-   VFHE_HIP_NTT_CHUNK_MB, VFHE_HIP_NTT_TILE_GRID, VFHE_HIP_XCD_LAG, VFHE_HIP_XCD_OCC,
-   VFHE_HIP_KS_FUSED, VFHE_HIP_FFT_FMA):
+   VFHE_HIP_NTT_CHUNK_MB, VFHE_HIP_NTT_TILE_GRID, VFHE_HIP_NTT_COL_WIDE, VFHE_HIP_XCD_LAG,
+   VFHE_HIP_XCD_OCC, VFHE_HIP_KS_FUSED, VFHE_HIP_FFT_FMA):
      "ntt.mode"          2: two launches per direction of the pipelined pass kernel (default);
                          0: two launches of the plain pass kernel; 1: persistent per-XCD kernel
      "ntt.chunk_mb"      two-launch modes: MiB of rows per sub-batch, 0 = whole batch
      "ntt.tile_grid"     mode 2: passes launched one tile per workgroup instead of persistent,
                          bit 0 forward column, 1 forward row, 2 inverse row, 3 inverse column,
-                         4 the fused product's row kernel (default 6: the row passes)
+                         4 the fused product's row kernel (default 15: the four passes)
+     "ntt.col_wide"      mode 2: 1 = column passes in 512-thread workgroups over 32-column
+                         tiles (default), 0 = 256 threads over 16 columns
      "ntt.xcd_lag"       persistent mode: queue slots between the two passes of a transform
      "ntt.xcd_occupancy" persistent mode: workgroups per CU, 2 or 4
      "ntt.spin_limit"    persistent mode: bound on a wait; exceeding it aborts at the next sync
@@ -342,9 +344,9 @@ void vfhe_hip_chacha20_blocks(const uint32_t *key, uint64_t nonce, uint64_t ctr0
 
 /* NTT schedule autotuner: times round trips of `rows` scratch rows of `ntt`'s
    transform size under each candidate schedule (two launches of the pipelined
-   pass kernel with the row passes one tile per workgroup or all passes persistent,
-   and over 256 MiB sub-batches; of the plain one over the whole batch or 256 / 128
-   MiB sub-batches; the persistent per-XCD kernel with queue lags 3, 5, 8 where
+   pass kernel: every pass one tile per workgroup with 512-thread column passes,
+   row passes one tile per workgroup, every pass persistent; of the plain one over
+   the whole batch or 256 / 128 MiB sub-batches; the persistent per-XCD kernel with queue lags 3, 5, 8 where
    available), keeps the fastest in the options above and returns its index;
    us_out (9 entries, may be NULL) receives each candidate's microseconds per
    round trip, -1 when not applicable. */

@@ -130,7 +130,8 @@ struct vh_opts {
     long ntt_mode = 2;        // 0: two launches per direction, 1: persistent per-XCD kernel,
                               // 2: two launches of the pipelined pass kernel
     long ntt_chunk_mb = 0;    // two-launch mode: rows per sub-batch (MiB), 0 = whole batch
-    long ntt_tile_grid = 6;   // pipelined passes launched one tile per workgroup instead of
+    long ntt_col_wide = 1;    // pipelined column passes: 512-thread workgroups over 32-column tiles
+    long ntt_tile_grid = 15;   // pipelined passes launched one tile per workgroup instead of
                               // persistent: bit 0 forward column, 1 forward row, 2 inverse row,
                               // 3 inverse column, 4 the fused product's row kernel
     long xcd_lag = 3;         // persistent mode: queue slots between an item's two passes
@@ -164,6 +165,7 @@ static void vh_ensure()
     if (const char *e = getenv("VFHE_HIP_NTT_MODE")) OPT.ntt_mode = atol(e);
     if (const char *e = getenv("VFHE_HIP_NTT_CHUNK_MB")) OPT.ntt_chunk_mb = atol(e);
     if (const char *e = getenv("VFHE_HIP_NTT_TILE_GRID")) OPT.ntt_tile_grid = atol(e);
+    if (const char *e = getenv("VFHE_HIP_NTT_COL_WIDE")) OPT.ntt_col_wide = atol(e);
     if (const char *e = getenv("VFHE_HIP_XCD_LAG")) OPT.xcd_lag = atol(e);
     if (const char *e = getenv("VFHE_HIP_XCD_OCC")) OPT.xcd_occ = atol(e);
     if (const char *e = getenv("VFHE_HIP_KS_FUSED")) OPT.ks_fused = atol(e);
@@ -637,11 +639,11 @@ static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const v
                     if (pipe) {
                         auto nb = [&](int bit) -> unsigned { return (OPT.ntt_tile_grid >> bit) & 1 ? UINT_MAX : C.pipe_blocks; };
                         if (dir == 0) {
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0, nb(0))); }
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0, nb(0), (int)OPT.ntt_col_wide)); }
                             { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1, nb(1))); }
                         } else {
                             { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0, nb(2))); }
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1, nb(3))); }
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1, nb(3), (int)OPT.ntt_col_wide)); }
                         }
                     } else if (dir == 0) {
                         { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0)); }
@@ -1024,6 +1026,7 @@ extern "C" int vfhe_hip_set_option(const char *key, long value)
     if (!strcmp(key, "ntt.mode")) OPT.ntt_mode = value;
     else if (!strcmp(key, "ntt.chunk_mb")) OPT.ntt_chunk_mb = value;
     else if (!strcmp(key, "ntt.tile_grid")) OPT.ntt_tile_grid = value;
+    else if (!strcmp(key, "ntt.col_wide")) OPT.ntt_col_wide = value;
     else if (!strcmp(key, "ntt.xcd_lag")) OPT.xcd_lag = value < 0 ? 0 : value;
     else if (!strcmp(key, "ntt.xcd_occupancy")) OPT.xcd_occ = value;
     else if (!strcmp(key, "ntt.spin_limit")) OPT.spin_limit = value < 1 ? 1 : value;
@@ -1042,6 +1045,7 @@ extern "C" long vfhe_hip_get_option(const char *key)
     if (!strcmp(key, "ntt.mode")) return OPT.ntt_mode;
     if (!strcmp(key, "ntt.chunk_mb")) return OPT.ntt_chunk_mb;
     if (!strcmp(key, "ntt.tile_grid")) return OPT.ntt_tile_grid;
+    if (!strcmp(key, "ntt.col_wide")) return OPT.ntt_col_wide;
     if (!strcmp(key, "ntt.xcd_lag")) return OPT.xcd_lag;
     if (!strcmp(key, "ntt.xcd_occupancy")) return OPT.xcd_occ;
     if (!strcmp(key, "ntt.spin_limit")) return OPT.spin_limit;
@@ -2735,12 +2739,12 @@ extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x
                 const bool pipe = OPT.ntt_mode == 2 && C.pipe_blocks > 0;
                 const unsigned nb0 = (OPT.ntt_tile_grid & 1) ? UINT_MAX : C.pipe_blocks;
                 const unsigned nb3 = ((OPT.ntt_tile_grid >> 3) & 1) ? UINT_MAX : C.pipe_blocks;
-                if (which == 0 && pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0, nb0));
+                if (which == 0 && pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0, nb0, (int)OPT.ntt_col_wide));
                 else if (which == 0) VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0));
                 else if (which == 1 && pipe)
                     VH_CHECK(vh_launch_ntt16p_row_mul(S(), d, cnt, P->d_limbs, logn, C.cus, (int)((OPT.ntt_tile_grid >> 4) & 1)));
                 else if (which == 1) VH_CHECK(vh_launch_ntt16_row_mul_inv(S(), d, cnt, P->d_limbs, logn));
-                else if (pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1, nb3));
+                else if (pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1, nb3, (int)OPT.ntt_col_wide));
                 else VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1));
             }
         };
@@ -3175,18 +3179,18 @@ extern "C" RNS_MLWE_Key mlwe_new_RNS_gaussian_key(uint64_t N, uint64_t r, uint64
 // NTT schedule autotuner: times forward+inverse round trips of `rows` scratch
 // rows (limbs of `ntt` in turn) under each candidate schedule and keeps the
 // fastest in the engine options.  Candidates: two launches per transform of the
-// pipelined pass kernel (row passes one tile per workgroup, or every pass
-// persistent; whole batch or 256 MiB sub-batches), of the plain pass kernel
-// (whole batch or 256 / 128 MiB sub-batches), and (when the XCC placement probe
-// passed and 2^13 <= N <= 2^16) the persistent per-XCD kernel with queue lags
-// 3, 5 and 8.
+// pipelined pass kernel (every pass one tile per workgroup with 512-thread column
+// passes; row passes one tile per workgroup; every pass persistent), of the plain
+// pass kernel (whole batch or 256 / 128 MiB sub-batches), and (when the XCC
+// placement probe passed and 2^13 <= N <= 2^16) the persistent per-XCD kernel
+// with queue lags 3, 5 and 8.
 // us_out[k] receives candidate k's microseconds per round trip (-1 if skipped);
 // returns the index of the chosen candidate.
 // ---------------------------------------------------------------------------
-struct vh_sched { long mode, chunk_mb, lag, occ, tile_grid; };
-static const vh_sched VH_SCHEDS[] = {{2, 0, 3, 2, 6},   {2, 0, 3, 2, 0},   {2, 256, 3, 2, 6},
-                                     {0, 0, 3, 2, 6},   {0, 256, 3, 2, 6}, {0, 128, 3, 2, 6},
-                                     {1, 0, 3, 2, 6},   {1, 0, 5, 2, 6},   {1, 0, 8, 2, 6}};
+struct vh_sched { long mode, chunk_mb, lag, occ, tile_grid, col_wide; };
+static const vh_sched VH_SCHEDS[] = {{2, 0, 3, 2, 15, 1},  {2, 0, 3, 2, 6, 0},  {2, 0, 3, 2, 0, 0},
+                                     {0, 0, 3, 2, 15, 1},  {0, 256, 3, 2, 15, 1}, {0, 128, 3, 2, 15, 1},
+                                     {1, 0, 3, 2, 15, 1},  {1, 0, 5, 2, 15, 1},  {1, 0, 8, 2, 15, 1}};
 
 extern "C" long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, double *us_out)
 {
@@ -3224,7 +3228,7 @@ extern "C" long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, doubl
             const vh_sched &sc = VH_SCHEDS[k];
             if (sc.mode == 1 && !(C.xcd_ok && logn >= 13 && logn <= 16)) continue;
             OPT.ntt_mode = sc.mode; OPT.ntt_chunk_mb = sc.chunk_mb; OPT.xcd_lag = sc.lag; OPT.xcd_occ = sc.occ;
-            OPT.ntt_tile_grid = sc.tile_grid;
+            OPT.ntt_tile_grid = sc.tile_grid; OPT.ntt_col_wide = sc.col_wide;
             for (int w = 0; w < (round == 0 && k == 0 ? 3 : 1); w++) {   // warm-up
                 run_ntt(items, P->d_limbs, P->h_limbs.data(), n, 0);
                 run_ntt(items, P->d_limbs, P->h_limbs.data(), n, 1);
@@ -3251,7 +3255,7 @@ extern "C" long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, doubl
     OPT = saved;
     OPT.ntt_mode = VH_SCHEDS[best].mode; OPT.ntt_chunk_mb = VH_SCHEDS[best].chunk_mb;
     OPT.xcd_lag = VH_SCHEDS[best].lag; OPT.xcd_occ = VH_SCHEDS[best].occ;
-    OPT.ntt_tile_grid = VH_SCHEDS[best].tile_grid;
+    OPT.ntt_tile_grid = VH_SCHEDS[best].tile_grid; OPT.ntt_col_wide = VH_SCHEDS[best].col_wide;
     vh_dfree(d);
     return best;
 }

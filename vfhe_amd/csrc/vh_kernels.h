@@ -79,10 +79,11 @@ hipError_t vh_launch_ntt_pass(hipStream_t st, const vh_item *items, unsigned nit
 hipError_t vh_launch_ntt16(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                            int dir, int logn, int s0, int R, int col, int first, int last);
 // Pipelined pass (FP64, N = 2^13..2^16): same arguments as vh_launch_ntt16 (first = 2 for the
-// forward column pass only), nblocks persistent workgroups (at most one per tile); needs
-// vh_limb::rowf_* for row passes.
+// forward column pass only), nblocks persistent workgroups of 256 threads (at most one per tile);
+// needs vh_limb::rowf_* for row passes.  wide: a column pass runs 512-thread workgroups over
+// twice the columns per tile (nblocks is halved).
 hipError_t vh_launch_ntt16p(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,
-                            int logn, int s0, int R, int col, int first, int last, unsigned nblocks);
+                            int logn, int s0, int R, int col, int first, int last, unsigned nblocks, int wide = 0);
 int vh_ntt16p_occupancy();
 // Persistent two-pass NTT with per-XCD queues; ctl holds 16 + nitems zeroed uint32 counters.
 hipError_t vh_launch_ntt_xcd(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,

@@ -4,6 +4,7 @@
 // number of (polynomial, limb) rows: the reference's per-limb loops
 // (modules/arith/c/src/rns_polynomial.c:582-614 and friends) become the y
 // dimension of a single grid.
+#include <climits>
 #include <type_traits>
 #include "vh_device.h"
 #include "vh_kernels.h"
@@ -359,14 +360,16 @@ k_ntt16(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, in
 // are the same for its 16 threads: thread e < 15 forms twiddle e once per tile
 // into the row's padding slots of the exchange buffer, which no exchange writes.
 // ---------------------------------------------------------------------------
-template <int DIR, int R, bool COL, bool FIRST, bool LAST, bool SHARED = false>
-__global__ void __launch_bounds__(256, 4)
+template <int DIR, int R, bool COL, bool FIRST, bool LAST, bool SHARED = false, int NT = 256>
+__global__ void __launch_bounds__(NT, 1024 / NT)
 k_ntt16p(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, int logn, int s0,
          unsigned tiles_per_item, unsigned ntiles)
 {
-    constexpr int S = 1 << R, TPS = S / 16, G = 256 / TPS;
-    static_assert(COL || R == 8, "row passes: 256-point rows (15 padding slots per row)");
-    __shared__ double lds[vh_ntt16_lds<R, COL>()];
+    // NT threads: 256, or 512 for a column pass over twice as many columns per tile
+    constexpr int S = 1 << R, TPS = S / 16, G = NT / TPS;
+    static_assert(COL || (R == 8 && NT == 256), "row passes: 256-point rows (15 padding slots per row)");
+    static_assert(NT == 256 || NT == 512, "256 or 512 threads");
+    __shared__ double lds[COL ? S * (G + 1) : G * (S + S / 16)];
     // Table head tw[0..255]: column passes use (w, w/q) pairs; row passes multiply the
     // entry by a row factor and use w alone, which keeps their LDS at 40 KiB (four
     // workgroups per CU).
@@ -432,8 +435,10 @@ k_ntt16p(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, i
         if (it.limb != cur_limb) {   // uniform: reload the table head
             cur_limb = it.limb;
             const VH_GLOBAL double *tw = vh_global(reinterpret_cast<const double *>(DIR == 0 ? L.tw_fwd : L.tw_inv));
-            if (COL) reinterpret_cast<double2 *>(headw)[tid] = make_double2(tw[2 * tid], tw[2 * tid + 1]);
-            else headw[tid] = tw[2 * tid];
+            if (tid < 256) {
+                if (COL) reinterpret_cast<double2 *>(headw)[tid] = make_double2(tw[2 * tid], tw[2 * tid + 1]);
+                else headw[tid] = tw[2 * tid];
+            }
             __syncthreads();
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -1714,27 +1719,33 @@ hipError_t vh_launch_ntt16(hipStream_t st, const vh_item *items, unsigned nitems
 }
 
 hipError_t vh_launch_ntt16p(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,
-                            int logn, int s0, int R, int col, int first, int last, unsigned nblocks)
+                            int logn, int s0, int R, int col, int first, int last, unsigned nblocks, int wide)
 {
     if (logn < 13 || logn > 16 || R < 5 || R > 8 || (!col && R != 8) || nblocks == 0) return hipErrorInvalidValue;
     if (col ? (s0 != 0 || R != logn - 8) : (s0 != logn - 8)) return hipErrorInvalidValue;
-    const unsigned G = 256u >> (R - 4);
+    if (!col) wide = 0;
+    const unsigned G = (wide ? 512u : 256u) >> (R - 4);
     const unsigned nsub = 1u << (logn - R);
     if (nsub % G) return hipErrorInvalidValue;
     const unsigned tpi = nsub / G, ntiles = tpi * nitems;
+    if (wide && nblocks != UINT_MAX) nblocks = nblocks / 2 ? nblocks / 2 : 1;   // same threads per CU
     const dim3 grid(nblocks < ntiles ? nblocks : ntiles);
-#define VH_P(D, RR, CC, FF, LL, SS) hipLaunchKernelGGL((k_ntt16p<D, RR, CC, FF, LL, SS>), grid, dim3(256), 0, st, items, limbs, logn, s0, tpi, ntiles)
-#define VH_P_R(D, CC, FF, LL, SS) \
-    switch (R) { case 5: VH_P(D, 5, CC, FF, LL, SS); break; case 6: VH_P(D, 6, CC, FF, LL, SS); break; \
-                 case 7: VH_P(D, 7, CC, FF, LL, SS); break; default: VH_P(D, 8, CC, FF, LL, SS); break; }
+#define VH_P(D, RR, CC, FF, LL, SS) hipLaunchKernelGGL((k_ntt16p<D, RR, CC, FF, LL, SS, 256>), grid, dim3(256), 0, st, items, limbs, logn, s0, tpi, ntiles)
+#define VH_PC(D, RR, FF, LL, SS) \
+    do { if (wide) hipLaunchKernelGGL((k_ntt16p<D, RR, true, FF, LL, SS, 512>), grid, dim3(512), 0, st, items, limbs, logn, s0, tpi, ntiles); \
+         else VH_P(D, RR, true, FF, LL, SS); } while (0)
+#define VH_P_R(D, FF, LL, SS) \
+    switch (R) { case 5: VH_PC(D, 5, FF, LL, SS); break; case 6: VH_PC(D, 6, FF, LL, SS); break; \
+                 case 7: VH_PC(D, 7, FF, LL, SS); break; default: VH_PC(D, 8, FF, LL, SS); break; }
     // first == 2: forward first pass whose source rows are shared by items (ModUp lift)
-    if (dir == 0 && col && first == 2 && !last) { VH_P_R(0, true, true, false, true) }
-    else if (dir == 0 && col && first && !last) { VH_P_R(0, true, true, false, false) }
+    if (dir == 0 && col && first == 2 && !last) { VH_P_R(0, true, false, true) }
+    else if (dir == 0 && col && first && !last) { VH_P_R(0, true, false, false) }
     else if (dir == 0 && !col && !first && last) { VH_P(0, 8, false, false, true, false); }
     else if (dir == 1 && !col && first && !last) { VH_P(1, 8, false, true, false, false); }
-    else if (dir == 1 && col && !first && last) { VH_P_R(1, true, false, true, false) }
+    else if (dir == 1 && col && !first && last) { VH_P_R(1, false, true, false) }
     else return hipErrorInvalidValue;
 #undef VH_P_R
+#undef VH_PC
 #undef VH_P
     return hipGetLastError();
 }
