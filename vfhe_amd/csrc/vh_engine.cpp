@@ -2098,7 +2098,7 @@ static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<st
                 // flight share two or three digit rows, which stay in the L2).
                 if (OPT.ntt_mode == 2 && C.pipe_blocks > 0)
                     VH_CHECK(vh_launch_ntt16p(S(), vh_stage_vec(chunk), cnt, P->d_limbs, 0, logn, 0, logn - 8, 1, 2, 0,
-                                              UINT_MAX));
+                                              UINT_MAX, (int)OPT.ntt_col_wide));
                 else
                     VH_CHECK(vh_launch_ntt16(S(), vh_stage_vec(chunk), cnt, P->d_limbs, 0, logn, 0, logn - 8, 1, 2, 0));
             }
