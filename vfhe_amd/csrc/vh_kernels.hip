@@ -343,11 +343,15 @@ k_ntt16(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, in
 
 // ---------------------------------------------------------------------------
 // Pipelined pass (FP64 path, N = 2^13 .. 2^16): the same stages, ownership and
-// arithmetic as ntt16_task, but a persistent workgroup walks the tiles of the
-// launch (tile u = G sub-transforms of row item u / tiles_per_item), issuing the
-// loads of its next tile before transforming the current one, and takes no
-// twiddle from global memory inside the transform, so those loads stay in
-// flight through it.  Twiddles come from the table head tw[0..255] staged in
+// arithmetic as ntt16_task, but a workgroup walks a contiguous range of the
+// launch's tiles (tile u = G sub-transforms of row item u / tiles_per_item),
+// issuing the loads of its next tile before transforming the current one, and
+// takes no twiddle from global memory inside the transform, so those loads stay
+// in flight through it.  With a grid of one workgroup per tile the range is one
+// tile: the workgroups in flight then stream neighbouring tiles, which measures
+// faster than persistent ranges for every pass here (the host picks the grid).
+// NT = 512 (column passes) doubles the columns of a tile, so each row segment a
+// load instruction touches is 256 bytes instead of 128.  Twiddles come from the table head tw[0..255] staged in
 // LDS (column passes: s0 = 0 and sub-independent indices) and, in row passes,
 // from the product of a head entry and one of the row's 8 factors
 // (vh_limb::rowf_*): tw[2^s + (sub << s') + local] = tw[2^s' + local] * b[sub][s'].
