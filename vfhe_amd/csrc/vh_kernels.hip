@@ -349,7 +349,8 @@ k_ntt16(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, in
 // takes no twiddle from global memory inside the transform, so those loads stay
 // in flight through it.  With a grid of one workgroup per tile the range is one
 // tile: the workgroups in flight then stream neighbouring tiles, which measures
-// faster than persistent ranges for every pass here (the host picks the grid).
+// faster than persistent ranges for the row passes and the 512-thread column
+// passes (the host picks the grid).
 // NT = 512 (column passes) doubles the columns of a tile, so each row segment a
 // load instruction touches is 256 bytes instead of 128.  Twiddles come from the table head tw[0..255] staged in
 // LDS (column passes: s0 = 0 and sub-independent indices) and, in row passes,
