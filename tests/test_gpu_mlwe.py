@@ -90,6 +90,7 @@ def test_ghs_keyswitch(lib, N, L, K, r, relin):
     (8192, 3, 1, 1, True, 0, 50),      # two-pass NTT + FP64 inner product
     (16384, 2, 2, 2, True, 1, 50),     # three output components: FP64 inner product
     (8192, 3, 1, 1, True, 1, 60),      # primes above 2^50: integer path throughout
+    (32768, 3, 2, 1, True, 1, 50),     # fused at 2^15 (7-stage column pass)
     (65536, 2, 1, 1, False, 1, 50),    # largest fused size
     (131072, 2, 1, 1, False, 1, 50),   # largest ring (2^17): generic passes + FP64 inner product
 ])
