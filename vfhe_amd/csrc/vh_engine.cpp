@@ -596,6 +596,15 @@ static void vh_check_device_error()
     if (e) vh_die("a persistent NTT launch left transforms incomplete (results are invalid)", __FILE__, __LINE__);
 }
 
+// Column passes (N = 2^13..2^16) use 512-thread workgroups over twice the columns per tile
+// (ntt.col_wide, 2^(logn-13) tiles per row item) when that still gives every CU two
+// workgroups; a smaller launch keeps the 256-thread tiles, twice as many workgroups.
+static int col_wide(unsigned items, int logn)
+{
+    const uint64_t tiles = (uint64_t)items << (logn - 13);
+    return OPT.ntt_col_wide && tiles >= 2 * (uint64_t)C.cus ? 1 : 0;
+}
+
 // Forward (dir 0) or inverse (dir 1) transforms of n-point rows.
 static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const vh_limb *h_limbs, uint64_t n, int dir)
 {
@@ -639,11 +648,11 @@ static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const v
                     if (pipe) {
                         auto nb = [&](int bit) -> unsigned { return (OPT.ntt_tile_grid >> bit) & 1 ? UINT_MAX : C.pipe_blocks; };
                         if (dir == 0) {
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0, nb(0), (int)OPT.ntt_col_wide)); }
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0, nb(0), col_wide(cc, logn))); }
                             { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1, nb(1))); }
                         } else {
                             { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0, nb(2))); }
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1, nb(3), (int)OPT.ntt_col_wide)); }
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1, nb(3), col_wide(cc, logn))); }
                         }
                     } else if (dir == 0) {
                         { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0)); }
@@ -2098,7 +2107,7 @@ static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<st
                 // flight share two or three digit rows, which stay in the L2).
                 if (OPT.ntt_mode == 2 && C.pipe_blocks > 0)
                     VH_CHECK(vh_launch_ntt16p(S(), vh_stage_vec(chunk), cnt, P->d_limbs, 0, logn, 0, logn - 8, 1, 2, 0,
-                                              UINT_MAX, (int)OPT.ntt_col_wide));
+                                              UINT_MAX, col_wide(cnt, logn)));
                 else
                     VH_CHECK(vh_launch_ntt16(S(), vh_stage_vec(chunk), cnt, P->d_limbs, 0, logn, 0, logn - 8, 1, 2, 0));
             }
@@ -2739,12 +2748,12 @@ extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x
                 const bool pipe = OPT.ntt_mode == 2 && C.pipe_blocks > 0;
                 const unsigned nb0 = (OPT.ntt_tile_grid & 1) ? UINT_MAX : C.pipe_blocks;
                 const unsigned nb3 = ((OPT.ntt_tile_grid >> 3) & 1) ? UINT_MAX : C.pipe_blocks;
-                if (which == 0 && pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0, nb0, (int)OPT.ntt_col_wide));
+                if (which == 0 && pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0, nb0, col_wide(cnt, logn)));
                 else if (which == 0) VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0));
                 else if (which == 1 && pipe)
                     VH_CHECK(vh_launch_ntt16p_row_mul(S(), d, cnt, P->d_limbs, logn, C.cus, (int)((OPT.ntt_tile_grid >> 4) & 1)));
                 else if (which == 1) VH_CHECK(vh_launch_ntt16_row_mul_inv(S(), d, cnt, P->d_limbs, logn));
-                else if (pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1, nb3, (int)OPT.ntt_col_wide));
+                else if (pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1, nb3, col_wide(cnt, logn)));
                 else VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1));
             }
         };
