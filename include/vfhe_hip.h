@@ -323,8 +323,9 @@ const char *vfhe_hip_device_name(void);
      "ntt.tile_grid"     mode 2: passes launched one tile per workgroup instead of persistent,
                          bit 0 forward column, 1 forward row, 2 inverse row, 3 inverse column,
                          4 the fused product's row kernel (default 15: the four passes)
-     "ntt.col_wide"      mode 2: 1 = column passes in 512-thread workgroups over 32-column
-                         tiles (default), 0 = 256 threads over 16 columns
+     "ntt.col_wide"      mode 2: 1 = column passes in 512-thread workgroups over twice the
+                         columns per tile when that leaves at least two workgroups per CU
+                         (default), 0 = always 256 threads
      "ntt.xcd_lag"       persistent mode: queue slots between the two passes of a transform
      "ntt.xcd_occupancy" persistent mode: workgroups per CU, 2 or 4
      "ntt.spin_limit"    persistent mode: bound on a wait; exceeding it aborts at the next sync
