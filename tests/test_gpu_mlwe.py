@@ -176,3 +176,82 @@ def test_ckks_config3_shape_multiply_rescale(lib):
         s.octx.to_coeff(ref[c], ref[c], mref)
     m2 = s.octx.mlwe_round_division(ref, r, mref, top)
     s.check(s.pool.read_sample(o), ref, m2)
+
+
+def _oracle_multiply_rescale(s, d1, d2, rlk_np, top):
+    """Oracle: mlwe_multiply (mlwe.c:711-760), then INTT and round division by the top prime (ckks.py:128-148)."""
+    r, m = 1, s.lvl_mask
+    ref = np.zeros((r + 1, s.L + s.K, s.N), dtype=np.uint64)
+    mref = s.octx.multiply(ref, d1, d2, r, m, rlk_np, s.ext_mask)
+    for c in range(r + 1):
+        s.octx.to_coeff(ref[c], ref[c], mref)
+    m2 = s.octx.mlwe_round_division(ref, r, mref, top)
+    return ref, m2
+
+
+@pytest.mark.slow
+def test_ckks_config3_batch32_multiply_rescale(lib):
+    """The benchmarked batch: 32 ciphertext pairs at N = 2^16, L = 24, K = 4 in one batched call (the
+    key-switch row pass remaps workgroups across XCDs only when several ciphertexts share a key block).
+    Every output equals the same pair multiplied alone (batch of one), and two outputs -- the first and
+    the last -- equal the oracle bit for bit."""
+    import concurrent.futures as cf
+    B = 32
+    s = Setup(65536, 24, 4, 43)
+    r, m = 1, s.lvl_mask
+    rlk = [s.key(s.L, r), None]
+    rlk_arr = s.outer(rlk)
+    pairs = [(s.ct(r, m), s.ct(r, m)) for _ in range(B)]
+    top = 1 << (s.L - 1)
+    outs = [s.pool.sample(r, s.ext_mask) for _ in range(B)]
+    print(f"[ckks batch] B={B} N=65536 L=24 K=4", flush=True)
+    lib.vfhe_hip_mlwe_multiply_batch(_native.voidp_array(outs), _native.voidp_array([p[0][1] for p in pairs]),
+                                     _native.voidp_array([p[1][1] for p in pairs]), B, rlk_arr, top)
+    with cf.ThreadPoolExecutor(2) as ex:   # the oracle runs beside the GPU checks (ctypes drops the GIL)
+        futs = {k: ex.submit(_oracle_multiply_rescale, s, pairs[k][0][0], pairs[k][1][0], [rlk[0][0], None], top)
+                for k in (0, B - 1)}
+        one = s.pool.sample(r, s.ext_mask)
+        for k in range(B):
+            lib.vfhe_hip_mlwe_multiply_batch(_native.voidp_array([one]), _native.voidp_array([pairs[k][0][1]]),
+                                             _native.voidp_array([pairs[k][1][1]]), 1, rlk_arr, top)
+            got, alone = s.pool.read_sample(outs[k]), s.pool.read_sample(one)
+            assert s.pool.sample_masks(outs[k]) == s.pool.sample_masks(one)
+            s.check(got, alone, s.pool.sample_masks(one)[0])
+        for k, f in futs.items():
+            ref, m2 = f.result()
+            assert s.pool.sample_masks(outs[k]) == [m2] * (r + 1)
+            s.check(s.pool.read_sample(outs[k]), ref, m2)
+    lib.free_mlwe_RNS_sample(one)
+    for o in outs:
+        lib.free_mlwe_RNS_sample(o)
+    for (_, c1), (_, c2) in pairs:
+        lib.free_mlwe_RNS_sample(c1)
+        lib.free_mlwe_RNS_sample(c2)
+
+
+@pytest.mark.parametrize("N,L,split", [(4096, 4, 1), (65536, 3, 1), (4096, 3, 4)])
+@pytest.mark.parametrize("mode,fn", [(0, "mlwe_RNS_mul_by_poly"), (1, "mlwe_RNS_mul_addto_by_poly"),
+                                     (2, "mlwe_RNS_mul_subto_by_poly")])
+def test_mlwe_mul_by_poly(lib, N, L, split, mode, fn):
+    """mlwe_RNS_mul{,_addto,_subto}_by_poly (mlwe.c:254-279): each of the r+1 components times one
+    NTT-form polynomial (polynomial_mul{,_addto,_subto}_RNS_polynomial, rns_polynomial.c:311-435; with
+    split degree 4 the products wrap through w), against the oracle, bit-exact.  The output's mask is
+    the input's AND the polynomial's."""
+    r = 2
+    s = Setup(N, L, 0, 71 + N + mode + split, split=split)
+    mc, mp = s.lvl_mask, s.lvl_mask & ~0b10 if L > 2 else s.lvl_mask
+    din, cin = s.ct(r, mc)
+    pdat = s.rows(mp)
+    poly = s.pool.poly(pdat, mp)
+    dout, cout = s.ct(r, s.lvl_mask)          # starting values for the accumulating forms
+    print(f"[{fn}] N={N} L={L} split={split}", flush=True)
+    getattr(lib, fn)(cout, cin, poly)
+    ref = dout.copy()
+    for c in range(r + 1):
+        mo = s.octx.mul(ref[c], din[c], mc, pdat, mp, mode)
+    got = s.pool.read_sample(cout)
+    assert s.pool.sample_masks(cout) == [mo] * (r + 1)
+    s.check(got, ref, mo)
+    lib.free_mlwe_RNS_sample(cin)
+    lib.free_mlwe_RNS_sample(cout)
+    s.pool.free(poly)

@@ -109,3 +109,64 @@ def test_one_of_each_against_oracle(lib, rings):
         if (m >> i) & 1:
             assert np.array_equal(g[:, i], out[:, i]), i
     lib.free_mlwe_RNS_sample(got)
+
+
+BIG_NS, BIG_L, BIG_K, BIG_LEVELS = (32768, 65536), 24, 4, (22, 24)
+
+
+@pytest.fixture(scope="module")
+def big_rings(lib):
+    rs = {N: serving.Ring(lib, N, L=BIG_L, K=BIG_K, rot_steps=STEPS, levels=BIG_LEVELS) for N in BIG_NS}
+    yield rs
+    lib.vfhe_hip_synchronize()
+    for r in rs.values():
+        r.free()
+
+
+@pytest.mark.slow
+def test_large_rings_same_alone_and_in_mix(lib, big_rings):
+    """The benchmarked upper end of the mix: N = 2^15 and 2^16 at levels 22..24 of a 24 + 4 prime ring.
+    Every output of the served mix equals the same request run alone, bit for bit."""
+    wl = serving.make_workload(20000.0, 0.003, BIG_NS, levels=BIG_LEVELS, rot_steps=STEPS, seed=5)
+    assert len(wl) > 30 and any(r.N == 65536 and r.level == 24 for r in wl)
+    print(f"[serving large] {len(wl)} requests", flush=True)
+    res = serving.run(lib, big_rings, wl, memory_budget=8 << 30, lanes=4, max_batch=8, keep_outputs=True)
+    print({k: v for k, v in res.items() if k != "outputs"}, flush=True)
+    assert res["requests"] == len(wl) and res["pending"] == 0
+    assert res["batches"] < len(wl)
+    for k, (req, out) in enumerate(zip(wl, res["outputs"])):
+        ring = big_rings[req.N]
+        ref = alone(lib, ring, req)
+        masks = ring.pool.sample_masks(out)
+        assert masks == ring.pool.sample_masks(ref), k
+        rows = [i for i in range(len(ring.primes)) if (masks[0] >> i) & 1]
+        assert np.array_equal(ring.pool.read_sample(out)[:, rows], ring.pool.read_sample(ref)[:, rows]), (k, req)
+        lib.free_mlwe_RNS_sample(ref)
+        lib.free_mlwe_RNS_sample(out)
+
+
+@pytest.mark.slow
+def test_level24_n65536_against_oracle(lib, big_rings):
+    """A served multiply + relinearize + rescale at N = 2^16, level 24, against the oracle bit for bit."""
+    N, lvl = 65536, 24
+    ring = big_rings[N]
+    c = O.Ctx(N, ring.primes)
+    lm, ext = (1 << lvl) - 1, ((1 << lvl) - 1) | ring.special
+    c1, c2 = ring.cts[lvl]
+    x1, x2 = ring.pool.read_sample(c1), ring.pool.read_sample(c2)
+    key = np.ascontiguousarray(np.stack([ring.pool.read_sample(ring.rlk_top[d]) for d in range(lvl)]))
+    wl = [serving.Request(0.0, serving.REQ_MULTIPLY, N, lvl, 0)]
+    print("[serving large] one multiply at N=65536 level 24 against the oracle", flush=True)
+    res = serving.run(lib, big_rings, wl, memory_budget=8 << 30, lanes=2, max_batch=4, keep_outputs=True)
+    got = res["outputs"][0]
+    out = np.zeros((2, len(ring.primes), N), dtype=np.uint64)
+    m = c.multiply(out, x1, x2, 1, lm, [key, None], ext)
+    for comp in range(2):
+        c.to_coeff(out[comp], out[comp], m)
+        mm = c.division(out[comp], m, 1 << (lvl - 1))
+    assert ring.pool.sample_masks(got) == [mm, mm]
+    g = ring.pool.read_sample(got)
+    for i in range(len(ring.primes)):
+        if (mm >> i) & 1:
+            assert np.array_equal(g[:, i], out[:, i]), i
+    lib.free_mlwe_RNS_sample(got)

@@ -258,6 +258,93 @@ def test_division_closed_form(rnd):
     assert mism > 0
 
 
+@pytest.mark.parametrize("bits", [20, 31, 40, 49, 50, 58, 62])
+def test_reduce_mp_against_bigint_and_reference(bits):
+    """o_reduce_mp (mod_reduce_array_mp, mod.c:261-276): (hi*2^64 + lo) mod q for every 128-bit input.
+    Checked against Python big ints; against the reference's portable mod_reduce_array_mp
+    (mod_portable.c:153-161, exact for every input) on all inputs; and against both builds of oracle/_ref
+    on the reference test's own inputs (test_mod.c:167-176).  The avx512ifma build's kernel for q < 2^32
+    (mod32.c:366-411) forms (hi mod q)*(2^64 mod q) + lo in 64 bits, which wraps once lo + (q-1)^2
+    reaches 2^64; the reference's test inputs stay below that, and the oracle follows the exact
+    portable definition."""
+    n = 512
+    q = int(O.lib().o_next_special_prime(1 << bits, n, 1))
+    rng = np.random.default_rng(bits)
+    hi = rng.integers(0, 2**63, n, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, n, dtype=np.uint64)
+    lo = rng.integers(0, 2**63, n, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, n, dtype=np.uint64)
+    # edge values: 0, 2^64 - 1 halves, multiples of q, q - 1, the largest 128-bit value
+    edges = [(0, 0), (0, 1), (0, q - 1), (0, q), (1, 0), (2**64 - 1, 2**64 - 1), (q - 1, 2**64 - 1),
+             (0, 2**64 - 1), (2**63, 0), ((q << 1) >> 64, (q << 1) & (2**64 - 1))]
+    for k, (h, l_) in enumerate(edges):
+        hi[k], lo[k] = h, l_
+
+    def check(hi, lo, engines):
+        out = np.zeros(n, dtype=np.uint64)
+        O.lib().o_reduce_mp(O.ptr(out), O.ptr(hi), O.ptr(lo), n, q)
+        exp = [((int(h) << 64) | int(l_)) % q for h, l_ in zip(hi, lo)]
+        assert [int(v) for v in out] == exp
+        for engine in engines:
+            L = O.ref_lib(engine)
+            proc = L.ntt_new_proc(n, q)
+            o, h, l_ = O.aligned(n), O.aligned(n), O.aligned(n)
+            h[:], l_[:] = hi, lo
+            L.mod_reduce_array_mp(O.ptr(o), O.ptr(h), O.ptr(l_), n, proc)
+            L.ntt_free_proc(proc)
+            assert np.array_equal(o, out), (engine, q)
+        return exp
+
+    exp = check(hi, lo, [e for e in O.ref_engines() if e == "portable" or q >= 2**32])
+    i = np.arange(n, dtype=np.uint64)
+    check(np.uint64(0xAAAAAAAAAAAAAAAA) ^ (i * np.uint64(0x11111111)),
+          np.uint64(0x5555555555555555) ^ (i * np.uint64(0x22222222)), O.ref_engines())
+    # a planted mistake -- the high word dropped -- fails the big-int check
+    bad = (lo % np.uint64(q)).astype(np.uint64)
+    assert [int(v) for v in bad] != exp
+
+
+@pytest.mark.parametrize("split", [1, 4])
+def test_add_integer_closed_forms(split):
+    """o_add_integer (rns_polynomial.c:474-514).  Coefficient form: the CRT value of coefficient 0
+    moves by v (signed, reduced mod Q) and nothing else changes.  NTT form: adding v to every slot of
+    the first block is what the NTT makes of adding v to coefficient 0 (X^0 evaluates to 1 at every
+    root), for positive and negative v."""
+    N = 32
+    primes = O.gen_primes(N, [50] * 4, split)
+    c = O.Ctx(N, primes, split)
+    L = len(primes)
+    Q = math.prod(primes)
+    rng = np.random.default_rng(split)
+    for m in (0b1111, 0b1011):
+        act = [i for i in range(L) if (m >> i) & 1]
+        Qm = math.prod(primes[i] for i in act)
+        for v in (0, 1, 7, 2**63 - 1, -1, -5, -(2**62), -primes[0], -(2**63), primes[1] * 3):
+            x = np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in primes])
+            y = np.zeros_like(x)
+            assert c.add_integer(y, x, m, v, False) == m
+            X0 = crt([int(x[i][0]) for i in act], [primes[i] for i in act])
+            Y0 = crt([int(y[i][0]) for i in act], [primes[i] for i in act])
+            assert Y0 == (X0 + v) % Qm, v
+            for i in act:
+                assert np.array_equal(y[i][1:], x[i][1:])
+            # NTT form: NTT(x + v) == NTT(x) (+) v on the first block
+            xn, yn = x.copy(), y.copy()
+            c.to_ntt(xn, xn, m)
+            c.to_ntt(yn, yn, m)
+            zn = np.zeros_like(xn)
+            c.add_integer(zn, xn, m, v, True)
+            n_blk = N // split
+            for i in act:
+                assert np.array_equal(zn[i][:n_blk], yn[i][:n_blk]), (v, i)
+                assert np.array_equal(zn[i][n_blk:], xn[i][n_blk:]), (v, i)
+    # a planted mistake -- a negative v taken as its unsigned 64-bit value -- fails the CRT check
+    x = np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in primes])
+    y = np.zeros_like(x)
+    c.add_integer(y, x, 0b1111, 2**64 - 5, False)   # what a sign-blind oracle would compute for v = -5
+    Y0 = crt([int(y[i][0]) for i in range(L)], primes)
+    X0 = crt([int(x[i][0]) for i in range(L)], primes)
+    assert Y0 != (X0 + 2**64 - 5) % Q
+
+
 def test_mod_reduce_lifted_closed_form():
     primes, c, rng = small_ctx()
     x = np.stack([rng.integers(0, q, 16, dtype=np.uint64) for q in primes])

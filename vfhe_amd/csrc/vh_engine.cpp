@@ -597,12 +597,18 @@ static void vh_check_device_error()
 }
 
 // Column passes (N = 2^13..2^16) use 512-thread workgroups over twice the columns per tile
-// (ntt.col_wide, 2^(logn-13) tiles per row item) when that still gives every CU two
+// (ntt.col_wide = 1, 2^(logn-13) tiles per row item) when that still gives every CU two
 // workgroups; a smaller launch keeps the 256-thread tiles, twice as many workgroups.
+// ntt.col_wide = 2 takes the wide tiles at every launch size (tests use it to reach the
+// 512-thread instances with small batches).  Every wide launch is counted
+// (vfhe_hip_get_option("stat.col_wide_launches")).
+static long g_col_wide_launches = 0;
 static int col_wide(unsigned items, int logn)
 {
     const uint64_t tiles = (uint64_t)items << (logn - 13);
-    return OPT.ntt_col_wide && tiles >= 2 * (uint64_t)C.cus ? 1 : 0;
+    const int w = OPT.ntt_col_wide >= 2 || (OPT.ntt_col_wide == 1 && tiles >= 2 * (uint64_t)C.cus) ? 1 : 0;
+    g_col_wide_launches += w;
+    return w;
 }
 
 // Forward (dir 0) or inverse (dir 1) transforms of n-point rows.
@@ -1060,6 +1066,7 @@ extern "C" long vfhe_hip_get_option(const char *key)
     if (!strcmp(key, "ntt.spin_limit")) return OPT.spin_limit;
     if (!strcmp(key, "ks.fused")) return OPT.ks_fused;
     if (!strcmp(key, "fft.fma")) return OPT.fft_fma;
+    if (!strcmp(key, "stat.col_wide_launches")) return g_col_wide_launches;
     return -1;
 }
 
@@ -3232,12 +3239,15 @@ extern "C" long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, doubl
     long best = 0;
     double best_us = 1e300;
     std::vector<double> us(nc, -1.0);
+    std::vector<bool> wide_ran(nc, false);
     for (int round = 0; round < 2; round++)   // two interleaved rounds, the faster of each kept
         for (long k = 0; k < nc; k++) {
             const vh_sched &sc = VH_SCHEDS[k];
             if (sc.mode == 1 && !(C.xcd_ok && logn >= 13 && logn <= 16)) continue;
             OPT.ntt_mode = sc.mode; OPT.ntt_chunk_mb = sc.chunk_mb; OPT.xcd_lag = sc.lag; OPT.xcd_occ = sc.occ;
-            OPT.ntt_tile_grid = sc.tile_grid; OPT.ntt_col_wide = sc.col_wide;
+            OPT.ntt_tile_grid = (saved.ntt_tile_grid & ~15L) | sc.tile_grid;
+            OPT.ntt_col_wide = sc.col_wide;
+            const long wide0 = g_col_wide_launches;
             for (int w = 0; w < (round == 0 && k == 0 ? 3 : 1); w++) {   // warm-up
                 run_ntt(items, P->d_limbs, P->h_limbs.data(), n, 0);
                 run_ntt(items, P->d_limbs, P->h_limbs.data(), n, 1);
@@ -3254,6 +3264,7 @@ extern "C" long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, doubl
             VH_CHECK(hipEventElapsedTime(&ms, e0, e1));
             const double t = ms * 1e3 / reps;
             if (us[k] < 0 || t < us[k]) us[k] = t;
+            wide_ran[k] = g_col_wide_launches > wide0;
         }
     for (long k = 0; k < nc; k++) {
         if (us_out) us_out[k] = us[k];
@@ -3261,10 +3272,14 @@ extern "C" long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, doubl
     }
     VH_CHECK(hipEventDestroy(e0));
     VH_CHECK(hipEventDestroy(e1));
+    // Adopt the winner's timed settings only: the fused product's row-kernel bit (tile_grid
+    // bit 4) keeps its value, and the 512-thread column rule is enabled only when the winning
+    // candidate's timed launches actually ran wide (otherwise the caller's setting stays).
     OPT = saved;
     OPT.ntt_mode = VH_SCHEDS[best].mode; OPT.ntt_chunk_mb = VH_SCHEDS[best].chunk_mb;
     OPT.xcd_lag = VH_SCHEDS[best].lag; OPT.xcd_occ = VH_SCHEDS[best].occ;
-    OPT.ntt_tile_grid = VH_SCHEDS[best].tile_grid; OPT.ntt_col_wide = VH_SCHEDS[best].col_wide;
+    OPT.ntt_tile_grid = (saved.ntt_tile_grid & ~15L) | VH_SCHEDS[best].tile_grid;
+    if (VH_SCHEDS[best].col_wide == 0 || wide_ran[best]) OPT.ntt_col_wide = VH_SCHEDS[best].col_wide;
     vh_dfree(d);
     return best;
 }
