@@ -80,6 +80,23 @@ typedef struct _RNS_MLWE_Key {
     double sigma;
 } *RNS_MLWE_Key;
 
+/* LWE key and sample: modules/mlwe/c/include/mlwe.h, cdef mlwe.cdef:11-25.
+   Host-resident arrays (a[limb][n], b[limb], s[limb][n]). */
+typedef struct _LWE_Key {
+    uint64_t **s;
+    uint64_t n;
+    uint64_t l;
+    incNTT ntt;
+    double sigma;
+} *LWE_Key;
+typedef struct _LWE {
+    uint64_t **a;
+    uint64_t *b;
+    uint64_t n;
+    uint64_t l;
+    incNTT ntt;
+} *LWE;
+
 /* Fast base conversion parameters: modules/arith/c/include/arith.h:211-221 */
 typedef struct _RNS_BaseConversionParams {
     uint64_t in_mask;
@@ -184,6 +201,13 @@ void polynomial_base_conversion_RNSc(RNSc_Polynomial out, RNSc_Polynomial in, RN
 void rns_compute_scaling_factors(uint64_t *delta_out, incNTT ntt, uint64_t in_mask, uint64_t out_mask); /* :1463-1494 */
 void polynomial_RNSc_scaled_lift(RNSc_Polynomial out, RNSc_Polynomial in, uint64_t *delta);           /* :1497-1525 */
 void polynomial_RNSc_permute(RNSc_Polynomial out, RNSc_Polynomial in, uint64_t gen);                  /* :996-1035 */
+/* Slots of NTT-form polynomials (device): */
+void polynomial_RNS_broadcast_slot(RNS_Polynomial out, RNS_Polynomial in, uint64_t slot_idx);           /* :1228-1244 */
+void polynomial_RNS_rotate_slot(RNS_Polynomial out, RNS_Polynomial in, uint64_t rot);                   /* :1262-1285 */
+void polynomial_RNS_copy_slot(RNS_Polynomial out, uint64_t dst, RNS_Polynomial in, uint64_t src);       /* :1287-1300 */
+/* Slot-wise inverse (:1384-1432): 0, -2 for a zero slot; -1 for split degree > 1, whose
+   extension-field slots (field.c) this engine does not implement. */
+int polynomial_RNS_inverse(RNS_Polynomial out, RNS_Polynomial in);
 
 /* ------------------------------------------------------------------------ */
 /* MLWE: modules/mlwe/c/src/mlwe.c, gadget.c                                 */
@@ -212,7 +236,29 @@ void mlwe_RNS_add_polynomial(RNS_MLWE out, RNS_MLWE in1, RNS_Polynomial in2);   
 void mlwe_RNS_sub_polynomial(RNS_MLWE out, RNS_MLWE in1, RNS_Polynomial in2);                          /* mlwe.c:479-482 */
 void mlwe_scale_RNS_mlwe_RNS(RNS_MLWE c, uint64_t *scale);                                            /* mlwe.c:217-225 */
 void mlwe_round_division_RNSc(RNSc_MLWE out, uint64_t divide_mask);                                   /* mlwe.c:762-772 */
-uint64_t mlwe_extended_rank(uint64_t r);                                                              /* mlwe.c:659-663 */
+uint64_t mlwe_extended_rank(uint64_t r);
+/* Sample extraction (mlwe.c:40-68): coefficient idx of a coefficient-form sample as an
+   LWE sample of dimension r*N over the active limbs of a[0] (host arrays). */
+LWE mlwe_extract_LWE(RNSc_MLWE in, uint64_t idx);
+/* Trace and packing (mlwe.c:328-445, 596-657), on the device. */
+void mlwe_partial_trace(RNSc_MLWE out, RNSc_MLWE in, uint64_t *gens, RNS_MLWE ***ksks, uint64_t size,
+                        uint64_t lvl);                                                                   /* mlwe.c:328-341 */
+void mlwe_trace(RNSc_MLWE out, RNSc_MLWE in, RNS_MLWE ***ksks, uint64_t lvl);                           /* mlwe.c:436-445 */
+void mlwe_full_packing_keyswitch(RNS_MLWE out, LWE *in, uint64_t size, RNS_MLWE **ksk, uint64_t lvl);    /* mlwe.c:343-434 */
+void mlwe_full_packing_keyswitch_scaled(RNSc_MLWE *vec, uint64_t ell, RNS_MLWE ***ksks, uint64_t lvl);  /* mlwe.c:600-657 */
+
+/* LWE (modules/mlwe/c/src/lwe.c), host arrays; randomness from the engine's ChaCha20 key. */
+LWE_Key lwe_alloc_key(uint64_t n, uint64_t l, incNTT ntt);                                      /* lwe.c:6-18 */
+LWE lwe_alloc_sample(uint64_t n, uint64_t l, incNTT ntt);                                       /* lwe.c:20-33 */
+void free_lwe_sample(LWE c);                                                                    /* lwe.c:35-44 */
+void free_lwe_key(LWE_Key key);                                                                 /* lwe.c:46-54 */
+LWE_Key lwe_new_key(uint64_t n, uint64_t l, incNTT ntt, double sec_sigma, double err_sigma);    /* lwe.c:56-70 */
+LWE_Key lwe_new_sparse_ternary_key(uint64_t n, uint64_t l, incNTT ntt, uint64_t h, double err_sigma); /* lwe.c:72-95 */
+void lwe_sample(LWE c, uint64_t *m, LWE_Key key);                                               /* lwe.c:97-123 */
+LWE lwe_new_sample(uint64_t *m, LWE_Key key);                                                   /* lwe.c:125-130 */
+LWE lwe_new_trivial_sample(uint64_t *m, uint64_t n, uint64_t l, incNTT ntt);                    /* lwe.c:132-142 */
+void lwe_phase(uint64_t *out, LWE c, LWE_Key key);                                              /* lwe.c:144-158 */
+void lwe_subto(LWE out, LWE in);                                                                /* lwe.c:160-169 */                                                              /* mlwe.c:659-663 */
 void mlwe_tensor_product(RNS_Polynomial *out, RNS_MLWE in1, RNS_MLWE in2);                            /* mlwe.c:665-709 */
 void mlwe_RNSc_GHS_hybrid_keyswitch(RNSc_MLWE out, RNSc_MLWE in, RNS_MLWE **ksk, uint64_t lvl);       /* mlwe.c:557-598 + gadget.c:22-37 */
 void mlwe_multiply(RNS_MLWE out, RNS_MLWE in1, RNS_MLWE in2, RNS_MLWE **ksk);                         /* mlwe.c:711-760 */
@@ -225,6 +271,7 @@ void mlwe_RNS_phase(RNS_Polynomial out, RNS_MLWE in, RNS_MLWE_Key key);         
 /* Engine identity: modules/misc/python/cdef/misc.cdef:8, vfhe_cpu.cdef:6     */
 /* ------------------------------------------------------------------------ */
 const char *vfhe_engine_active(void);        /* returns "hip" */
+void *safe_aligned_malloc(uint64_t size);    /* misc_tp.c:48-62 (arith.cdef:26): 64-byte aligned host memory, freed with libc free */
 int vfhe_cpu_supports(const char *capability); /* "hip": a gfx950 device is visible */
 
 /* ------------------------------------------------------------------------ */
@@ -373,6 +420,18 @@ void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polyn
    when divide_mask == 0. */
 void vfhe_hip_mlwe_multiply_batch(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLWE *in2, uint64_t count,
                                   RNS_MLWE **rlk, uint64_t divide_mask);
+/* Graph form of vfhe_hip_mlwe_multiply_batch (same arguments, same results bit for
+   bit).  The first call of a batch shape (ring, count, rank, every operand's and key
+   entry's limb mask, divide mask, lane, schedule options) runs eagerly and captures
+   the batch's launch sequence -- tensor product, INTT, ModUp column pass, key-switch
+   row pass + inner product, INTT, ModDown + rescale -- into a HIP graph owned by the
+   engine; later calls of that shape rebuild only the work lists for their operands
+   and replay the graph (one launch, no device allocation: the temporaries are the
+   plan's own).  Operands written from the host must be marked dirty as usual; they
+   are uploaded before the replay.  vfhe_hip_graph_clear frees every plan. */
+void vfhe_hip_mlwe_multiply_batch_graph(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLWE *in2, uint64_t count,
+                                        RNS_MLWE **rlk, uint64_t divide_mask);
+void vfhe_hip_graph_clear(void);
 
 /* mgsw_CMUX for `count` independent (out, in1, in2, mgsw) quadruples in one
    launch sequence (a batch of blind rotations stepping through one key index). */

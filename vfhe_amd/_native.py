@@ -64,6 +64,7 @@ _SIGNATURES = {
     "next_special_prime": (U64, [U64, U64, ctypes.c_bool]),
     "bit_rev": (None, [U64P, U64P, U64, U64]),
     "vfhe_engine_active": (ctypes.c_char_p, []),
+    "safe_aligned_malloc": (VP, [U64]),
     "vfhe_cpu_supports": (I32, [ctypes.c_char_p]),
     # single-limb kernels
     "ntt_new_proc": (VP, [U64, U64]),
@@ -217,9 +218,32 @@ _SIGNATURES = {
     "vfhe_hip_get_option": (ctypes.c_long, [ctypes.c_char_p]),
     "vfhe_hip_profile_start": (None, []),
     "vfhe_hip_profile_stop": (None, [ctypes.POINTER(ctypes.c_double), U64P]),
+    # slots, LWE, extraction, trace and packing
+    "polynomial_RNS_broadcast_slot": (None, [VP, VP, U64]),
+    "polynomial_RNS_rotate_slot": (None, [VP, VP, U64]),
+    "polynomial_RNS_copy_slot": (None, [VP, U64, VP, U64]),
+    "polynomial_RNS_inverse": (I32, [VP, VP]),
+    "mlwe_extract_LWE": (VP, [VP, U64]),
+    "mlwe_partial_trace": (None, [VP, VP, U64P, VP, U64, U64]),
+    "mlwe_trace": (None, [VP, VP, VP, U64]),
+    "mlwe_full_packing_keyswitch": (None, [VP, VPP, U64, VPP, U64]),
+    "mlwe_full_packing_keyswitch_scaled": (None, [VPP, U64, VP, U64]),
+    "lwe_alloc_key": (VP, [U64, U64, VP]),
+    "lwe_alloc_sample": (VP, [U64, U64, VP]),
+    "free_lwe_sample": (None, [VP]),
+    "free_lwe_key": (None, [VP]),
+    "lwe_new_key": (VP, [U64, U64, VP, ctypes.c_double, ctypes.c_double]),
+    "lwe_new_sparse_ternary_key": (VP, [U64, U64, VP, U64, ctypes.c_double]),
+    "lwe_sample": (None, [VP, U64P, VP]),
+    "lwe_new_sample": (VP, [U64P, VP]),
+    "lwe_new_trivial_sample": (VP, [U64P, U64, U64, VP]),
+    "lwe_phase": (None, [U64P, VP, VP]),
+    "lwe_subto": (None, [VP, VP]),
     "vfhe_hip_ntt_batch": (None, [VPP, U64, I32]),
     "vfhe_hip_ntt_mul_intt_batch": (None, [VPP, VPP, VPP, U64]),
     "vfhe_hip_mlwe_multiply_batch": (None, [VPP, VPP, VPP, U64, VPP, U64]),
+    "vfhe_hip_mlwe_multiply_batch_graph": (None, [VPP, VPP, VPP, U64, VPP, U64]),
+    "vfhe_hip_graph_clear": (None, []),
 }
 
 _lib = None

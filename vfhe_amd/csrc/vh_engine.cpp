@@ -44,6 +44,42 @@ typedef unsigned __int128 u128;
 #define ACTIVE(mask, i) ((((mask) >> (i)) & 1ULL) != 0)
 
 // ---------------------------------------------------------------------------
+// Graph capture state (vfhe_hip_mlwe_multiply_batch_graph).  A plan holds one
+// batch shape's captured launch sequence.  Modes:
+//   LOG     the batch runs eagerly; the sizes of its staged work lists and
+//           device allocations are logged in order;
+//   RECORD  the host logic runs again under stream capture: work lists are
+//           written into the plan's pinned buffer (copied to the device by the
+//           graph's first node), allocations come from the plan's own blocks,
+//           frees are no-ops;
+//   DRY     the host logic runs once more to rebuild the work lists for new
+//           operands, with every launch skipped; the graph is then replayed.
+// A staged size, allocation size or launch count that differs from the plan
+// raises vh_plan_mismatch (the caller falls back to the eager path).
+// ---------------------------------------------------------------------------
+enum { VH_CAP_OFF = 0, VH_CAP_LOG = 1, VH_CAP_RECORD = 2, VH_CAP_DRY = 3 };
+struct vh_plan_mismatch {};
+struct vh_graph_plan {
+    std::vector<uint64_t> key;
+    std::vector<size_t> stage_sz, block_sz;
+    std::vector<void *> blocks;
+    size_t wl_bytes = 0;
+    char *h_wl = nullptr, *d_wl = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
+    bool launched = false;
+    long launches = 0;
+};
+static int g_cap = VH_CAP_OFF;
+static vh_graph_plan *g_plan = nullptr;
+static size_t g_cap_stage = 0, g_cap_off = 0, g_cap_block = 0;
+static long g_cap_launches = 0;
+static std::vector<vh_graph_plan *> g_plans;
+static long g_graph_captures = 0, g_graph_replays = 0, g_graph_fallbacks = 0;
+#define VH_LAUNCH(x) do { if (g_cap >= VH_CAP_RECORD) g_cap_launches++; if (g_cap != VH_CAP_DRY) VH_CHECK(x); } while (0)
+
+// ---------------------------------------------------------------------------
 // Engine-private tails of the public structs
 // ---------------------------------------------------------------------------
 struct _NTT_proc {
@@ -178,7 +214,7 @@ static void vh_ensure()
         const unsigned nb = (unsigned)C.cus * 4;
         unsigned *d = nullptr;
         VH_CHECK(hipMalloc((void **)&d, nb * sizeof(unsigned)));
-        VH_CHECK(vh_launch_xcc_probe(S(), d, nb));
+        VH_LAUNCH(vh_launch_xcc_probe(S(), d, nb));
         std::vector<unsigned> h(nb);
         VH_CHECK(hipMemcpyAsync(h.data(), d, nb * sizeof(unsigned), hipMemcpyDeviceToHost, S()));
         VH_CHECK(hipStreamSynchronize(S()));
@@ -199,6 +235,16 @@ static void *vh_stage(const void *src, size_t bytes)
 {
     vh_lane &L = LN();
     const size_t need = (bytes + 255) & ~(size_t)255, half = L.ring_size / 2;
+    if (g_cap == VH_CAP_LOG) {
+        g_plan->stage_sz.push_back(need);
+    } else if (g_cap >= VH_CAP_RECORD) {
+        if (g_cap_stage >= g_plan->stage_sz.size() || g_plan->stage_sz[g_cap_stage] != need) throw vh_plan_mismatch();
+        memcpy(g_plan->h_wl + g_cap_off, src, bytes);
+        char *d = g_plan->d_wl + g_cap_off;
+        g_cap_off += need;
+        g_cap_stage++;
+        return d;
+    }
     VH_REQUIRE(need <= half);
     if (L.ring_off + need > (size_t)(L.cur_half + 1) * half) {
         VH_CHECK(hipEventRecord(L.half_ev[L.cur_half], L.stream));
@@ -244,7 +290,7 @@ struct vh_prof_scope {
     int tag; hipEvent_t a = nullptr, b = nullptr;
     explicit vh_prof_scope(int t) : tag(t)
     {
-        if (!g_prof) return;
+        if (!g_prof || g_cap >= VH_CAP_RECORD) return;
         a = prof_event(); b = prof_event();
         VH_CHECK(hipEventRecord(a, S()));
     }
@@ -317,6 +363,12 @@ static void *vh_take_parked(int ln, size_t c)
 static void *vh_dmalloc(size_t bytes)
 {
     const size_t c = vh_size_class(bytes);
+    if (g_cap == VH_CAP_LOG) {
+        g_plan->block_sz.push_back(c);
+    } else if (g_cap >= VH_CAP_RECORD) {
+        if (g_cap_block >= g_plan->block_sz.size() || g_plan->block_sz[g_cap_block] != c) throw vh_plan_mismatch();
+        return g_plan->blocks[g_cap_block++];
+    }
     if (void *p = vh_take_parked(g_lane, c)) return p;
     for (int ln = 0; ln < VH_MAX_LANES; ln++)
         if (ln != g_lane && !BC.free_[ln].empty())
@@ -339,7 +391,7 @@ static void *vh_dmalloc(size_t bytes)
 
 static void vh_dfree(void *p)
 {
-    if (!p) return;
+    if (!p || g_cap >= VH_CAP_RECORD) return;   // a plan's blocks stay with the plan
     auto it = BC.size_of.find(p);
     VH_REQUIRE(it != BC.size_of.end());
     const size_t c = it->second;
@@ -631,15 +683,15 @@ static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const v
                 // Persistent kernel: both passes of every transform on one XCD, the
                 // intermediate exchanged through that XCD's L2.
                 unsigned *ctl = (unsigned *)vh_dmalloc((16 + 2 * cnt) * sizeof(unsigned));
-                VH_CHECK(vh_launch_zero_u32(S(), ctl, 16 + 2 * cnt));
+                VH_LAUNCH(vh_launch_zero_u32(S(), ctl, 16 + 2 * cnt));
                 const int occ = OPT.xcd_occ >= 4 ? 4 : 2;
                 {
                     vh_prof_scope ps_(VH_PROF_NTT_FP64);
-                    VH_CHECK(vh_launch_ntt_xcd(S(), d_items, cnt, d_limbs, dir, logn, ctl, C.d_err,
+                    VH_LAUNCH(vh_launch_ntt_xcd(S(), d_items, cnt, d_limbs, dir, logn, ctl, C.d_err,
                                                (unsigned)OPT.xcd_lag, (unsigned)OPT.spin_limit,
                                                (unsigned)(C.cus * occ), occ));
                 }
-                VH_CHECK(vh_launch_ntt_xcd_verify(S(), ctl, cnt, logn, C.d_err));
+                VH_LAUNCH(vh_launch_ntt_xcd_verify(S(), ctl, cnt, logn, C.d_err));
                 vh_dfree(ctl);
             } else if (fp && logn >= 13 && logn <= 16) {
                 // Two register radix-16 passes (column pass of R1 = logn - 8 stages, row pass of 8),
@@ -654,32 +706,32 @@ static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const v
                     if (pipe) {
                         auto nb = [&](int bit) -> unsigned { return (OPT.ntt_tile_grid >> bit) & 1 ? UINT_MAX : C.pipe_blocks; };
                         if (dir == 0) {
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0, nb(0), col_wide(cc, logn))); }
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1, nb(1))); }
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0, nb(0), col_wide(cc, logn))); }
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1, nb(1))); }
                         } else {
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0, nb(2))); }
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1, nb(3), col_wide(cc, logn))); }
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0, nb(2))); }
+                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1, nb(3), col_wide(cc, logn))); }
                         }
                     } else if (dir == 0) {
-                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0)); }
-                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1)); }
+                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0)); }
+                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1)); }
                     } else {
-                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(S(), ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0)); }
-                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_CHECK(vh_launch_ntt16(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1)); }
+                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16(S(), ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0)); }
+                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1)); }
                     }
                 }
             } else if (logn <= 12) {
-                { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, dir, logn, 0, logn, 0, 1, 1, 1)); }
+                { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, dir, logn, 0, logn, 0, 1, 1, 1)); }
             } else {
                 const int R1 = logn - 8, R2 = 8;
                 const int Gc = std::min(1 << (logn - R1), 4096 >> R1);
                 const int Gr = std::min(1 << R1, 16);
                 if (dir == 0) {
-                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 0, logn, 0, R1, 1, Gc, 1, 0)); }
-                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 0, logn, R1, R2, 0, Gr, 0, 1)); }
+                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 0, logn, 0, R1, 1, Gc, 1, 0)); }
+                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 0, logn, R1, R2, 0, Gr, 0, 1)); }
                 } else {
-                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 1, logn, R1, R2, 0, Gr, 1, 0)); }
-                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_CHECK(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 1, logn, 0, R1, 1, Gc, 0, 1)); }
+                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 1, logn, R1, R2, 0, Gr, 1, 0)); }
+                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 1, logn, 0, R1, 1, Gc, 0, 1)); }
                 }
             }
         }
@@ -693,7 +745,7 @@ static void run_eltwise(std::vector<vh_item> &items, const vh_limb *d_limbs, int
         std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
         {
             vh_prof_scope ps_(VH_PROF_ELTWISE);
-            VH_CHECK(vh_launch_eltwise(S(), vh_stage_vec(chunk), cnt, d_limbs, op, n));
+            VH_LAUNCH(vh_launch_eltwise(S(), vh_stage_vec(chunk), cnt, d_limbs, op, n));
         }
     }
 }
@@ -914,6 +966,7 @@ static void dev_ready(RNS_Polynomial p)
 {
     vh_poly *v = VP(p);
     if (!v->host_dirty) return;
+    if (g_cap >= VH_CAP_RECORD) throw vh_plan_mismatch();   // uploads happen before capture
     VH_CHECK(hipMemcpyAsync(v->d, v->hblock, p->allocated_l * p->ntt->N * 8, hipMemcpyHostToDevice, S()));
     VH_CHECK(hipStreamSynchronize(S()));
     v->host_dirty = 0;
@@ -947,7 +1000,8 @@ static RNS_Polynomial new_poly(uint64_t N, uint64_t rns_mask, incNTT ntt, bool z
     for (uint64_t i = 0; i < l; i++) v->pub.coeffs[i] = nullptr;
     if (mirror) ensure_mirror(v);
     v->d = (uint64_t *)vh_dmalloc(l * N * 8);
-    if (zero) VH_CHECK(hipMemsetAsync(v->d, 0, l * N * 8, S()));
+    if (zero && g_cap >= VH_CAP_RECORD) VH_LAUNCH(vh_launch_zero_u32(S(), (unsigned *)v->d, (unsigned)(l * N * 2)));
+    else if (zero) VH_CHECK(hipMemsetAsync(v->d, 0, l * N * 8, S()));
     v->host_dirty = 0;
     return &v->pub;
 }
@@ -1067,6 +1121,10 @@ extern "C" long vfhe_hip_get_option(const char *key)
     if (!strcmp(key, "ks.fused")) return OPT.ks_fused;
     if (!strcmp(key, "fft.fma")) return OPT.fft_fma;
     if (!strcmp(key, "stat.col_wide_launches")) return g_col_wide_launches;
+    if (!strcmp(key, "stat.graph_captures")) return g_graph_captures;
+    if (!strcmp(key, "stat.graph_replays")) return g_graph_replays;
+    if (!strcmp(key, "stat.graph_fallbacks")) return g_graph_fallbacks;
+    if (!strcmp(key, "stat.graph_plans")) return (long)g_plans.size();
     return -1;
 }
 
@@ -1224,7 +1282,7 @@ static void int_rows_to_RNS(const std::vector<std::pair<RNS_Polynomial, const ui
         const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, items.size() - off);
         std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
         vh_prof_scope ps_(VH_PROF_OTHER);
-        VH_CHECK(vh_launch_scatter_split(S(), vh_stage_vec(chunk), cnt, DL(p0), p0->ntt->N, 1));
+        VH_LAUNCH(vh_launch_scatter_split(S(), vh_stage_vec(chunk), cnt, DL(p0), p0->ntt->N, 1));
     }
     ntt_rows(pairs, 0);
 }
@@ -1261,7 +1319,7 @@ extern "C" void array_to_RNS(RNS_Polynomial out, uint64_t **in)
     if (!items.empty())
         {
             vh_prof_scope ps_(VH_PROF_OTHER);
-            VH_CHECK(vh_launch_scatter_split(S(), vh_stage_vec(items), (unsigned)items.size(), DL(out), N, 0));
+            VH_LAUNCH(vh_launch_scatter_split(S(), vh_stage_vec(items), (unsigned)items.size(), DL(out), N, 0));
         }
     vh_dfree(tmp);
     ntt_rows({{out, out}}, 0);
@@ -1299,7 +1357,7 @@ static void run_muls(const std::vector<vh_mulreq> &reqs, int mode)
             std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
             {
                 vh_prof_scope ps_(VH_PROF_ELTWISE);
-                VH_CHECK(vh_launch_split_mul(S(), vh_stage_vec(chunk), cnt, P->d_limbs, mode, P->n, t->split_degree));
+                VH_LAUNCH(vh_launch_split_mul(S(), vh_stage_vec(chunk), cnt, P->d_limbs, mode, P->n, t->split_degree));
             }
         }
     }
@@ -1531,7 +1589,7 @@ static void run_division(const std::vector<RNS_Polynomial> &polys, uint64_t divi
                     std::vector<vh_div_job> chunkv(jobs.begin() + off, jobs.begin() + off + cnt);
                     {
                         vh_prof_scope ps_(VH_PROF_DIVISION);
-                        VH_CHECK(vh_launch_division(S(), vh_stage_vec(chunkv), cnt, (unsigned)outs.size(),
+                        VH_LAUNCH(vh_launch_division(S(), vh_stage_vec(chunkv), cnt, (unsigned)outs.size(),
                                                 VPOOL(t)->d_limbs, d_consts, t->N, (int)divs.size()));
                     }
                 }
@@ -1662,7 +1720,7 @@ extern "C" void polynomial_base_conversion_RNSc(RNSc_Polynomial out, RNSc_Polyno
         const vh_bconv_job *d_job = (const vh_bconv_job *)vh_stage(&jb, sizeof(jb));
         {
             vh_prof_scope ps_(VH_PROF_OTHER);
-            VH_CHECK(vh_launch_base_conversion(S(), d_job, 1, p->v, DL(out), d_consts, out->ntt->N));
+            VH_LAUNCH(vh_launch_base_conversion(S(), d_job, 1, p->v, DL(out), d_consts, out->ntt->N));
         }
     }
     if (!params) free_base_conversion_params(p);
@@ -1722,7 +1780,7 @@ static void permute_into(RNS_Polynomial out, RNS_Polynomial in, uint64_t gen)
     if (!perm.empty())
         {
             vh_prof_scope ps_(VH_PROF_OTHER);
-            VH_CHECK(vh_launch_permute(S(), vh_stage_vec(perm), (unsigned)perm.size(), DL(out), out->ntt->N));
+            VH_LAUNCH(vh_launch_permute(S(), vh_stage_vec(perm), (unsigned)perm.size(), DL(out), out->ntt->N));
         }
     run_eltwise(zero, DL(out), VH_OP_ZERO, out->ntt->N);
 }
@@ -1979,7 +2037,7 @@ static bool tensor1_batch(const std::vector<std::vector<RNS_Polynomial>> &outs, 
         const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, jobs.size() - off);
         std::vector<vh_tensor_job> chunk(jobs.begin() + off, jobs.begin() + off + cnt);
         vh_prof_scope ps_(VH_PROF_ELTWISE);
-        VH_CHECK(vh_launch_tensor1(S(), vh_stage_vec(chunk), cnt, DL(in1[0]->b), t->N));
+        VH_LAUNCH(vh_launch_tensor1(S(), vh_stage_vec(chunk), cnt, DL(in1[0]->b), t->N));
     }
     return true;
 }
@@ -2113,10 +2171,10 @@ static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<st
                 // One tile per workgroup (the grid runs tile-major, so the workgroups in
                 // flight share two or three digit rows, which stay in the L2).
                 if (OPT.ntt_mode == 2 && C.pipe_blocks > 0)
-                    VH_CHECK(vh_launch_ntt16p(S(), vh_stage_vec(chunk), cnt, P->d_limbs, 0, logn, 0, logn - 8, 1, 2, 0,
+                    VH_LAUNCH(vh_launch_ntt16p(S(), vh_stage_vec(chunk), cnt, P->d_limbs, 0, logn, 0, logn - 8, 1, 2, 0,
                                               UINT_MAX, col_wide(cnt, logn)));
                 else
-                    VH_CHECK(vh_launch_ntt16(S(), vh_stage_vec(chunk), cnt, P->d_limbs, 0, logn, 0, logn - 8, 1, 2, 0));
+                    VH_LAUNCH(vh_launch_ntt16(S(), vh_stage_vec(chunk), cnt, P->d_limbs, 0, logn, 0, logn - 8, 1, 2, 0));
             }
         } else {
             run_ntt(nt, P->d_limbs, P->h_limbs.data(), P->n, 0);
@@ -2149,10 +2207,10 @@ static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<st
                 {
                     vh_prof_scope ps_(VH_PROF_KSMAC);
                     if (fused_mac)
-                        VH_CHECK(vh_launch_ntt16_row_mac(S(), vh_stage_vec(chunk), cnt, (unsigned)maxnext, P->d_limbs,
+                        VH_LAUNCH(vh_launch_ntt16_row_mac(S(), vh_stage_vec(chunk), cnt, (unsigned)maxnext, P->d_limbs,
                                                          d_kptr, logn, (int)Cc));
                     else
-                        VH_CHECK(vh_launch_keyswitch_mac(S(), vh_stage_vec(chunk), cnt, (unsigned)maxnext, P->d_limbs,
+                        VH_LAUNCH(vh_launch_keyswitch_mac(S(), vh_stage_vec(chunk), cnt, (unsigned)maxnext, P->d_limbs,
                                                          d_kptr, N, fp_mac ? 1 : 0));
                 }
             }
@@ -2180,7 +2238,7 @@ static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<st
                             chunk.push_back(it);
                         }
                     vh_prof_scope ps_(VH_PROF_ELTWISE);
-                    VH_CHECK(vh_launch_split_mul(S(), vh_stage_vec(chunk), (unsigned)chunk.size(), P->d_limbs,
+                    VH_LAUNCH(vh_launch_split_mul(S(), vh_stage_vec(chunk), (unsigned)chunk.size(), P->d_limbs,
                                                  add ? 1 : 2, P->n, t->split_degree));
                 }
             }
@@ -2312,7 +2370,7 @@ static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector
             const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, mdr.size() - off);
             std::vector<vh_mdr_job> chunk(mdr.begin() + off, mdr.begin() + off + cnt);
             vh_prof_scope ps_(VH_PROF_DIVISION);
-            VH_CHECK(vh_launch_moddown_rescale(S(), vh_stage_vec(chunk), cnt, P->d_limbs, cd, cr, N, (int)dv.size()));
+            VH_LAUNCH(vh_launch_moddown_rescale(S(), vh_stage_vec(chunk), cnt, P->d_limbs, cd, cr, N, (int)dv.size()));
         }
         run_eltwise(zero, P->d_limbs, VH_OP_ZERO, N);
     }
@@ -2496,6 +2554,165 @@ extern "C" void vfhe_hip_mlwe_multiply_batch_multi(RNS_MLWE *out, RNS_MLWE *in1,
 }
 
 // ---------------------------------------------------------------------------
+// Graph-captured multiply batch.  The launch sequence of multiply_batch depends
+// only on the batch's shape: the ring, the count, the rank, every operand's and
+// key entry's limb mask, the rows allocated, the divide mask, the lane and the
+// schedule options.  The first call of a shape runs eagerly (LOG) and captures
+// the sequence (RECORD) into a plan; later calls rebuild the work lists for
+// their operands with launches skipped (DRY) and replay the plan's graph, whose
+// first node copies the work lists from the plan's pinned buffer to the device.
+// Temporaries (ModUp rows, tensor slots) are the plan's own device blocks, so a
+// replay allocates nothing.  Replays of one plan run on one stream in order; the
+// host waits for the previous replay of the plan before rewriting its pinned
+// work lists.
+// ---------------------------------------------------------------------------
+static void plan_free(vh_graph_plan *pl)
+{
+    if (pl->done) { VH_CHECK(hipEventSynchronize(pl->done)); VH_CHECK(hipEventDestroy(pl->done)); }
+    if (pl->exec) VH_CHECK(hipGraphExecDestroy(pl->exec));
+    if (pl->graph) VH_CHECK(hipGraphDestroy(pl->graph));
+    for (void *b : pl->blocks) if (b) VH_CHECK(hipFree(b));
+    if (pl->d_wl) VH_CHECK(hipFree(pl->d_wl));
+    if (pl->h_wl) VH_CHECK(hipHostFree(pl->h_wl));
+    delete pl;
+}
+
+static std::vector<uint64_t> multiply_key(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &in1,
+                                          const std::vector<RNS_MLWE> &in2, RNS_MLWE **rlk, uint64_t divide_mask)
+{
+    std::vector<uint64_t> k = {(uint64_t)(uintptr_t)in1[0]->b->ntt, outs.size(), in1[0]->r, divide_mask,
+                               (uint64_t)g_lane, (uint64_t)OPT.ntt_mode, (uint64_t)OPT.ntt_chunk_mb,
+                               (uint64_t)OPT.ntt_tile_grid, (uint64_t)OPT.ntt_col_wide, (uint64_t)OPT.ks_fused,
+                               (uint64_t)OPT.xcd_lag, (uint64_t)OPT.xcd_occ};
+    auto add = [&](RNS_MLWE c) {
+        k.push_back(c->r);
+        for (auto p : comps(c)) { k.push_back(p->rns_mask); k.push_back(p->allocated_l); }
+    };
+    for (size_t i = 0; i < outs.size(); i++) { add(outs[i]); add(in1[i]); add(in2[i]); }
+    const uint64_t R = mlwe_extended_rank(in1[0]->r);
+    for (uint64_t i = 0; i < R; i++) {
+        if (!rlk[i]) { k.push_back(~0ULL); continue; }
+        // digit entries the batch reads: one per active limb of the inputs
+        const uint64_t nd = (uint64_t)__builtin_popcountll(in1[0]->b->rns_mask);
+        for (uint64_t d = 0; d < nd; d++) add(rlk[i][d]);
+    }
+    return k;
+}
+
+static void multiply_batch_eager_or_plan(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &in1,
+                                         const std::vector<RNS_MLWE> &in2, RNS_MLWE **rlk, uint64_t divide_mask)
+{
+    const size_t B = outs.size();
+    const std::vector<RNS_MLWE **> ks(B, rlk);
+    const std::vector<uint64_t> dm(B, divide_mask);
+    // Uploads and lazily built tables happen outside capture.
+    for (size_t i = 0; i < B; i++) {
+        for (auto p : comps(outs[i])) dev_ready(p);
+        for (auto p : comps(in1[i])) dev_ready(p);
+        for (auto p : comps(in2[i])) dev_ready(p);
+    }
+    const uint64_t R = mlwe_extended_rank(in1[0]->r);
+    const uint64_t nd = (uint64_t)__builtin_popcountll(in1[0]->b->rns_mask);
+    for (uint64_t i = 0; i < R; i++)
+        if (rlk[i]) for (uint64_t d = 0; d < nd; d++) for (auto p : comps(rlk[i][d])) dev_ready(p);
+    const std::vector<uint64_t> key = multiply_key(outs, in1, in2, rlk, divide_mask);
+    vh_graph_plan *pl = nullptr;
+    for (auto q : g_plans) if (q->key == key) { pl = q; break; }
+    std::vector<uint64_t> masks0;
+    for (auto o : outs) for (auto p : comps(o)) masks0.push_back(p->rns_mask);
+    auto restore = [&](const std::vector<uint64_t> &m) {
+        size_t k = 0;
+        for (auto o : outs) for (auto p : comps(o)) p->rns_mask = m[k++];
+    };
+    if (pl) {
+        // Replay: rebuild the work lists, then launch the graph.
+        if (pl->launched) VH_CHECK(hipEventSynchronize(pl->done));
+        g_cap = VH_CAP_DRY; g_plan = pl; g_cap_stage = g_cap_off = g_cap_block = 0; g_cap_launches = 0;
+        try {
+            multiply_batch(outs, in1, in2, ks, dm);
+            if (g_cap_stage != pl->stage_sz.size() || g_cap_block != pl->block_sz.size() ||
+                g_cap_launches != pl->launches)
+                throw vh_plan_mismatch();
+        } catch (const vh_plan_mismatch &) {
+            g_cap = VH_CAP_OFF; g_plan = nullptr;
+            g_graph_fallbacks++;
+            restore(masks0);
+            multiply_batch(outs, in1, in2, ks, dm);
+            return;
+        }
+        g_cap = VH_CAP_OFF; g_plan = nullptr;
+        VH_CHECK(hipGraphLaunch(pl->exec, S()));
+        VH_CHECK(hipEventRecord(pl->done, S()));
+        pl->launched = true;
+        g_graph_replays++;
+        return;
+    }
+    // First call of this shape: run eagerly, logging sizes, then capture.
+    pl = new vh_graph_plan();
+    pl->key = key;
+    g_cap = VH_CAP_LOG; g_plan = pl;
+    multiply_batch(outs, in1, in2, ks, dm);
+    g_cap = VH_CAP_OFF; g_plan = nullptr;
+    std::vector<uint64_t> masks1;
+    for (auto o : outs) for (auto p : comps(o)) masks1.push_back(p->rns_mask);
+    for (size_t b : pl->stage_sz) pl->wl_bytes += b;
+    if (pl->wl_bytes) {
+        VH_CHECK(hipHostMalloc((void **)&pl->h_wl, pl->wl_bytes, hipHostMallocDefault));
+        VH_CHECK(hipMalloc((void **)&pl->d_wl, pl->wl_bytes));
+    }
+    for (size_t b : pl->block_sz) {
+        void *p = nullptr;
+        VH_CHECK(hipMalloc(&p, b));
+        pl->blocks.push_back(p);
+    }
+    VH_CHECK(hipEventCreateWithFlags(&pl->done, hipEventDisableTiming));
+    restore(masks0);
+    VH_CHECK(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
+    g_cap = VH_CAP_RECORD; g_plan = pl; g_cap_stage = g_cap_off = g_cap_block = 0; g_cap_launches = 0;
+    bool ok = true;
+    try {
+        if (pl->wl_bytes)
+            VH_CHECK(hipMemcpyAsync(pl->d_wl, pl->h_wl, pl->wl_bytes, hipMemcpyHostToDevice, S()));
+        multiply_batch(outs, in1, in2, ks, dm);
+        ok = g_cap_stage == pl->stage_sz.size() && g_cap_block == pl->block_sz.size();
+    } catch (const vh_plan_mismatch &) {
+        ok = false;
+    }
+    pl->launches = g_cap_launches;
+    g_cap = VH_CAP_OFF; g_plan = nullptr;
+    hipGraph_t graph = nullptr;
+    VH_CHECK(hipStreamEndCapture(S(), &graph));
+    pl->graph = graph;
+    restore(masks1);
+    if (!ok) {
+        // The host logic did not repeat itself: keep the eager result, drop the plan.
+        g_graph_fallbacks++;
+        plan_free(pl);
+        return;
+    }
+    VH_CHECK(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
+    g_plans.push_back(pl);
+    g_graph_captures++;
+}
+
+extern "C" void vfhe_hip_mlwe_multiply_batch_graph(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLWE *in2, uint64_t count,
+                                                   RNS_MLWE **rlk, uint64_t divide_mask)
+{
+    VH_ENTER;
+    if (count == 0) return;
+    VH_REQUIRE(out && in1 && in2 && rlk);
+    multiply_batch_eager_or_plan(std::vector<RNS_MLWE>(out, out + count), std::vector<RNS_MLWE>(in1, in1 + count),
+                                 std::vector<RNS_MLWE>(in2, in2 + count), rlk, divide_mask);
+}
+
+extern "C" void vfhe_hip_graph_clear(void)
+{
+    VH_ENTER;
+    for (auto pl : g_plans) plan_free(pl);
+    g_plans.clear();
+}
+
+// ---------------------------------------------------------------------------
 // MGSW (mlwe/c/src/mgsw.c) and monomial products (rns_polynomial.c:1048-1168,
 // mlwe.c:493-509)
 // ---------------------------------------------------------------------------
@@ -2519,7 +2736,7 @@ static void mul_xai_rows(RNS_Polynomial out, RNS_Polynomial in, uint64_t a, bool
     });
     if (items.empty()) return;
     vh_prof_scope ps_(VH_PROF_OTHER);
-    VH_CHECK(vh_launch_mul_xai(S(), vh_stage_vec(items), (unsigned)items.size(), DL(out), N, a));
+    VH_LAUNCH(vh_launch_mul_xai(S(), vh_stage_vec(items), (unsigned)items.size(), DL(out), N, a));
 }
 
 extern "C" void polynomial_RNSc_mul_by_xai(RNSc_Polynomial out, RNSc_Polynomial in, uint64_t a)
@@ -2710,6 +2927,37 @@ extern "C" void vfhe_hip_ntt_batch(RNS_Polynomial *polys, uint64_t count, int di
     ntt_rows(pairs, direction ? 1 : 0);
 }
 
+// The fused product's three launches (N = 2^13..2^16, FP64 limbs): forward column
+// passes of x into z and of y into its temporary row (col), forward row stages of
+// both, pointwise product and inverse row stages in one kernel (mid), inverse
+// column pass (inv).
+static void fused_product_launches(std::vector<vh_item> &col, std::vector<vh_item> &mid, std::vector<vh_item> &inv,
+                                   const vh_pool *P, int logn)
+{
+    const int R1 = logn - 8;
+    auto launch = [&](std::vector<vh_item> &v, int which) {
+        for (size_t off = 0; off < v.size(); off += VH_MAX_GRID_Y) {
+            const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, v.size() - off);
+            std::vector<vh_item> chunk(v.begin() + off, v.begin() + off + cnt);
+            const vh_item *d = vh_stage_vec(chunk);
+            vh_prof_scope ps_(VH_PROF_NTT_FP64);
+            const bool pipe = OPT.ntt_mode == 2 && C.pipe_blocks > 0;
+            const unsigned nb0 = (OPT.ntt_tile_grid & 1) ? UINT_MAX : C.pipe_blocks;
+            const unsigned nb3 = ((OPT.ntt_tile_grid >> 3) & 1) ? UINT_MAX : C.pipe_blocks;
+            if (which == 0 && pipe) VH_LAUNCH(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0, nb0, col_wide(cnt, logn)));
+            else if (which == 0) VH_LAUNCH(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0));
+            else if (which == 1 && pipe)
+                VH_LAUNCH(vh_launch_ntt16p_row_mul(S(), d, cnt, P->d_limbs, logn, C.cus, (int)((OPT.ntt_tile_grid >> 4) & 1)));
+            else if (which == 1) VH_LAUNCH(vh_launch_ntt16_row_mul_inv(S(), d, cnt, P->d_limbs, logn));
+            else if (pipe) VH_LAUNCH(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1, nb3, col_wide(cnt, logn)));
+            else VH_LAUNCH(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1));
+        }
+    };
+    launch(col, 0);
+    launch(mid, 1);
+    launch(inv, 2);
+}
+
 extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polynomial *y, uint64_t count)
 {
     VH_ENTER;
@@ -2745,28 +2993,7 @@ extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x
                 mid.push_back(mk_item(drow(z[i], li), drow(z[i], li), (uint32_t)li, drow(tmp[i], li)));
                 inv.push_back(mk_item(drow(z[i], li), drow(z[i], li), (uint32_t)li));
             }
-        const int R1 = logn - 8;
-        auto launch = [&](std::vector<vh_item> &v, int which) {
-            for (size_t off = 0; off < v.size(); off += VH_MAX_GRID_Y) {
-                const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, v.size() - off);
-                std::vector<vh_item> chunk(v.begin() + off, v.begin() + off + cnt);
-                const vh_item *d = vh_stage_vec(chunk);
-                vh_prof_scope ps_(VH_PROF_NTT_FP64);
-                const bool pipe = OPT.ntt_mode == 2 && C.pipe_blocks > 0;
-                const unsigned nb0 = (OPT.ntt_tile_grid & 1) ? UINT_MAX : C.pipe_blocks;
-                const unsigned nb3 = ((OPT.ntt_tile_grid >> 3) & 1) ? UINT_MAX : C.pipe_blocks;
-                if (which == 0 && pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0, nb0, col_wide(cnt, logn)));
-                else if (which == 0) VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 0, logn, 0, R1, 1, 1, 0));
-                else if (which == 1 && pipe)
-                    VH_CHECK(vh_launch_ntt16p_row_mul(S(), d, cnt, P->d_limbs, logn, C.cus, (int)((OPT.ntt_tile_grid >> 4) & 1)));
-                else if (which == 1) VH_CHECK(vh_launch_ntt16_row_mul_inv(S(), d, cnt, P->d_limbs, logn));
-                else if (pipe) VH_CHECK(vh_launch_ntt16p(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1, nb3, col_wide(cnt, logn)));
-                else VH_CHECK(vh_launch_ntt16(S(), d, cnt, P->d_limbs, 1, logn, 0, R1, 1, 0, 1));
-            }
-        };
-        launch(col, 0);
-        launch(mid, 1);
-        launch(inv, 2);
+        fused_product_launches(col, mid, inv, P, logn);
         for (auto p : tmp) free_RNS_polynomial(p);
         return;
     }
@@ -2801,6 +3028,7 @@ extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x
     ntt_rows(inv, 1);
     for (auto p : tmp) free_RNS_polynomial(p);
 }
+
 
 
 // ---------------------------------------------------------------------------
@@ -2848,7 +3076,7 @@ static void fft_rows(double *const *d_rows_dev, unsigned count, const vh_rous *w
 {
     VH_REQUIRE(ws && n <= ws->size);
     vh_prof_scope ps_(VH_PROF_OTHER);
-    VH_CHECK(vh_launch_fft(S(), d_rows_dev, count, ws->d, log2_exact(n), dir, OPT.fft_fma ? 1 : 0));
+    VH_LAUNCH(vh_launch_fft(S(), d_rows_dev, count, ws->d, log2_exact(n), dir, OPT.fft_fma ? 1 : 0));
 }
 
 static void fft_host_row(double *x, void *ws, uint64_t n, int dir)
@@ -2880,7 +3108,7 @@ static void bitrev_rows(double *const *d_rows, double *const *d_tmp, const std::
     VH_REQUIRE(prec <= 20 && N == ((uint64_t)1 << prec));
     {
         vh_prof_scope ps_(VH_PROF_OTHER);
-        VH_CHECK(vh_launch_fft_bitrev(S(), d_rows, d_tmp, (unsigned)rows.size(), N, prec));
+        VH_LAUNCH(vh_launch_fft_bitrev(S(), d_rows, d_tmp, (unsigned)rows.size(), N, prec));
     }
     for (size_t k = 0; k < rows.size(); k++)
         VH_CHECK(hipMemcpyAsync(rows[k], tmp[k], 2 * N * 8, hipMemcpyDeviceToDevice, S()));
@@ -2904,7 +3132,7 @@ extern "C" void complex_poly_scale_double(double *v, double scale, uint64_t N)
     double *d = (double *)vh_dmalloc(2 * N * 8);
     VH_CHECK(hipMemcpyAsync(d, v, 2 * N * 8, hipMemcpyHostToDevice, S()));
     std::vector<double *> rows = {d};
-    VH_CHECK(vh_launch_fft_scale(S(), vh_stage_vec(rows), 1, scale, 2 * N));
+    VH_LAUNCH(vh_launch_fft_scale(S(), vh_stage_vec(rows), 1, scale, 2 * N));
     VH_CHECK(hipMemcpyAsync(v, d, 2 * N * 8, hipMemcpyDeviceToHost, S()));
     VH_CHECK(hipStreamSynchronize(S()));
     vh_dfree(d);
@@ -2922,7 +3150,7 @@ extern "C" void complex_poly_round_to_RNS(RNS_Polynomial out, double *in, uint64
     VH_CHECK(hipMemcpyAsync(d, in, n * 8, hipMemcpyHostToDevice, S()));
     std::vector<double *> rows = {d};
     std::vector<uint64_t *> outs = {di};
-    VH_CHECK(vh_launch_fft_round(S(), vh_stage_vec(rows), vh_stage_vec(outs), 1, n));
+    VH_LAUNCH(vh_launch_fft_round(S(), vh_stage_vec(rows), vh_stage_vec(outs), 1, n));
     int_rows_to_RNS({{out, di}});
     VH_CHECK(hipStreamSynchronize(S()));
     vh_dfree(d);
@@ -2959,8 +3187,8 @@ extern "C" void complex_polys_ifft_scale_round_to_RNS_batch(void **rows_in, void
         double *const *d_rows = vh_stage_vec(r);
         bitrev_rows(d_rows, vh_stage_vec(t), r, t, n_complex, log_prec);
         fft_rows(d_rows, (unsigned)cnt, (const vh_rous *)gs_ws, n_complex, 1);
-        VH_CHECK(vh_launch_fft_scale(S(), d_rows, (unsigned)cnt, temp_delta / (double)n_complex, n2));
-        VH_CHECK(vh_launch_fft_round(S(), d_rows, vh_stage_vec(o), (unsigned)cnt, n2));
+        VH_LAUNCH(vh_launch_fft_scale(S(), d_rows, (unsigned)cnt, temp_delta / (double)n_complex, n2));
+        VH_LAUNCH(vh_launch_fft_round(S(), d_rows, vh_stage_vec(o), (unsigned)cnt, n2));
     }
     std::vector<std::pair<RNS_Polynomial, const uint64_t *>> jobs;
     for (uint64_t i = 0; i < count; i++) jobs.push_back({(RNS_Polynomial)outs_rns[i], ints[i]});
@@ -3036,7 +3264,7 @@ extern "C" void vfhe_hip_chacha20_blocks(const uint32_t *key, uint64_t nonce, ui
     memcpy(k.k, key, sizeof(k.k));
     k.nonce = nonce;
     uint32_t *d = (uint32_t *)vh_dmalloc(nblocks * 64);
-    VH_CHECK(vh_launch_chacha_raw(S(), k, ctr0, nblocks, d));
+    VH_LAUNCH(vh_launch_chacha_raw(S(), k, ctr0, nblocks, d));
     VH_CHECK(hipMemcpyAsync(out, d, nblocks * 64, hipMemcpyDeviceToHost, S()));
     VH_CHECK(hipStreamSynchronize(S()));
     vh_dfree(d);
@@ -3060,7 +3288,7 @@ static void sample_uniform_rows(const std::vector<std::pair<RNS_Polynomial, uint
         const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, items.size() - off);
         std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
         vh_prof_scope ps_(VH_PROF_OTHER);
-        VH_CHECK(vh_launch_sample_uniform(S(), vh_stage_vec(chunk), cnt, DL(p0), key, N));
+        VH_LAUNCH(vh_launch_sample_uniform(S(), vh_stage_vec(chunk), cnt, DL(p0), key, N));
     }
 }
 
@@ -3091,7 +3319,7 @@ static void sample_gaussian_polys(const std::vector<RNS_Polynomial> &polys, doub
         const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, jobs.size() - off);
         std::vector<vh_gauss_job> chunk(jobs.begin() + off, jobs.begin() + off + cnt);
         vh_prof_scope ps_(VH_PROF_OTHER);
-        VH_CHECK(vh_launch_sample_gaussian(S(), vh_stage_vec(chunk), cnt, d_items, DL(polys[0]), key, sigma, N));
+        VH_LAUNCH(vh_launch_sample_gaussian(S(), vh_stage_vec(chunk), cnt, d_items, DL(polys[0]), key, sigma, N));
     }
 }
 
@@ -3228,7 +3456,7 @@ extern "C" long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, doubl
         for (size_t off = 0; off < items.size(); off += VH_MAX_GRID_Y) {
             const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, items.size() - off);
             std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
-            VH_CHECK(vh_launch_sample_uniform(S(), vh_stage_vec(chunk), cnt, P->d_limbs, key, n));
+            VH_LAUNCH(vh_launch_sample_uniform(S(), vh_stage_vec(chunk), cnt, P->d_limbs, key, n));
         }
     }
     for (auto &it : items) it.src = it.dst;
@@ -3311,4 +3539,421 @@ extern "C" void vfhe_hip_profile_stop(double *ms, uint64_t *launches)
     }
     g_prof_recs.clear();
     g_prof = false;
+}
+
+// ---------------------------------------------------------------------------
+// Host allocation helper of the reference (misc_tp.c:48-62): 64-byte aligned,
+// exit on failure.  Buffers are released with libc free (misc.cdef).
+// ---------------------------------------------------------------------------
+extern "C" void *safe_aligned_malloc(uint64_t size)
+{
+    void *p = nullptr;
+    if (posix_memalign(&p, 64, size ? size : 64) != 0 || !p) exit(EXIT_FAILURE);
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// Slot operations on NTT-form polynomials (rns_polynomial.c:1228-1300) and the
+// slot-wise inverse (rns_polynomial.c:1384-1432), one launch per call.
+// ---------------------------------------------------------------------------
+static void slot_rows(RNS_Polynomial out, RNS_Polynomial in, int op, uint64_t scalar, uint32_t aux, unsigned *flag)
+{
+    incNTT t = in->ntt;
+    const uint64_t ps = t->N / t->split_degree;
+    std::vector<vh_item> items;
+    for_active(in->rns_mask, t->l, [&](uint64_t i) {
+        vh_item it = mk_item(drow(in, i), drow(out, i), (uint32_t)i, (const uint64_t *)flag, scalar);
+        it.aux = aux;
+        items.push_back(it);
+    });
+    for (size_t off = 0; off < items.size(); off += VH_MAX_GRID_Y) {
+        const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, items.size() - off);
+        std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
+        vh_prof_scope ps_(VH_PROF_ELTWISE);
+        VH_LAUNCH(vh_launch_slot(S(), vh_stage_vec(chunk), cnt, DL(in), op, t->N, ps));
+    }
+}
+
+static void slot_prepare(RNS_Polynomial out, RNS_Polynomial in)
+{
+    VH_REQUIRE(out && in && out->ntt->N == in->ntt->N && out->ntt == in->ntt);
+    dev_ready(in);
+    dev_ready(out);
+    out->rns_mask = in->rns_mask;
+    check_rows(out, out->rns_mask);
+}
+
+extern "C" void polynomial_RNS_broadcast_slot(RNS_Polynomial out, RNS_Polynomial in, uint64_t slot_idx)
+{
+    VH_ENTER;
+    VH_REQUIRE(slot_idx < in->ntt->N / in->ntt->split_degree);
+    slot_prepare(out, in);
+    slot_rows(out, in, 0, slot_idx, 0, nullptr);
+}
+
+extern "C" void polynomial_RNS_rotate_slot(RNS_Polynomial out, RNS_Polynomial in, uint64_t rot)
+{
+    VH_ENTER;
+    VH_REQUIRE(out != in && rot <= in->ntt->N / in->ntt->split_degree);
+    slot_prepare(out, in);
+    slot_rows(out, in, 1, rot, 0, nullptr);
+}
+
+extern "C" void polynomial_RNS_copy_slot(RNS_Polynomial out, uint64_t dst, RNS_Polynomial in, uint64_t src)
+{
+    VH_ENTER;
+    const uint64_t ps = in->ntt->N / in->ntt->split_degree;
+    VH_REQUIRE(dst < ps && src < ps);
+    slot_prepare(out, in);
+    slot_rows(out, in, 2, src, (uint32_t)dst, nullptr);
+}
+
+// Returns 0, -2 when a slot is zero (the reference's code, polynomial.py:236-245), or
+// -1 for a split degree above 1 (whose slots are extension-field elements; the
+// extension-field arithmetic, field.c, is not part of this engine).
+extern "C" int polynomial_RNS_inverse(RNS_Polynomial out, RNS_Polynomial in)
+{
+    VH_ENTER;
+    if (in->ntt->split_degree != 1) {
+        fprintf(stderr, "vfhe hip engine: polynomial_RNS_inverse supports split degree 1 only\n");
+        return -1;
+    }
+    slot_prepare(out, in);
+    unsigned *flag = (unsigned *)vh_dmalloc(sizeof(unsigned));
+    VH_LAUNCH(vh_launch_zero_u32(S(), flag, 1));
+    slot_rows(out, in, 3, 0, 0, flag);
+    unsigned h = 0;
+    VH_CHECK(hipMemcpyAsync(&h, flag, sizeof(h), hipMemcpyDeviceToHost, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
+    vh_dfree(flag);
+    return h ? -2 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// LWE samples and keys (modules/mlwe/c/src/lwe.c): host-resident arrays of a
+// few hundred words per limb, handled on the host like the reference does.
+// Randomness: the engine's ChaCha20 key (vfhe_prng_set_deterministic_seed pins
+// it), one nonce per call, expanded on the host by the same RFC 8439 block
+// function the device samplers use.
+// ---------------------------------------------------------------------------
+static inline uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+static void chacha20_host(const vh_rng_key &k, uint64_t ctr, uint32_t o[16])
+{
+    uint32_t x[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u,
+                      k.k[0], k.k[1], k.k[2], k.k[3], k.k[4], k.k[5], k.k[6], k.k[7],
+                      (uint32_t)ctr, (uint32_t)(ctr >> 32), (uint32_t)k.nonce, (uint32_t)(k.nonce >> 32)};
+    uint32_t s[16];
+    memcpy(s, x, sizeof(s));
+    auto qr = [&](int a, int b, int c, int d) {
+        s[a] += s[b]; s[d] = rotl32(s[d] ^ s[a], 16);
+        s[c] += s[d]; s[b] = rotl32(s[b] ^ s[c], 12);
+        s[a] += s[b]; s[d] = rotl32(s[d] ^ s[a], 8);
+        s[c] += s[d]; s[b] = rotl32(s[b] ^ s[c], 7);
+    };
+    for (int r = 0; r < 10; r++) {
+        qr(0, 4, 8, 12); qr(1, 5, 9, 13); qr(2, 6, 10, 14); qr(3, 7, 11, 15);
+        qr(0, 5, 10, 15); qr(1, 6, 11, 12); qr(2, 7, 8, 13); qr(3, 4, 9, 14);
+    }
+    for (int i = 0; i < 16; i++) o[i] = s[i] + x[i];
+}
+
+struct vh_host_stream {
+    vh_rng_key key;
+    uint64_t ctr = 0;
+    uint32_t buf[16];
+    int pos = 16;
+    vh_host_stream() : key(rng_next()) {}
+    uint64_t next64()
+    {
+        if (pos >= 16) { chacha20_host(key, ctr++, buf); pos = 0; }
+        const uint64_t v = (uint64_t)buf[pos] | ((uint64_t)buf[pos + 1] << 32);
+        pos += 2;
+        return v;
+    }
+    uint64_t residue(uint64_t q) { return (uint64_t)(((u128)next64() * q) >> 64); }
+    // rounded-toward-zero Box-Muller normal sample (misc_tp.c:26-31 with double2int, misc.c:22)
+    int64_t normal_trunc(double sigma)
+    {
+        const double u0 = ((double)(next64() >> 11) + 0.5) * 0x1p-53, u1 = ((double)(next64() >> 11) + 0.5) * 0x1p-53;
+        return (int64_t)(cos(6.283185307179586476925286766559 * u0) * sqrt(-2.0 * log(u1)) * sigma);
+    }
+};
+
+static inline uint64_t signed_mod(int64_t v, uint64_t q)
+{
+    const uint64_t mag = v < 0 ? (uint64_t)0 - (uint64_t)v : (uint64_t)v;
+    const uint64_t a = mag % q;
+    return (v < 0 && a) ? q - a : a;
+}
+
+extern "C" LWE_Key lwe_alloc_key(uint64_t n, uint64_t l, incNTT ntt)
+{
+    LWE_Key key = (LWE_Key)malloc(sizeof(*key));
+    if (!key) exit(EXIT_FAILURE);
+    key->s = (uint64_t **)malloc(sizeof(uint64_t *) * (l ? l : 1));
+    if (!key->s) exit(EXIT_FAILURE);
+    for (uint64_t i = 0; i < l; i++) key->s[i] = (uint64_t *)safe_aligned_malloc(sizeof(uint64_t) * n);
+    key->n = n;
+    key->l = l;
+    key->ntt = ntt;
+    key->sigma = 0;
+    return key;
+}
+
+extern "C" LWE lwe_alloc_sample(uint64_t n, uint64_t l, incNTT ntt)
+{
+    LWE c = (LWE)malloc(sizeof(*c));
+    if (!c) exit(EXIT_FAILURE);
+    c->a = (uint64_t **)malloc(sizeof(uint64_t *) * (l ? l : 1));
+    if (!c->a) exit(EXIT_FAILURE);
+    for (uint64_t i = 0; i < l; i++) c->a[i] = (uint64_t *)safe_aligned_malloc(sizeof(uint64_t) * n);
+    c->b = (uint64_t *)safe_aligned_malloc(sizeof(uint64_t) * (l ? l : 1));
+    c->n = n;
+    c->l = l;
+    c->ntt = ntt;
+    return c;
+}
+
+extern "C" void free_lwe_sample(LWE c)
+{
+    if (!c) return;
+    for (uint64_t i = 0; i < c->l; i++) free(c->a[i]);
+    free(c->a);
+    free(c->b);
+    free(c);
+}
+
+extern "C" void free_lwe_key(LWE_Key key)
+{
+    if (!key) return;
+    for (uint64_t i = 0; i < key->l; i++) free(key->s[i]);
+    free(key->s);
+    free(key);
+}
+
+// Key coefficients: rounded normal of deviation sec_sigma, the same integer in every limb.
+extern "C" LWE_Key lwe_new_key(uint64_t n, uint64_t l, incNTT ntt, double sec_sigma, double err_sigma)
+{
+    VH_ENTER;
+    VH_REQUIRE(ntt && l <= ntt->l);
+    LWE_Key key = lwe_alloc_key(n, l, ntt);
+    vh_host_stream rs;
+    for (uint64_t i = 0; i < n; i++) {
+        const int64_t s = rs.normal_trunc(sec_sigma);
+        for (uint64_t j = 0; j < l; j++) key->s[j][i] = signed_mod(s, ntt->ntt[j]->q);
+    }
+    key->sigma = err_sigma;
+    return key;
+}
+
+// Sparse ternary key of Hamming weight h, alternating +1 / -1 (misc.c:32-62: h/2 of each sign).
+extern "C" LWE_Key lwe_new_sparse_ternary_key(uint64_t n, uint64_t l, incNTT ntt, uint64_t h, double err_sigma)
+{
+    VH_ENTER;
+    VH_REQUIRE(ntt && l <= ntt->l && h <= n);
+    LWE_Key key = lwe_alloc_key(n, l, ntt);
+    std::vector<int> tern(n, 0);
+    vh_host_stream rs;
+    uint64_t placed = 0;
+    int sign = 1;
+    while (placed < h) {
+        const uint64_t k = rs.residue(n);
+        if (tern[k]) continue;
+        tern[k] = sign;
+        sign = -sign;
+        placed++;
+    }
+    for (uint64_t i = 0; i < n; i++)
+        for (uint64_t j = 0; j < l; j++) {
+            const uint64_t q = ntt->ntt[j]->q;
+            key->s[j][i] = tern[i] == 1 ? 1 : (tern[i] == -1 ? q - 1 : 0);
+        }
+    key->sigma = err_sigma;
+    return key;
+}
+
+// b = e + <a, s> (+ m) per limb, a uniform (lwe.c:97-123); one error value for every limb.
+extern "C" void lwe_sample(LWE c, uint64_t *m, LWE_Key key)
+{
+    VH_ENTER;
+    VH_REQUIRE(c && key && c->n == key->n && c->l == key->l);
+    vh_host_stream rs;
+    const int64_t e = rs.normal_trunc(key->sigma);
+    for (uint64_t i = 0; i < key->l; i++) {
+        const uint64_t q = key->ntt->ntt[i]->q;
+        u128 acc = 0;
+        for (uint64_t j = 0; j < key->n; j++) {
+            c->a[i][j] = rs.residue(q);
+            acc += (u128)c->a[i][j] * key->s[i][j] % q;
+        }
+        uint64_t b = (uint64_t)((acc + signed_mod(e, q)) % q);
+        if (m) b = add_modq(b, m[i] % q, q);
+        c->b[i] = b;
+    }
+}
+
+extern "C" LWE lwe_new_sample(uint64_t *m, LWE_Key key)
+{
+    LWE c = lwe_alloc_sample(key->n, key->l, key->ntt);
+    lwe_sample(c, m, key);
+    return c;
+}
+
+extern "C" LWE lwe_new_trivial_sample(uint64_t *m, uint64_t n, uint64_t l, incNTT ntt)
+{
+    LWE c = lwe_alloc_sample(n, l, ntt);
+    for (uint64_t i = 0; i < l; i++) {
+        memset(c->a[i], 0, sizeof(uint64_t) * n);
+        c->b[i] = m ? m[i] : 0;
+    }
+    return c;
+}
+
+// out[i] = b - <a, s> mod q_i (lwe.c:138-153)
+extern "C" void lwe_phase(uint64_t *out, LWE c, LWE_Key key)
+{
+    VH_REQUIRE(c && key && out && c->n == key->n && c->l <= key->l);
+    for (uint64_t i = 0; i < c->l; i++) {
+        const uint64_t q = key->ntt->ntt[i]->q;
+        u128 acc = 0;
+        for (uint64_t j = 0; j < key->n; j++) acc += (u128)(c->a[i][j] % q) * key->s[i][j] % q;
+        out[i] = sub_modq(c->b[i] % q, (uint64_t)(acc % q), q);
+    }
+}
+
+// out -= in (lwe.c:155-164)
+extern "C" void lwe_subto(LWE out, LWE in)
+{
+    VH_REQUIRE(out && in && out->n == in->n && out->l == in->l);
+    for (uint64_t i = 0; i < out->l; i++) {
+        const uint64_t q = out->ntt->ntt[i]->q;
+        for (uint64_t j = 0; j < out->n; j++) out->a[i][j] = sub_modq(out->a[i][j] % q, in->a[i][j] % q, q);
+        out->b[i] = sub_modq(out->b[i] % q, in->b[i] % q, q);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sample extraction (mlwe.c:40-68): the LWE sample of coefficient idx of a
+// coefficient-form MLWE sample, over the active limbs of a[0] (limb j of the
+// LWE = the j-th active limb).  The sample's rows are read back to the host.
+// ---------------------------------------------------------------------------
+extern "C" LWE mlwe_extract_LWE(RNSc_MLWE in, uint64_t idx)
+{
+    VH_ENTER;
+    const uint64_t N = in->a[0]->ntt->N, r = in->r, mask = in->a[0]->rns_mask;
+    VH_REQUIRE(idx < N);
+    std::vector<int> act;
+    for (uint64_t i = 0; i < in->a[0]->ntt->l; i++) if (ACTIVE(mask, i)) act.push_back((int)i);
+    LWE res = lwe_alloc_sample(r * N, act.size(), in->a[0]->ntt);
+    for (auto p : comps(in)) polynomial_sync_to_host(p);
+    for (size_t j = 0; j < act.size(); j++) {
+        const int g = act[j];
+        const uint64_t q = in->a[0]->ntt->ntt[g]->q;
+        for (uint64_t k = 0; k < r; k++) {
+            const uint64_t *a = in->a[k]->coeffs[g];
+            for (uint64_t i = 0; i <= idx; i++) res->a[j][k * N + i] = a[idx - i];
+            for (uint64_t i = idx + 1; i < N; i++) res->a[j][k * N + i] = negate_modq(a[N + idx - i], q);
+        }
+        res->b[j] = in->b->coeffs[g][idx];
+    }
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// Trace and packing (mlwe.c:328-445, 600-657): sequences of automorphisms with
+// key-switching, additions and monomial products on the device.
+// ---------------------------------------------------------------------------
+extern "C" void mlwe_partial_trace(RNSc_MLWE out, RNSc_MLWE in, uint64_t *gens, RNS_MLWE ***ksks, uint64_t size,
+                                   uint64_t lvl)
+{
+    VH_ENTER;
+    RNSc_MLWE acc = alloc_sample_raw(out->a[0]->ntt->N, out->r, out->a[0]->rns_mask, out->a[0]->ntt);
+    mlwe_copy_RNS_sample(acc, in);
+    for (uint64_t i = 0; i < size; i++) {
+        mlwe_automorphism_RNSc_GHS(out, acc, gens[i], ksks[i], lvl);
+        mlwe_add_RNSc_sample(acc, acc, out);
+    }
+    mlwe_copy_RNS_sample(out, acc);
+    free_mlwe_RNS_sample(acc);
+}
+
+extern "C" void mlwe_trace(RNSc_MLWE out, RNSc_MLWE in, RNS_MLWE ***ksks, uint64_t lvl)
+{
+    const uint64_t logN = (uint64_t)ilog2(in->a[0]->ntt->N);
+    std::vector<uint64_t> gens(logN);
+    for (uint64_t i = 1; i <= logN; i++) gens[i - 1] = (1ULL << (logN - i + 1)) + 1;
+    mlwe_partial_trace(out, in, gens.data(), ksks, logN, lvl);
+}
+
+// Packs `size` LWE samples (dimension n = in[k]->n, limbs = the active limbs of
+// out) into one MLWE sample: out = -sum_i ksk[i] (x) a_i(X) + sum_k b_k X^k, with
+// a_i(X) = sum_k in[k]->a[.][i] X^k and the gadget product of gadget.c:22-37,
+// all n gadget products in one batched ModUp + key inner product.
+extern "C" void mlwe_full_packing_keyswitch(RNS_MLWE out, LWE *in, uint64_t size, RNS_MLWE **ksk, uint64_t lvl)
+{
+    (void)lvl;
+    VH_ENTER;
+    VH_REQUIRE(out && in && ksk && size > 0);
+    incNTT t = out->b->ntt;
+    const uint64_t N = t->N, n = in[0]->n, lwe_l = in[0]->l;
+    const uint64_t target = out->b->rns_mask, ext = ksk[0][0]->b->rns_mask, divide = ext & ~target;
+    VH_REQUIRE(size <= N);
+    std::vector<int> act;
+    for (uint64_t i = 0; i < t->l; i++) if (ACTIVE(target, i)) act.push_back((int)i);
+    VH_REQUIRE(lwe_l <= act.size());
+    // a_i(X) for every LWE coordinate i, coefficient form on the target limbs
+    std::vector<RNS_Polynomial> polys(n);
+    for (uint64_t i = 0; i < n; i++) {
+        polys[i] = new_poly(N, target, t, false, true);
+        for (int g : act) memset(polys[i]->coeffs[g], 0, N * 8);
+        for (uint64_t limb = 0; limb < lwe_l; limb++)
+            for (uint64_t k = 0; k < size; k++) polys[i]->coeffs[act[limb]][k] = in[k]->a[limb][i];
+        VP(polys[i])->host_dirty = 1;
+    }
+    std::vector<vh_gsrc> srcs;
+    for (uint64_t i = 0; i < n; i++) srcs.push_back({polys[i], ksk[i]});
+    gadget_batch({out}, {srcs}, false);        // out = -sum_i <lift(a_i), ksk[i]>, NTT form on ext
+    RNS_Polynomial body = new_poly(N, target, t, false, true);
+    for (int g : act) memset(body->coeffs[g], 0, N * 8);
+    for (uint64_t limb = 0; limb < lwe_l; limb++)
+        for (uint64_t k = 0; k < size; k++) body->coeffs[act[limb]][k] = in[k]->b[limb];
+    VP(body)->host_dirty = 1;
+    dev_ready(body);
+    mlwe_ntt(out, out, 1);
+    if (divide) run_division(comps(out), divide, true);
+    binop(out->b, out->b, body, VH_OP_ADD);
+    mlwe_ntt(out, out, 0);
+    for (auto p : polys) free_RNS_polynomial(p);
+    free_RNS_polynomial(body);
+}
+
+static void packing_rec(std::vector<RNSc_MLWE> vec, uint64_t ell, RNS_MLWE ***ksks, uint64_t lvl)
+{
+    if (ell == 0) return;
+    const uint64_t half = 1ULL << (ell - 1);
+    std::vector<RNSc_MLWE> even(half), odd(half);
+    for (uint64_t i = 0; i < half; i++) { even[i] = vec[2 * i]; odd[i] = vec[2 * i + 1]; }
+    packing_rec(even, ell - 1, ksks, lvl);
+    packing_rec(odd, ell - 1, ksks, lvl);
+    RNSc_MLWE c = even[0];
+    const uint64_t N = vec[0]->b->ntt->N, r = vec[0]->r;
+    incNTT t = vec[0]->b->ntt;
+    RNSc_MLWE tmp = alloc_sample_raw(N, r, vec[0]->b->rns_mask, t);
+    RNSc_MLWE tmp2 = alloc_sample_raw(N, r, ksks[ell - 1][0][0]->b->rns_mask, t);
+    mlwe_RNSc_mul_by_xai(tmp, odd[0], N >> ell);            // tmp = odd * X^(N / 2^ell)
+    mlwe_sub_RNSc_sample(c, even[0], tmp);                  // c = even - tmp
+    mlwe_automorphism_RNSc_GHS(tmp2, c, (1ULL << ell) + 1, ksks[ell - 1], lvl);
+    for (auto p : comps(tmp)) scale_op(p, p, nullptr, 2, VH_OP_SCALE);   // tmp = 2 tmp
+    mlwe_add_RNSc_sample(c, c, tmp2);
+    mlwe_add_RNSc_sample(c, c, tmp);
+    free_mlwe_RNS_sample(tmp);
+    free_mlwe_RNS_sample(tmp2);
+}
+
+extern "C" void mlwe_full_packing_keyswitch_scaled(RNSc_MLWE *vec, uint64_t ell, RNS_MLWE ***ksks, uint64_t lvl)
+{
+    VH_ENTER;
+    if (ell == 0) return;
+    packing_rec(std::vector<RNSc_MLWE>(vec, vec + (1ULL << ell)), ell, ksks, lvl);
 }

@@ -154,3 +154,9 @@ hipError_t vh_launch_mul_xai(hipStream_t st, const vh_item *items, unsigned nite
                              uint64_t a);
 hipError_t vh_launch_scatter_split(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                                    uint64_t N, int signed_in);
+
+// Slot operations on NTT-form rows of n = split * ps values (op 0 broadcast, 1 rotate,
+// 2 copy one slot, 3 per-slot inverse with a zero flag at src2).
+hipError_t vh_launch_slot(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int op,
+                          uint64_t n, uint64_t ps);
+

@@ -1971,3 +1971,51 @@ hipError_t vh_launch_zero_u32(hipStream_t st, unsigned *p, unsigned n)
     hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Slot operations on NTT-form rows (rns_polynomial.c:1228-1300, 1384-1432).  A
+// row of n = split * ps values holds `split` blocks of ps slots.
+//   op 0 broadcast: dst[i] = src[block(i) + scalar]
+//   op 1 rotate:    dst[i] = src[block(i) + (slot(i) + scalar) mod ps]
+//   op 2 copy:      dst[b*ps + aux] = src[b*ps + scalar] for every block b (i < split)
+//   op 3 inverse:   dst[i] = src[i]^(q-2) mod q; a zero slot sets *(unsigned *)src2
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_slot(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, int op, uint64_t n, uint64_t ps)
+{
+    const vh_item it = items[blockIdx.y];
+    const vh_limb L = limbs[it.limb];
+    const uint64_t cnt = op == 2 ? n / ps : n;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (op == 0) {
+            it.dst[i] = it.src[(i / ps) * ps + it.scalar];
+        } else if (op == 1) {
+            const uint64_t b = (i / ps) * ps, s = i - b;
+            it.dst[i] = it.src[b + (s + it.scalar) % ps];
+        } else if (op == 2) {
+            it.dst[i * ps + it.aux] = it.src[i * ps + it.scalar];
+        } else {
+            const uint64_t a = vh_reduce64(it.src[i], L.q, L.mu64);
+            if (a == 0) {
+                atomicOr((unsigned *)it.src2, 1u);
+                it.dst[i] = 0;
+                continue;
+            }
+            uint64_t r = 1, base = a, e = L.q - 2;
+            while (e) {
+                if (e & 1) r = vh_mulmod(r, base, L);
+                base = vh_mulmod(base, base, L);
+                e >>= 1;
+            }
+            it.dst[i] = r;
+        }
+    }
+}
+
+hipError_t vh_launch_slot(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int op,
+                          uint64_t n, uint64_t ps)
+{
+    const uint64_t cnt = op == 2 ? n / ps : n;
+    hipLaunchKernelGGL(k_slot, dim3(vh_blocks(cnt, 256, 64), nitems), dim3(256), 0, st, items, limbs, op, n, ps);
+    return hipGetLastError();
+}
