@@ -17,7 +17,13 @@ Also reported on the same line:
   * plain     -- the same round trips as one C-ABI call per polynomial, with and
                  without host residency;
   * ckks      -- CKKS multiply + relinearize + rescale at N = 2^16, L = 24 + 4
-                 special limbs, batch of 32 ciphertext pairs (mults/s, p50/p99).
+                 special limbs, batch of 32 ciphertext pairs (mults/s, p50/p99 over
+                 50 batches), eager and graph-replayed, with two products checked
+                 bit for bit against the reference kernels' composition and that
+                 composition timed on the host's CPUs;
+  * cggi      -- CGGI16 functional bootstrap of 1024 LWE ciphertexts (n = 630)
+                 into N = 1024 (bootstraps/s, every output decrypted);
+  * serving   -- Poisson CKKS requests of mixed N and level (requests/s, p50/p99).
 """
 from __future__ import annotations
 
@@ -53,7 +59,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ckks", action="store_true")
     ap.add_argument("--ckks-batch", type=int, default=32)
-    ap.add_argument("--ckks-steps", type=int, default=5)
+    ap.add_argument("--ckks-steps", type=int, default=50)
+    ap.add_argument("--no-cggi", action="store_true")
+    ap.add_argument("--cggi-steps", type=int, default=3)
+    ap.add_argument("--only-cggi", action="store_true", help="run the bootstrap leg alone (profiling)")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=1.5, help="wall time per offered load")
     ap.add_argument("--only-ntt", action="store_true",
@@ -94,8 +103,38 @@ def timed(lib, fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def cpu_info() -> dict:
+    """The host CPU as /proc/cpuinfo describes it: model, logical CPUs, physical cores, SMT."""
+    model, cores, logical = "?", set(), 0
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    logical += 1
+                elif k == "model name":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                    cores.add((phys, core))
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    ncores = len(cores) or logical
+    return {"model": model, "logical_cpus": logical, "physical_cores": ncores,
+            "smt": round(logical / ncores, 2) if ncores else None, "usable_cpus": usable}
+
+
+def cpu_threads() -> int:
+    return max(1, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1))
+
+
 def cpu_baseline(primes, seconds):
-    """Reference ntt_forward + ntt_reverse on this host's cores (oracle/_ref)."""
+    """Reference ntt_forward + ntt_reverse on every CPU this process may run on (oracle/_ref)."""
     from oracle import oracle as O
 
     engines = O.ref_engines()
@@ -103,7 +142,7 @@ def cpu_baseline(primes, seconds):
         return None
     eng = engines[0]
     procs = [O.RefProc(eng, N, q) for q in primes]
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = cpu_threads()
     counts = [0] * threads
     rng = np.random.default_rng(5)
     xs = [rng.integers(0, q, N, dtype=np.uint64) for q in primes]
@@ -134,6 +173,7 @@ def cpu_baseline(primes, seconds):
         "unit": "round-trips/s",
         "cores": threads,
         "kind": "reference",
+        "cpu": cpu_info(),
         "sample": f"{limb_rt} limb round trips (ntt_forward+ntt_reverse, reference {eng} engine, N=2^16, "
                   f"the 24 ~50-bit primes) over {wall:.2f} s on {threads} threads; value = limb round trips / 24 / s",
     }
@@ -155,6 +195,9 @@ def main():
     if args.only_serving:
         print(json.dumps(bench_serving(lib, args)), flush=True)
         return
+    if args.only_cggi:
+        print(json.dumps(bench_cggi(lib, args)), flush=True)
+        return
     primes = params.gen_primes(N, [50] * L)
     log("device", lib.vfhe_hip_device_name().decode(), "primes", primes[0], "...", primes[-1])
     sched = {k: int(lib.vfhe_hip_get_option(k.encode())) for k in
@@ -163,7 +206,7 @@ def main():
     log("schedule", sched, "probe", [int(lib.vfhe_hip_get_option(f"probe.xcd{k}".encode())) for k in range(8)])
     pool = rt.Pool(N, primes)
     tuned = None
-    if not (args.no_autotune or args.only_ntt or args.only_fused or args.only_ckks or args.only_serving):
+    if not (args.no_autotune or args.only_ntt or args.only_fused or args.only_ckks or args.only_serving or args.only_cggi):
         us = (ctypes.c_double * 9)()
         best = lib.vfhe_hip_autotune_ntt(pool.h, POLYS * L, 2, us)
         tuned = {"chosen": int(best), "us_per_round_trip": [round(x, 1) for x in us]}
@@ -271,6 +314,10 @@ def main():
     if not args.no_ckks:
         ckks = bench_ckks(lib, _native, params, rt, args)
 
+    cggi = None
+    if not args.no_cggi:
+        cggi = bench_cggi(lib, args)
+
     serve = None
     if not args.no_serving:
         serve = bench_serving(lib, args)
@@ -312,38 +359,93 @@ def main():
                   "batched_over_device_plain": round(plain_dev_dt / dt, 2),
                   "batched_over_host_plain": round(plain_host_dt / dt, 2)},
         "ckks": ckks,
+        "cggi": cggi,
         "serving": serve,
     }
     print(json.dumps(line), flush=True)
 
 
 def bench_ckks(lib, _native, params, rt, args):
-    """CKKS multiply + relinearize + rescale, N=2^16, L=24 level limbs + 4 special limbs, r=1."""
+    """CKKS multiply + relinearize + rescale, N=2^16, L=24 level limbs + 4 special limbs, r=1, batch of
+    32 pairs: the eager batched entry point and its graph-replayed form, p50/p99 over --ckks-steps
+    batches, parity of two products against the reference kernels' composition, and the same
+    composition timed on the host's CPUs."""
     K, B = 4, args.ckks_batch
     primes = params.gen_primes(N, [50] * (L + K))
     pool = rt.Pool(N, primes)
     lvl, ext = (1 << L) - 1, (1 << (L + K)) - 1
     rng = np.random.default_rng(99)
     log("ckks: building relinearization key (24 digits x 2 x 28 limbs)")
-    entries = [pool.sample(1, ext, np.stack([rt.random_rows(rng, primes, N, ext) for _ in range(2)])) for _ in range(L)]
+    kdata = np.stack([np.stack([rt.random_rows(rng, primes, N, ext) for _ in range(2)]) for _ in range(L)])
+    entries = [pool.sample(1, ext, kdata[d]) for d in range(L)]
     key = _native.voidp_array(entries)
     rlk = _native.voidp_array([key, None])
-    in1 = [pool.sample(1, lvl, np.stack([rt.random_rows(rng, primes, N, lvl) for _ in range(2)])) for _ in range(B)]
-    in2 = [pool.sample(1, lvl, np.stack([rt.random_rows(rng, primes, N, lvl) for _ in range(2)])) for _ in range(B)]
+    keep = []   # plaintext rows of the first two pairs, for the parity check
+    in1, in2 = [], []
+    for k in range(B):
+        d1 = np.stack([rt.random_rows(rng, primes, N, lvl) for _ in range(2)])
+        d2 = np.stack([rt.random_rows(rng, primes, N, lvl) for _ in range(2)])
+        in1.append(pool.sample(1, lvl, d1))
+        in2.append(pool.sample(1, lvl, d2))
+        if k < 2:
+            keep.append((d1[:, :L].copy(), d2[:, :L].copy()))
     outs = [pool.sample(1, ext) for _ in range(B)]
     a1, a2, ao = _native.voidp_array(in1), _native.voidp_array(in2), _native.voidp_array(outs)
     top = 1 << (L - 1)
-    lat = []
-    lib.vfhe_hip_mlwe_multiply_batch(ao, a1, a2, B, rlk, top)
-    lib.vfhe_hip_synchronize()
-    with Prof(lib) as prof:
-        for _ in range(args.ckks_steps):
+
+    def run(fn, steps):
+        lat, enq = [], []
+        for _ in range(steps):
             for c in outs:
                 pool.set_sample_mask(c, ext)
             t0 = time.perf_counter()
-            lib.vfhe_hip_mlwe_multiply_batch(ao, a1, a2, B, rlk, top)
+            fn(ao, a1, a2, B, rlk, top)
+            t1 = time.perf_counter()
             lib.vfhe_hip_synchronize()
             lat.append(time.perf_counter() - t0)
+            enq.append(t1 - t0)
+        return np.array(lat) * 1e3, np.array(enq) * 1e3
+
+    run(lib.vfhe_hip_mlwe_multiply_batch, 2)
+    with Prof(lib) as prof:
+        lat, enq = run(lib.vfhe_hip_mlwe_multiply_batch, args.ckks_steps)
+    # parity of the eager result: two products against the reference kernels (oracle/_ref)
+    parity = None
+    cpu = None
+    try:
+        from oracle import oracle as O
+        engines = O.ref_engines()
+    except Exception:
+        engines = []
+    got = [pool.read_sample(outs[k]) for k in range(len(keep))]
+    if engines:
+        ct1 = np.stack([d1 for d1, _ in keep])
+        ct2 = np.stack([d2 for _, d2 in keep])
+        ref = O.ref_ckks_multiply(engines[0], N, L, K, primes, ct1, ct2, kdata, threads=2)
+        same = all(np.array_equal(got[k][:, : L - 1], ref[k]) for k in range(len(keep)))
+        parity = f"{'bit-exact' if same else 'MISMATCH'}: {len(keep)} products vs the reference {engines[0]} " \
+                 f"kernels composed in the reference's call order (oracle/ref_ckks.c)"
+        log("ckks parity:", parity)
+        if not args.no_cpu:
+            T = cpu_threads()
+            log(f"ckks cpu baseline: {T} products on {T} threads")
+            t0 = time.perf_counter()
+            O.ref_ckks_multiply(engines[0], N, L, K, primes, ct1, ct2, kdata, threads=T, count=T)
+            wall = time.perf_counter() - t0
+            cpu = {"value": round(T / wall, 3), "unit": "mults/s", "cores": T, "kind": "reference",
+                   "cpu": cpu_info(),
+                   "sample": f"{T} multiply+relinearize+rescale products at N=2^16, L=24+4 (one per thread, "
+                             f"reference {engines[0]} kernels in the reference's call order, oracle/ref_ckks.c) "
+                             f"in {wall:.2f} s"}
+            log("ckks cpu baseline:", cpu)
+    # graph-replayed form: the first call captures the batch's launch sequence
+    g0 = {k: int(lib.vfhe_hip_get_option(f"stat.graph_{k}".encode())) for k in ("captures", "replays", "fallbacks")}
+    run(lib.vfhe_hip_mlwe_multiply_batch_graph, 2)
+    glat, genq = run(lib.vfhe_hip_mlwe_multiply_batch_graph, args.ckks_steps)
+    g1 = {k: int(lib.vfhe_hip_get_option(f"stat.graph_{k}".encode())) - g0[k] for k in g0}
+    ggot = [pool.read_sample(outs[k]) for k in range(len(keep))]
+    graph_same = all(np.array_equal(ggot[k], got[k]) for k in range(len(keep)))
+    lib.vfhe_hip_graph_clear()
     one = []
     o1 = _native.voidp_array(outs[:1])
     for _ in range(3):
@@ -352,23 +454,86 @@ def bench_ckks(lib, _native, params, rt, args):
         lib.vfhe_hip_mlwe_multiply_batch(o1, _native.voidp_array(in1[:1]), _native.voidp_array(in2[:1]), 1, rlk, top)
         lib.vfhe_hip_synchronize()
         one.append(time.perf_counter() - t0)
-    lat_ms = np.array(lat) * 1e3
     names = ["ntt_fp64", "ntt_int", "eltwise", "division", "keyswitch_mac", "other"]
     res = {
         "unit": "mults/s",
-        "value": round(B / float(np.median(lat)), 2),
+        "value": round(B / float(np.median(lat) / 1e3), 2),
         "batch": B,
-        "batch_ms_p50": round(float(np.percentile(lat_ms, 50)), 3),
-        "batch_ms_p99": round(float(np.percentile(lat_ms, 99)), 3),
-        "request_latency_p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "batches_timed": int(args.ckks_steps),
+        "batch_ms_p50": round(float(np.percentile(lat, 50)), 3),
+        "batch_ms_p99": round(float(np.percentile(lat, 99)), 3),
+        "request_latency_p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "host_enqueue_ms_p50": round(float(np.percentile(enq, 50)), 3),
+        "graph": {"value": round(B / float(np.median(glat) / 1e3), 2), "batch_ms_p50": round(float(np.percentile(glat, 50)), 3),
+                  "batch_ms_p99": round(float(np.percentile(glat, 99)), 3),
+                  "host_enqueue_ms_p50": round(float(np.percentile(genq, 50)), 3), "counters": g1,
+                  "equals_eager": bool(graph_same)},
         "unbatched_mults_per_s": round(1.0 / float(np.median(one)), 2),
         "kernel_ms_per_batch": {n: round(m / args.ckks_steps, 3) for n, m in zip(names, prof.ms)},
+        "parity": parity,
+        "cpu_baseline": cpu,
         "workload": "multiply+relinearize+rescale, N=2^16, L=24+4 special, rank 1, batch of 32 pairs, "
                     "random rlk (throughput does not depend on key validity)",
     }
     log("ckks:", res)
     for c in in1 + in2 + outs + entries:
         lib.free_mlwe_RNS_sample(c)
+    return res
+
+
+def bench_cggi(lib, args):
+    """CGGI16 functional bootstrap of 1024 LWE ciphertexts (n = 630, binary key) into N = 1024 (rank 1,
+    two 50-bit level primes + one 50-bit special prime, the reference's RNS variant), 4-bit messages,
+    identity table: bootstraps/s over --cggi-steps repetitions, every output decrypted."""
+    from vfhe_amd.arith import Ring
+    from vfhe_amd.cggi import BootstrapKey, Bootstrapper, LweBatch, LweKey, extract, lwe_key_of, pack_lut
+    from vfhe_amd.fhe import Chain, SecretKey
+
+    prec, n, Nc, count = 5, 630, 1024, 1024
+    lib.vfhe_prng_set_deterministic_seed(630)
+    rng = np.random.default_rng(630)
+    Rq = Ring(Nc, prime_size=[50, 50, 50], split_degree=1)
+    in_ring = Rq.quotient_ring(ell=2)
+    lwe_key = LweKey(in_ring, [int(b) for b in rng.integers(0, 2, n)])
+    chain = Chain.nested(Rq, special=1, levels=1)
+    sk = [0] * Nc
+    for k in rng.choice(Nc, size=64, replace=False):
+        sk[int(k)] = 1
+    out_key = SecretKey(chain, [sk], 3.2)
+    t0 = time.perf_counter()
+    bk = BootstrapKey(chain, lwe_key, out_key)
+    keygen_s = time.perf_counter() - t0
+    tv = pack_lut(chain, list(range(1 << (prec - 1))), 1 << (prec - 1), prec)
+    q = in_ring.q_l
+    msgs = [int(v) for v in rng.integers(0, 16, count)]
+    lwes = LweBatch.encrypt(lwe_key, [round((v * q) / (1 << prec)) % q for v in msgs])
+    boot = Bootstrapper(chain, bk)
+    out_key_lwe = lwe_key_of(out_key, in_ring)
+    times, rot_times, errors = [], [], 0
+    for rep in range(args.cggi_steps + 1):
+        lib.vfhe_hip_synchronize()
+        t0 = time.perf_counter()
+        acc = boot.blind_rotate(tv, lwes, 1 << (prec - 1))
+        lib.vfhe_hip_synchronize()
+        t1 = time.perf_counter()
+        out = extract(acc, 0)
+        t2 = time.perf_counter()
+        if rep == 0:
+            got = [round((ph * (1 << prec)) / q) % (1 << prec) for ph in out.phases(out_key_lwe)]
+            errors = sum(1 for a, b in zip(got, msgs) if a != b)
+            continue
+        rot_times.append(t1 - t0)
+        times.append(t2 - t0)
+        del acc, out
+    t = float(np.median(times))
+    res = {"unit": "bootstraps/s", "value": round(count / t, 2), "ms_per_batch": round(t * 1e3, 2),
+           "blind_rotation_ms_per_batch": round(float(np.median(rot_times)) * 1e3, 2), "batch": count,
+           "decrypt_errors": errors, "bootstrap_key_s": round(keygen_s, 2),
+           "workload": "CGGI16 functional bootstrap (RNS variant of the reference, cggi16.py:46-140): LWE n=630 "
+                       "(binary key), GLWE N=1024 rank 1 over two 50-bit primes + one special, gadget = one digit "
+                       "per limb (ell=2), 1024 LWE ciphertexts of uniform 4-bit messages, identity table; the "
+                       "32-bit-torus FFT variant has no counterpart in the reference"}
+    log("cggi:", res)
     return res
 
 

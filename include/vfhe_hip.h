@@ -158,6 +158,7 @@ uint64_t **incNTT_get_rou_matrix(incNTT ntt);                                   
 RNS_Polynomial polynomial_new_RNS_polynomial(uint64_t N, uint64_t rns_mask, incNTT ntt);        /* :65-78 */
 RNS_Polynomial *polynomial_new_RNS_polynomial_array(uint64_t size, uint64_t N, uint64_t rns_mask, incNTT ntt); /* :80-90 */
 RNS_Polynomial *polynomial_new_array_of_RNS_polynomials(uint64_t N, uint64_t rns_mask, uint64_t size, incNTT ntt); /* :174-181 */
+/* Frees are ordered on the calling thread's lane (see vfhe_hip_set_lane for work on other lanes). */
 void free_RNS_polynomial(void *p);                                                              /* :154-163 */
 void free_RNS_polynomial_array(uint64_t size, RNS_Polynomial *p);                               /* :165-172 */
 void polynomial_copy_RNS_polynomial(RNS_Polynomial out, RNS_Polynomial in);                     /* :126-136 */
@@ -355,8 +356,12 @@ void vfhe_hip_synchronize(void);
 /* Select the calling thread's lane (0..7): a HIP stream with its own staging
    ring; lane 0 is the default.  Work on different lanes may overlap on the
    device, so a caller that uses several lanes orders dependent work itself
-   (vfhe_hip_synchronize waits for every lane).  Returns the previous lane, or
-   -1 for an invalid one. */
+   (vfhe_hip_synchronize waits for every lane).  This includes frees: a
+   polynomial or sample freed on one lane returns its device block to that
+   lane, whose next allocation may reuse it at once, ordered only by that
+   lane's stream; work still reading or writing it on another lane must have
+   completed (vfhe_hip_synchronize) before the free.  Returns the previous
+   lane, or -1 for an invalid one. */
 int vfhe_hip_set_lane(int lane);
 int vfhe_hip_last_error(void);
 const char *vfhe_hip_device_name(void);
@@ -433,10 +438,20 @@ void vfhe_hip_mlwe_multiply_batch_graph(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLWE *
                                         RNS_MLWE **rlk, uint64_t divide_mask);
 void vfhe_hip_graph_clear(void);
 
+/* mlwe_RNSc_GHS_hybrid_keyswitch (mlwe.c:557-598) of count coefficient-form inputs that share one
+   key, one launch sequence: out[k] is in[k] switched by ksk (outputs allocated on the extended ring,
+   left on the level limbs in coefficient form). */
+void vfhe_hip_mlwe_keyswitch_batch(RNS_MLWE *out, RNS_MLWE *in, uint64_t count, RNS_MLWE **ksk);
 /* mgsw_CMUX for `count` independent (out, in1, in2, mgsw) quadruples in one
    launch sequence (a batch of blind rotations stepping through one key index). */
 void vfhe_hip_mgsw_cmux_batch(RNS_MLWE *out, RNSc_MLWE *in1, RNSc_MLWE *in2, RNS_MLWE **mgsw, uint64_t count,
                               uint64_t ell, uint64_t special_primes);
+/* CGGI16 blind rotation of `count` accumulators at once (cggi16.py:70-90): acc[k] enters
+   as X^-rot_b * tv in coefficient form; for i = 0..n-1 every accumulator with a non-zero
+   exponent a = steps[k*n + i] becomes CMUX(acc, X^a * acc, bk[i]) (bk[i] the (r+1)*ell
+   NTT-form samples of one MGSW encryption), then back to coefficient form.  Results in acc. */
+void vfhe_hip_blind_rotate_batch(RNS_MLWE *acc, uint64_t count, RNS_MLWE **bk, uint64_t n, const uint64_t *steps,
+                                 uint64_t ell, uint64_t special_primes);
 /* mlwe_automorphism_RNSc_GHS (mlwe.c:314-326) for `count` inputs sharing gen and ksk. */
 void vfhe_hip_mlwe_automorphism_batch(RNS_MLWE *out, RNS_MLWE *in, uint64_t count, uint64_t gen, RNS_MLWE **ksk);
 /* The same two batches with a Galois element / key (and rescale mask) per job:
@@ -473,17 +488,23 @@ void *vfhe_hip_current_stream(void);
    (1..7) engine lanes. */
 void *vfhe_hip_server_new(uint64_t memory_budget, uint32_t lanes, uint32_t max_batch);
 /* Thread-safe; returns the request id (>= 1), or 0 for an invalid request.
-   Inputs and keys must stay alive and unchanged until the request completes. */
+   Inputs and keys must stay alive and unchanged until the request completes.
+   Inputs may still be being written by work the caller enqueued on its current
+   lane: the request's batch waits for that lane's work submitted before this call. */
 uint64_t vfhe_hip_server_submit(void *srv, int op, RNS_MLWE in1, RNS_MLWE in2, RNS_MLWE **ksk, uint64_t gen,
                                 uint64_t divide_mask);
 /* Up to `max` completed requests: id, output sample (the caller owns it) and
    latency from submit to completion in seconds.  wait != 0 blocks until one
    completes or none is pending.  Returns the number written. */
 uint64_t vfhe_hip_server_poll(void *srv, uint64_t *ids, RNS_MLWE *outs, double *latency_s, uint64_t max, int wait);
-/* stats[9]: batches, requests launched, mean batch size, peak in-flight bytes,
+/* vfhe_hip_server_poll plus each request's latency split in three (parts[3k..3k+2], seconds):
+   submit -> admitted into a batch, admitted -> its batch enqueued, enqueued -> completion seen. */
+uint64_t vfhe_hip_server_poll_timed(void *srv, uint64_t *ids, RNS_MLWE *outs, double *latency_s, double *parts,
+                                    uint64_t max, int wait);
+/* stats[10]: batches, requests launched, mean batch size, peak in-flight bytes,
    most lanes busy at once, requests pending, seconds the worker spent
    enqueueing batches, summed device seconds of the batches (each from its
-   lane's events), seconds since the server started. */
+   lane's events), seconds since the server started, longest single batch enqueue (s). */
 void vfhe_hip_server_stats(void *srv, double *stats);
 /* Drains every submitted request, then stops the worker. */
 void vfhe_hip_server_free(void *srv);

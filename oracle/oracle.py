@@ -263,8 +263,31 @@ def ref_lib(engine: str):
         L.is_prime.argtypes = [U64]
         L.inverse_mod.restype = U64
         L.inverse_mod.argtypes = [U64, U64]
+        L.ref_ckks_multiply.restype = ctypes.c_int
+        L.ref_ckks_multiply.argtypes = [U64, U64, U64, U64P, U64P, U64P, U64P, U64P, U64, U64, ctypes.c_int]
         _ref[engine] = L
     return _ref[engine]
+
+
+def ref_ckks_multiply(engine: str, N: int, L: int, K: int, primes, ct1: np.ndarray, ct2: np.ndarray,
+                      rlk: np.ndarray, threads: int = 1, count: int | None = None) -> np.ndarray:
+    """CKKS multiply + relinearize + rescale composed from the reference's own kernels
+    (oracle/ref_ckks.c): ct1, ct2 (n, 2, L, N) in NTT form, rlk (L, 2, L+K, N) in NTT form;
+    returns the n products (n, 2, L-1, N) in coefficient form.  count > n repeats the inputs
+    (pair k = input k mod n) to make a timing sample of `count` products on `threads` threads."""
+    n = ct1.shape[0]
+    count = n if count is None else count
+    a1 = aligned(ct1.size)
+    a1[:] = ct1.reshape(-1)
+    a2 = aligned(ct2.size)
+    a2[:] = ct2.reshape(-1)
+    k = aligned(rlk.size)
+    k[:] = rlk.reshape(-1)
+    out = aligned(n * 2 * (L - 1) * N)
+    pr = np.array([int(p) for p in primes[: L + K]], dtype=np.uint64)
+    rc = ref_lib(engine).ref_ckks_multiply(N, L, K, ptr(pr), ptr(a1), ptr(a2), ptr(k), ptr(out), n, count, threads)
+    assert rc == 0
+    return out.reshape(n, 2, L - 1, N).copy()
 
 
 def aligned(n: int) -> np.ndarray:

@@ -9,6 +9,8 @@ import os
 import re
 import subprocess
 
+import pytest
+
 from vfhe_amd import _native
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -59,3 +61,68 @@ def test_struct_layout_matches_reference_cdef():
     M = _native.RNSMLWE
     assert [M.a.offset, M.b.offset, M.r.offset] == [0, 8, 16]
     assert ctypes.sizeof(M) == 24
+
+
+# Functions the reference cdefs declare that this engine deliberately does not export: they are
+# off the RNS polynomial path (hashing needs BLAKE3, an empty submodule in the reference; mp
+# polynomials, prime fields and GP25 belong to the proof-system and GP25 stacks), or libc's own.
+ABSENT = {
+    "polynomial_RNS_get_hash": "BLAKE3 hashing (rns_polynomial.c:884-903), not on the path",
+    "polynomial_RNS_get_hash_p": "BLAKE3 hashing, not on the path",
+    "get_mp_vector_size": "mp polynomials (proof-system stack)",
+    "mp_scale": "mp polynomials", "mp_sub": "mp polynomials", "mp_load": "mp polynomials",
+    "load_m512": "mp polynomials (AVX-512 register loader)",
+    "new_mp_polynomial": "mp polynomials", "mp_polynomial_from_RNS": "mp polynomials",
+    "mp_polynomial_to_RNSc": "mp polynomials",
+    "field_new_proc": "prime fields (proof-system stack)", "field_ext_add": "prime fields",
+    "field_ext_sub": "prime fields", "field_ext_neg": "prime fields", "field_ext_mul": "prime fields",
+    "field_ext_pow": "prime fields", "field_ext_inv": "prime fields",
+    "field_sample_random_element": "prime fields", "field_hash_element": "prime fields (BLAKE3)",
+    "field_ext_is_equal": "prime fields", "field_base_conversion": "prime fields",
+    "gp25_RGSW_monomial_mul": "GP25 scheme", "gp25_RGSW_monomial_mul_mt": "GP25 scheme",
+    "gp25_sub_a_mt": "GP25 scheme",
+    "free": "libc",
+}
+REF = "/root/reference/modules"
+CDEFS = ["arith/python/cdef/arith.cdef", "mlwe/python/cdef/mlwe.cdef", "fhe/python/cdef/fhe.cdef",
+         "misc/python/cdef/misc.cdef", "misc/python/cdef/vfhe_cpu.cdef"]
+
+
+def _arity(params: str) -> int:
+    params = params.strip()
+    if params in ("", "void"):
+        return 0
+    depth, n = 0, 1
+    for ch in params:
+        depth += ch in "(["
+        depth -= ch in ")]"
+        n += ch == "," and depth == 0
+    return n
+
+
+def _signatures(text: str) -> dict:
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    return {name: _arity(params) for name, params in
+            re.findall(r"\b([A-Za-z_]\w*)\s*\(([^;{()]*(?:\([^()]*\)[^;{()]*)*)\)\s*;", text)
+            if name not in ("if", "while", "sizeof")}
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference sources absent")
+def test_reference_cdefs_are_covered():
+    """Every function of the reference's cdefs (arith, mlwe, fhe, misc, vfhe_cpu) is declared in
+    include/vfhe_hip.h with the same number of arguments, or is listed above as deliberately
+    absent; a cffi module over the covered cdefs therefore links against this library."""
+    ours = _signatures(open(HEADER).read())
+    missing, arity = [], []
+    for rel in CDEFS:
+        for name, n in _signatures(open(os.path.join(REF, rel)).read()).items():
+            if name in ABSENT:
+                assert name not in ours, f"{name} is exported but listed as absent"
+                continue
+            if name not in ours:
+                missing.append(f"{rel}:{name}")
+            elif ours[name] != n:
+                arity.append(f"{name}: reference {n}, here {ours[name]}")
+    assert not missing, missing
+    assert not arity, arity

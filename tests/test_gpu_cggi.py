@@ -1,24 +1,25 @@
-"""CGGI16 functional bootstrap on the hip engine through the host mirror.
+"""CGGI16 functional bootstrap on the hip engine (vfhe_amd.cggi).
 
-Ports modules/fhe/python/test/test_cggi16.py:20-62 (same ring, keys, LUT and
-message precision): bootstrap-key generation (MGSW encryptions of the LWE key),
-LUT packing, blind rotation (mlwe_RNSc_mul_by_xai + mgsw_CMUX) and sample
-extraction, checked by decrypting the extracted LWE sample and reading the
-lookup table's entry for the message.  Keys and encryption randomness come from
-the seeded host sampler, so the run is deterministic.  The batched blind
-rotation (one batched CMUX per key index over several inputs) must give the
-same ciphertexts as the one-at-a-time form, bit for bit.
+Ports modules/fhe/python/test/test_cggi16.py:20-62 (same ring, key weights, LUT
+and message precision): bootstrapping key (MGSW encryptions of the LWE key),
+LUT packing, blind rotation and sample extraction, checked by decrypting the
+extracted LWE sample and reading the lookup table's entry for the message.
+The batched blind rotation (vfhe_hip_blind_rotate_batch) is checked bit for bit
+against the reference's one-at-a-time sequence made of the reference-named
+entry points (mlwe_RNSc_mul_by_xai, mgsw_CMUX, mlwe_RNS_to_RNSc), whose pieces
+are pinned against the oracle in test_gpu_mgsw.py.  The last test runs the
+benchmark's bootstrap shape: LWE dimension 630, N = 1024, 1024 ciphertexts of
+4-bit messages, identity table, every output decrypted exactly.
 """
 import random
 
 import numpy as np
 import pytest
 
-from vfhe_amd import arith
-from vfhe_amd.arith import Ring, repr
-from vfhe_amd.cggi import CGGI16, _extract_lwe_key
-from vfhe_amd.lwe import LWE, LWE_Key, extract_lwe
-from vfhe_amd.mlwe import MLWE, MLWE_Scheme
+from vfhe_amd import _native, arith
+from vfhe_amd.arith import Ring
+from vfhe_amd.cggi import BootstrapKey, Bootstrapper, LweBatch, LweKey, extract, lwe_key_of, pack_lut, rotation_amounts
+from vfhe_amd.fhe import Batch, Chain, SecretKey
 
 pytestmark = pytest.mark.gpu
 
@@ -29,81 +30,121 @@ def mod_switch(v, q, p):
     return round((v * p) / q) % p
 
 
+def binary_key(rng, n, h):
+    s = [0] * n
+    for k in rng.choice(n, size=h, replace=False):
+        s[int(k)] = 1
+    return s
+
+
 @pytest.fixture(scope="module")
 def setup():
     arith.NTT_processor_instance.reset()
-    arith.set_seed(0xB007C0DE)
+    lib = _native.load()
+    lib.vfhe_prng_set_deterministic_seed(0xB007C0DE)
+    rng = np.random.default_rng(0xB007C0DE)
     rnd = random.Random(0xB007C0DE)
     Rq = Ring(256, prime_size=[50, 50, 50], split_degree=1)
     in_ring = Rq.quotient_ring(ell=2)
-    in_scheme = MLWE_Scheme(in_ring, special_primes=0, module_rank=1)
-    out_scheme = MLWE_Scheme(Rq, special_primes=1, module_rank=1, max_lvl=1)
-    input_key = in_scheme.key_gen_sparse(17, 3.2, ternary=False)
-    input_lwe_key = _extract_lwe_key(input_key)
-    output_key = out_scheme.key_gen_sparse(64, 3.2, ternary=False)
-    cg = CGGI16(out_scheme)
+    lwe_key = LweKey(in_ring, binary_key(rng, 256, 17))           # key_gen_sparse(17, ternary=False)
+    chain = Chain.nested(Rq, special=1, levels=1)
+    out_key = SecretKey(chain, [binary_key(rng, 256, 64)], 3.2)    # key_gen_sparse(64, ternary=False)
     print("[cggi] bootstrap key: 256 MGSW encryptions, N=256", flush=True)
-    bk = cg.generate_bootstrap_key(input_lwe_key, output_key)
+    bk = BootstrapKey(chain, lwe_key, out_key)
     lut_size = 1 << (MSG_PREC - 1)
     lut = [rnd.randint(0, (1 << (MSG_PREC - 1)) - 1) for _ in range(lut_size)]
-    tv = cg.LUT_packing(lut, lut_size, MSG_PREC)
-    out_lwe_key = _extract_lwe_key(output_key)
-    out_key_in_ring = LWE_Key(ring=in_ring, key=out_lwe_key.get_s(), n=out_lwe_key.n)
-    return dict(in_ring=in_ring, key=input_lwe_key, cg=cg, bk=bk, lut=lut, tv=tv, out_key=out_key_in_ring,
-                out_scheme=out_scheme, output_key=output_key, rnd=rnd)
+    tv = pack_lut(chain, lut, lut_size, MSG_PREC)
+    return dict(in_ring=in_ring, lwe_key=lwe_key, chain=chain, out_key=out_key, bk=bk, lut=lut, tv=tv,
+                boot=Bootstrapper(chain, bk), out_lwe=lwe_key_of(out_key, in_ring))
 
 
-def encrypt_msg(s, msg_val):
-    ring = s["in_ring"]
-    m_scaled = mod_switch(msg_val, 1 << MSG_PREC, ring.q_l)
-    return LWE(ring=ring, m=[m_scaled % q for q in ring.primes], key=s["key"])
+def encrypt_msgs(s, msgs):
+    q = s["in_ring"].q_l
+    return LweBatch.encrypt(s["lwe_key"], [mod_switch(v, 1 << MSG_PREC, q) for v in msgs])
 
 
-def test_lwe_mirror_phase_and_extraction(setup):
-    """lwe.c:95-156: phase(sample(m)) == m (error-free key); mlwe.c:40-66: the extracted sample's
-    phase is coefficient idx of the MLWE phase."""
+def test_lwe_phase_and_extraction(setup):
+    """lwe.c:97-158: phase(sample(m)) == m under an error-free key; mlwe.c:40-66: the extracted
+    sample's phase is coefficient idx of the MLWE phase."""
     s = setup
-    ring = s["in_ring"]
-    for v in (0, 5, 15):
-        m = mod_switch(v, 1 << MSG_PREC, ring.q_l)
-        c = LWE(ring=ring, m=[m % q for q in ring.primes], key=s["key"])
-        assert c.phase(s["key"], recompose=True) == m % ring.q_l
-    sch, key = s["out_scheme"], s["output_key"]
-    msg = arith.Polynomial(sch.rings[0]).from_array([(7 * i) % 97 for i in range(256)])
-    ct = sch.sample(msg, key)
-    ph = sch.phase(ct, key)
-    coeffs = ph.get_polynomial()
-    lwe_key = _extract_lwe_key(key)
-    k = LWE_Key(ring=sch.rings[0], key=lwe_key.get_s(), n=lwe_key.n)
+    q = s["in_ring"].q_l
+    msgs = [mod_switch(v, 1 << MSG_PREC, q) for v in (0, 5, 15)]
+    assert LweBatch.encrypt(s["lwe_key"], msgs).phases(s["lwe_key"]) == [m % q for m in msgs]
+    chain, key = s["chain"], s["out_key"]
+    msg = arith.Polynomial(chain.levels[0]).from_array([(7 * i) % 97 for i in range(256)])
+    ct = Batch.encrypt(chain, key, [msg])
+    coeffs = ct.decrypt(key)[0].get_polynomial()
     for idx in (0, 1, 100, 255):
-        assert extract_lwe(ct, idx).phase(k, recompose=True) == coeffs[idx]
+        assert extract(ct, idx).phases(s["out_lwe"])[0] == coeffs[idx]
 
 
 def test_functional_bootstrap(setup):
-    """test_cggi16.py:20-62: the bootstrapped sample decrypts to lut[msg] for several messages."""
+    """test_cggi16.py:20-62: each bootstrapped sample decrypts to lut[msg]."""
     s = setup
-    cg, bk, lut, tv = s["cg"], s["bk"], s["lut"], s["tv"]
     msgs = [0, 3, 7, 12, 15]
-    for v in msgs:
-        print(f"[cggi] bootstrap msg={v}", flush=True)
-        lw = encrypt_msg(s, v)
-        out = cg.functional_bootstrap(None, tv, lw, bk, torus_base=1 << (MSG_PREC - 1))
-        phase = out.phase(s["out_key"], recompose=True)
-        assert mod_switch(phase, s["in_ring"].q_l, 1 << MSG_PREC) == lut[v], v
+    print(f"[cggi] bootstrap of {len(msgs)} messages", flush=True)
+    out = s["boot"].bootstrap(s["tv"], encrypt_msgs(s, msgs), torus_base=1 << (MSG_PREC - 1))
+    q = s["in_ring"].q_l
+    for v, ph in zip(msgs, out.phases(s["out_lwe"])):
+        assert mod_switch(ph, q, 1 << MSG_PREC) == s["lut"][v], v
 
 
-def test_blind_rotate_batch_matches_sequential(setup):
-    """The batched blind rotation equals functional_bootstrap_wo_extract input by input."""
+def _reference_sequence(s, lwes, k, torus_base):
+    """The reference's functional_bootstrap_wo_extract (cggi16.py:46-90) for input k, one
+    reference-named engine call at a time."""
+    lib = _native.load()
+    chain, bk, tv = s["chain"], s["bk"], s["tv"]
+    rot, steps = rotation_amounts(*lwes.arrays(k), lwes.ring.primes[: lwes.l], chain.N, torus_base)
+    tv.to_coeff()
+    out = Batch.alloc(chain, 0, 1, form="coeff")
+    lib.mlwe_RNSc_mul_by_xai(out.handles[0], tv.handles[0], rot)
+    for i, a in enumerate(steps):
+        if a == 0:
+            continue
+        out.to_coeff()
+        rotated = Batch.alloc(chain, 0, 1, form="coeff")
+        lib.mlwe_RNSc_mul_by_xai(rotated.handles[0], out.handles[0], a)
+        res = Batch.alloc(chain, 0, 1, form="ntt")
+        lib.mgsw_CMUX(res.handles[0], out.handles[0], rotated.handles[0], bk.arrays[i], bk.ell, chain.special)
+        lib.mlwe_copy_RNS_sample(out.handles[0], res.handles[0])
+        out.form = "ntt"
+    return out.to_coeff()
+
+
+def test_blind_rotate_batch_matches_reference_sequence(setup):
     s = setup
-    cg, bk, tv = s["cg"], s["bk"], s["tv"]
-    lwes = [encrypt_msg(s, v) for v in (1, 9, 14)]
-    print("[cggi] batched blind rotation of 3 inputs", flush=True)
-    got = cg.blind_rotate_batch([tv] * 3, lwes, bk, torus_base=1 << (MSG_PREC - 1))
-    for lw, g in zip(lwes, got):
-        ref = MLWE(cg.scheme)
-        cg.functional_bootstrap_wo_extract(ref, tv, lw, bk, torus_base=1 << (MSG_PREC - 1))
-        ref.to_coeff()
-        g.to_coeff()
-        for j in range(ref.r):
-            assert ref.get_a_poly(j).get_coeff_matrix(repr.coeff) == g.get_a_poly(j).get_coeff_matrix(repr.coeff)
-        assert ref.get_b_poly().get_coeff_matrix(repr.coeff) == g.get_b_poly().get_coeff_matrix(repr.coeff)
+    tb = 1 << (MSG_PREC - 1)
+    lwes = encrypt_msgs(s, [1, 9, 14])
+    print("[cggi] batched blind rotation of 3 inputs against the one-call-at-a-time sequence", flush=True)
+    got = s["boot"].blind_rotate(s["tv"], lwes, tb)
+    for k in range(len(lwes)):
+        ref = _reference_sequence(s, lwes, k, tb)
+        for p, r in zip(got.polys(k), ref.polys(0)):
+            assert p == r, k
+
+
+def test_bootstrap_batch_shape_n630_N1024():
+    """The bootstrap benchmark's shape: LWE n = 630 (binary key), GLWE N = 1024 (rank 1, two
+    50-bit level primes + one special), 1024 LWE ciphertexts of uniform 4-bit messages, identity
+    table; every output decrypts to its message."""
+    arith.NTT_processor_instance.reset()
+    lib = _native.load()
+    lib.vfhe_prng_set_deterministic_seed(0x630)
+    rng = np.random.default_rng(0x630)
+    Rq = Ring(1024, prime_size=[50, 50, 50], split_degree=1)
+    in_ring = Rq.quotient_ring(ell=2)
+    lwe_key = LweKey(in_ring, [int(b) for b in rng.integers(0, 2, 630)])
+    chain = Chain.nested(Rq, special=1, levels=1)
+    out_key = SecretKey(chain, [binary_key(rng, 1024, 64)], 3.2)
+    print("[cggi] bootstrap key: 630 MGSW encryptions, N=1024", flush=True)
+    bk = BootstrapKey(chain, lwe_key, out_key)
+    lut = list(range(1 << (MSG_PREC - 1)))
+    tv = pack_lut(chain, lut, len(lut), MSG_PREC)
+    msgs = [int(v) for v in rng.integers(0, 16, 1024)]
+    q = in_ring.q_l
+    lwes = LweBatch.encrypt(lwe_key, [mod_switch(v, 1 << MSG_PREC, q) for v in msgs])
+    print("[cggi] bootstrapping 1024 ciphertexts", flush=True)
+    out = Bootstrapper(chain, bk).bootstrap(tv, lwes, torus_base=1 << (MSG_PREC - 1))
+    phases = out.phases(lwe_key_of(out_key, in_ring))
+    got = [mod_switch(ph, q, 1 << MSG_PREC) for ph in phases]
+    assert got == msgs

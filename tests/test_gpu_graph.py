@@ -78,6 +78,9 @@ def test_graph_replays_match_eager(lib, N, L, K, B, calls):
         for (_, c1), (_, c2) in pairs:
             lib.free_mlwe_RNS_sample(c1)
             lib.free_mlwe_RNS_sample(c2)
+    print("[graph] captures", _stat(lib, "graph_captures") - cap0, "replays", _stat(lib, "graph_replays") - rep0,
+          "fallbacks", _stat(lib, "graph_fallbacks"), "why", _stat(lib, "graph_why"), _stat(lib, "graph_why_want"),
+          _stat(lib, "graph_why_got"), flush=True)
     assert _stat(lib, "graph_captures") - cap0 == 1
     assert _stat(lib, "graph_replays") - rep0 == calls - 1
     lib.vfhe_hip_graph_clear()

@@ -759,3 +759,49 @@ def test_cmux_selects_with_valid_mgsw():
         # planted mistake: the other message is far away
         c.addsub(diff, ph, lvl, msgs[1 - bit], lvl, True)
         assert _centered_max(c, primes, diff, lvl) > 2**8
+
+
+@pytest.mark.skipif(not O.ref_engines(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("N,L,K", [(4096, 4, 1), (8192, 5, 2)])
+def test_multiply_rescale_matches_reference_kernels(N, L, K):
+    """The oracle's mlwe_multiply + CKKS rescale (its restatement of mlwe.c:557-760, gadget.c:22-37
+    and rns_polynomial.c:939-976) equals the same sequence composed from the reference engine's own
+    kernels (oracle/ref_ckks.c over oracle/_ref), bit for bit, for every engine built here; the
+    oracle fed its key digits in reverse order (a planted indexing slip) does not."""
+    primes = O.gen_primes(N, [50] * (L + K))
+    c = O.Ctx(N, primes)
+    rng = np.random.default_rng(N + L)
+    lvl, ext = (1 << L) - 1, (1 << (L + K)) - 1
+
+    def rows(mask):
+        out = np.zeros((L + K, N), dtype=np.uint64)
+        for i, q in enumerate(primes):
+            if (mask >> i) & 1:
+                out[i] = rng.integers(0, q, N, dtype=np.uint64)
+        return out
+
+    kdata = np.stack([np.stack([rows(ext), rows(ext)]) for _ in range(L)])
+    pairs = [(np.stack([rows(lvl), rows(lvl)]), np.stack([rows(lvl), rows(lvl)])) for _ in range(2)]
+    top = 1 << (L - 1)
+    expect = []
+    for d1, d2 in pairs:
+        ref = np.zeros((2, L + K, N), dtype=np.uint64)
+        m = c.multiply(ref, d1, d2, 1, lvl, [kdata, None], ext)
+        for comp in range(2):
+            c.to_coeff(ref[comp], ref[comp], m)
+        m2 = c.mlwe_round_division(ref, 1, m, top)
+        assert m2 == lvl & ~top
+        expect.append(ref[:, : L - 1])
+    ct1 = np.stack([d1[:, :L] for d1, _ in pairs])
+    ct2 = np.stack([d2[:, :L] for _, d2 in pairs])
+    for eng in O.ref_engines():
+        got = O.ref_ckks_multiply(eng, N, L, K, primes, ct1, ct2, kdata, threads=2)
+        for k in range(len(pairs)):
+            assert np.array_equal(got[k], expect[k]), (eng, k)
+    d1, d2 = pairs[0]
+    ref = np.zeros((2, L + K, N), dtype=np.uint64)
+    m = c.multiply(ref, d1, d2, 1, lvl, [kdata[::-1].copy(), None], ext)
+    for comp in range(2):
+        c.to_coeff(ref[comp], ref[comp], m)
+    c.mlwe_round_division(ref, 1, m, top)
+    assert not np.array_equal(got[0], ref[:, : L - 1])

@@ -186,6 +186,8 @@ _SIGNATURES = {
     "mlwe_RNSc_sample": (None, [VP, VP, VP]),
     "vfhe_hip_chacha20_blocks": (None, [VP, U64, U64, U64, VP]),
     "vfhe_hip_mlwe_automorphism_batch": (None, [VPP, VPP, U64, U64, VPP]),
+    "vfhe_hip_mlwe_keyswitch_batch": (None, [VPP, VPP, U64, VPP]),
+    "vfhe_hip_blind_rotate_batch": (None, [VPP, U64, VPP, U64, U64P, U64, U64]),
     "vfhe_hip_mlwe_multiply_batch_multi": (None, [VPP, VPP, VPP, U64, VPP, U64P]),
     "polynomial_RNSc_mul_by_xai": (None, [VP, VP, U64]),
     "polynomial_RNSc_mul_by_xai_minus1": (None, [VP, VP, U64]),
@@ -202,6 +204,8 @@ _SIGNATURES = {
     "vfhe_hip_server_new": (VP, [U64, ctypes.c_uint32, ctypes.c_uint32]),
     "vfhe_hip_server_submit": (U64, [VP, I32, VP, VP, VPP, U64, U64]),
     "vfhe_hip_server_poll": (U64, [VP, U64P, VPP, ctypes.POINTER(ctypes.c_double), U64, I32]),
+    "vfhe_hip_server_poll_timed": (U64, [VP, U64P, VPP, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double), U64, I32]),
     "vfhe_hip_server_stats": (None, [VP, ctypes.POINTER(ctypes.c_double)]),
     "vfhe_hip_server_free": (None, [VP]),
     "load_rous_CT": (VP, [DP, DP, U64]),

@@ -1,148 +1,237 @@
-"""Host mirror of vfhe.fhe.cggi16 (CGGI16 functional bootstrap) on the hip engine.
+"""CGGI16 functional bootstrapping, RNS variant, on the hip engine, batch first.
 
-Follows modules/fhe/python/src/vfhe/fhe/cggi16.py:14-140: bootstrap key = one
-MGSW encryption of each LWE key coefficient (:25-44), blind rotation = an
-initial X^-round(2N(b + offset)/Q) rotation of the test vector, then for every
-input coefficient a_i a CMUX between the accumulator and its X^round(2N a_i/Q)
-rotation selected by bk[i] (:46-90), LUT packing (:92-116), sample extraction
-(:118-140).  The arithmetic steps are the engine's mlwe_RNSc_mul_by_xai and
-mgsw_CMUX; `blind_rotate_batch` advances many independent bootstraps one key
-index at a time, so each step is one batched inverse NTT and one batched CMUX
-(vfhe_hip_ntt_batch, vfhe_hip_mgsw_cmux_batch) over all of them.
+The reference bootstraps one LWE ciphertext at a time
+(modules/fhe/python/src/vfhe/fhe/cggi16.py:46-140): an initial rotation of the
+test vector, then one CMUX per LWE coefficient, then sample extraction.  Here
+a whole batch of LWE ciphertexts moves through the bootstrapping key together:
+the rotation amounts of every input are computed on the host
+(`rotation_amounts`), the accumulators live in HBM, and
+`vfhe_hip_blind_rotate_batch` steps all of them through key index i with one
+monomial-product launch and one batched CMUX (mgsw.c:51-71) per index.  The
+result of each accumulator is the reference's, bit for bit, since every
+accumulator sees the same sequence of engine operations the reference makes.
+
+LWE samples and keys are the engine's host-resident `LWE` / `LWE_Key` structs
+(lwe.c semantics, include/vfhe_hip.h), wrapped by `LweKey` / `LweBatch`.
+
+Restated from the reference because they fix bits of the result:
+  * the MGSW plaintext layout (`gsw_plaintexts`, mgsw.py:17-53);
+  * the rotation amounts from an LWE sample (`rotation_amounts`, cggi16.py:54-86),
+    with Python's exact-rational true division and round-half-even;
+  * the test-vector packing of a lookup table (`pack_lut`, cggi16.py:92-116).
 """
 from __future__ import annotations
 
+import ctypes
 import math
 
+import numpy as np
+
 from . import _native
-from .arith import Polynomial, crt, lib, repr
-from .lwe import LWE, LWE_Key, extract_lwe
-from .mgsw import CMUX, MGSW_Scheme, cmux_batch
-from .mlwe import MLWE, MLWE_Key
+from .arith import Polynomial, crt
+from .fhe import Batch, Chain, SecretKey, _handles
+
+U64 = _native.U64
+U64P = _native.U64P
 
 
-class CGGI16_Key:
-    def __init__(self):
-        self.bk = []
-        self.b_prec = 0
+def lib():
+    return _native.load()
 
 
-def _extract_lwe_key(key: MLWE_Key) -> LWE_Key:
-    """mlwe.py:513-518: the key polynomials' coefficients, concatenated."""
-    coeffs = [c for comp in key.key for c in comp]
-    return LWE_Key(ring=key.scheme.rings[0], key=coeffs, n=len(coeffs))
+class _LweStruct(ctypes.Structure):
+    """struct _LWE (modules/mlwe/python/cdef/mlwe.cdef:18-24)."""
+    _fields_ = [("a", ctypes.POINTER(U64P)), ("b", U64P), ("n", U64), ("l", U64), ("ntt", ctypes.c_void_p)]
 
 
-class CGGI16:
-    def __init__(self, scheme, gsw_ell: int | None = None):
-        self.scheme = scheme
-        self.mgsw_scheme = MGSW_Scheme(scheme, ell=gsw_ell)
-        self.ring = scheme.ring
+class _LweKeyStruct(ctypes.Structure):
+    """struct _LWE_Key (modules/mlwe/python/cdef/mlwe.cdef:11-17)."""
+    _fields_ = [("s", ctypes.POINTER(U64P)), ("n", U64), ("l", U64), ("ntt", ctypes.c_void_p),
+                ("sigma", ctypes.c_double)]
 
-    def generate_bootstrap_key(self, input_key, output_key: MLWE_Key) -> CGGI16_Key:
-        """cggi16.py:25-44."""
-        bk = CGGI16_Key()
-        lwe_key = _extract_lwe_key(input_key) if isinstance(input_key, MLWE_Key) else input_key
-        N = self.ring.N
-        for s_i in lwe_key.get_s():
-            poly = Polynomial(self.ring).from_array([s_i] + [0] * (N - 1))
-            mgsw = self.mgsw_scheme.encrypt(poly, output_key)
-            mgsw.to_NTT()
-            bk.bk.append(mgsw)
-        return bk
 
-    # -- rotation amounts, exactly as cggi16.py:54-86 computes them ------------------
-    def _rotations(self, rlwe_in: LWE, torus_base: int):
-        N = self.ring.N
-        primes = rlwe_in.ring.primes[: rlwe_in.l]
-        q_lwe = math.prod(primes)
-        b_val = crt(rlwe_in.get_b(), primes)
-        a_limbs = rlwe_in.get_a()
-        a_vals = [crt([a_limbs[j][i] for j in range(rlwe_in.l)], primes) for i in range(rlwe_in.n)]
-        prec_offset = q_lwe // (4 * torus_base)
-        rot_b = round(((b_val + prec_offset) * (2 * N)) / q_lwe) % (2 * N)
-        rot_tv = (2 * N - rot_b) % (2 * N)
-        steps = [round((a * (2 * N)) / q_lwe) % (2 * N) for a in a_vals]
-        return rot_tv, steps
+# ---------------------------------------------------------------------------------------------
+# LWE
+# ---------------------------------------------------------------------------------------------
+class LweKey:
+    """An LWE secret of n signed coefficients over the first l limbs of a ring's pool."""
 
-    def functional_bootstrap_wo_extract(self, out: MLWE, tv: MLWE, rlwe_in: LWE, bk: CGGI16_Key, torus_base: int):
-        """cggi16.py:46-90: blind rotation of tv by the phase of rlwe_in, result in out."""
-        rot_tv, steps = self._rotations(rlwe_in, torus_base)
-        out.to_coeff()
+    def __init__(self, ring, coeffs, sigma: float = 0.0, limbs: int | None = None):
+        self.ring = ring
+        self.coeffs = [int(v) for v in coeffs]
+        self.n = len(self.coeffs)
+        self.l = ring.ell if limbs is None else limbs
+        self.h = lib().lwe_alloc_key(self.n, self.l, ring.NTT)
+        k = ctypes.cast(self.h, ctypes.POINTER(_LweKeyStruct)).contents
+        k.sigma = float(sigma)
+        for j, q in enumerate(ring.primes[: self.l]):
+            row = np.array([v % q for v in self.coeffs], dtype=np.uint64)
+            ctypes.memmove(k.s[j], row.ctypes.data, 8 * self.n)
+
+    def __del__(self):
+        try:
+            lib().free_lwe_key(self.h)
+        except Exception:
+            pass
+
+
+class LweBatch:
+    """Engine LWE samples (host arrays a[limb][n], b[limb])."""
+
+    def __init__(self, ring, handles, n: int, l: int):
+        self.ring, self.handles, self.n, self.l = ring, list(handles), n, l
+
+    @classmethod
+    def encrypt(cls, key: LweKey, messages: list[int]) -> "LweBatch":
+        """b = <a, s> + m + e per limb (lwe.c:97-123); m an integer modulo the limbs' product."""
+        hs = []
+        for m in messages:
+            ms = np.array([int(m) % q for q in key.ring.primes[: key.l]], dtype=np.uint64)
+            hs.append(lib().lwe_new_sample(ms.ctypes.data_as(U64P), key.h))
+        return cls(key.ring, hs, key.n, key.l)
+
+    def __del__(self):
+        try:
+            for h in self.handles:
+                lib().free_lwe_sample(h)
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.handles)
+
+    def arrays(self, k: int):
+        """(a as l x n uint64, b as l uint64) of member k."""
+        s = ctypes.cast(self.handles[k], ctypes.POINTER(_LweStruct)).contents
+        l, n = int(s.l), int(s.n)
+        a = np.stack([np.ctypeslib.as_array(s.a[j], shape=(n,)).copy() for j in range(l)])
+        b = np.ctypeslib.as_array(s.b, shape=(l,)).copy()
+        return a, b
+
+    def phases(self, key: LweKey) -> list[int]:
+        """b - <a, s> per member (lwe.c:144-158), recomposed over the limbs by CRT."""
+        primes = self.ring.primes[: self.l]
+        out = []
+        buf = np.zeros(self.l, dtype=np.uint64)
+        for h in self.handles:
+            lib().lwe_phase(buf.ctypes.data_as(U64P), h, key.h)
+            out.append(crt([int(v) for v in buf], primes))
+        return out
+
+
+def extract(batch: Batch, idx: int = 0) -> LweBatch:
+    """Coefficient idx of every member's phase as an LWE sample of dimension r*N (mlwe.c:40-66)."""
+    batch.to_coeff()
+    hs = [lib().mlwe_extract_LWE(h, idx) for h in batch.handles]
+    return LweBatch(batch.ring, hs, batch.rank * batch.ring.N, batch.ring.ell)
+
+
+# ---------------------------------------------------------------------------------------------
+# MGSW encryptions
+# ---------------------------------------------------------------------------------------------
+def gsw_plaintexts(key: SecretKey, msg: Polynomial, ring, ell: int, special_modulus: int) -> list[Polynomial]:
+    """The (r+1)*ell plaintexts of an MGSW encryption of msg on `ring`
+    (modules/mlwe/python/src/vfhe/mlwe/mgsw.py:17-53): for every key component j and digit i,
+    -s_j * msg * f_i, then for every digit i, msg * f_i, where f_i holds P mod q_i at the ring's
+    i-th limb and 0 elsewhere (P the key-switching primes' product).  NTT form."""
+    msg.to_NTT()
+    factors = [[special_modulus % p if k == i else 0 for k, p in enumerate(ring.primes)] for i in range(ell)]
+    out = []
+    for j in range(key.chain.rank):
+        sm = -(key.poly(j, ring) * msg)
+        out += [sm * f for f in factors]
+    out += [msg * f for f in factors]
+    return out
+
+
+class BootstrapKey:
+    """One MGSW encryption per LWE key coefficient, each (r+1)*ell NTT-form samples on the
+    extended ring of level 0 (cggi16.py:25-44)."""
+
+    def __init__(self, chain: Chain, lwe_key: LweKey, out_key: SecretKey, ell: int | None = None):
+        self.chain = chain
+        self.ring = chain.ext[0]
+        self.ell = ell if ell else chain.levels[0].ell
+        self.n = lwe_key.n
+        N = chain.N
+        self.rows = []
+        for s in lwe_key.coeffs:
+            m = Polynomial(self.ring).from_array([s] + [0] * (N - 1))
+            plains = gsw_plaintexts(out_key, m, self.ring, self.ell, chain.special_modulus)
+            for p in plains:
+                p.to_coeff()
+            b = Batch.encrypt(chain, out_key, plains, ring=self.ring)
+            b.to_ntt()
+            self.rows.append(b)
+        self.arrays = [_handles(b.handles) for b in self.rows]
+        self.table = _handles([ctypes.cast(a, ctypes.c_void_p).value for a in self.arrays])
+
+
+# ---------------------------------------------------------------------------------------------
+# Bootstrapping
+# ---------------------------------------------------------------------------------------------
+def rotation_amounts(a: np.ndarray, b: np.ndarray, primes: list[int], N: int, torus_base: int):
+    """Rotation of the test vector and per-coefficient exponents of one LWE sample
+    (cggi16.py:54-86): with q the limbs' product and b, a_i recomposed by CRT,
+    rot_b = round((b + q // (4 torus_base)) * 2N / q) mod 2N, the initial exponent
+    2N - rot_b, and a_i -> round(a_i * 2N / q) mod 2N; the divisions are Python's correctly
+    rounded true division and the rounding is half to even, as there."""
+    q = math.prod(primes)
+    bv = crt([int(v) for v in b], primes)
+    off = q // (4 * torus_base)
+    rot_b = round(((bv + off) * (2 * N)) / q) % (2 * N)
+    steps = []
+    for i in range(a.shape[1]):
+        ai = crt([int(a[j, i]) for j in range(a.shape[0])], primes)
+        steps.append(round((ai * (2 * N)) / q) % (2 * N))
+    return (2 * N - rot_b) % (2 * N), steps
+
+
+def pack_lut(chain: Chain, lut: list[int], size: int, prec: int) -> Batch:
+    """Trivial encryption (a = 0) of the packed lookup table (cggi16.py:92-116): entry k scaled to
+    lut[k] * q // 2^prec (q the level-0 modulus) fills N // size consecutive coefficients."""
+    ring = chain.levels[0]
+    N, q = ring.N, ring.q_l
+    scaled = [(v * q) // (1 << prec) for v in lut]
+    coeffs = [0] * N
+    for i in range(N):
+        k = i // (N // size)
+        if k < len(scaled):
+            coeffs[i] = scaled[k]
+    poly = Polynomial(ring).from_bigint_array(coeffs)
+    tv = Batch.alloc(chain, 0, 1, form="ntt")
+    lib().mlwe_RNS_trivial_sample_of_zero(tv.handles[0])
+    lib().mlwe_RNS_add_polynomial(tv.handles[0], tv.handles[0], poly.obj)
+    return tv
+
+
+class Bootstrapper:
+    """Functional bootstrapping of LWE batches into the level-0 ring of `chain`."""
+
+    def __init__(self, chain: Chain, bk: BootstrapKey):
+        self.chain, self.bk = chain, bk
+
+    def blind_rotate(self, tv: Batch, lwes: LweBatch, torus_base: int) -> Batch:
+        """The accumulators X^(phase-dependent) * tv of every input (cggi16.py:46-90), coefficient
+        form on the level-0 ring."""
+        chain, N = self.chain, self.chain.N
+        assert lwes.n == self.bk.n
+        primes = lwes.ring.primes[: lwes.l]
+        amounts = [rotation_amounts(*lwes.arrays(k), primes, N, torus_base) for k in range(len(lwes))]
         tv.to_coeff()
-        lib().mlwe_RNSc_mul_by_xai(out.obj, tv.obj, rot_tv)
-        out.repr = repr.coeff
-        for i, a_i in enumerate(steps):
-            if a_i == 0:
-                continue
-            out.to_coeff()
-            rotated = MLWE(self.scheme)
-            lib().mlwe_RNSc_mul_by_xai(rotated.obj, out.obj, a_i)
-            rotated.repr = repr.coeff
-            res = CMUX(out, rotated, bk.bk[i])
-            out.copy_from(res)
+        acc = Batch.alloc(chain, 0, len(lwes), form="coeff")
+        for h, (rot, _) in zip(acc.handles, amounts):
+            lib().mlwe_RNSc_mul_by_xai(h, tv.handles[0], rot)
+        steps = np.array([s for _, st in amounts for s in st], dtype=np.uint64)
+        lib().vfhe_hip_blind_rotate_batch(_handles(acc.handles), len(acc), self.bk.table, self.bk.n,
+                                          steps.ctypes.data_as(U64P), self.bk.ell, chain.special)
+        return acc
 
-    def blind_rotate_batch(self, tvs: list, lwes: list, bk: CGGI16_Key, torus_base: int) -> list:
-        """functional_bootstrap_wo_extract for many inputs at once: the accumulators advance
-        together through the key indices; step i is one batched inverse NTT of the accumulators
-        that rotate at i, their rotations, and one batched CMUX selected by bk.bk[i]."""
-        accs = []
-        plans = []
-        for tv, lw in zip(tvs, lwes):
-            rot_tv, steps = self._rotations(lw, torus_base)
-            acc = MLWE(self.scheme)
-            tv.to_coeff()
-            lib().mlwe_RNSc_mul_by_xai(acc.obj, tv.obj, rot_tv)
-            acc.repr = repr.coeff
-            accs.append(acc)
-            plans.append(steps)
-        n = max((len(p) for p in plans), default=0)
-        for i in range(n):
-            idx = [k for k, p in enumerate(plans) if i < len(p) and p[i] != 0]
-            if not idx:
-                continue
-            ntt_rows = [accs[k] for k in idx if accs[k].repr != repr.coeff]
-            if ntt_rows:
-                comps = []
-                for c in ntt_rows:
-                    comps += [c.obj_a_i(j) for j in range(c.r)] + [c.obj_b()]
-                lib().vfhe_hip_ntt_batch(_native.voidp_array(comps), len(comps), 1)
-                for c in ntt_rows:
-                    c.repr = repr.coeff
-            rotated = []
-            for k in idx:
-                r = MLWE(self.scheme)
-                lib().mlwe_RNSc_mul_by_xai(r.obj, accs[k].obj, plans[k][i])
-                r.repr = repr.coeff
-                rotated.append(r)
-            res = cmux_batch([accs[k] for k in idx], rotated, [bk.bk[i]] * len(idx))
-            for k, r in zip(idx, res):
-                accs[k] = r
-        return accs
+    def bootstrap(self, tv: Batch, lwes: LweBatch, torus_base: int) -> LweBatch:
+        """Blind rotation, then extraction of coefficient 0 (cggi16.py:118-140)."""
+        return extract(self.blind_rotate(tv, lwes, torus_base), 0)
 
-    def LUT_packing(self, lut: list, size: int, LUT_prec: int) -> MLWE:
-        """cggi16.py:92-116: a trivial encryption of the scaled table, each entry repeated N/size times."""
-        rlwe_tv = MLWE(self.scheme)
-        N = self.ring.N
-        q = self.ring.quotient_ring(ell=self.ring.ell).q_l
-        normal_ring = self.scheme.rings[0]
-        scaled_lut = [(val * q) // (1 << LUT_prec) for val in lut]
-        coeffs = [0] * N
-        for i in range(N):
-            k = i // (N // size)
-            if k < len(scaled_lut):
-                coeffs[i] = scaled_lut[k]
-        poly = Polynomial(normal_ring).from_bigint_array(coeffs)
-        lib().mlwe_RNS_trivial_sample_of_zero(rlwe_tv.obj)
-        rlwe_tv.repr = repr.ntt
-        return rlwe_tv + poly
 
-    def functional_bootstrap(self, out: LWE | None, tv: MLWE, rlwe_in: LWE, bk: CGGI16_Key, torus_base: int) -> LWE:
-        """cggi16.py:118-140: blind rotation, then extraction of coefficient 0 (dimension r*N)."""
-        rotated = MLWE(self.scheme)
-        self.functional_bootstrap_wo_extract(rotated, tv, rlwe_in, bk, torus_base)
-        res = extract_lwe(rotated, 0)
-        if out is not None:
-            out.a, out.b, out.n, out.l = res.a, res.b, res.n, res.l
-            return out
-        return res
+def lwe_key_of(key: SecretKey, ring) -> LweKey:
+    """The LWE secret an extracted sample decrypts under (the key's coefficients, concatenated)."""
+    return LweKey(ring, key.lwe_coeffs())

@@ -77,7 +77,13 @@ static size_t g_cap_stage = 0, g_cap_off = 0, g_cap_block = 0;
 static long g_cap_launches = 0;
 static std::vector<vh_graph_plan *> g_plans;
 static long g_graph_captures = 0, g_graph_replays = 0, g_graph_fallbacks = 0;
-#define VH_LAUNCH(x) do { if (g_cap >= VH_CAP_RECORD) g_cap_launches++; if (g_cap != VH_CAP_DRY) VH_CHECK(x); } while (0)
+// Why the last plan mismatch happened: kind * 1000000 + detail (1 staged work list
+// size at index, 2 device block size at index, 3 upload under capture, 4 launch
+// count, 5 staged list count, 6 block count).
+static long g_graph_why = 0, g_graph_why_want = 0, g_graph_why_got = 0;
+// In DRY mode the launch expression still runs (its work lists are staged into the
+// plan's buffer) but the launch functions launch nothing (g_vh_suppress_launch).
+#define VH_LAUNCH(x) do { if (g_cap >= VH_CAP_RECORD) g_cap_launches++; VH_CHECK(x); } while (0)
 
 // ---------------------------------------------------------------------------
 // Engine-private tails of the public structs
@@ -172,6 +178,8 @@ struct vh_opts {
                               // 3 inverse column, 4 the fused product's row kernel
     long xcd_lag = 3;         // persistent mode: queue slots between an item's two passes
     long xcd_occ = 2;         // persistent mode: workgroups per CU (2 or 4)
+    long xcd_l2 = 1;          // persistent mode: the first pass stores its intermediate with
+                              // ordinary (L2-allocating) stores; 0 streams it non-temporally
     long spin_limit = 1 << 22;
     long ks_fused = 1;        // key-switch: fuse the forward row pass with the key inner product
     long fft_fma = 1;         // complex FFT: 1 = avx512ifma engine's fused complex product, 0 = portable's
@@ -204,6 +212,7 @@ static void vh_ensure()
     if (const char *e = getenv("VFHE_HIP_NTT_COL_WIDE")) OPT.ntt_col_wide = atol(e);
     if (const char *e = getenv("VFHE_HIP_XCD_LAG")) OPT.xcd_lag = atol(e);
     if (const char *e = getenv("VFHE_HIP_XCD_OCC")) OPT.xcd_occ = atol(e);
+    if (const char *e = getenv("VFHE_HIP_XCD_L2")) OPT.xcd_l2 = atol(e);
     if (const char *e = getenv("VFHE_HIP_KS_FUSED")) OPT.ks_fused = atol(e);
     if (const char *e = getenv("VFHE_HIP_FFT_FMA")) OPT.fft_fma = atol(e);
     VH_CHECK(hipMalloc((void **)&C.d_err, 256));
@@ -235,10 +244,18 @@ static void *vh_stage(const void *src, size_t bytes)
 {
     vh_lane &L = LN();
     const size_t need = (bytes + 255) & ~(size_t)255, half = L.ring_size / 2;
+    static const bool dbg = getenv("VFHE_HIP_GRAPH_DEBUG") != nullptr;
+    if (dbg && g_cap != VH_CAP_OFF)
+        fprintf(stderr, "[vh graph] mode %d stage %zu bytes %zu\n", g_cap, g_cap == VH_CAP_LOG ? g_plan->stage_sz.size() : g_cap_stage, bytes);
     if (g_cap == VH_CAP_LOG) {
         g_plan->stage_sz.push_back(need);
     } else if (g_cap >= VH_CAP_RECORD) {
-        if (g_cap_stage >= g_plan->stage_sz.size() || g_plan->stage_sz[g_cap_stage] != need) throw vh_plan_mismatch();
+        if (g_cap_stage >= g_plan->stage_sz.size() || g_plan->stage_sz[g_cap_stage] != need) {
+            g_graph_why = 1000000L + (long)g_cap_stage;
+            g_graph_why_want = g_cap_stage < g_plan->stage_sz.size() ? (long)g_plan->stage_sz[g_cap_stage] : -1;
+            g_graph_why_got = (long)need;
+            throw vh_plan_mismatch();
+        }
         memcpy(g_plan->h_wl + g_cap_off, src, bytes);
         char *d = g_plan->d_wl + g_cap_off;
         g_cap_off += need;
@@ -324,6 +341,10 @@ struct vh_block_cache {
     size_t cached = 0;
 };
 static vh_block_cache BC;
+// Allocation counters (vfhe_hip_get_option "stat.cache_*"): blocks handed out from the
+// cache, blocks taken from the stream-ordered allocator, out-of-memory flushes of the
+// cache (each a synchronisation of every lane under the engine lock).
+static long g_cache_hits = 0, g_cache_misses = 0, g_cache_flushes = 0;
 
 static size_t vh_size_class(size_t bytes)
 {
@@ -366,17 +387,24 @@ static void *vh_dmalloc(size_t bytes)
     if (g_cap == VH_CAP_LOG) {
         g_plan->block_sz.push_back(c);
     } else if (g_cap >= VH_CAP_RECORD) {
-        if (g_cap_block >= g_plan->block_sz.size() || g_plan->block_sz[g_cap_block] != c) throw vh_plan_mismatch();
+        if (g_cap_block >= g_plan->block_sz.size() || g_plan->block_sz[g_cap_block] != c) {
+            g_graph_why = 2000000L + (long)g_cap_block;
+            g_graph_why_want = g_cap_block < g_plan->block_sz.size() ? (long)g_plan->block_sz[g_cap_block] : -1;
+            g_graph_why_got = (long)c;
+            throw vh_plan_mismatch();
+        }
         return g_plan->blocks[g_cap_block++];
     }
-    if (void *p = vh_take_parked(g_lane, c)) return p;
+    if (void *p = vh_take_parked(g_lane, c)) { g_cache_hits++; return p; }
     for (int ln = 0; ln < VH_MAX_LANES; ln++)
         if (ln != g_lane && !BC.free_[ln].empty())
-            if (void *p = vh_take_parked(ln, c)) return p;
+            if (void *p = vh_take_parked(ln, c)) { g_cache_hits++; return p; }
     void *p = nullptr;
+    g_cache_misses++;
     if (hipMallocAsync(&p, c, S()) != hipSuccess) {
         // out of memory with blocks parked in the cache: give them back and retry once
         (void)hipGetLastError();
+        g_cache_flushes++;
         sync_all_lanes();
         for (auto &lane : BC.free_)
             for (auto &kv : lane)
@@ -689,7 +717,7 @@ static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const v
                     vh_prof_scope ps_(VH_PROF_NTT_FP64);
                     VH_LAUNCH(vh_launch_ntt_xcd(S(), d_items, cnt, d_limbs, dir, logn, ctl, C.d_err,
                                                (unsigned)OPT.xcd_lag, (unsigned)OPT.spin_limit,
-                                               (unsigned)(C.cus * occ), occ));
+                                               (unsigned)(C.cus * occ), occ, OPT.xcd_l2 ? 1 : 0));
                 }
                 VH_LAUNCH(vh_launch_ntt_xcd_verify(S(), ctl, cnt, logn, C.d_err));
                 vh_dfree(ctl);
@@ -966,7 +994,7 @@ static void dev_ready(RNS_Polynomial p)
 {
     vh_poly *v = VP(p);
     if (!v->host_dirty) return;
-    if (g_cap >= VH_CAP_RECORD) throw vh_plan_mismatch();   // uploads happen before capture
+    if (g_cap >= VH_CAP_RECORD) { g_graph_why = 3000000L; throw vh_plan_mismatch(); }   // uploads happen before capture
     VH_CHECK(hipMemcpyAsync(v->d, v->hblock, p->allocated_l * p->ntt->N * 8, hipMemcpyHostToDevice, S()));
     VH_CHECK(hipStreamSynchronize(S()));
     v->host_dirty = 0;
@@ -1098,6 +1126,7 @@ extern "C" int vfhe_hip_set_option(const char *key, long value)
     else if (!strcmp(key, "ntt.col_wide")) OPT.ntt_col_wide = value;
     else if (!strcmp(key, "ntt.xcd_lag")) OPT.xcd_lag = value < 0 ? 0 : value;
     else if (!strcmp(key, "ntt.xcd_occupancy")) OPT.xcd_occ = value;
+    else if (!strcmp(key, "ntt.xcd_l2")) OPT.xcd_l2 = value;
     else if (!strcmp(key, "ntt.spin_limit")) OPT.spin_limit = value < 1 ? 1 : value;
     else if (!strcmp(key, "ks.fused")) OPT.ks_fused = value;
     else if (!strcmp(key, "fft.fma")) OPT.fft_fma = value;
@@ -1117,13 +1146,21 @@ extern "C" long vfhe_hip_get_option(const char *key)
     if (!strcmp(key, "ntt.col_wide")) return OPT.ntt_col_wide;
     if (!strcmp(key, "ntt.xcd_lag")) return OPT.xcd_lag;
     if (!strcmp(key, "ntt.xcd_occupancy")) return OPT.xcd_occ;
+    if (!strcmp(key, "ntt.xcd_l2")) return OPT.xcd_l2;
     if (!strcmp(key, "ntt.spin_limit")) return OPT.spin_limit;
     if (!strcmp(key, "ks.fused")) return OPT.ks_fused;
     if (!strcmp(key, "fft.fma")) return OPT.fft_fma;
     if (!strcmp(key, "stat.col_wide_launches")) return g_col_wide_launches;
+    if (!strcmp(key, "stat.cache_hits")) return g_cache_hits;
+    if (!strcmp(key, "stat.cache_misses")) return g_cache_misses;
+    if (!strcmp(key, "stat.cache_flushes")) return g_cache_flushes;
+    if (!strcmp(key, "stat.cache_parked_bytes")) return (long)BC.cached;
     if (!strcmp(key, "stat.graph_captures")) return g_graph_captures;
     if (!strcmp(key, "stat.graph_replays")) return g_graph_replays;
     if (!strcmp(key, "stat.graph_fallbacks")) return g_graph_fallbacks;
+    if (!strcmp(key, "stat.graph_why")) return g_graph_why;
+    if (!strcmp(key, "stat.graph_why_want")) return g_graph_why_want;
+    if (!strcmp(key, "stat.graph_why_got")) return g_graph_why_got;
     if (!strcmp(key, "stat.graph_plans")) return (long)g_plans.size();
     return -1;
 }
@@ -2418,6 +2455,16 @@ static void automorphism_batch(const std::vector<RNS_MLWE> &outs, const std::vec
     for (auto t : tmp) free_mlwe_RNS_sample(t);
 }
 
+// mlwe_RNSc_GHS_hybrid_keyswitch (mlwe.c:557-598) over `count` inputs sharing one key.
+extern "C" void vfhe_hip_mlwe_keyswitch_batch(RNS_MLWE *out, RNS_MLWE *in, uint64_t count, RNS_MLWE **ksk)
+{
+    VH_ENTER;
+    if (count == 0) return;
+    VH_REQUIRE(out && in && ksk);
+    keyswitch_batch(std::vector<RNS_MLWE>(out, out + count), std::vector<RNS_MLWE>(in, in + count),
+                    std::vector<RNS_MLWE **>(count, ksk));
+}
+
 extern "C" void vfhe_hip_mlwe_automorphism_batch(RNS_MLWE *out, RNS_MLWE *in, uint64_t count, uint64_t gen,
                                                  RNS_MLWE **ksk)
 {
@@ -2489,6 +2536,13 @@ static void multiply_batch(const std::vector<RNS_MLWE> &outs, const std::vector<
                            const std::vector<uint64_t> &divide_masks)
 {
     const size_t B = outs.size();
+    static const bool dbg = getenv("VFHE_HIP_GRAPH_DEBUG") != nullptr;
+    if (dbg && g_cap != VH_CAP_OFF)
+        for (size_t bi = 0; bi < B; bi++)
+            fprintf(stderr, "[vh graph] mode %d job %zu r %lu masks a1 %lx b1 %lx a2 %lx b2 %lx out %lx\n", g_cap, bi,
+                    (unsigned long)in1[bi]->r, (unsigned long)in1[bi]->a[0]->rns_mask, (unsigned long)in1[bi]->b->rns_mask,
+                    (unsigned long)in2[bi]->a[0]->rns_mask, (unsigned long)in2[bi]->b->rns_mask,
+                    (unsigned long)outs[bi]->b->rns_mask);
     VH_REQUIRE(B > 0 && in1.size() == B && in2.size() == B && ksks.size() == B && divide_masks.size() == B);
     const uint64_t N = in1[0]->b->ntt->N, r = in1[0]->r, R = mlwe_extended_rank(r);
     incNTT t = in1[0]->b->ntt;
@@ -2583,7 +2637,7 @@ static std::vector<uint64_t> multiply_key(const std::vector<RNS_MLWE> &outs, con
     std::vector<uint64_t> k = {(uint64_t)(uintptr_t)in1[0]->b->ntt, outs.size(), in1[0]->r, divide_mask,
                                (uint64_t)g_lane, (uint64_t)OPT.ntt_mode, (uint64_t)OPT.ntt_chunk_mb,
                                (uint64_t)OPT.ntt_tile_grid, (uint64_t)OPT.ntt_col_wide, (uint64_t)OPT.ks_fused,
-                               (uint64_t)OPT.xcd_lag, (uint64_t)OPT.xcd_occ};
+                               (uint64_t)OPT.xcd_lag, (uint64_t)OPT.xcd_occ, (uint64_t)OPT.xcd_l2};
     auto add = [&](RNS_MLWE c) {
         k.push_back(c->r);
         for (auto p : comps(c)) { k.push_back(p->rns_mask); k.push_back(p->allocated_l); }
@@ -2628,18 +2682,24 @@ static void multiply_batch_eager_or_plan(const std::vector<RNS_MLWE> &outs, cons
         // Replay: rebuild the work lists, then launch the graph.
         if (pl->launched) VH_CHECK(hipEventSynchronize(pl->done));
         g_cap = VH_CAP_DRY; g_plan = pl; g_cap_stage = g_cap_off = g_cap_block = 0; g_cap_launches = 0;
+        g_vh_suppress_launch = 1;
         try {
             multiply_batch(outs, in1, in2, ks, dm);
+            if (g_cap_launches != pl->launches) g_graph_why = 4000000L + std::min(999999L, std::labs(g_cap_launches - pl->launches));
+            else if (g_cap_stage != pl->stage_sz.size()) g_graph_why = 5000000L + (long)g_cap_stage;
+            else if (g_cap_block != pl->block_sz.size()) g_graph_why = 6000000L + (long)g_cap_block;
             if (g_cap_stage != pl->stage_sz.size() || g_cap_block != pl->block_sz.size() ||
                 g_cap_launches != pl->launches)
                 throw vh_plan_mismatch();
         } catch (const vh_plan_mismatch &) {
+            g_vh_suppress_launch = 0;
             g_cap = VH_CAP_OFF; g_plan = nullptr;
             g_graph_fallbacks++;
             restore(masks0);
             multiply_batch(outs, in1, in2, ks, dm);
             return;
         }
+        g_vh_suppress_launch = 0;
         g_cap = VH_CAP_OFF; g_plan = nullptr;
         VH_CHECK(hipGraphLaunch(pl->exec, S()));
         VH_CHECK(hipEventRecord(pl->done, S()));
@@ -2732,11 +2792,11 @@ static void mul_xai_rows(RNS_Polynomial out, RNS_Polynomial in, uint64_t a, bool
         return;
     }
     for_active(out->rns_mask, out->ntt->l, [&](uint64_t j) {
-        items.push_back(mk_item(drow(in, j), drow(out, j), (uint32_t)j, nullptr, 0, 0, minus1 ? 1u : 0u));
+        items.push_back(mk_item(drow(in, j), drow(out, j), (uint32_t)j, nullptr, a, 0, minus1 ? 1u : 0u));
     });
     if (items.empty()) return;
     vh_prof_scope ps_(VH_PROF_OTHER);
-    VH_LAUNCH(vh_launch_mul_xai(S(), vh_stage_vec(items), (unsigned)items.size(), DL(out), N, a));
+    VH_LAUNCH(vh_launch_mul_xai(S(), vh_stage_vec(items), (unsigned)items.size(), DL(out), N));
 }
 
 extern "C" void polynomial_RNSc_mul_by_xai(RNSc_Polynomial out, RNSc_Polynomial in, uint64_t a)
@@ -2877,6 +2937,71 @@ extern "C" void vfhe_hip_mgsw_cmux_batch(RNS_MLWE *out, RNSc_MLWE *in1, RNSc_MLW
     cmux_batch(std::vector<RNS_MLWE>(out, out + count), std::vector<RNS_MLWE>(in1, in1 + count),
                std::vector<RNS_MLWE>(in2, in2 + count), std::vector<RNS_MLWE *>(mgsw, mgsw + count), ell,
                special_primes);
+}
+
+// Blind rotation of a batch (cggi16.py:46-90 for every accumulator at once).  acc[k]
+// enters as X^-rot_b * tv in coefficient form; for every bootstrapping-key index i,
+// each accumulator whose exponent a = steps[k*n + i] is non-zero becomes
+// CMUX(acc, X^a * acc, bk[i]) (mgsw.c:51-71, NTT form) and is brought back to
+// coefficient form for the next index, the reference's sequence of calls.  One
+// monomial-product launch per index covers every accumulator (exponents per row).
+// The results are left in acc (coefficient form).
+extern "C" void vfhe_hip_blind_rotate_batch(RNS_MLWE *acc, uint64_t count, RNS_MLWE **bk, uint64_t n,
+                                            const uint64_t *steps, uint64_t ell, uint64_t special_primes)
+{
+    VH_ENTER;
+    if (count == 0 || n == 0) return;
+    VH_REQUIRE(acc && bk && steps);
+    incNTT t = acc[0]->b->ntt;
+    const uint64_t N = t->N, r = acc[0]->r;
+    VH_REQUIRE(t->split_degree == 1);
+    for (uint64_t k = 0; k < count; k++) {
+        VH_REQUIRE(acc[k]->b->ntt == t && acc[k]->r == r);
+        for (auto p : comps(acc[k])) dev_ready(p);
+    }
+    std::vector<RNS_MLWE> rot(count), out(count);
+    for (uint64_t k = 0; k < count; k++) {
+        rot[k] = alloc_sample_raw(N, r, acc[k]->b->rns_mask, t);
+        out[k] = alloc_sample_raw(N, r, acc[k]->b->rns_mask, t);
+    }
+    for (uint64_t i = 0; i < n; i++) {
+        VH_REQUIRE(bk[i]);
+        std::vector<RNS_MLWE> a_sel, r_sel, o_sel;
+        std::vector<vh_item> items;
+        for (uint64_t k = 0; k < count; k++) {
+            const uint64_t a = steps[k * n + i] & ((N << 1) - 1);
+            if (a == 0) continue;
+            const auto x = comps(acc[k]), y = comps(rot[k]);
+            for (size_t c = 0; c < x.size(); c++) {
+                y[c]->rns_mask = x[c]->rns_mask;
+                for_active(x[c]->rns_mask, t->l, [&](uint64_t j) {
+                    items.push_back(mk_item(drow(x[c], j), drow(y[c], j), (uint32_t)j, nullptr, a));
+                });
+            }
+            a_sel.push_back(acc[k]); r_sel.push_back(rot[k]); o_sel.push_back(out[k]);
+        }
+        if (a_sel.empty()) continue;
+        for (size_t off = 0; off < items.size(); off += VH_MAX_GRID_Y) {
+            const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, items.size() - off);
+            std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
+            vh_prof_scope ps_(VH_PROF_OTHER);
+            VH_LAUNCH(vh_launch_mul_xai(S(), vh_stage_vec(chunk), cnt, DL(acc[0]->b), N));
+        }
+        for (auto o : o_sel) for (auto p : comps(o)) p->rns_mask = acc[0]->b->rns_mask;
+        cmux_batch(o_sel, a_sel, r_sel, std::vector<RNS_MLWE *>(a_sel.size(), bk[i]), ell, special_primes);
+        std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> back;
+        for (auto o : o_sel) for (auto p : comps(o)) back.push_back({p, p});
+        ntt_rows(back, 1);
+        // the CMUX output becomes the accumulator: exchange the device rows
+        for (size_t s = 0; s < a_sel.size(); s++) {
+            const auto x = comps(a_sel[s]), o = comps(o_sel[s]);
+            for (size_t c = 0; c < x.size(); c++) {
+                std::swap(VP(x[c])->d, VP(o[c])->d);
+                x[c]->rns_mask = o[c]->rns_mask;
+            }
+        }
+    }
+    for (uint64_t k = 0; k < count; k++) { free_mlwe_RNS_sample(rot[k]); free_mlwe_RNS_sample(out[k]); }
 }
 
 // ---------------------------------------------------------------------------

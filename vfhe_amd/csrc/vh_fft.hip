@@ -142,11 +142,11 @@ hipError_t vh_launch_fft(hipStream_t st, double *const *rows, unsigned nrows, co
         thr = thr < 64 ? 64 : (thr > 256 ? 256 : thr);
         const size_t lds = (size_t)G * S * sizeof(double2);
         if (dir == 0) {
-            if (fma) hipLaunchKernelGGL((k_fft_pass<0, 1>), grid, dim3(thr), lds, st, rows, ws, logn, s0, R, col, G);
-            else hipLaunchKernelGGL((k_fft_pass<0, 0>), grid, dim3(thr), lds, st, rows, ws, logn, s0, R, col, G);
+            if (fma) VH_KL((k_fft_pass<0, 1>), grid, dim3(thr), lds, st, rows, ws, logn, s0, R, col, G);
+            else VH_KL((k_fft_pass<0, 0>), grid, dim3(thr), lds, st, rows, ws, logn, s0, R, col, G);
         } else {
-            if (fma) hipLaunchKernelGGL((k_fft_pass<1, 1>), grid, dim3(thr), lds, st, rows, ws, logn, s0, R, col, G);
-            else hipLaunchKernelGGL((k_fft_pass<1, 0>), grid, dim3(thr), lds, st, rows, ws, logn, s0, R, col, G);
+            if (fma) VH_KL((k_fft_pass<1, 1>), grid, dim3(thr), lds, st, rows, ws, logn, s0, R, col, G);
+            else VH_KL((k_fft_pass<1, 0>), grid, dim3(thr), lds, st, rows, ws, logn, s0, R, col, G);
         }
     };
     if (logn <= 11) {
@@ -169,18 +169,18 @@ hipError_t vh_launch_fft(hipStream_t st, double *const *rows, unsigned nrows, co
 hipError_t vh_launch_fft_bitrev(hipStream_t st, double *const *rows, double *const *tmp, unsigned nrows, uint64_t n,
                                 uint32_t prec)
 {
-    hipLaunchKernelGGL(k_fft_bitrev, dim3(vh_fblocks(n), nrows), dim3(256), 0, st, rows, tmp, n, prec);
+    VH_KL(k_fft_bitrev, dim3(vh_fblocks(n), nrows), dim3(256), 0, st, rows, tmp, n, prec);
     return hipGetLastError();
 }
 
 hipError_t vh_launch_fft_scale(hipStream_t st, double *const *rows, unsigned nrows, double scale, uint64_t count)
 {
-    hipLaunchKernelGGL(k_fft_scale, dim3(vh_fblocks(count), nrows), dim3(256), 0, st, rows, scale, count);
+    VH_KL(k_fft_scale, dim3(vh_fblocks(count), nrows), dim3(256), 0, st, rows, scale, count);
     return hipGetLastError();
 }
 
 hipError_t vh_launch_fft_round(hipStream_t st, double *const *rows, uint64_t *const *outs, unsigned nrows, uint64_t n)
 {
-    hipLaunchKernelGGL(k_fft_round, dim3(vh_fblocks(n), nrows), dim3(256), 0, st, rows, outs, n);
+    VH_KL(k_fft_round, dim3(vh_fblocks(n), nrows), dim3(256), 0, st, rows, outs, n);
     return hipGetLastError();
 }

@@ -8,6 +8,11 @@
 #define VH_MAX_LIMBS 64   // masks are uint64_t, as in the reference
 #define VH_MAX_COMPS 8    // MLWE components (rank + 1) of one key-switch output
 
+// While set, the launch functions below evaluate their arguments and launch nothing:
+// the engine rebuilds a captured graph's work lists this way before replaying it.
+extern int g_vh_suppress_launch;
+#define VH_KL(...) do { if (!g_vh_suppress_launch) hipLaunchKernelGGL(__VA_ARGS__); } while (0)
+
 // Kernel families reported by the engine's launch profiler (vfhe_hip_profile_stop).
 enum {
     VH_PROF_NTT_FP64 = 0, VH_PROF_NTT_INT, VH_PROF_ELTWISE, VH_PROF_DIVISION, VH_PROF_KSMAC, VH_PROF_OTHER,
@@ -88,7 +93,7 @@ int vh_ntt16p_occupancy();
 // Persistent two-pass NTT with per-XCD queues; ctl holds 16 + nitems zeroed uint32 counters.
 hipError_t vh_launch_ntt_xcd(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,
                              int logn, unsigned *ctl, unsigned *err, unsigned lag, unsigned spin_limit, unsigned nblocks,
-                             int occ);
+                             int occ, int l2x);
 hipError_t vh_launch_ntt_xcd_verify(hipStream_t st, const unsigned *ctl, unsigned nitems, int logn, unsigned *err);
 hipError_t vh_launch_xcc_probe(hipStream_t st, unsigned *out, unsigned nblocks);
 hipError_t vh_launch_zero_u32(hipStream_t st, unsigned *p, unsigned n);
@@ -149,9 +154,8 @@ hipError_t vh_launch_fft_bitrev(hipStream_t st, double *const *rows, double *con
                                 uint32_t prec);
 hipError_t vh_launch_fft_scale(hipStream_t st, double *const *rows, unsigned nrows, double scale, uint64_t count);
 hipError_t vh_launch_fft_round(hipStream_t st, double *const *rows, uint64_t *const *outs, unsigned nrows, uint64_t n);
-// out = X^a * in (item.aux = 0) or (X^a - 1) * in (aux = 1) mod X^n + 1, a < 2n.
-hipError_t vh_launch_mul_xai(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, uint64_t n,
-                             uint64_t a);
+// out = X^a * in (item.aux = 0) or (X^a - 1) * in (aux = 1) mod X^n + 1, a = item.scalar < 2n.
+hipError_t vh_launch_mul_xai(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, uint64_t n);
 hipError_t vh_launch_scatter_split(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                                    uint64_t N, int signed_in);
 

@@ -9,6 +9,8 @@
 #include "vh_device.h"
 #include "vh_kernels.h"
 
+int g_vh_suppress_launch = 0;
+
 // ---------------------------------------------------------------------------
 // NTT: one pass of R consecutive stages of the merged negacyclic transform.
 //
@@ -178,11 +180,14 @@ template <int R, bool COL> constexpr int vh_ntt16_lds()
 }
 
 // One task: the G sub-transforms starting at sub-transform blk*G of one row item.
-// COHERENT: read the input with device-coherent loads (it was written by other
-// workgroups of the same launch and the L1 may hold stale lines of it).
+// COHERENT: read the input with L1-bypassing loads (sc0: a relaxed workgroup-scope
+// atomic load on gfx950), served by the L2: it was written by other workgroups of
+// the same launch on the same XCD, and this CU's L1 may hold stale lines of it.
 // SHARED: the input row is read by many items (the ModUp lift: one digit row
 // transformed modulo every extended prime), so it is loaded through the caches.
-template <int DIR, int R, bool COL, bool FIRST, bool LAST, bool COHERENT, bool SHARED = false>
+// NTS: results stored non-temporally (streamed past the caches); false keeps them
+// in the L2 for a reader on the same XCD.
+template <int DIR, int R, bool COL, bool FIRST, bool LAST, bool COHERENT, bool SHARED = false, bool NTS = true>
 __device__ __forceinline__ void ntt16_task(const vh_item &it, const vh_limb &L, int logn, int s0, size_t blk,
                                            double *__restrict__ lds)
 {
@@ -204,6 +209,10 @@ __device__ __forceinline__ void ntt16_task(const vh_item &it, const vh_limb &L, 
         const uint64_t raw = COHERENT ? __scoped_atomic_load_n(in + j, __ATOMIC_RELAXED, __MEMORY_SCOPE_WRKGRP)
                                       : (SHARED ? in[j] : vh_ld(in + j));
         return FIRST ? vh_u2d(raw) : __longlong_as_double((long long)raw);
+    };
+    auto st = [&](VH_GLOBAL uint64_t *p, uint64_t x) {
+        if (NTS) vh_st(p, x);
+        else *p = x;
     };
     auto finish = [&](double x) -> uint64_t {
         return LAST ? vh_d2u(vh_canon(x, q, qinv)) : (uint64_t)__double_as_longlong(vh_center(x, q, qinv));
@@ -254,10 +263,10 @@ __device__ __forceinline__ void ntt16_task(const vh_item &it, const vh_limb &L, 
         }
         if (R == 4) {
 #pragma unroll
-            for (int m = 0; m < 16; m++) vh_st(out + gidx(t + TPS * m), finish(v[m]));
+            for (int m = 0; m < 16; m++) st(out + gidx(t + TPS * m), finish(v[m]));
         } else if (COL) {
 #pragma unroll
-            for (int m = 0; m < 16; m++) vh_st(out + gidx(16 * t + m), finish(v[m]));
+            for (int m = 0; m < 16; m++) st(out + gidx(16 * t + m), finish(v[m]));
         } else {
             // row pass: back to the stride ownership through LDS, so each store
             // instruction writes whole 128-B runs
@@ -267,7 +276,7 @@ __device__ __forceinline__ void ntt16_task(const vh_item &it, const vh_limb &L, 
             vh_wave_sync();
 #pragma unroll
             for (int m = 0; m < 16; m++)
-                vh_st(out + gidx(t + TPS * m), (uint64_t)__double_as_longlong(lds[lidx(t + TPS * m)]));
+                st(out + gidx(t + TPS * m), (uint64_t)__double_as_longlong(lds[lidx(t + TPS * m)]));
         }
     } else {
         // ---- load into contiguous ownership ----
@@ -327,7 +336,7 @@ __device__ __forceinline__ void ntt16_task(const vh_item &it, const vh_limb &L, 
 #pragma unroll
         for (int m = 0; m < 16; m++) {
             const double x = LAST ? vh_mulmod_tw(v[m], L.ninv_d, L.ninv_q, q) : v[m];
-            vh_st(out + gidx(t + TPS * m), finish(x));
+            st(out + gidx(t + TPS * m), finish(x));
         }
     }
 }
@@ -762,8 +771,8 @@ hipError_t vh_launch_ntt16_row_mac(hipStream_t st, const vh_mac_job *jobs, unsig
     if (logn < 13 || logn > 16 || comps < 1 || comps > 2) return hipErrorInvalidValue;
     const dim3 grid(njobs, 1u << (logn - 12), next);
     const int xcd = (grid.y * grid.z) % 8 == 0;
-    if (comps == 1) hipLaunchKernelGGL(k_ntt16_row_mac<1>, grid, dim3(256), 0, st, jobs, limbs, kptr, logn, xcd);
-    else hipLaunchKernelGGL(k_ntt16_row_mac<2>, grid, dim3(256), 0, st, jobs, limbs, kptr, logn, xcd);
+    if (comps == 1) VH_KL(k_ntt16_row_mac<1>, grid, dim3(256), 0, st, jobs, limbs, kptr, logn, xcd);
+    else VH_KL(k_ntt16_row_mac<2>, grid, dim3(256), 0, st, jobs, limbs, kptr, logn, xcd);
     return hipGetLastError();
 }
 
@@ -1110,7 +1119,7 @@ hipError_t vh_launch_ntt16p_row_mul(hipStream_t st, const vh_item *items, unsign
     if (occ < 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ntt16p_row_mul, 256, 0) != hipSuccess) occ = -1;
     if (occ <= 0) return hipErrorInvalidValue;
     const unsigned tpi = 1u << (logn - 12), ntiles = tpi * nitems, nb = one_tile ? ntiles : cus * (unsigned)occ;
-    hipLaunchKernelGGL(k_ntt16p_row_mul, dim3(nb < ntiles ? nb : ntiles), dim3(256), 0, st, items, limbs, logn, tpi,
+    VH_KL(k_ntt16p_row_mul, dim3(nb < ntiles ? nb : ntiles), dim3(256), 0, st, items, limbs, logn, tpi,
                        ntiles);
     return hipGetLastError();
 }
@@ -1119,7 +1128,7 @@ hipError_t vh_launch_ntt16_row_mul_inv(hipStream_t st, const vh_item *items, uns
                                        int logn)
 {
     if (logn < 13 || logn > 16) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_ntt16_row_mul_inv, dim3(1u << (logn - 12), nitems), dim3(256), 0, st, items, limbs, logn);
+    VH_KL(k_ntt16_row_mul_inv, dim3(1u << (logn - 12), nitems), dim3(256), 0, st, items, limbs, logn);
     return hipGetLastError();
 }
 
@@ -1142,7 +1151,7 @@ hipError_t vh_launch_ntt16_row_mul_inv(hipStream_t st, const vh_item *items, uns
 // and k_ntt_xcd_verify ORs 2 for any item not fully transformed (both are checked
 // by the host at its next synchronisation).
 // ---------------------------------------------------------------------------
-template <int DIR, int R1, int OCC>
+template <int DIR, int R1, int OCC, bool L2X>
 __global__ void __launch_bounds__(256, OCC)
 k_ntt_xcd(const vh_item *__restrict__ items, unsigned nitems, const vh_limb *__restrict__ limbs, int logn,
           unsigned *__restrict__ ctl, unsigned *__restrict__ err, unsigned lag, unsigned spin_limit)
@@ -1167,8 +1176,8 @@ k_ntt_xcd(const vh_item *__restrict__ items, unsigned nitems, const vh_limb *__r
             if (j >= mine) continue;
             const unsigned i = xcd + 8 * j;
             const vh_item &it = items[i];
-            if (DIR == 0) ntt16_task<0, R1, true, true, false, false>(it, limbs[it.limb], logn, 0, r, lds);
-            else ntt16_task<1, 8, false, true, false, false>(it, limbs[it.limb], logn, R1, r, lds);
+            if (DIR == 0) ntt16_task<0, R1, true, true, false, false, false, !L2X>(it, limbs[it.limb], logn, 0, r, lds);
+            else ntt16_task<1, 8, false, true, false, false, false, !L2X>(it, limbs[it.limb], logn, R1, r, lds);
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             if (threadIdx.x == 0) __hip_atomic_fetch_add(done + i, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1435,10 +1444,10 @@ hipError_t vh_launch_moddown_rescale(hipStream_t st, const vh_mdr_job *jobs, uns
 {
     const dim3 grid((unsigned)((n + 255) / 256), njobs);
     if (kmax <= 4)
-        hipLaunchKernelGGL(k_moddown_rescale<4>, grid, dim3(256), 0, st, jobs, limbs, (const vh_divc *)cdown,
+        VH_KL(k_moddown_rescale<4>, grid, dim3(256), 0, st, jobs, limbs, (const vh_divc *)cdown,
                            (const vh_divc *)cresc, n);
     else
-        hipLaunchKernelGGL(k_moddown_rescale<VH_MAX_DIV>, grid, dim3(256), 0, st, jobs, limbs, (const vh_divc *)cdown,
+        VH_KL(k_moddown_rescale<VH_MAX_DIV>, grid, dim3(256), 0, st, jobs, limbs, (const vh_divc *)cdown,
                            (const vh_divc *)cresc, n);
     return hipGetLastError();
 }
@@ -1605,11 +1614,11 @@ hipError_t vh_launch_ntt_pass(hipStream_t st, const vh_item *items, unsigned nit
     thr = thr < 64 ? 64 : (thr > 256 ? 256 : thr);
     const size_t lds = (size_t)G * S * 8;
     if (fp) {
-        if (dir == 0) hipLaunchKernelGGL((k_ntt_pass<0, true>), grid, dim3(thr), lds, st, items, limbs, logn, s0, R, col, G, first, last);
-        else hipLaunchKernelGGL((k_ntt_pass<1, true>), grid, dim3(thr), lds, st, items, limbs, logn, s0, R, col, G, first, last);
+        if (dir == 0) VH_KL((k_ntt_pass<0, true>), grid, dim3(thr), lds, st, items, limbs, logn, s0, R, col, G, first, last);
+        else VH_KL((k_ntt_pass<1, true>), grid, dim3(thr), lds, st, items, limbs, logn, s0, R, col, G, first, last);
     } else {
-        if (dir == 0) hipLaunchKernelGGL((k_ntt_pass<0, false>), grid, dim3(thr), lds, st, items, limbs, logn, s0, R, col, G, first, last);
-        else hipLaunchKernelGGL((k_ntt_pass<1, false>), grid, dim3(thr), lds, st, items, limbs, logn, s0, R, col, G, first, last);
+        if (dir == 0) VH_KL((k_ntt_pass<0, false>), grid, dim3(thr), lds, st, items, limbs, logn, s0, R, col, G, first, last);
+        else VH_KL((k_ntt_pass<1, false>), grid, dim3(thr), lds, st, items, limbs, logn, s0, R, col, G, first, last);
     }
     return hipGetLastError();
 }
@@ -1635,21 +1644,21 @@ k_tensor1(const vh_tensor_job *__restrict__ jobs, const vh_limb *__restrict__ li
 hipError_t vh_launch_tensor1(hipStream_t st, const vh_tensor_job *jobs, unsigned njobs, const vh_limb *limbs,
                              uint64_t n)
 {
-    hipLaunchKernelGGL(k_tensor1, dim3(vh_blocks(n, 256, 64), njobs), dim3(256), 0, st, jobs, limbs, n);
+    VH_KL(k_tensor1, dim3(vh_blocks(n, 256, 64), njobs), dim3(256), 0, st, jobs, limbs, n);
     return hipGetLastError();
 }
 
 hipError_t vh_launch_eltwise(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                              int op, uint64_t n)
 {
-    hipLaunchKernelGGL(k_eltwise, dim3(vh_blocks(n, 256, 64), nitems), dim3(256), 0, st, items, limbs, op, n);
+    VH_KL(k_eltwise, dim3(vh_blocks(n, 256, 64), nitems), dim3(256), 0, st, items, limbs, op, n);
     return hipGetLastError();
 }
 
 hipError_t vh_launch_split_mul(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                                int mode, uint64_t n, uint64_t S)
 {
-    hipLaunchKernelGGL(k_split_mul, dim3(vh_blocks(n * S, 256, 64), nitems), dim3(256), 0, st, items, limbs, mode, n);
+    VH_KL(k_split_mul, dim3(vh_blocks(n * S, 256, 64), nitems), dim3(256), 0, st, items, limbs, mode, n);
     return hipGetLastError();
 }
 
@@ -1658,10 +1667,10 @@ hipError_t vh_launch_division(hipStream_t st, const vh_div_job *jobs, unsigned n
 {
     (void)nout;
     if (kmax <= 4)
-        hipLaunchKernelGGL(k_division<4>, dim3((unsigned)((n + 255) / 256), njobs), dim3(256), 0, st, jobs, limbs,
+        VH_KL(k_division<4>, dim3((unsigned)((n + 255) / 256), njobs), dim3(256), 0, st, jobs, limbs,
                            (const vh_divc *)consts, n);
     else
-        hipLaunchKernelGGL(k_division<VH_MAX_DIV>, dim3((unsigned)((n + 255) / 256), njobs), dim3(256), 0, st, jobs,
+        VH_KL(k_division<VH_MAX_DIV>, dim3((unsigned)((n + 255) / 256), njobs), dim3(256), 0, st, jobs,
                            limbs, (const vh_divc *)consts, n);
     return hipGetLastError();
 }
@@ -1669,14 +1678,14 @@ hipError_t vh_launch_division(hipStream_t st, const vh_div_job *jobs, unsigned n
 hipError_t vh_launch_base_conversion(hipStream_t st, const vh_bconv_job *jobs, unsigned njobs, unsigned nout,
                                      const vh_limb *limbs, const void *consts, uint64_t n)
 {
-    hipLaunchKernelGGL(k_base_conversion, dim3((unsigned)((n + 255) / 256), njobs, nout), dim3(256), 0, st, jobs,
+    VH_KL(k_base_conversion, dim3((unsigned)((n + 255) / 256), njobs, nout), dim3(256), 0, st, jobs,
                        limbs, (const ulonglong2 *)consts, n);
     return hipGetLastError();
 }
 
 hipError_t vh_launch_permute(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, uint64_t N)
 {
-    hipLaunchKernelGGL(k_permute, dim3(vh_blocks(N, 256, 64), nitems), dim3(256), 0, st, items, limbs, N);
+    VH_KL(k_permute, dim3(vh_blocks(N, 256, 64), nitems), dim3(256), 0, st, items, limbs, N);
     return hipGetLastError();
 }
 
@@ -1684,15 +1693,15 @@ hipError_t vh_launch_keyswitch_mac(hipStream_t st, const vh_mac_job *jobs, unsig
                                    const vh_limb *limbs, const uint64_t *const *kptr, uint64_t n, int fp)
 {
     const dim3 grid(njobs, (unsigned)((n + 255) / 256), next);
-    if (fp) hipLaunchKernelGGL(k_keyswitch_mac<true>, grid, dim3(256), 0, st, jobs, limbs, kptr, n);
-    else hipLaunchKernelGGL(k_keyswitch_mac<false>, grid, dim3(256), 0, st, jobs, limbs, kptr, n);
+    if (fp) VH_KL(k_keyswitch_mac<true>, grid, dim3(256), 0, st, jobs, limbs, kptr, n);
+    else VH_KL(k_keyswitch_mac<false>, grid, dim3(256), 0, st, jobs, limbs, kptr, n);
     return hipGetLastError();
 }
 
 hipError_t vh_launch_scatter_split(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                                    uint64_t N, int signed_in)
 {
-    hipLaunchKernelGGL(k_scatter_split, dim3(vh_blocks(N, 256, 64), nitems), dim3(256), 0, st, items, limbs, N,
+    VH_KL(k_scatter_split, dim3(vh_blocks(N, 256, 64), nitems), dim3(256), 0, st, items, limbs, N,
                        signed_in);
     return hipGetLastError();
 }
@@ -1707,7 +1716,7 @@ hipError_t vh_launch_ntt16(hipStream_t st, const vh_item *items, unsigned nitems
     dim3 grid(nsub / G, nitems);
     // (dir, first, last, col) take the four combinations a two-pass transform uses;
     // first == 2 marks a forward first pass whose source rows are shared by items.
-#define VH_NTT16(D, RR, CC, FF, LL, SS) hipLaunchKernelGGL((k_ntt16<D, RR, CC, FF, LL, SS>), grid, dim3(256), 0, st, items, limbs, logn, s0)
+#define VH_NTT16(D, RR, CC, FF, LL, SS) VH_KL((k_ntt16<D, RR, CC, FF, LL, SS>), grid, dim3(256), 0, st, items, limbs, logn, s0)
 #define VH_NTT16_R(D, CC, FF, LL, SS) \
     switch (R) { case 4: VH_NTT16(D, 4, CC, FF, LL, SS); break; case 5: VH_NTT16(D, 5, CC, FF, LL, SS); break; \
                  case 6: VH_NTT16(D, 6, CC, FF, LL, SS); break; case 7: VH_NTT16(D, 7, CC, FF, LL, SS); break; \
@@ -1735,9 +1744,9 @@ hipError_t vh_launch_ntt16p(hipStream_t st, const vh_item *items, unsigned nitem
     const unsigned tpi = nsub / G, ntiles = tpi * nitems;
     if (wide && nblocks != UINT_MAX) nblocks = nblocks / 2 ? nblocks / 2 : 1;   // same threads per CU
     const dim3 grid(nblocks < ntiles ? nblocks : ntiles);
-#define VH_P(D, RR, CC, FF, LL, SS) hipLaunchKernelGGL((k_ntt16p<D, RR, CC, FF, LL, SS, 256>), grid, dim3(256), 0, st, items, limbs, logn, s0, tpi, ntiles)
+#define VH_P(D, RR, CC, FF, LL, SS) VH_KL((k_ntt16p<D, RR, CC, FF, LL, SS, 256>), grid, dim3(256), 0, st, items, limbs, logn, s0, tpi, ntiles)
 #define VH_PC(D, RR, FF, LL, SS) \
-    do { if (wide) hipLaunchKernelGGL((k_ntt16p<D, RR, true, FF, LL, SS, 512>), grid, dim3(512), 0, st, items, limbs, logn, s0, tpi, ntiles); \
+    do { if (wide) VH_KL((k_ntt16p<D, RR, true, FF, LL, SS, 512>), grid, dim3(512), 0, st, items, limbs, logn, s0, tpi, ntiles); \
          else VH_P(D, RR, true, FF, LL, SS); } while (0)
 #define VH_P_R(D, FF, LL, SS) \
     switch (R) { case 5: VH_PC(D, 5, FF, LL, SS); break; case 6: VH_PC(D, 6, FF, LL, SS); break; \
@@ -1771,10 +1780,14 @@ int vh_ntt16p_occupancy()
 
 hipError_t vh_launch_ntt_xcd(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,
                              int logn, unsigned *ctl, unsigned *err, unsigned lag, unsigned spin_limit, unsigned nblocks,
-                             int occ)
+                             int occ, int l2x)
 {
     if (logn < 13 || logn > 16) return hipErrorInvalidValue;
-#define VH_XCD(D, RR, OO) hipLaunchKernelGGL((k_ntt_xcd<D, RR, OO>), dim3(nblocks), dim3(256), 0, st, items, nitems, limbs, logn, ctl, err, lag, spin_limit)
+#define VH_XCD(D, RR, OO) \
+    do { \
+        if (l2x) VH_KL((k_ntt_xcd<D, RR, OO, true>), dim3(nblocks), dim3(256), 0, st, items, nitems, limbs, logn, ctl, err, lag, spin_limit); \
+        else VH_KL((k_ntt_xcd<D, RR, OO, false>), dim3(nblocks), dim3(256), 0, st, items, nitems, limbs, logn, ctl, err, lag, spin_limit); \
+    } while (0)
 #define VH_XCD_R(D, OO) \
     switch (logn - 8) { case 5: VH_XCD(D, 5, OO); break; case 6: VH_XCD(D, 6, OO); break; \
                         case 7: VH_XCD(D, 7, OO); break; default: VH_XCD(D, 8, OO); }
@@ -1793,7 +1806,7 @@ __global__ void k_ntt_xcd_verify(const unsigned *__restrict__ ctl, unsigned nite
 
 hipError_t vh_launch_ntt_xcd_verify(hipStream_t st, const unsigned *ctl, unsigned nitems, int logn, unsigned *err)
 {
-    hipLaunchKernelGGL(k_ntt_xcd_verify, dim3((nitems + 255) / 256), dim3(256), 0, st, ctl, nitems,
+    VH_KL(k_ntt_xcd_verify, dim3((nitems + 255) / 256), dim3(256), 0, st, ctl, nitems,
                        1u << (logn - 12), err);
     return hipGetLastError();
 }
@@ -1899,7 +1912,7 @@ k_chacha_raw(vh_rng_key key, uint64_t ctr0, uint64_t nblocks, uint32_t *__restri
 
 hipError_t vh_launch_chacha_raw(hipStream_t st, vh_rng_key key, uint64_t ctr0, uint64_t nblocks, uint32_t *out)
 {
-    hipLaunchKernelGGL(k_chacha_raw, dim3(vh_blocks(nblocks, 256, 64)), dim3(256), 0, st, key, ctr0, nblocks, out);
+    VH_KL(k_chacha_raw, dim3(vh_blocks(nblocks, 256, 64)), dim3(256), 0, st, key, ctr0, nblocks, out);
     return hipGetLastError();
 }
 
@@ -1907,7 +1920,7 @@ hipError_t vh_launch_sample_uniform(hipStream_t st, const vh_item *items, unsign
                                     vh_rng_key key, uint64_t n)
 {
     if (n % 8) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_sample_uniform, dim3(vh_blocks(n / 8, 256, 64), nitems), dim3(256), 0, st, items, limbs, key, n);
+    VH_KL(k_sample_uniform, dim3(vh_blocks(n / 8, 256, 64), nitems), dim3(256), 0, st, items, limbs, key, n);
     return hipGetLastError();
 }
 
@@ -1915,7 +1928,7 @@ hipError_t vh_launch_sample_gaussian(hipStream_t st, const vh_gauss_job *jobs, u
                                      const vh_limb *limbs, vh_rng_key key, double sigma, uint64_t n)
 {
     if (n % 4) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_sample_gaussian, dim3(vh_blocks(n / 4, 256, 64), njobs), dim3(256), 0, st, jobs, items, limbs,
+    VH_KL(k_sample_gaussian, dim3(vh_blocks(n / 4, 256, 64), njobs), dim3(256), 0, st, jobs, items, limbs,
                        key, sigma, n);
     return hipGetLastError();
 }
@@ -1927,9 +1940,10 @@ hipError_t vh_launch_sample_gaussian(hipStream_t st, const vh_gauss_job *jobs, u
 // -in[u - n].  One thread per output coefficient; out != in.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_mul_xai(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, uint64_t n, uint64_t a)
+k_mul_xai(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, uint64_t n)
 {
     const vh_item it = items[blockIdx.y];
+    const uint64_t a = it.scalar;   // the monomial's exponent, per row (< 2n)
     const vh_limb L = limbs[it.limb];
     const uint64_t q = L.q, mu = L.mu64, n2 = 2 * n;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -1941,11 +1955,9 @@ k_mul_xai(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, 
     }
 }
 
-hipError_t vh_launch_mul_xai(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, uint64_t n,
-                             uint64_t a)
+hipError_t vh_launch_mul_xai(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, uint64_t n)
 {
-    if (a >= 2 * n) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_mul_xai, dim3(vh_blocks(n, 256, 64), nitems), dim3(256), 0, st, items, limbs, n, a);
+    VH_KL(k_mul_xai, dim3(vh_blocks(n, 256, 64), nitems), dim3(256), 0, st, items, limbs, n);
     return hipGetLastError();
 }
 
@@ -1957,7 +1969,7 @@ __global__ void k_xcc_probe(unsigned *out)
 
 hipError_t vh_launch_xcc_probe(hipStream_t st, unsigned *out, unsigned nblocks)
 {
-    hipLaunchKernelGGL(k_xcc_probe, dim3(nblocks), dim3(64), 0, st, out);
+    VH_KL(k_xcc_probe, dim3(nblocks), dim3(64), 0, st, out);
     return hipGetLastError();
 }
 
@@ -1968,7 +1980,7 @@ __global__ void k_zero_u32(unsigned *p, unsigned n)
 
 hipError_t vh_launch_zero_u32(hipStream_t st, unsigned *p, unsigned n)
 {
-    hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024), dim3(256), 0, st, p, n);
+    VH_KL(k_zero_u32, dim3((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
 
@@ -2016,6 +2028,6 @@ hipError_t vh_launch_slot(hipStream_t st, const vh_item *items, unsigned nitems,
                           uint64_t n, uint64_t ps)
 {
     const uint64_t cnt = op == 2 ? n / ps : n;
-    hipLaunchKernelGGL(k_slot, dim3(vh_blocks(cnt, 256, 64), nitems), dim3(256), 0, st, items, limbs, op, n, ps);
+    VH_KL(k_slot, dim3(vh_blocks(cnt, 256, 64), nitems), dim3(256), 0, st, items, limbs, op, n, ps);
     return hipGetLastError();
 }

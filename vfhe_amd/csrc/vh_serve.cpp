@@ -13,6 +13,11 @@
 // submit -> completion observed.  Outputs are allocated by the runtime and
 // handed to the caller, who frees them (free_mlwe_RNS_sample).
 //
+// Inputs may still be in flight on the submitting thread's lane: submit records
+// an event on that lane's stream, and the batch's lane waits for the events of
+// all its requests before its first launch, so a ciphertext produced
+// asynchronously by the caller is read only once it is complete.
+//
 // The runtime calls only the engine's public C ABI (include/vfhe_hip.h).
 #include <hip/hip_runtime.h>
 
@@ -53,7 +58,8 @@ struct Req {
     RNS_MLWE **ksk;
     uint64_t gen, divide_mask;
     uint64_t bytes;         // footprint reserved while in flight
-    double t_submit, t_done;
+    hipEvent_t ready;       // recorded on the submitter's stream: the inputs are complete
+    double t_submit, t_admit, t_launched, t_done;
     RNS_MLWE out;
 };
 
@@ -103,8 +109,10 @@ struct Server {
     bool stop = false;
     std::thread worker;
     uint64_t n_batches = 0, n_reqs = 0, max_busy = 0;
-    double host_s = 0, device_s = 0, t_start = now_s();
+    double host_s = 0, device_s = 0, t_start = now_s(), max_launch_s = 0;
     std::vector<hipEvent_t> ev_free;
+
+    std::vector<hipEvent_t> ready_free;   // guarded by mu (submitters and the worker)
 
     hipEvent_t event()
     {
@@ -114,12 +122,28 @@ struct Server {
         return e;
     }
 
+    // with mu held
+    hipEvent_t ready_event()
+    {
+        if (!ready_free.empty()) { hipEvent_t e = ready_free.back(); ready_free.pop_back(); return e; }
+        hipEvent_t e;
+        srv_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreateWithFlags");
+        return e;
+    }
+
     void launch(std::vector<Req *> &reqs, int lane)
     {
         const double th0 = now_s();
         const int prev = vfhe_hip_set_lane(lane + 1);   // lanes 1..lanes; lane 0 stays the caller's
+        hipStream_t st = (hipStream_t)vfhe_hip_current_stream();
+        for (Req *r : reqs) srv_check(hipStreamWaitEvent(st, r->ready, 0), "hipStreamWaitEvent");
+        {
+            // the waits above captured the events' state: they can be recorded again
+            std::lock_guard<std::mutex> g(mu);
+            for (Req *r : reqs) { ready_free.push_back(r->ready); r->ready = nullptr; }
+        }
         hipEvent_t ev0 = event();
-        srv_check(hipEventRecord(ev0, (hipStream_t)vfhe_hip_current_stream()), "hipEventRecord");
+        srv_check(hipEventRecord(ev0, st), "hipEventRecord");
         const size_t B = reqs.size();
         std::vector<RNS_MLWE> out(B), in1(B), in2(B);
         uint64_t bytes = 0;
@@ -148,13 +172,16 @@ struct Server {
         b.ev = event();
         b.lane = lane;
         b.bytes = bytes;
-        srv_check(hipEventRecord(b.ev, (hipStream_t)vfhe_hip_current_stream()), "hipEventRecord");
+        srv_check(hipEventRecord(b.ev, st), "hipEventRecord");
         vfhe_hip_set_lane(prev);
         lane_busy[lane] = 1;
         inflight.push_back(b);
         n_batches++;
         n_reqs += B;
-        host_s += now_s() - th0;
+        const double th1 = now_s();
+        for (Req *r : reqs) r->t_launched = th1;
+        host_s += th1 - th0;
+        max_launch_s = std::max(max_launch_s, th1 - th0);
     }
 
     // One worker step; returns true when it made progress.
@@ -207,7 +234,8 @@ struct Server {
                     it = waiting.erase(it);
                 }
                 if (batch.empty()) break;
-                for (Req *r : batch) in_use += r->bytes;
+                const double ta = now_s();
+                for (Req *r : batch) { in_use += r->bytes; r->t_admit = ta; }
                 peak = std::max(peak, in_use);
             }
             launch(batch, lane);
@@ -270,6 +298,11 @@ extern "C" uint64_t vfhe_hip_server_submit(void *srv, int op, RNS_MLWE in1, RNS_
     r->bytes = footprint(r);
     {
         std::lock_guard<std::mutex> g(s->mu);
+        r->ready = s->ready_event();
+    }
+    srv_check(hipEventRecord(r->ready, (hipStream_t)vfhe_hip_current_stream()), "hipEventRecord");
+    {
+        std::lock_guard<std::mutex> g(s->mu);
         r->id = s->next_id++;
         r->t_submit = now_s();
         s->incoming.push_back(r);
@@ -279,10 +312,9 @@ extern "C" uint64_t vfhe_hip_server_submit(void *srv, int op, RNS_MLWE in1, RNS_
     return r->id;
 }
 
-extern "C" uint64_t vfhe_hip_server_poll(void *srv, uint64_t *ids, RNS_MLWE *outs, double *latency_s, uint64_t max,
-                                         int wait)
+static uint64_t poll_impl(Server *s, uint64_t *ids, RNS_MLWE *outs, double *latency_s, double *parts, uint64_t max,
+                          int wait)
 {
-    Server *s = (Server *)srv;
     std::unique_lock<std::mutex> g(s->mu);
     if (wait) s->cv_done.wait(g, [&] { return !s->done.empty() || s->pending == 0; });
     const uint64_t n = std::min<uint64_t>(max, s->done.size());
@@ -291,10 +323,27 @@ extern "C" uint64_t vfhe_hip_server_poll(void *srv, uint64_t *ids, RNS_MLWE *out
         ids[k] = r->id;
         outs[k] = r->out;
         latency_s[k] = r->t_done - r->t_submit;
+        if (parts) {
+            parts[3 * k] = r->t_admit - r->t_submit;        // waiting for a lane and memory
+            parts[3 * k + 1] = r->t_launched - r->t_admit;  // host enqueue of its batch
+            parts[3 * k + 2] = r->t_done - r->t_launched;   // device time + completion observed
+        }
         delete r;
     }
     s->done.erase(s->done.begin(), s->done.begin() + (long)n);
     return n;
+}
+
+extern "C" uint64_t vfhe_hip_server_poll(void *srv, uint64_t *ids, RNS_MLWE *outs, double *latency_s, uint64_t max,
+                                         int wait)
+{
+    return poll_impl((Server *)srv, ids, outs, latency_s, nullptr, max, wait);
+}
+
+extern "C" uint64_t vfhe_hip_server_poll_timed(void *srv, uint64_t *ids, RNS_MLWE *outs, double *latency_s,
+                                               double *parts, uint64_t max, int wait)
+{
+    return poll_impl((Server *)srv, ids, outs, latency_s, parts, max, wait);
 }
 
 extern "C" void vfhe_hip_server_stats(void *srv, double *stats)
@@ -310,6 +359,7 @@ extern "C" void vfhe_hip_server_stats(void *srv, double *stats)
     stats[6] = s->host_s;
     stats[7] = s->device_s;
     stats[8] = now_s() - s->t_start;
+    stats[9] = s->max_launch_s;
 }
 
 extern "C" void vfhe_hip_server_free(void *srv)
@@ -323,6 +373,7 @@ extern "C" void vfhe_hip_server_free(void *srv)
     s->cv_work.notify_all();
     s->worker.join();
     for (hipEvent_t e : s->ev_free) (void)hipEventDestroy(e);
+    for (hipEvent_t e : s->ready_free) (void)hipEventDestroy(e);
     for (Req *r : s->done) delete r;   // outputs of unpolled requests stay with the engine's allocator
     delete s;
 }
