@@ -442,6 +442,16 @@ void vfhe_hip_graph_clear(void);
    key, one launch sequence: out[k] is in[k] switched by ksk (outputs allocated on the extended ring,
    left on the level limbs in coefficient form). */
 void vfhe_hip_mlwe_keyswitch_batch(RNS_MLWE *out, RNS_MLWE *in, uint64_t count, RNS_MLWE **ksk);
+/* No counterpart in the reference: the allocation-free form of vfhe_hip_ntt_batch, for
+   callers that cannot allocate (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc).  Same transforms and results; the persistent
+   per-XCD kernel's work list and queue counters (and any other path's work lists)
+   live in a per-device scratch array of 1 MiB (VH_SCRATCH_BYTES) held for the life
+   of the process, so no device or host memory is allocated.  Returns 0, or -1 with
+   nothing done when the batch needs more than the scratch (about 18,000 limb rows).
+   Calls that overlap on one device (several lanes or threads) share the scratch and
+   race: order them. */
+int vfhe_hip_ntt_batch_noalloc(RNS_Polynomial *polys, uint64_t count, int direction);
 /* mgsw_CMUX for `count` independent (out, in1, in2, mgsw) quadruples in one
    launch sequence (a batch of blind rotations stepping through one key index). */
 void vfhe_hip_mgsw_cmux_batch(RNS_MLWE *out, RNSc_MLWE *in1, RNSc_MLWE *in2, RNS_MLWE **mgsw, uint64_t count,

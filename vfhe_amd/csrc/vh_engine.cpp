@@ -146,6 +146,12 @@ struct vh_ctx {
     unsigned pipe_blocks = 0;    // persistent workgroups of the pipelined NTT pass (CUs x occupancy)
 };
 static vh_ctx C;
+// Device address of each device's scratch array (vh_kernels.hip), resolved at set-up.
+static void *g_scratch_of[64] = {nullptr};
+// While an allocation-free call runs: work lists and scratch blocks are carved from
+// g_na_base (that device's scratch array) by a bump offset, nothing is allocated.
+static char *g_na_base = nullptr;
+static size_t g_na_off = 0;
 static thread_local int g_lane = 0;
 static inline vh_lane &LN() { return C.lanes[g_lane]; }
 static inline hipStream_t S() { return C.lanes[g_lane].stream; }
@@ -215,6 +221,8 @@ static void vh_ensure()
     if (const char *e = getenv("VFHE_HIP_XCD_L2")) OPT.xcd_l2 = atol(e);
     if (const char *e = getenv("VFHE_HIP_KS_FUSED")) OPT.ks_fused = atol(e);
     if (const char *e = getenv("VFHE_HIP_FFT_FMA")) OPT.fft_fma = atol(e);
+    VH_REQUIRE(C.device >= 0 && C.device < 64);
+    VH_CHECK(vh_scratch_resolve(&g_scratch_of[C.device]));
     VH_CHECK(hipMalloc((void **)&C.d_err, 256));
     VH_CHECK(hipMemset(C.d_err, 0, 256));
     {
@@ -274,6 +282,14 @@ static void *vh_stage(const void *src, size_t bytes)
     const size_t off = L.ring_off;
     memcpy(L.h_ring + off, src, bytes);
     L.ring_off += need;
+    if (g_na_base) {
+        // allocation-free call: the list goes to the device's scratch array
+        VH_REQUIRE(g_na_off + need <= VH_SCRATCH_BYTES);
+        char *d = g_na_base + g_na_off;
+        g_na_off += need;
+        VH_CHECK(hipMemcpyAsync(d, L.h_ring + off, bytes, hipMemcpyHostToDevice, L.stream));
+        return d;
+    }
     // Small work lists are read by the kernels straight from pinned host memory
     // (one read per workgroup); larger ones are copied to HBM first.
     if (bytes <= VH_STAGE_INPLACE) return L.hd_ring + off;
@@ -395,6 +411,14 @@ static void *vh_dmalloc(size_t bytes)
         }
         return g_plan->blocks[g_cap_block++];
     }
+    if (g_na_base) {
+        // allocation-free call: a block of the scratch array (sized by the entry point)
+        const size_t need = (bytes + 255) & ~(size_t)255;
+        VH_REQUIRE(g_na_off + need <= VH_SCRATCH_BYTES);
+        void *p = g_na_base + g_na_off;
+        g_na_off += need;
+        return p;
+    }
     if (void *p = vh_take_parked(g_lane, c)) { g_cache_hits++; return p; }
     for (int ln = 0; ln < VH_MAX_LANES; ln++)
         if (ln != g_lane && !BC.free_[ln].empty())
@@ -420,6 +444,7 @@ static void *vh_dmalloc(size_t bytes)
 static void vh_dfree(void *p)
 {
     if (!p || g_cap >= VH_CAP_RECORD) return;   // a plan's blocks stay with the plan
+    if (g_na_base && (char *)p >= g_na_base && (char *)p < g_na_base + VH_SCRATCH_BYTES) return;
     auto it = BC.size_of.find(p);
     VH_REQUIRE(it != BC.size_of.end());
     const size_t c = it->second;
@@ -3050,6 +3075,51 @@ extern "C" void vfhe_hip_ntt_batch(RNS_Polynomial *polys, uint64_t count, int di
     std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pairs;
     for (uint64_t i = 0; i < count; i++) { dev_ready(polys[i]); pairs.push_back({polys[i], polys[i]}); }
     ntt_rows(pairs, direction ? 1 : 0);
+}
+
+// Allocation-free form of vfhe_hip_ntt_batch: the same transforms, run by the
+// persistent per-XCD kernel (one launch per direction) wherever it applies (FP64
+// limbs, N = 2^13..2^16, XCD placement probe passed), with that kernel's work list
+// and queue counters, and every other path's work lists, in this device's scratch
+// array (vh_kernels.hip) instead of engine allocations.  The counters are cleared
+// by a fill kernel before each launch.  Calls that overlap on one device (other
+// lanes, other threads' streams) share the array and race.  Returns 0, or -1 with
+// nothing done when the batch needs more than VH_SCRATCH_BYTES.
+extern "C" int vfhe_hip_ntt_batch_noalloc(RNS_Polynomial *polys, uint64_t count, int direction)
+{
+    VH_ENTER;
+    char *base = (char *)g_scratch_of[C.device];
+    VH_REQUIRE(base != nullptr && (count == 0 || polys));
+    uint64_t rows = 0;
+    for (uint64_t i = 0; i < count; i++) {
+        const incNTT t = polys[i]->ntt;
+        const uint64_t m = polys[i]->rns_mask & (t->l >= 64 ? ~0ULL : ((1ULL << t->l) - 1));
+        rows += (uint64_t)__builtin_popcountll(m) * t->split_degree;
+    }
+    // per row: a 48-byte work item and two 4-byte counters; per launch group (at most
+    // two: FP64 and integer limbs) the 256-byte roundings and the 16 queue tickets
+    const size_t need = rows * (sizeof(vh_item) + 8) + 2048;
+    if (need > VH_SCRATCH_BYTES) {
+        fprintf(stderr, "vfhe hip engine: vfhe_hip_ntt_batch_noalloc: %lu limb rows need %zu bytes of scratch, "
+                        "more than its %zu\n", (unsigned long)rows, need, (size_t)VH_SCRATCH_BYTES);
+        return -1;
+    }
+    std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pairs;
+    for (uint64_t i = 0; i < count; i++) { dev_ready(polys[i]); pairs.push_back({polys[i], polys[i]}); }
+    const long mode = OPT.ntt_mode;
+    if (C.xcd_ok) OPT.ntt_mode = 1;
+    g_na_base = base;
+    g_na_off = 0;
+    try {
+        ntt_rows(pairs, direction ? 1 : 0);
+    } catch (...) {
+        g_na_base = nullptr;
+        OPT.ntt_mode = mode;
+        throw;
+    }
+    g_na_base = nullptr;
+    OPT.ntt_mode = mode;
+    return 0;
 }
 
 // The fused product's three launches (N = 2^13..2^16, FP64 limbs): forward column

@@ -13,6 +13,14 @@
 extern int g_vh_suppress_launch;
 #define VH_KL(...) do { if (!g_vh_suppress_launch) hipLaunchKernelGGL(__VA_ARGS__); } while (0)
 
+// Per-device scratch of the allocation-free entry point (vfhe_hip_ntt_batch_noalloc):
+// a file-scope device array in vh_kernels.hip, VH_SCRATCH_BYTES on every device the
+// library runs on, for the life of the process.
+#define VH_SCRATCH_BYTES ((size_t)1 << 20)
+// Resolves the calling thread's current device's copy of the scratch array to a
+// device address (hipGetSymbolAddress); call once per device with it current.
+hipError_t vh_scratch_resolve(void **ptr);
+
 // Kernel families reported by the engine's launch profiler (vfhe_hip_profile_stop).
 enum {
     VH_PROF_NTT_FP64 = 0, VH_PROF_NTT_INT, VH_PROF_ELTWISE, VH_PROF_DIVISION, VH_PROF_KSMAC, VH_PROF_OTHER,

@@ -1973,6 +1973,20 @@ hipError_t vh_launch_xcc_probe(hipStream_t st, unsigned *out, unsigned nblocks)
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point: work lists and the persistent NTT's
+// queue counters of vfhe_hip_ntt_batch_noalloc.  Host code never passes the
+// array's name to a kernel or a copy: it is a host-side stand-in there.  The
+// device address comes from vh_scratch_resolve, called once per device at the
+// engine's set-up with that device current.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[VH_SCRATCH_BYTES];
+
+hipError_t vh_scratch_resolve(void **ptr)
+{
+    return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+}
+
 __global__ void k_zero_u32(unsigned *p, unsigned n)
 {
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
