@@ -354,3 +354,56 @@ class RefProc:
         src[:] = x
         self.L.ntt_reverse(ptr(out), ptr(src), self.h)
         return out.copy()
+
+
+# ---------------------------------------------------------------------------
+# Slot operations on NTT-form rows (numpy restatements; rows are [limb][N], a row of
+# N = split * ps values holds `split` blocks of ps slots)
+# ---------------------------------------------------------------------------
+
+def slot_broadcast(rows: np.ndarray, mask: int, split: int, slot: int) -> np.ndarray:
+    """polynomial_RNS_broadcast_slot (modules/arith/c/src/rns_polynomial.c:1228-1245): every slot of
+    each block takes the value of slot `slot` of that block, on the active limbs."""
+    out = np.zeros_like(rows)
+    ps = rows.shape[1] // split
+    for i in range(rows.shape[0]):
+        if (mask >> i) & 1:
+            for j in range(split):
+                out[i, j * ps:(j + 1) * ps] = rows[i, j * ps + slot]
+    return out
+
+
+def slot_rotate(rows: np.ndarray, mask: int, split: int, rot: int) -> np.ndarray:
+    """polynomial_RNS_rotate_slot (rns_polynomial.c:1262-1284): out[k] = in[(k + rot) mod ps] in each block."""
+    out = np.zeros_like(rows)
+    ps = rows.shape[1] // split
+    for i in range(rows.shape[0]):
+        if (mask >> i) & 1:
+            for j in range(split):
+                out[i, j * ps:(j + 1) * ps] = np.roll(rows[i, j * ps:(j + 1) * ps], -rot)
+    return out
+
+
+def slot_copy(out_rows: np.ndarray, dst: int, rows: np.ndarray, mask: int, split: int, src: int) -> np.ndarray:
+    """polynomial_RNS_copy_slot (rns_polynomial.c:1286-1300): slot dst of each block of out takes slot src
+    of the same block of in, on in's active limbs; the rest of out is kept."""
+    out = out_rows.copy()
+    ps = rows.shape[1] // split
+    for i in range(rows.shape[0]):
+        if (mask >> i) & 1:
+            for j in range(split):
+                out[i, j * ps + dst] = rows[i, j * ps + src]
+    return out
+
+
+def slot_inverse(rows: np.ndarray, mask: int, primes) -> tuple[int, np.ndarray]:
+    """polynomial_RNS_inverse at split degree 1 (rns_polynomial.c:1383-1429): the slot-wise inverse mod
+    q_i on every active limb, or status -2 when a slot is zero (returns (status, rows))."""
+    out = np.zeros_like(rows)
+    for i, q in enumerate(primes):
+        if (mask >> i) & 1:
+            row = [int(v) for v in rows[i]]
+            if any(v == 0 for v in row):
+                return -2, out
+            out[i] = np.array([pow(v, -1, int(q)) for v in row], dtype=np.uint64)
+    return 0, out

@@ -805,3 +805,33 @@ def test_multiply_rescale_matches_reference_kernels(N, L, K):
         c.to_coeff(ref[comp], ref[comp], m)
     c.mlwe_round_division(ref, 1, m, top)
     assert not np.array_equal(got[0], ref[:, : L - 1])
+
+
+def test_slot_restatements_pinned():
+    """Slot operations of the oracle against what fixes them: rotation by 1 is the index shift
+    k -> k+1 within each block and rotating by r then ps - r is the identity; a broadcast equals
+    copying the slot into every position; the slot-wise inverse times the input is 1 mod q, and a
+    zero slot reports -2.  Inactive limbs stay zero."""
+    q = [97, 193]
+    split, ps = 2, 8
+    rows = np.zeros((3, split * ps), dtype=np.uint64)
+    rows[0] = np.arange(split * ps) % 97 + 1
+    rows[1] = (np.arange(split * ps) * 5 + 3) % 193 + 1
+    mask = 0b011
+    rot1 = O.slot_rotate(rows, mask, split, 1)
+    for j in range(split):
+        for k in range(ps):
+            assert rot1[0, j * ps + k] == rows[0, j * ps + (k + 1) % ps]
+    assert np.array_equal(O.slot_rotate(O.slot_rotate(rows, mask, split, 3), mask, split, ps - 3), rows)
+    b = O.slot_broadcast(rows, mask, split, 5)
+    c = np.zeros_like(rows)
+    for dst in range(ps):
+        c = O.slot_copy(c, dst, rows, mask, split, 5)
+    assert np.array_equal(b, c)
+    assert not b[2].any() and not rot1[2].any()
+    st, inv = O.slot_inverse(rows, mask, q + [1])
+    assert st == 0
+    for i in range(2):
+        assert all((int(a) * int(x)) % q[i] == 1 for a, x in zip(rows[i], inv[i]))
+    rows[1, 3] = 0
+    assert O.slot_inverse(rows, mask, q + [1])[0] == -2

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <tuple>
 #include <mutex>
@@ -252,9 +253,6 @@ static void *vh_stage(const void *src, size_t bytes)
 {
     vh_lane &L = LN();
     const size_t need = (bytes + 255) & ~(size_t)255, half = L.ring_size / 2;
-    static const bool dbg = getenv("VFHE_HIP_GRAPH_DEBUG") != nullptr;
-    if (dbg && g_cap != VH_CAP_OFF)
-        fprintf(stderr, "[vh graph] mode %d stage %zu bytes %zu\n", g_cap, g_cap == VH_CAP_LOG ? g_plan->stage_sz.size() : g_cap_stage, bytes);
     if (g_cap == VH_CAP_LOG) {
         g_plan->stage_sz.push_back(need);
     } else if (g_cap >= VH_CAP_RECORD) {
@@ -717,10 +715,72 @@ static int col_wide(unsigned items, int logn)
 }
 
 // Forward (dir 0) or inverse (dir 1) transforms of n-point rows.
+// Launches the transforms of `cnt` staged work items (device list d_items, all of one
+// arithmetic path: fp = FP64 limbs) of n-point rows, direction dir.
+static void ntt_dispatch(const vh_item *d_items, unsigned cnt, const vh_limb *d_limbs, int fp, uint64_t n, int dir)
+{
+    const int logn = ilog2(n);
+    if (fp && logn >= 13 && logn <= 16 && OPT.ntt_mode == 1 && C.xcd_ok) {
+        // Persistent kernel: both passes of every transform on one XCD, the
+        // intermediate exchanged through that XCD's L2.
+        unsigned *ctl = (unsigned *)vh_dmalloc((16 + 2 * cnt) * sizeof(unsigned));
+        VH_LAUNCH(vh_launch_zero_u32(S(), ctl, 16 + 2 * cnt));
+        const int occ = OPT.xcd_occ >= 4 ? 4 : 2;
+        {
+            vh_prof_scope ps_(VH_PROF_NTT_FP64);
+            VH_LAUNCH(vh_launch_ntt_xcd(S(), d_items, cnt, d_limbs, dir, logn, ctl, C.d_err,
+                                       (unsigned)OPT.xcd_lag, (unsigned)OPT.spin_limit,
+                                       (unsigned)(C.cus * occ), occ, OPT.xcd_l2 ? 1 : 0));
+        }
+        VH_LAUNCH(vh_launch_ntt_xcd_verify(S(), ctl, cnt, logn, C.d_err));
+        vh_dfree(ctl);
+    } else if (fp && logn >= 13 && logn <= 16) {
+        // Two register radix-16 passes (column pass of R1 = logn - 8 stages, row pass of 8),
+        // optionally over sub-batches of ntt_chunk_mb MiB of rows.
+        const int R1 = logn - 8;
+        const uint64_t chunk = OPT.ntt_chunk_mb > 0 ? (uint64_t)OPT.ntt_chunk_mb << 20 : UINT64_MAX;
+        const unsigned per = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(cnt, chunk / (n * 8)));
+        const bool pipe = OPT.ntt_mode == 2 && C.pipe_blocks > 0;
+        for (unsigned c0 = 0; c0 < cnt; c0 += per) {
+            const unsigned cc = std::min(per, cnt - c0);
+            const vh_item *ci = d_items + c0;
+            if (pipe) {
+                auto nb = [&](int bit) -> unsigned { return (OPT.ntt_tile_grid >> bit) & 1 ? UINT_MAX : C.pipe_blocks; };
+                if (dir == 0) {
+                    { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0, nb(0), col_wide(cc, logn))); }
+                    { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1, nb(1))); }
+                } else {
+                    { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0, nb(2))); }
+                    { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1, nb(3), col_wide(cc, logn))); }
+                }
+            } else if (dir == 0) {
+                { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0)); }
+                { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1)); }
+            } else {
+                { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16(S(), ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0)); }
+                { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1)); }
+            }
+        }
+    } else if (logn <= 12) {
+        { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, dir, logn, 0, logn, 0, 1, 1, 1)); }
+    } else {
+        const int R1 = logn - 8, R2 = 8;
+        const int Gc = std::min(1 << (logn - R1), 4096 >> R1);
+        const int Gr = std::min(1 << R1, 16);
+        if (dir == 0) {
+            { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 0, logn, 0, R1, 1, Gc, 1, 0)); }
+            { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 0, logn, R1, R2, 0, Gr, 0, 1)); }
+        } else {
+            { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 1, logn, R1, R2, 0, Gr, 1, 0)); }
+            { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 1, logn, 0, R1, 1, Gc, 0, 1)); }
+        }
+    }
+}
+
+// Forward (dir 0) or inverse (dir 1) transforms of n-point rows.
 static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const vh_limb *h_limbs, uint64_t n, int dir)
 {
     if (items.empty()) return;
-    const int logn = ilog2(n);
     std::vector<vh_item> grp[2];
     for (auto &it : items) grp[h_limbs[it.limb].fp64 ? 1 : 0].push_back(it);
     // Limb-major order: workgroups running together share one limb's twiddle table in L2.
@@ -731,62 +791,7 @@ static void run_ntt(std::vector<vh_item> &items, const vh_limb *d_limbs, const v
         for (size_t off = 0; off < v.size(); off += VH_MAX_GRID_Y) {
             const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, v.size() - off);
             std::vector<vh_item> chunk(v.begin() + off, v.begin() + off + cnt);
-            const vh_item *d_items = vh_stage_vec(chunk);
-            if (fp && logn >= 13 && logn <= 16 && OPT.ntt_mode == 1 && C.xcd_ok) {
-                // Persistent kernel: both passes of every transform on one XCD, the
-                // intermediate exchanged through that XCD's L2.
-                unsigned *ctl = (unsigned *)vh_dmalloc((16 + 2 * cnt) * sizeof(unsigned));
-                VH_LAUNCH(vh_launch_zero_u32(S(), ctl, 16 + 2 * cnt));
-                const int occ = OPT.xcd_occ >= 4 ? 4 : 2;
-                {
-                    vh_prof_scope ps_(VH_PROF_NTT_FP64);
-                    VH_LAUNCH(vh_launch_ntt_xcd(S(), d_items, cnt, d_limbs, dir, logn, ctl, C.d_err,
-                                               (unsigned)OPT.xcd_lag, (unsigned)OPT.spin_limit,
-                                               (unsigned)(C.cus * occ), occ, OPT.xcd_l2 ? 1 : 0));
-                }
-                VH_LAUNCH(vh_launch_ntt_xcd_verify(S(), ctl, cnt, logn, C.d_err));
-                vh_dfree(ctl);
-            } else if (fp && logn >= 13 && logn <= 16) {
-                // Two register radix-16 passes (column pass of R1 = logn - 8 stages, row pass of 8),
-                // optionally over sub-batches of ntt_chunk_mb MiB of rows.
-                const int R1 = logn - 8;
-                const uint64_t chunk = OPT.ntt_chunk_mb > 0 ? (uint64_t)OPT.ntt_chunk_mb << 20 : UINT64_MAX;
-                const unsigned per = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(cnt, chunk / (n * 8)));
-                const bool pipe = OPT.ntt_mode == 2 && C.pipe_blocks > 0;
-                for (unsigned c0 = 0; c0 < cnt; c0 += per) {
-                    const unsigned cc = std::min(per, cnt - c0);
-                    const vh_item *ci = d_items + c0;
-                    if (pipe) {
-                        auto nb = [&](int bit) -> unsigned { return (OPT.ntt_tile_grid >> bit) & 1 ? UINT_MAX : C.pipe_blocks; };
-                        if (dir == 0) {
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0, nb(0), col_wide(cc, logn))); }
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16p(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1, nb(1))); }
-                        } else {
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0, nb(2))); }
-                            { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16p(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1, nb(3), col_wide(cc, logn))); }
-                        }
-                    } else if (dir == 0) {
-                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, 0, R1, 1, 1, 0)); }
-                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16(S(), ci, cc, d_limbs, 0, logn, R1, 8, 0, 0, 1)); }
-                    } else {
-                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16(S(), ci, cc, d_limbs, 1, logn, R1, 8, 0, 1, 0)); }
-                        { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1)); }
-                    }
-                }
-            } else if (logn <= 12) {
-                { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, dir, logn, 0, logn, 0, 1, 1, 1)); }
-            } else {
-                const int R1 = logn - 8, R2 = 8;
-                const int Gc = std::min(1 << (logn - R1), 4096 >> R1);
-                const int Gr = std::min(1 << R1, 16);
-                if (dir == 0) {
-                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 0, logn, 0, R1, 1, Gc, 1, 0)); }
-                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 0, logn, R1, R2, 0, Gr, 0, 1)); }
-                } else {
-                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 1, logn, R1, R2, 0, Gr, 1, 0)); }
-                    { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, 1, logn, 0, R1, 1, Gc, 0, 1)); }
-                }
-            }
+            ntt_dispatch(vh_stage_vec(chunk), cnt, d_limbs, fp, n, dir);
         }
     }
 }
@@ -2561,13 +2566,6 @@ static void multiply_batch(const std::vector<RNS_MLWE> &outs, const std::vector<
                            const std::vector<uint64_t> &divide_masks)
 {
     const size_t B = outs.size();
-    static const bool dbg = getenv("VFHE_HIP_GRAPH_DEBUG") != nullptr;
-    if (dbg && g_cap != VH_CAP_OFF)
-        for (size_t bi = 0; bi < B; bi++)
-            fprintf(stderr, "[vh graph] mode %d job %zu r %lu masks a1 %lx b1 %lx a2 %lx b2 %lx out %lx\n", g_cap, bi,
-                    (unsigned long)in1[bi]->r, (unsigned long)in1[bi]->a[0]->rns_mask, (unsigned long)in1[bi]->b->rns_mask,
-                    (unsigned long)in2[bi]->a[0]->rns_mask, (unsigned long)in2[bi]->b->rns_mask,
-                    (unsigned long)outs[bi]->b->rns_mask);
     VH_REQUIRE(B > 0 && in1.size() == B && in2.size() == B && ksks.size() == B && divide_masks.size() == B);
     const uint64_t N = in1[0]->b->ntt->N, r = in1[0]->r, R = mlwe_extended_rank(r);
     incNTT t = in1[0]->b->ntt;
@@ -2965,68 +2963,175 @@ extern "C" void vfhe_hip_mgsw_cmux_batch(RNS_MLWE *out, RNSc_MLWE *in1, RNSc_MLW
 }
 
 // Blind rotation of a batch (cggi16.py:46-90 for every accumulator at once).  acc[k]
-// enters as X^-rot_b * tv in coefficient form; for every bootstrapping-key index i,
-// each accumulator whose exponent a = steps[k*n + i] is non-zero becomes
-// CMUX(acc, X^a * acc, bk[i]) (mgsw.c:51-71, NTT form) and is brought back to
-// coefficient form for the next index, the reference's sequence of calls.  One
-// monomial-product launch per index covers every accumulator (exponents per row).
-// The results are left in acc (coefficient form).
+// enters as X^-rot_b * tv in coefficient form; for every bootstrapping-key index i the
+// reference sets acc = CMUX(acc, X^a * acc, bk[i]) (mgsw.c:51-71: ExtProd(bk[i],
+// X^a acc - acc) + NTT(acc), NTT form) and brings it back to coefficient form for the
+// next index.  Per index this runs, over every accumulator at once:
+//   diff = (X^a - 1) * acc                      one monomial-product launch (per-row a)
+//   lift + NTT of diff's digits                 one NTT launch (the lift folded in on FP64 limbs)
+//   out = sum_d NTT(digit_d) * bk[i][d]         one inner-product launch (the external product)
+//   INTT of out, round division by the special limbs (mgsw.c:6-49)
+//   acc = acc + out                             (INTT(NTT(div) + NTT(acc)) = div + acc exactly)
+// An accumulator with a = 0 gets diff = 0, hence out = 0 and acc unchanged, which is the
+// reference's skip.  Temporaries are allocated once per call; the work lists that do not
+// depend on i are built once.  The results are left in acc (coefficient form).
 extern "C" void vfhe_hip_blind_rotate_batch(RNS_MLWE *acc, uint64_t count, RNS_MLWE **bk, uint64_t n,
                                             const uint64_t *steps, uint64_t ell, uint64_t special_primes)
 {
     VH_ENTER;
     if (count == 0 || n == 0) return;
-    VH_REQUIRE(acc && bk && steps);
+    VH_REQUIRE(acc && bk && steps && bk[0] && bk[0][0]);
     incNTT t = acc[0]->b->ntt;
-    const uint64_t N = t->N, r = acc[0]->r;
-    VH_REQUIRE(t->split_degree == 1);
+    const uint64_t N = t->N, r = acc[0]->r, Cc = r + 1, l = t->l;
+    VH_REQUIRE(t->split_degree == 1 && Cc <= VH_MAX_COMPS && count <= VH_MAX_GRID_Y);
+    const uint64_t m = acc[0]->b->rns_mask, ext = bk[0][0]->b->rns_mask;
     for (uint64_t k = 0; k < count; k++) {
         VH_REQUIRE(acc[k]->b->ntt == t && acc[k]->r == r);
-        for (auto p : comps(acc[k])) dev_ready(p);
+        for (auto p : comps(acc[k])) { VH_REQUIRE(p->rns_mask == m); dev_ready(p); }
     }
-    std::vector<RNS_MLWE> rot(count), out(count);
+    // digits: the level limbs of every component; extended limbs: the key's; the
+    // division removes the top `special_primes` extended limbs (external_product_batch)
+    std::vector<int> J, E, divs, keep;
+    for (uint64_t i = 0; i < l; i++) { if (ACTIVE(m, i)) J.push_back((int)i); if (ACTIVE(ext, i)) E.push_back((int)i); }
+    uint64_t dm = 0, cnt = 0;
+    for (int idx = 63; idx >= 0 && cnt < special_primes; idx--)
+        if (ACTIVE(ext, (uint64_t)idx)) { dm |= 1ULL << idx; cnt++; }
+    for (int e : E) (ACTIVE(dm, (uint64_t)e) ? divs : keep).push_back(e);
+    VH_REQUIRE((m & ~ext) == 0 && (ext & ~dm) == m && J.size() <= ell && divs.size() <= VH_MAX_DIV &&
+               E.size() <= VH_MAX_LIMBS);
+    vh_pool *P = VPOOL(t);
+    const uint64_t nd = Cc * J.size(), next = E.size();
+    bool fp = true;   // FP64 limbs only: the lift folds into the NTT and the inner product runs in FP64
+    for (int e : E) fp = fp && P->h_limbs[e].fp64;
+    for (int j : J) fp = fp && P->h_limbs[j].fp64;
+    // temporaries
+    std::vector<RNS_MLWE> diff(count), out(count);
     for (uint64_t k = 0; k < count; k++) {
-        rot[k] = alloc_sample_raw(N, r, acc[k]->b->rns_mask, t);
-        out[k] = alloc_sample_raw(N, r, acc[k]->b->rns_mask, t);
+        diff[k] = alloc_sample_raw(N, r, m, t);
+        out[k] = alloc_sample_raw(N, r, ext, t);
     }
+    uint64_t *modup = (uint64_t *)vh_dmalloc(count * nd * next * N * 8);
+    // The work lists do not depend on the key index: built and copied to the device once.
+    std::vector<vh_item> rot, lift, nt, inv, zero, adds;
+    std::vector<vh_mac_job> jobs(count);
+    std::vector<vh_div_job> dj;
+    for (uint64_t k = 0; k < count; k++) {
+        const auto dc = comps(diff[k]), oc = comps(out[k]), ac = comps(acc[k]);
+        for (uint64_t c = 0; c < Cc; c++)
+            for (size_t jj = 0; jj < J.size(); jj++) {
+                // diff = (X^a - 1) * acc, a = the accumulator's entry of the exponent table
+                rot.push_back(mk_item(drow(ac[c], J[jj]), drow(dc[c], J[jj]), (uint32_t)J[jj], nullptr, k, 0, 1u));
+                const uint64_t d = c * J.size() + jj;
+                for (uint64_t ti = 0; ti < next; ti++) {
+                    uint64_t *dst = modup + ((k * nd + d) * next + ti) * N;
+                    lift.push_back(mk_item(drow(dc[c], J[jj]), dst, (uint32_t)E[ti]));
+                    nt.push_back(mk_item(fp ? drow(dc[c], J[jj]) : dst, dst, (uint32_t)E[ti]));
+                }
+            }
+        vh_mac_job &jb = jobs[k];
+        memset(&jb, 0, sizeof(jb));
+        for (uint64_t c = 0; c < Cc; c++) jb.out[c] = VP(oc[c])->d;
+        jb.modup = modup + k * nd * next * N;
+        jb.kptr_off = 0;   // every accumulator reads the same key rows
+        jb.comps = (int)Cc;
+        jb.add = 1;
+        jb.digits = (int)nd;
+        jb.next = (int)next;
+        for (uint64_t ti = 0; ti < next; ti++) jb.limb[ti] = E[ti];
+        for (uint64_t c = 0; c < Cc; c++) {
+            for (int e : E) inv.push_back(mk_item(drow(oc[c], e), drow(oc[c], e), (uint32_t)e));
+            for (int d : divs) zero.push_back(mk_item(nullptr, drow(oc[c], d), (uint32_t)d));
+            for (int j : J) adds.push_back(mk_item(drow(ac[c], j), drow(ac[c], j), (uint32_t)j, drow(oc[c], j)));
+            vh_div_job x;
+            memset(&x, 0, sizeof(x));
+            x.base = VP(oc[c])->d;
+            x.stride = N;
+            x.K = (int)divs.size();
+            x.nl = (int)l;
+            for (size_t q = 0; q < divs.size(); q++) x.div_limb[q] = divs[q];
+            for (size_t q = 0; q < keep.size(); q++) x.out_limb[q] = keep[q];
+            x.nout = (int)keep.size();
+            dj.push_back(x);
+        }
+    }
+    auto by_limb = [](const vh_item &a, const vh_item &b) { return a.limb < b.limb; };
+    std::stable_sort(nt.begin(), nt.end(), by_limb);    // workgroups in flight share a limb's twiddles
+    std::stable_sort(inv.begin(), inv.end(), by_limb);
+    std::vector<uint64_t> amounts(count * n);
+    for (uint64_t k = 0; k < count * n; k++) amounts[k] = steps[k] & ((N << 1) - 1);
+    std::vector<void *> held;
+    auto persist = [&](const void *src, size_t bytes) -> void * {
+        void *d = vh_dmalloc(bytes ? bytes : 8);
+        if (bytes) VH_CHECK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, S()));
+        held.push_back(d);
+        return d;
+    };
+    const vh_item *d_rot = (const vh_item *)persist(rot.data(), rot.size() * sizeof(vh_item));
+    const vh_item *d_lift = fp ? nullptr : (const vh_item *)persist(lift.data(), lift.size() * sizeof(vh_item));
+    const vh_item *d_nt = (const vh_item *)persist(nt.data(), nt.size() * sizeof(vh_item));
+    const vh_item *d_inv = (const vh_item *)persist(inv.data(), inv.size() * sizeof(vh_item));
+    const vh_item *d_zero = (const vh_item *)persist(zero.data(), zero.size() * sizeof(vh_item));
+    const vh_item *d_adds = (const vh_item *)persist(adds.data(), adds.size() * sizeof(vh_item));
+    const vh_mac_job *d_jobs = (const vh_mac_job *)persist(jobs.data(), jobs.size() * sizeof(vh_mac_job));
+    const vh_div_job *d_dj = (const vh_div_job *)persist(dj.data(), dj.size() * sizeof(vh_div_job));
+    const uint64_t *d_amounts = (const uint64_t *)persist(amounts.data(), amounts.size() * 8);
+    VH_CHECK(hipStreamSynchronize(S()));   // the host lists may go once copied
+    const void *d_consts = div_consts(t, ext, dm, divs, true);
+    auto chunks = [&](size_t total, const std::function<void(size_t, unsigned)> &f) {
+        for (size_t off = 0; off < total; off += VH_MAX_GRID_Y) f(off, (unsigned)std::min<size_t>(VH_MAX_GRID_Y, total - off));
+    };
+    std::vector<const uint64_t *> kptr(next * nd * Cc);
     for (uint64_t i = 0; i < n; i++) {
         VH_REQUIRE(bk[i]);
-        std::vector<RNS_MLWE> a_sel, r_sel, o_sel;
-        std::vector<vh_item> items;
-        for (uint64_t k = 0; k < count; k++) {
-            const uint64_t a = steps[k * n + i] & ((N << 1) - 1);
-            if (a == 0) continue;
-            const auto x = comps(acc[k]), y = comps(rot[k]);
-            for (size_t c = 0; c < x.size(); c++) {
-                y[c]->rns_mask = x[c]->rns_mask;
-                for_active(x[c]->rns_mask, t->l, [&](uint64_t j) {
-                    items.push_back(mk_item(drow(x[c], j), drow(y[c], j), (uint32_t)j, nullptr, a));
-                });
-            }
-            a_sel.push_back(acc[k]); r_sel.push_back(rot[k]); o_sel.push_back(out[k]);
-        }
-        if (a_sel.empty()) continue;
-        for (size_t off = 0; off < items.size(); off += VH_MAX_GRID_Y) {
-            const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, items.size() - off);
-            std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cnt);
+        // 1. diff = (X^a - 1) * acc (a = 0 gives 0: the reference's skipped index)
+        chunks(rot.size(), [&](size_t off, unsigned cn) {
             vh_prof_scope ps_(VH_PROF_OTHER);
-            VH_LAUNCH(vh_launch_mul_xai(S(), vh_stage_vec(chunk), cnt, DL(acc[0]->b), N));
-        }
-        for (auto o : o_sel) for (auto p : comps(o)) p->rns_mask = acc[0]->b->rns_mask;
-        cmux_batch(o_sel, a_sel, r_sel, std::vector<RNS_MLWE *>(a_sel.size(), bk[i]), ell, special_primes);
-        std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> back;
-        for (auto o : o_sel) for (auto p : comps(o)) back.push_back({p, p});
-        ntt_rows(back, 1);
-        // the CMUX output becomes the accumulator: exchange the device rows
-        for (size_t s = 0; s < a_sel.size(); s++) {
-            const auto x = comps(a_sel[s]), o = comps(o_sel[s]);
-            for (size_t c = 0; c < x.size(); c++) {
-                std::swap(VP(x[c])->d, VP(o[c])->d);
-                x[c]->rns_mask = o[c]->rns_mask;
+            VH_LAUNCH(vh_launch_mul_xai(S(), d_rot + off, cn, P->d_limbs, N, d_amounts, n, i));
+        });
+        // 2. digits lifted to the extended limbs, forward NTT
+        if (!fp) chunks(lift.size(), [&](size_t off, unsigned cn) {
+            vh_prof_scope ps_(VH_PROF_ELTWISE);
+            VH_LAUNCH(vh_launch_eltwise(S(), d_lift + off, cn, P->d_limbs, VH_OP_REDUCE, N));
+        });
+        chunks(nt.size(), [&](size_t off, unsigned cn) { ntt_dispatch(d_nt + off, cn, P->d_limbs, fp, N, 0); });
+        // 3. external product with bk[i]: row (ti, d, c) of the key table
+        for (uint64_t d = 0; d < nd; d++) {
+            const uint64_t src = d / J.size(), jj = d % J.size();
+            RNS_MLWE e = bk[i][src * ell + jj];
+            VH_REQUIRE(e && e->r == r && e->b->ntt == t);
+            for (uint64_t c = 0; c < Cc; c++) {
+                RNS_Polynomial kp = c < r ? e->a[c] : e->b;
+                dev_ready(kp);
+                for (uint64_t ti = 0; ti < next; ti++) kptr[(ti * nd + d) * Cc + c] = drow(kp, E[ti]);
             }
         }
+        const uint64_t *const *d_kptr = vh_stage_vec(kptr);
+        chunks(jobs.size(), [&](size_t off, unsigned cn) {
+            vh_prof_scope ps_(VH_PROF_KSMAC);
+            VH_LAUNCH(vh_launch_keyswitch_mac(S(), d_jobs + off, cn, (unsigned)next, P->d_limbs, d_kptr, N, fp ? 1 : 0));
+        });
+        // 4. back to coefficient form, round-divided by the special limbs (their rows cleared)
+        chunks(inv.size(), [&](size_t off, unsigned cn) { ntt_dispatch(d_inv + off, cn, P->d_limbs, fp, N, 1); });
+        if (!divs.empty()) {
+            chunks(dj.size(), [&](size_t off, unsigned cn) {
+                vh_prof_scope ps_(VH_PROF_DIVISION);
+                VH_LAUNCH(vh_launch_division(S(), d_dj + off, cn, (unsigned)keep.size(), P->d_limbs, d_consts, N,
+                                             (int)divs.size()));
+            });
+            chunks(zero.size(), [&](size_t off, unsigned cn) {
+                vh_prof_scope ps_(VH_PROF_ELTWISE);
+                VH_LAUNCH(vh_launch_eltwise(S(), d_zero + off, cn, P->d_limbs, VH_OP_ZERO, N));
+            });
+        }
+        // 5. acc += out
+        chunks(adds.size(), [&](size_t off, unsigned cn) {
+            vh_prof_scope ps_(VH_PROF_ELTWISE);
+            VH_LAUNCH(vh_launch_eltwise(S(), d_adds + off, cn, P->d_limbs, VH_OP_ADD, N));
+        });
     }
-    for (uint64_t k = 0; k < count; k++) { free_mlwe_RNS_sample(rot[k]); free_mlwe_RNS_sample(out[k]); }
+    for (void *d : held) vh_dfree(d);
+    vh_dfree(modup);
+    for (uint64_t k = 0; k < count; k++) { free_mlwe_RNS_sample(diff[k]); free_mlwe_RNS_sample(out[k]); }
 }
 
 // ---------------------------------------------------------------------------

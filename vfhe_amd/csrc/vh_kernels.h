@@ -162,8 +162,10 @@ hipError_t vh_launch_fft_bitrev(hipStream_t st, double *const *rows, double *con
                                 uint32_t prec);
 hipError_t vh_launch_fft_scale(hipStream_t st, double *const *rows, unsigned nrows, double scale, uint64_t count);
 hipError_t vh_launch_fft_round(hipStream_t st, double *const *rows, uint64_t *const *outs, unsigned nrows, uint64_t n);
-// out = X^a * in (item.aux = 0) or (X^a - 1) * in (aux = 1) mod X^n + 1, a = item.scalar < 2n.
-hipError_t vh_launch_mul_xai(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, uint64_t n);
+// out = X^a * in (item.aux = 0) or (X^a - 1) * in (aux = 1) mod X^n + 1, a < 2n: a = item.scalar,
+// or with a table, a = amounts[item.scalar * stride + col].
+hipError_t vh_launch_mul_xai(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, uint64_t n,
+                             const uint64_t *amounts = nullptr, uint64_t stride = 0, uint64_t col = 0);
 hipError_t vh_launch_scatter_split(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                                    uint64_t N, int signed_in);
 

@@ -1940,10 +1940,12 @@ hipError_t vh_launch_sample_gaussian(hipStream_t st, const vh_gauss_job *jobs, u
 // -in[u - n].  One thread per output coefficient; out != in.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_mul_xai(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, uint64_t n)
+k_mul_xai(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, uint64_t n,
+          const uint64_t *__restrict__ amounts, uint64_t stride, uint64_t col)
 {
     const vh_item it = items[blockIdx.y];
-    const uint64_t a = it.scalar;   // the monomial's exponent, per row (< 2n)
+    // the monomial's exponent (< 2n): the row's own, or entry (row's scalar, col) of a table
+    const uint64_t a = amounts ? amounts[it.scalar * stride + col] : it.scalar;
     const vh_limb L = limbs[it.limb];
     const uint64_t q = L.q, mu = L.mu64, n2 = 2 * n;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -1955,9 +1957,10 @@ k_mul_xai(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs, 
     }
 }
 
-hipError_t vh_launch_mul_xai(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, uint64_t n)
+hipError_t vh_launch_mul_xai(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, uint64_t n,
+                             const uint64_t *amounts, uint64_t stride, uint64_t col)
 {
-    VH_KL(k_mul_xai, dim3(vh_blocks(n, 256, 64), nitems), dim3(256), 0, st, items, limbs, n);
+    VH_KL(k_mul_xai, dim3(vh_blocks(n, 256, 64), nitems), dim3(256), 0, st, items, limbs, n, amounts, stride, col);
     return hipGetLastError();
 }
 

@@ -134,11 +134,14 @@ class Server:
         return rid
 
     def poll(self, max_n: int = 256, wait: bool = True):
+        """Completed requests: (id, output, latency s, (queued s, enqueued s, device + completion s))."""
         ids = (ctypes.c_uint64 * max_n)()
         outs = (ctypes.c_void_p * max_n)()
         lat = (ctypes.c_double * max_n)()
-        n = self.lib.vfhe_hip_server_poll(self.h, ids, outs, lat, max_n, int(wait))
-        return [(int(ids[k]), outs[k], float(lat[k])) for k in range(n)]
+        parts = (ctypes.c_double * (3 * max_n))()
+        n = self.lib.vfhe_hip_server_poll_timed(self.h, ids, outs, lat, parts, max_n, int(wait))
+        return [(int(ids[k]), outs[k], float(lat[k]), (parts[3 * k], parts[3 * k + 1], parts[3 * k + 2]))
+                for k in range(n)]
 
     def free_outputs(self, samples):
         """Frees a list of output samples with one engine call."""
@@ -146,16 +149,23 @@ class Server:
         self.lib.vfhe_hip_free_mlwe_samples(arr, len(samples))
 
     def stats(self) -> dict:
-        s = (ctypes.c_double * 9)()
+        s = (ctypes.c_double * 10)()
         self.lib.vfhe_hip_server_stats(self.h, s)
         return {"batches": int(s[0]), "requests": int(s[1]), "mean_batch": round(s[2], 2),
                 "peak_inflight_bytes": int(s[3]), "max_lanes_busy": int(s[4]), "pending": int(s[5]),
-                "host_enqueue_s": round(s[6], 4), "device_batch_s": round(s[7], 4), "server_s": round(s[8], 4)}
+                "host_enqueue_s": round(s[6], 4), "device_batch_s": round(s[7], 4), "server_s": round(s[8], 4),
+                "max_batch_enqueue_ms": round(s[9] * 1e3, 3)}
 
     def close(self):
         if self.h:
             self.lib.vfhe_hip_server_free(self.h)
             self.h = None
+
+
+def cache_counters(lib) -> dict:
+    """The engine's device block cache counters (hits, misses, out-of-memory flushes, parked bytes)."""
+    return {k: int(lib.vfhe_hip_get_option(f"stat.cache_{k}".encode())) for k in ("hits", "misses", "flushes",
+                                                                                   "parked_bytes")}
 
 
 def run(lib, rings: dict, workload: list[Request], memory_budget: int, lanes: int = 4, max_batch: int = 32,
@@ -172,6 +182,8 @@ def run(lib, rings: dict, workload: list[Request], memory_budget: int, lanes: in
     n = len(workload)
     lat = [None] * n
     svc = [None] * n
+    parts = [None] * n
+    cache0 = cache_counters(lib)
     late = [0.0] * n
     outs = [None] * n
     index = {}
@@ -188,12 +200,13 @@ def run(lib, rings: dict, workload: list[Request], memory_budget: int, lanes: in
                 time.sleep(0.0002)
                 continue
             frees = []
-            for rid, out, l in batch:
+            for rid, out, l, pt in batch:
                 k = index.pop(rid, None)
                 while k is None:          # submit() returns after the runtime may already have finished it
                     time.sleep(0)
                     k = index.pop(rid, None)
                 svc[k] = l
+                parts[k] = pt
                 lat[k] = l + late[k]
                 if keep_outputs:
                     outs[k] = out
@@ -234,6 +247,20 @@ def run(lib, rings: dict, workload: list[Request], memory_budget: int, lanes: in
     if sv:
         res["service_p99_ms"] = round(float(np.percentile(np.asarray(sv) * 1e3, 99)), 3)
         res["max_lateness_ms"] = round(max(late) * 1e3, 3)
+        res["client_late_p99_ms"] = round(float(np.percentile(np.asarray(late) * 1e3, 99)), 3)
+    pt = np.array([p for p in parts if p is not None]) * 1e3
+    if len(pt):
+        # where the tail comes from: each phase's p99, and the phases of the slowest request
+        worst = int(np.argmax([(s or 0.0) + lt for s, lt in zip(svc, late)]))
+        res["phase_p99_ms"] = {name: round(float(np.percentile(pt[:, i], 99)), 3)
+                               for i, name in enumerate(("queued", "enqueue", "device"))}
+        if parts[worst] is not None:
+            res["slowest_request_ms"] = {"client_late": round(late[worst] * 1e3, 3),
+                                         **{name: round(parts[worst][i] * 1e3, 3)
+                                            for i, name in enumerate(("queued", "enqueue", "device"))}}
+    cache1 = cache_counters(lib)
+    res["block_cache"] = {k: cache1[k] - cache0[k] for k in ("hits", "misses", "flushes")}
+    res["block_cache"]["parked_bytes"] = cache1["parked_bytes"]
     res.update(stats)
     if keep_outputs:
         res["outputs"] = outs
