@@ -65,6 +65,7 @@ def parse():
     ap.add_argument("--only-cggi", action="store_true", help="run the bootstrap leg alone (profiling)")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=1.5, help="wall time per offered load")
+    ap.add_argument("--serving-lanes", type=int, default=4, help="HIP streams the serving runtime uses (1..7)")
     ap.add_argument("--only-ntt", action="store_true",
                     help="run the batched NTT round trip alone (the command profiled with rocprofv3)")
     ap.add_argument("--only-ckks", action="store_true", help="run the CKKS leg alone (profiling)")
@@ -546,7 +547,7 @@ def bench_serving(lib, args):
     t0 = time.perf_counter()
     rings = {n: serving.Ring(lib, n, L=24, K=4, rot_steps=4, levels=(4, 24)) for n in Ns}
     log(f"serving: setup {time.perf_counter() - t0:.1f} s")
-    budget, lanes, max_batch = 32 << 30, 4, 32
+    budget, lanes, max_batch = 32 << 30, args.serving_lanes, 32
     burst = serving.make_workload(40000.0, 0.2, Ns, seed=1)
     sat = serving.run(lib, rings, burst, budget, lanes, max_batch)
     cap = sat["requests_per_s"]

@@ -462,6 +462,14 @@ void vfhe_hip_mgsw_cmux_batch(RNS_MLWE *out, RNSc_MLWE *in1, RNSc_MLWE *in2, RNS
    NTT-form samples of one MGSW encryption), then back to coefficient form.  Results in acc. */
 void vfhe_hip_blind_rotate_batch(RNS_MLWE *acc, uint64_t count, RNS_MLWE **bk, uint64_t n, const uint64_t *steps,
                                  uint64_t ell, uint64_t special_primes);
+/* The rotation amounts of a CGGI16 blind rotation for `count` LWE samples, replacing the
+   per-sample CRT and rounding of cggi16.py:54-86 (host arithmetic): rot_tv[k] = 2N - rot_b
+   mod 2N, steps[k*n + i] = round(a_i * 2N / q) mod 2N, q the product of the sample's primes.
+   The reference rounds a double quotient; an entry whose exact quotient lies within that
+   double's rounding error of a half is set to UINT64_MAX for the caller to recompute.
+   Returns the number of such entries, or -1 when q * 2N does not fit 127 bits. */
+int64_t vfhe_hip_cggi_rotations(LWE *in, uint64_t count, uint64_t N, uint64_t torus_base, uint64_t *rot_tv,
+                                uint64_t *steps);
 /* mlwe_automorphism_RNSc_GHS (mlwe.c:314-326) for `count` inputs sharing gen and ksk. */
 void vfhe_hip_mlwe_automorphism_batch(RNS_MLWE *out, RNS_MLWE *in, uint64_t count, uint64_t gen, RNS_MLWE **ksk);
 /* The same two batches with a Galois element / key (and rescale mask) per job:

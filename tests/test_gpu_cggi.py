@@ -11,6 +11,7 @@ are pinned against the oracle in test_gpu_mgsw.py.  The last test runs the
 benchmark's bootstrap shape: LWE dimension 630, N = 1024, 1024 ciphertexts of
 4-bit messages, identity table, every output decrypted exactly.
 """
+import ctypes
 import random
 
 import numpy as np
@@ -19,7 +20,8 @@ import pytest
 from vfhe_amd import _native, arith
 from vfhe_amd.arith import Ring
 from vfhe_amd.cggi import BootstrapKey, Bootstrapper, LweBatch, LweKey, extract, lwe_key_of, pack_lut, rotation_amounts
-from vfhe_amd.fhe import Batch, Chain, SecretKey
+from vfhe_amd.cggi import _LweStruct as LweStruct
+from vfhe_amd.fhe import Batch, Chain, SecretKey, _handles
 
 pytestmark = pytest.mark.gpu
 
@@ -121,6 +123,36 @@ def test_blind_rotate_batch_matches_reference_sequence(setup):
         ref = _reference_sequence(s, lwes, k, tb)
         for p, r in zip(got.polys(k), ref.polys(0)):
             assert p == r, k
+
+
+def test_rotation_amounts_engine_helper(setup):
+    """vfhe_hip_cggi_rotations (cggi16.py:54-86 in 128-bit integers) equals the exact-rational
+    restatement on random samples, and marks coefficients whose quotient sits next to a
+    rounding half (set here by hand: a_0 = floor((2k+1) q / 4N)), which Bootstrapper.amounts then
+    takes through the restatement."""
+    s = setup
+    tb = 1 << (MSG_PREC - 1)
+    lwes = encrypt_msgs(s, [0, 1, 2, 7, 8, 15, 3, 11])
+    primes = lwes.ring.primes[: lwes.l]
+    N, q = s["chain"].N, s["in_ring"].q_l
+    st = ctypes.cast(lwes.handles[0], ctypes.POINTER(LweStruct)).contents
+    for k, i in ((3, 0), (200, 5)):
+        v = ((2 * k + 1) * q) // (4 * N)
+        for j, p in enumerate(primes):
+            st.a[j][i] = v % p
+    lib = _native.load()
+    n = lwes.n
+    rots = np.zeros(len(lwes), dtype=np.uint64)
+    steps = np.zeros(len(lwes) * n, dtype=np.uint64)
+    marked = lib.vfhe_hip_cggi_rotations(_handles(lwes.handles), len(lwes), N, tb,
+                                         rots.ctypes.data_as(_native.U64P), steps.ctypes.data_as(_native.U64P))
+    assert marked == 2
+    assert steps[0] == steps[5] == np.uint64(2**64 - 1)
+    rots, steps = s["boot"].amounts(lwes, tb)
+    for k in range(len(lwes)):
+        rot, ref = rotation_amounts(*lwes.arrays(k), primes, N, tb)
+        assert int(rots[k]) == rot, k
+        assert [int(v) for v in steps[k * n:(k + 1) * n]] == ref, k
 
 
 def test_bootstrap_batch_shape_n630_N1024():

@@ -5,7 +5,7 @@ The reference bootstraps one LWE ciphertext at a time
 test vector, then one CMUX per LWE coefficient, then sample extraction.  Here
 a whole batch of LWE ciphertexts moves through the bootstrapping key together:
 the rotation amounts of every input are computed on the host
-(`rotation_amounts`), the accumulators live in HBM, and
+(`Bootstrapper.amounts`, exact 128-bit arithmetic in the engine), the accumulators live in HBM, and
 `vfhe_hip_blind_rotate_batch` steps all of them through key index i with one
 monomial-product launch and one batched CMUX (mgsw.c:51-71) per index.  The
 result of each accumulator is the reference's, bit for bit, since every
@@ -33,6 +33,7 @@ from .fhe import Batch, Chain, SecretKey, _handles
 
 U64 = _native.U64
 U64P = _native.U64P
+_MARK = np.uint64(0xFFFFFFFFFFFFFFFF)
 
 
 def lib():
@@ -216,16 +217,34 @@ class Bootstrapper:
         form on the level-0 ring."""
         chain, N = self.chain, self.chain.N
         assert lwes.n == self.bk.n
-        primes = lwes.ring.primes[: lwes.l]
-        amounts = [rotation_amounts(*lwes.arrays(k), primes, N, torus_base) for k in range(len(lwes))]
+        rots, steps = self.amounts(lwes, torus_base)
         tv.to_coeff()
         acc = Batch.alloc(chain, 0, len(lwes), form="coeff")
-        for h, (rot, _) in zip(acc.handles, amounts):
-            lib().mlwe_RNSc_mul_by_xai(h, tv.handles[0], rot)
-        steps = np.array([s for _, st in amounts for s in st], dtype=np.uint64)
+        for h, rot in zip(acc.handles, rots):
+            lib().mlwe_RNSc_mul_by_xai(h, tv.handles[0], int(rot))
         lib().vfhe_hip_blind_rotate_batch(_handles(acc.handles), len(acc), self.bk.table, self.bk.n,
                                           steps.ctypes.data_as(U64P), self.bk.ell, chain.special)
         return acc
+
+    def amounts(self, lwes: LweBatch, torus_base: int):
+        """(rot_tv per input, steps as count x n) of cggi16.py:54-86, computed by the engine's
+        exact 128-bit helper; the rare entries it marks as too close to a rounding half for
+        its bound, or a modulus too wide for it, go through `rotation_amounts`."""
+        n, N = lwes.n, self.chain.N
+        rots = np.zeros(len(lwes), dtype=np.uint64)
+        steps = np.zeros(len(lwes) * n, dtype=np.uint64)
+        marked = lib().vfhe_hip_cggi_rotations(_handles(lwes.handles), len(lwes), N, torus_base,
+                                               rots.ctypes.data_as(U64P), steps.ctypes.data_as(U64P))
+        if marked:
+            primes = lwes.ring.primes[: lwes.l]
+            st2 = steps.reshape(len(lwes), n)
+            todo = range(len(lwes)) if marked < 0 else [
+                k for k in range(len(lwes)) if rots[k] == _MARK or (st2[k] == _MARK).any()]
+            for k in todo:
+                rot, st = rotation_amounts(*lwes.arrays(k), primes, N, torus_base)
+                rots[k] = rot
+                st2[k] = st
+        return rots, steps
 
     def bootstrap(self, tv: Batch, lwes: LweBatch, torus_base: int) -> LweBatch:
         """Blind rotation, then extraction of coefficient 0 (cggi16.py:118-140)."""

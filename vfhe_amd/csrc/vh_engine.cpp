@@ -4134,6 +4134,97 @@ extern "C" void lwe_subto(LWE out, LWE in)
 }
 
 // ---------------------------------------------------------------------------
+// Rotation amounts of a CGGI16 blind rotation (cggi16.py:54-86) for a batch of LWE
+// samples: with q the product of the sample's primes and b, a_i recomposed by CRT,
+//   rot_tv = (2N - round((b + q // (4 torus_base)) * 2N / q) mod 2N) mod 2N,
+//   steps[i] = round(a_i * 2N / q) mod 2N,
+// where the reference divides with Python's correctly rounded true division and rounds
+// half to even.  With v * 2N = f q + r (q odd, so never an exact tie), the double rounds
+// to f + (2r > q) unless f + r/q lies within the double's rounding error of the half;
+// such an entry (about one in 2^37 for N = 1024) is marked UINT64_MAX for the caller
+// to compute exactly.  Needs (q + q/4) * 2N < 2^128; returns the number of marked
+// entries, or -1 when q is too large.
+// ---------------------------------------------------------------------------
+static int clz128(u128 x)
+{
+    const uint64_t hi = (uint64_t)(x >> 64), lo = (uint64_t)x;
+    return hi ? __builtin_clzll(hi) : 64 + (lo ? __builtin_clzll(lo) : 64);
+}
+
+static uint64_t round_2n_over_q(u128 v, u128 q, uint64_t N, bool *ambiguous)
+{
+    const u128 X = v * (u128)(2 * N);
+    const u128 f = X / q, r = X % q;
+    const u128 twor = r << 1;
+    const u128 gap = twor > q ? twor - q : q - twor;
+    // A double of magnitude below 2^e carries a rounding error up to 2^(e-53); the float quotient
+    // can round across the half only when |frac - 1/2| is within that, i.e. |2r - q| <= q 2^(e-52).
+    // Such samples are marked (margin 2^3) and left to an exact restatement of the float path.
+    const int e = f ? 128 - clz128(f) : 0;
+    if (e >= 48 || gap <= (q >> (49 - e)) + 1) { *ambiguous = true; return UINT64_MAX; }
+    return (uint64_t)((f + (twor > q ? 1 : 0)) % (u128)(2 * N));
+}
+
+extern "C" int64_t vfhe_hip_cggi_rotations(LWE *in, uint64_t count, uint64_t N, uint64_t torus_base,
+                                           uint64_t *rot_tv, uint64_t *steps)
+{
+    VH_REQUIRE(in && rot_tv && steps && torus_base > 0 && N > 0);
+    int64_t marked = 0;
+    for (uint64_t k = 0; k < count; k++) {
+        LWE c = in[k];
+        VH_REQUIRE(c && c->l >= 1 && c->ntt && c->l <= c->ntt->l);
+        // q and the CRT basis of the sample's limbs
+        u128 q = 1;
+        int bits = 0;
+        for (uint64_t j = 0; j < c->l; j++) {
+            const uint64_t p = c->ntt->ntt[j]->q;
+            bits += 64 - __builtin_clzll(p);
+            // (value + q/4) * 2N must stay below 2^128
+            if (bits + 1 + (64 - __builtin_clzll(2 * N)) > 127) return -1;
+            q *= p;
+        }
+        // Garner's mixed-radix recomposition: x = sum_j c_j R_j, R_j = q_0 ... q_{j-1},
+        // c_j = (x_j - (sum_{i<j} c_i R_i mod q_j)) * R_j^-1 mod q_j; every product fits 128 bits
+        std::vector<uint64_t> qs(c->l), rinv(c->l);
+        std::vector<u128> R(c->l);
+        for (uint64_t j = 0; j < c->l; j++) qs[j] = c->ntt->ntt[j]->q;
+        for (uint64_t j = 0; j < c->l; j++) {
+            R[j] = j ? R[j - 1] * qs[j - 1] : 1;
+            rinv[j] = inverse_mod((uint64_t)(R[j] % qs[j]), qs[j]);
+        }
+        std::vector<uint64_t> cj(c->l);
+        auto crt = [&](auto limb) -> u128 {
+            u128 x = 0;
+            for (uint64_t j = 0; j < c->l; j++) {
+                const uint64_t qj = qs[j];
+                u128 vmod = 0, w = 1;
+                for (uint64_t i = 0; i < j; i++) {
+                    vmod = (vmod + (u128)cj[i] * (uint64_t)w) % qj;
+                    w = w * qs[i] % qj;
+                }
+                const uint64_t d = (uint64_t)(((u128)(limb(j) % qj) + qj - (uint64_t)vmod) % qj);
+                cj[j] = (uint64_t)((u128)d * rinv[j] % qj);
+                x += (u128)cj[j] * R[j];
+            }
+            return x;
+        };
+        bool amb = false;
+        const u128 b = crt([&](uint64_t j) { return c->b[j]; });
+        const u128 off = q / (4 * torus_base);
+        const uint64_t rb = round_2n_over_q(b + off, q, N, &amb);
+        rot_tv[k] = amb ? UINT64_MAX : (2 * N - rb) % (2 * N);
+        marked += amb;
+        for (uint64_t i = 0; i < c->n; i++) {
+            bool ai = false;
+            const u128 a = crt([&](uint64_t j) { return c->a[j][i]; });
+            steps[k * c->n + i] = round_2n_over_q(a, q, N, &ai);
+            marked += ai;
+        }
+    }
+    return marked;
+}
+
+// ---------------------------------------------------------------------------
 // Sample extraction (mlwe.c:40-68): the LWE sample of coefficient idx of a
 // coefficient-form MLWE sample, over the active limbs of a[0] (limb j of the
 // LWE = the j-th active limb).  The sample's rows are read back to the host.

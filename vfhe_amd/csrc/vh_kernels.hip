@@ -185,8 +185,10 @@ template <int R, bool COL> constexpr int vh_ntt16_lds()
 // the same launch on the same XCD, and this CU's L1 may hold stale lines of it.
 // SHARED: the input row is read by many items (the ModUp lift: one digit row
 // transformed modulo every extended prime), so it is loaded through the caches.
-// NTS: results stored non-temporally (streamed past the caches); false keeps them
-// in the L2 for a reader on the same XCD.
+// NTS: rows loaded and results stored non-temporally (streamed past the caches);
+// false loads and stores them as ordinary L2-allocating accesses, so the rows stay
+// in the L2 for a reader on the same XCD (a non-temporal load leaves its line at
+// streaming priority, and an ordinary store into that line does not raise it).
 template <int DIR, int R, bool COL, bool FIRST, bool LAST, bool COHERENT, bool SHARED = false, bool NTS = true>
 __device__ __forceinline__ void ntt16_task(const vh_item &it, const vh_limb &L, int logn, int s0, size_t blk,
                                            double *__restrict__ lds)
@@ -207,7 +209,7 @@ __device__ __forceinline__ void ntt16_task(const vh_item &it, const vh_limb &L, 
     auto lidx = [&](int k) -> int { return COL ? k * (G + 1) + g : g * (S + S / 16) + k + (k >> 4); };
     auto load = [&](size_t j) -> double {
         const uint64_t raw = COHERENT ? __scoped_atomic_load_n(in + j, __ATOMIC_RELAXED, __MEMORY_SCOPE_WRKGRP)
-                                      : (SHARED ? in[j] : vh_ld(in + j));
+                                      : ((SHARED || !NTS) ? in[j] : vh_ld(in + j));
         return FIRST ? vh_u2d(raw) : __longlong_as_double((long long)raw);
     };
     auto st = [&](VH_GLOBAL uint64_t *p, uint64_t x) {
