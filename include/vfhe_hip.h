@@ -503,7 +503,8 @@ void *vfhe_hip_current_stream(void);
    (the oldest request is admitted alone when nothing is in flight); requests
    of one class (op, ring, rank; levels, keys, gens and rescale masks may
    differ) are batched, up to max_batch per launch, and batches run on `lanes`
-   (1..7) engine lanes. */
+   (1..7) engine lanes; lanes beyond the fourth take a batch only while at most 8
+   requests wait (heavy load grows the batches on the first four instead). */
 void *vfhe_hip_server_new(uint64_t memory_budget, uint32_t lanes, uint32_t max_batch);
 /* Thread-safe; returns the request id (>= 1), or 0 for an invalid request.
    Inputs and keys must stay alive and unchanged until the request completes.

@@ -8,7 +8,11 @@
 // runs batches on several engine lanes (HIP streams of default priority) so
 // that work of different classes overlaps on the device.  A batch launched on
 // a free lane does not wait for batches on other lanes; requests that arrive
-// while batches run are admitted as soon as a lane and memory are free.
+// while batches run are admitted as soon as a lane and memory are free.  Lanes
+// beyond the first four take a batch only while the queue is short (light load:
+// more small batches in flight shorten the wait); under heavy load the queue
+// grows past that and batches form on the first four lanes, where each batch
+// grows instead of more batches competing for the device.
 // Completion is observed with a HIP event per batch; a request's latency is
 // submit -> completion observed.  Outputs are allocated by the runtime and
 // handed to the caller, who frees them (free_mlwe_RNS_sample).
@@ -81,6 +85,10 @@ uint64_t popcount(uint64_t x) { return (uint64_t)__builtin_popcountll(x); }
 
 // Device bytes a request holds while in flight: its output, the engine's
 // temporaries (tensor / permuted input) and the ModUp buffer of its digits.
+// Lanes 0..kBaseLanes-1 always take work; the others only while at most
+// kShortQueue requests wait.
+constexpr unsigned kBaseLanes = 4, kShortQueue = 8;
+
 uint64_t footprint(const Req *r)
 {
     const incNTT t = r->in1->b->ntt;
@@ -220,6 +228,7 @@ struct Server {
                 std::lock_guard<std::mutex> g(mu);
                 while (!incoming.empty()) { waiting.push_back(incoming.front()); incoming.pop_front(); }
                 if (waiting.empty()) break;
+                if ((unsigned)lane >= kBaseLanes && waiting.size() > kShortQueue) break;
                 Req *head = waiting.front();
                 uint64_t room = budget > in_use ? budget - in_use : 0;
                 for (auto it = waiting.begin(); it != waiting.end() && batch.size() < max_batch;) {
