@@ -65,7 +65,8 @@ def parse():
     ap.add_argument("--only-cggi", action="store_true", help="run the bootstrap leg alone (profiling)")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=1.5, help="wall time per offered load")
-    ap.add_argument("--serving-lanes", type=int, default=4, help="HIP streams the serving runtime uses (1..7)")
+    ap.add_argument("--serving-lanes", type=int, default=7,
+                    help="HIP streams the serving runtime uses (1..7; beyond 4 only while the queue is short)")
     ap.add_argument("--only-ntt", action="store_true",
                     help="run the batched NTT round trip alone (the command profiled with rocprofv3)")
     ap.add_argument("--only-ckks", action="store_true", help="run the CKKS leg alone (profiling)")

@@ -145,6 +145,44 @@ def test_multiply_and_batched_rescale(lib):
         s.check(s.pool.read_sample(outs_batch[k]), ref, m2)
 
 
+@pytest.mark.parametrize("K", [1, 2, 3, 4])
+def test_moddown_rescale_fp64_against_integer_and_oracle(lib, K):
+    """The FP64 ModDown + pass-through + rescale kernel (ks.mdr_fp64 = 1: the sequential round
+    divisions folded into per-limb affine constants) against the integer kernel (ks.mdr_fp64 = 0)
+    and, for the first pair, the oracle: 1 to 4 special primes, a batch of 3."""
+    s = Setup(4096, 5, K, 40 + K)
+    r, m = 1, s.lvl_mask
+    rlk = [s.key(s.L, r), None]
+    rlk_arr = s.outer(rlk)
+    cts = [(s.ct(r, m), s.ct(r, m)) for _ in range(3)]
+    top = 1 << (s.L - 1)
+    got = {}
+    saved = lib.vfhe_hip_get_option(b"ks.mdr_fp64")
+    try:
+        for fp in (1, 0):
+            assert lib.vfhe_hip_set_option(b"ks.mdr_fp64", fp) == 0
+            outs = [s.pool.sample(r, s.ext_mask) for _ in cts]
+            n0 = lib.vfhe_hip_get_option(b"stat.mdr_fp64_launches")
+            lib.vfhe_hip_mlwe_multiply_batch(_native.voidp_array(outs), _native.voidp_array([c[0][1] for c in cts]),
+                                             _native.voidp_array([c[1][1] for c in cts]), len(cts), rlk_arr, top)
+            assert (lib.vfhe_hip_get_option(b"stat.mdr_fp64_launches") > n0) == bool(fp)
+            got[fp] = [s.pool.read_sample(o) for o in outs]
+            masks = [s.pool.sample_masks(o) for o in outs]
+    finally:
+        lib.vfhe_hip_set_option(b"ks.mdr_fp64", saved)
+    m2 = m & ~top
+    assert masks == [[m2] * (r + 1)] * len(cts)
+    for a, b in zip(got[1], got[0]):
+        s.check(a, b, m2)
+    (d1, _), (d2, _) = cts[0]
+    ref = np.zeros((r + 1, s.L + s.K, s.N), dtype=np.uint64)
+    mref = s.octx.multiply(ref, d1, d2, r, m, [rlk[0][0], None], s.ext_mask)
+    for c in range(r + 1):
+        s.octx.to_coeff(ref[c], ref[c], mref)
+    assert s.octx.mlwe_round_division(ref, r, mref, top) == m2
+    s.check(got[1][0], ref, m2)
+
+
 def test_automorphism(lib):
     s = Setup(2048, 3, 1, 31)
     r = 1

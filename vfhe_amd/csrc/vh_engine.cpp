@@ -189,6 +189,7 @@ struct vh_opts {
                               // ordinary (L2-allocating) stores; 0 streams it non-temporally
     long spin_limit = 1 << 22;
     long ks_fused = 1;        // key-switch: fuse the forward row pass with the key inner product
+    long mdr_fp64 = 1;        // ModDown + rescale on the FP64 path when every limb allows it
     long fft_fma = 1;         // complex FFT: 1 = avx512ifma engine's fused complex product, 0 = portable's
 };
 static vh_opts OPT;
@@ -221,6 +222,7 @@ static void vh_ensure()
     if (const char *e = getenv("VFHE_HIP_XCD_OCC")) OPT.xcd_occ = atol(e);
     if (const char *e = getenv("VFHE_HIP_XCD_L2")) OPT.xcd_l2 = atol(e);
     if (const char *e = getenv("VFHE_HIP_KS_FUSED")) OPT.ks_fused = atol(e);
+    if (const char *e = getenv("VFHE_HIP_MDR_FP64")) OPT.mdr_fp64 = atol(e);
     if (const char *e = getenv("VFHE_HIP_FFT_FMA")) OPT.fft_fma = atol(e);
     VH_REQUIRE(C.device >= 0 && C.device < 64);
     VH_CHECK(vh_scratch_resolve(&g_scratch_of[C.device]));
@@ -706,6 +708,7 @@ static void vh_check_device_error()
 // 512-thread instances with small batches).  Every wide launch is counted
 // (vfhe_hip_get_option("stat.col_wide_launches")).
 static long g_col_wide_launches = 0;
+static long g_mdr_fp64_launches = 0;   // launches of the FP64 ModDown + rescale kernel
 static int col_wide(unsigned items, int logn)
 {
     const uint64_t tiles = (uint64_t)items << (logn - 13);
@@ -1159,6 +1162,7 @@ extern "C" int vfhe_hip_set_option(const char *key, long value)
     else if (!strcmp(key, "ntt.xcd_l2")) OPT.xcd_l2 = value;
     else if (!strcmp(key, "ntt.spin_limit")) OPT.spin_limit = value < 1 ? 1 : value;
     else if (!strcmp(key, "ks.fused")) OPT.ks_fused = value;
+    else if (!strcmp(key, "ks.mdr_fp64")) OPT.mdr_fp64 = value;
     else if (!strcmp(key, "fft.fma")) OPT.fft_fma = value;
     else return -1;
     return 0;
@@ -1179,8 +1183,10 @@ extern "C" long vfhe_hip_get_option(const char *key)
     if (!strcmp(key, "ntt.xcd_l2")) return OPT.xcd_l2;
     if (!strcmp(key, "ntt.spin_limit")) return OPT.spin_limit;
     if (!strcmp(key, "ks.fused")) return OPT.ks_fused;
+    if (!strcmp(key, "ks.mdr_fp64")) return OPT.mdr_fp64;
     if (!strcmp(key, "fft.fma")) return OPT.fft_fma;
     if (!strcmp(key, "stat.col_wide_launches")) return g_col_wide_launches;
+    if (!strcmp(key, "stat.mdr_fp64_launches")) return g_mdr_fp64_launches;
     if (!strcmp(key, "stat.cache_hits")) return g_cache_hits;
     if (!strcmp(key, "stat.cache_misses")) return g_cache_misses;
     if (!strcmp(key, "stat.cache_flushes")) return g_cache_flushes;
@@ -1612,6 +1618,68 @@ static const void *div_consts(incNTT t, uint64_t mask, uint64_t chunk, const std
         VH_CHECK(hipMemcpy(d_consts, consts.data(), consts.size() * sizeof(vh_divc_h), hipMemcpyHostToDevice));
     }
     return d_consts;
+}
+
+// Constant rows of k_moddown_rescale_fp for the limbs of `mask`, dividing by the special
+// primes dv (ascending, the limbs of `down`) and then by limb `resc`: the sequential round
+// divisions (rns_polynomial.c:939-976) written as one affine map per limb.  With
+// inv_j(q) = p_j^-1 mod q and h_j(q) = floor(p_j/2) mod q:
+//   special k:  G = prod_{j<k} inv_j,  g_kj = -prod_{j<=j'<k} inv_j',
+//               c = floor(p_k/2) + sum_{j<k} h_j prod_{j<=j'<k} inv_j'            (mod p_k)
+//   limb i:     G_k = prod_{k<=k'<K} inv_k'  (mod q_i);  for i = resc: X = G_0, T_k = -G_k,
+//               C = floor(q_r/2) + sum_k h_k G_k;  otherwise, with A = q_r^-1 mod q_i:
+//               X = G_0 A, T_k = -G_k A, C = (h_r + sum_k h_k G_k) A, R = -A.
+static const void *mdr_fp_consts(incNTT t, uint64_t mask, uint64_t down, const std::vector<int> &dv, int resc)
+{
+    const uint64_t l = t->l;
+    void *&d = VPOOL(t)->div_consts[std::make_tuple(mask, down, 0x100 + resc)];
+    if (d) return d;
+    const int W = VH_MDR_W, K = (int)dv.size();
+    VH_REQUIRE(K >= 1 && K <= 4 && W >= 4 + 4);
+    std::vector<double2> cs((size_t)(4 + l) * W, make_double2(0.0, 0.0));
+    auto put = [&](size_t at, uint64_t w, uint64_t q) {
+        const double c = centred(w % q, q);
+        cs[at] = make_double2(c, c / (double)q);
+    };
+    auto neg = [](uint64_t w, uint64_t q) { return w % q ? q - w % q : 0; };
+    for (int k = 0; k < K; k++) {
+        const uint64_t pk = t->ntt[dv[k]]->q;
+        uint64_t G = 1, c = pk / 2;
+        for (int j = 0; j < k; j++) G = mulmod_h(G, inverse_mod(t->ntt[dv[j]]->q % pk, pk), pk);
+        for (int j = 0; j < k; j++) {
+            uint64_t g = 1;
+            for (int jj = j; jj < k; jj++) g = mulmod_h(g, inverse_mod(t->ntt[dv[jj]]->q % pk, pk), pk);
+            c = (c + mulmod_h((t->ntt[dv[j]]->q / 2) % pk, g, pk)) % pk;
+            put((size_t)k * W + 2 + j, neg(g, pk), pk);
+        }
+        put((size_t)k * W, G, pk);
+        cs[(size_t)k * W + 1] = make_double2(centred(c, pk), 0.0);
+    }
+    const uint64_t qr = t->ntt[resc]->q;
+    for (uint64_t i = 0; i < l; i++) {
+        if (!ACTIVE(mask, i) || ACTIVE(down, i)) continue;
+        const uint64_t qi = t->ntt[i]->q;
+        std::vector<uint64_t> Gk(K + 1, 1);
+        for (int k = K - 1; k >= 0; k--) Gk[k] = mulmod_h(Gk[k + 1], inverse_mod(t->ntt[dv[k]]->q % qi, qi), qi);
+        uint64_t hs = 0;
+        for (int k = 0; k < K; k++) hs = (hs + mulmod_h((t->ntt[dv[k]]->q / 2) % qi, Gk[k], qi)) % qi;
+        const size_t row = (size_t)(4 + i) * W;
+        if ((int)i == resc) {
+            put(row, Gk[0], qi);
+            cs[row + 2] = make_double2(centred((hs + qr / 2) % qi, qi), 0.0);
+            for (int k = 0; k < K; k++) put(row + 4 + k, neg(Gk[k], qi), qi);
+        } else {
+            const uint64_t A = inverse_mod(qr % qi, qi);
+            put(row, mulmod_h(Gk[0], A, qi), qi);
+            put(row + 1, A, qi);
+            cs[row + 2] = make_double2(centred(mulmod_h((hs + (qr / 2) % qi) % qi, A, qi), qi), 0.0);
+            put(row + 3, neg(A, qi), qi);
+            for (int k = 0; k < K; k++) put(row + 4 + k, neg(mulmod_h(Gk[k], A, qi), qi), qi);
+        }
+    }
+    VH_CHECK(hipMalloc(&d, cs.size() * sizeof(double2)));
+    VH_CHECK(hipMemcpy(d, cs.data(), cs.size() * sizeof(double2), hipMemcpyHostToDevice));
+    return d;
 }
 
 static void run_division(const std::vector<RNS_Polynomial> &polys, uint64_t divide_mask, bool round)
@@ -2431,13 +2499,20 @@ static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector
         std::vector<int> dv, rv = {__builtin_ctzll(mdr_resc)};
         for (uint64_t i = 0; i < l; i++) if (ACTIVE(mdr_down, i)) dv.push_back((int)i);
         incNTT t = outs[0]->b->ntt;
-        const void *cd = div_consts(t, mdr_mask, mdr_down, dv, true);
-        const void *cr = div_consts(t, mdr_mask & ~mdr_down, mdr_resc, rv, true);
+        bool fp = OPT.mdr_fp64 && dv.size() <= 4;
+        for_active(mdr_mask, l, [&](uint64_t i) { fp = fp && P->h_limbs[i].fp64; });
+        const void *cd = fp ? nullptr : div_consts(t, mdr_mask, mdr_down, dv, true);
+        const void *cr = fp ? nullptr : div_consts(t, mdr_mask & ~mdr_down, mdr_resc, rv, true);
+        const void *cf = fp ? mdr_fp_consts(t, mdr_mask, mdr_down, dv, rv[0]) : nullptr;
         for (size_t off = 0; off < mdr.size(); off += VH_MAX_GRID_Y) {
             const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, mdr.size() - off);
             std::vector<vh_mdr_job> chunk(mdr.begin() + off, mdr.begin() + off + cnt);
             vh_prof_scope ps_(VH_PROF_DIVISION);
-            VH_LAUNCH(vh_launch_moddown_rescale(S(), vh_stage_vec(chunk), cnt, P->d_limbs, cd, cr, N, (int)dv.size()));
+            if (fp) {
+                VH_LAUNCH(vh_launch_moddown_rescale_fp(S(), vh_stage_vec(chunk), cnt, P->d_limbs, cf, N));
+                g_mdr_fp64_launches++;
+            }
+            else VH_LAUNCH(vh_launch_moddown_rescale(S(), vh_stage_vec(chunk), cnt, P->d_limbs, cd, cr, N, (int)dv.size()));
         }
         run_eltwise(zero, P->d_limbs, VH_OP_ZERO, N);
     }
@@ -2660,7 +2735,8 @@ static std::vector<uint64_t> multiply_key(const std::vector<RNS_MLWE> &outs, con
     std::vector<uint64_t> k = {(uint64_t)(uintptr_t)in1[0]->b->ntt, outs.size(), in1[0]->r, divide_mask,
                                (uint64_t)g_lane, (uint64_t)OPT.ntt_mode, (uint64_t)OPT.ntt_chunk_mb,
                                (uint64_t)OPT.ntt_tile_grid, (uint64_t)OPT.ntt_col_wide, (uint64_t)OPT.ks_fused,
-                               (uint64_t)OPT.xcd_lag, (uint64_t)OPT.xcd_occ, (uint64_t)OPT.xcd_l2};
+                               (uint64_t)OPT.xcd_lag, (uint64_t)OPT.xcd_occ, (uint64_t)OPT.xcd_l2,
+                               (uint64_t)OPT.mdr_fp64};
     auto add = [&](RNS_MLWE c) {
         k.push_back(c->r);
         for (auto p : comps(c)) { k.push_back(p->rns_mask); k.push_back(p->allocated_l); }

@@ -114,6 +114,11 @@ hipError_t vh_launch_split_mul(hipStream_t st, const vh_item *items, unsigned ni
 // cdown: K x nl division constants of the special primes; cresc: 1 x nl of the rescale prime.
 hipError_t vh_launch_moddown_rescale(hipStream_t st, const vh_mdr_job *jobs, unsigned njobs, const vh_limb *limbs,
                                      const void *cdown, const void *cresc, uint64_t n, int kmax);
+// FP64 form (every limb below 2^50, K <= 4): cs holds the constant rows built by the
+// engine (4 special rows, then one row per pool limb, VH_MDR_W double2 each).
+#define VH_MDR_W 8
+hipError_t vh_launch_moddown_rescale_fp(hipStream_t st, const vh_mdr_job *jobs, unsigned njobs, const vh_limb *limbs,
+                                        const void *cs, uint64_t n);
 hipError_t vh_launch_division(hipStream_t st, const vh_div_job *jobs, unsigned njobs, unsigned nout,
                               const vh_limb *limbs, const void *consts, uint64_t n, int kmax);
 hipError_t vh_launch_base_conversion(hipStream_t st, const vh_bconv_job *jobs, unsigned njobs, unsigned nout,

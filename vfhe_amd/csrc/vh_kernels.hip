@@ -1441,6 +1441,79 @@ k_moddown_rescale(const vh_mdr_job *__restrict__ jobs, const vh_limb *__restrict
     }
 }
 
+// The same ModDown + pass-through + rescale on the FP64 path (every limb below 2^50,
+// at most 4 special primes).  The K sequential round divisions of the reference
+// (rns_polynomial.c:939-976, ascending special primes, then the rescale prime) are
+// affine in the row values and in the carried t_k = (special row k after the
+// divisions before it + floor(p_k/2)) mod p_k, so with the products of inverses
+// folded into per-limb constants (built on the host, exact):
+//   t_k   = canon(v_k * G_k + c_k + sum_{j<k} t_j * g_kj)                 mod p_k
+//   t_r   = canon(x_r * X_r + add_r + C_r + sum_k t_k * T_rk)             mod q_r
+//   out_i = canon(x_i * X_i + add_i * A_i + C_i + t_r * R_i + sum_k t_k * T_ik)  mod q_i
+// Each product is vh_mulmod_tw against a centred constant ({w, RN(w/q)}): |term| <= 3q/4,
+// so with K <= 4 every sum stays below 6q < 2^53 and the result is the reference's,
+// bit for bit.  Constant rows (stride VH_MDR_W double2): rows 0..3 the special primes
+// [G, c, g_k0..g_k3], then row 4 + limb for every limb of the pool [X, A, C, R, T_0..T_3]
+// (C in .x only).
+__global__ void __launch_bounds__(256)
+k_moddown_rescale_fp(const vh_mdr_job *__restrict__ jobs, const vh_limb *__restrict__ limbs,
+                     const double2 *__restrict__ cs, uint64_t n)
+{
+    constexpr int KMAX = 4, W = VH_MDR_W;
+    const vh_mdr_job &jb = jobs[blockIdx.y];
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const int K = jb.K;
+    const VH_GLOBAL uint64_t *base = vh_global(jb.base);
+    const VH_GLOBAL uint64_t *add = jb.add ? vh_global(jb.add) : nullptr;
+    auto row = [&](const VH_GLOBAL uint64_t *b, int limb) -> double { return vh_u2d(b[(uint64_t)limb * jb.stride + x]); };
+    double t[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        if (k >= K) continue;   // (continue, not break: the loop unrolls fully)
+        const int lk = jb.div_limb[k];
+        const vh_limb &L = limbs[lk];
+        const double2 *c = cs + k * W;
+        double s = c[1].x + vh_mulmod_tw(row(base, lk), c[0].x, c[0].y, L.qd);
+#pragma unroll
+        for (int j = 0; j < KMAX; j++)
+            if (j < k) s += vh_mulmod_tw(t[j], c[2 + j].x, c[2 + j].y, L.qd);
+        t[k] = vh_canon(s, L.qd, L.qinv);
+    }
+    const int lr = jb.resc;
+    double tr;
+    {
+        const vh_limb &L = limbs[lr];
+        const double2 *c = cs + (KMAX + lr) * W;
+        double s = c[2].x + vh_mulmod_tw(row(base, lr), c[0].x, c[0].y, L.qd);
+        if (add) s += row(add, lr);
+#pragma unroll
+        for (int k = 0; k < KMAX; k++)
+            if (k < K) s += vh_mulmod_tw(t[k], c[4 + k].x, c[4 + k].y, L.qd);
+        tr = vh_canon(s, L.qd, L.qinv);
+    }
+    VH_GLOBAL uint64_t *outb = vh_global(jb.base);
+    for (int o = 0; o < jb.nout; o++) {
+        const int li = jb.out_limb[o];
+        const vh_limb &L = limbs[li];
+        const double2 *c = cs + (KMAX + li) * W;
+        double s = c[2].x + vh_mulmod_tw(row(base, li), c[0].x, c[0].y, L.qd) + vh_mulmod_tw(tr, c[3].x, c[3].y, L.qd);
+        if (add) s += vh_mulmod_tw(row(add, li), c[1].x, c[1].y, L.qd);
+#pragma unroll
+        for (int k = 0; k < KMAX; k++)
+            if (k < K) s += vh_mulmod_tw(t[k], c[4 + k].x, c[4 + k].y, L.qd);
+        outb[(uint64_t)li * jb.stride + x] = vh_d2u(vh_canon(s, L.qd, L.qinv));
+    }
+}
+
+hipError_t vh_launch_moddown_rescale_fp(hipStream_t st, const vh_mdr_job *jobs, unsigned njobs, const vh_limb *limbs,
+                                        const void *cs, uint64_t n)
+{
+    const dim3 grid((unsigned)((n + 255) / 256), njobs);
+    VH_KL(k_moddown_rescale_fp, grid, dim3(256), 0, st, jobs, limbs, (const double2 *)cs, n);
+    return hipGetLastError();
+}
+
 hipError_t vh_launch_moddown_rescale(hipStream_t st, const vh_mdr_job *jobs, unsigned njobs, const vh_limb *limbs,
                                      const void *cdown, const void *cresc, uint64_t n, int kmax)
 {
