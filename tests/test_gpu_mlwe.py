@@ -183,6 +183,44 @@ def test_moddown_rescale_fp64_against_integer_and_oracle(lib, K):
     s.check(got[1][0], ref, m2)
 
 
+@pytest.mark.parametrize("rescale", [False, True])
+def test_multiply_pass_through_folded_into_key_mac(lib, rescale):
+    """ks.fold = 1: the tensor's linear and constant slots stay in NTT form and enter the fused
+    key inner product as P * d (k_ntt16_row_mac), instead of being added after the ModDown.
+    Bit-equal to ks.fold = 0 and to the oracle (mlwe.c:711-760, then ckks.py:128-148)."""
+    s = Setup(8192, 4, 2, 77 + rescale)
+    r, m = 1, s.lvl_mask
+    rlk = [s.key(s.L, r), None]
+    rlk_arr = s.outer(rlk)
+    cts = [(s.ct(r, m), s.ct(r, m)) for _ in range(3)]
+    top = (1 << (s.L - 1)) if rescale else 0
+    got = {}
+    saved = lib.vfhe_hip_get_option(b"ks.fold")
+    try:
+        for fold in (1, 0):
+            assert lib.vfhe_hip_set_option(b"ks.fold", fold) == 0
+            outs = [s.pool.sample(r, s.ext_mask) for _ in cts]
+            n0 = lib.vfhe_hip_get_option(b"stat.ks_fold_jobs")
+            lib.vfhe_hip_mlwe_multiply_batch(_native.voidp_array(outs), _native.voidp_array([c[0][1] for c in cts]),
+                                             _native.voidp_array([c[1][1] for c in cts]), len(cts), rlk_arr, top)
+            assert (lib.vfhe_hip_get_option(b"stat.ks_fold_jobs") - n0) == (len(cts) if fold else 0)
+            got[fold] = [s.pool.read_sample(o) for o in outs]
+    finally:
+        lib.vfhe_hip_set_option(b"ks.fold", saved)
+    m2 = m & ~top
+    for a, b in zip(got[1], got[0]):
+        s.check(a, b, m2)
+    (d1, _), (d2, _) = cts[1]
+    ref = np.zeros((r + 1, s.L + s.K, s.N), dtype=np.uint64)
+    mref = s.octx.multiply(ref, d1, d2, r, m, [rlk[0][0], None], s.ext_mask)
+    if rescale:
+        for c in range(r + 1):
+            s.octx.to_coeff(ref[c], ref[c], mref)
+        mref = s.octx.mlwe_round_division(ref, r, mref, top)
+    assert mref == m2
+    s.check(got[1][1], ref, m2)
+
+
 def test_automorphism(lib):
     s = Setup(2048, 3, 1, 31)
     r = 1

@@ -111,6 +111,39 @@ def test_one_of_each_against_oracle(lib, rings):
     lib.free_mlwe_RNS_sample(got)
 
 
+def test_submit_output_still_in_flight_on_lane0(lib, rings):
+    """A ciphertext produced asynchronously on the caller's lane (lane 0, no synchronisation) is
+    submitted at once: the batch's lane waits for the submit-time event, so the served result
+    equals the one computed after a synchronisation."""
+    N, lvl = NS[1], LEVELS[1]
+    ring = rings[N]
+    op, c1, c2, ksk, gen, dm = ring.args(serving.Request(0.0, serving.REQ_MULTIPLY, N, lvl, 0))
+    srv = serving.Server(lib, 1 << 30, lanes=3, max_batch=4)
+    try:
+        x = ring.pool.sample(1, ring.pool.mask(_native_b(c1)))
+        lib.mlwe_multiply(x, c1, c2, ksk)               # enqueued on lane 0, not waited for
+        rid = srv.submit(op, x, c2, ksk, gen, dm)
+        done = []
+        while not done:
+            done = srv.poll(wait=True)
+        assert [d[0] for d in done] == [rid]
+        got = done[0][1]
+        lib.vfhe_hip_synchronize()
+        ref = ring.pool.sample(1, ring.pool.mask(_native_b(c1)))
+        lib.mlwe_multiply(ref, x, c2, ksk)
+        lib.mlwe_RNS_to_RNSc(ref, ref)
+        lib.mlwe_round_division_RNSc(ref, dm)
+        masks = ring.pool.sample_masks(got)
+        assert masks == ring.pool.sample_masks(ref)
+        rows = [i for i in range(len(ring.primes)) if (masks[0] >> i) & 1]
+        assert np.array_equal(ring.pool.read_sample(got)[:, rows], ring.pool.read_sample(ref)[:, rows])
+        for c in (got, ref, x):
+            lib.free_mlwe_RNS_sample(c)
+    finally:
+        lib.vfhe_hip_synchronize()
+        srv.close()
+
+
 BIG_NS, BIG_L, BIG_K, BIG_LEVELS = (32768, 65536), 24, 4, (22, 24)
 
 

@@ -190,6 +190,8 @@ struct vh_opts {
     long spin_limit = 1 << 22;
     long ks_fused = 1;        // key-switch: fuse the forward row pass with the key inner product
     long mdr_fp64 = 1;        // ModDown + rescale on the FP64 path when every limb allows it
+    long ks_fold = 1;         // multiply: pass-through components enter the fused key inner
+                              // product as P * d (NTT form) instead of after the ModDown
     long fft_fma = 1;         // complex FFT: 1 = avx512ifma engine's fused complex product, 0 = portable's
 };
 static vh_opts OPT;
@@ -223,6 +225,7 @@ static void vh_ensure()
     if (const char *e = getenv("VFHE_HIP_XCD_L2")) OPT.xcd_l2 = atol(e);
     if (const char *e = getenv("VFHE_HIP_KS_FUSED")) OPT.ks_fused = atol(e);
     if (const char *e = getenv("VFHE_HIP_MDR_FP64")) OPT.mdr_fp64 = atol(e);
+    if (const char *e = getenv("VFHE_HIP_KS_FOLD")) OPT.ks_fold = atol(e);
     if (const char *e = getenv("VFHE_HIP_FFT_FMA")) OPT.fft_fma = atol(e);
     VH_REQUIRE(C.device >= 0 && C.device < 64);
     VH_CHECK(vh_scratch_resolve(&g_scratch_of[C.device]));
@@ -709,6 +712,7 @@ static void vh_check_device_error()
 // (vfhe_hip_get_option("stat.col_wide_launches")).
 static long g_col_wide_launches = 0;
 static long g_mdr_fp64_launches = 0;   // launches of the FP64 ModDown + rescale kernel
+static long g_ks_fold_jobs = 0;        // multiply jobs whose pass-through went into the key MAC
 static int col_wide(unsigned items, int logn)
 {
     const uint64_t tiles = (uint64_t)items << (logn - 13);
@@ -1163,6 +1167,7 @@ extern "C" int vfhe_hip_set_option(const char *key, long value)
     else if (!strcmp(key, "ntt.spin_limit")) OPT.spin_limit = value < 1 ? 1 : value;
     else if (!strcmp(key, "ks.fused")) OPT.ks_fused = value;
     else if (!strcmp(key, "ks.mdr_fp64")) OPT.mdr_fp64 = value;
+    else if (!strcmp(key, "ks.fold")) OPT.ks_fold = value;
     else if (!strcmp(key, "fft.fma")) OPT.fft_fma = value;
     else return -1;
     return 0;
@@ -1184,6 +1189,8 @@ extern "C" long vfhe_hip_get_option(const char *key)
     if (!strcmp(key, "ntt.spin_limit")) return OPT.spin_limit;
     if (!strcmp(key, "ks.fused")) return OPT.ks_fused;
     if (!strcmp(key, "ks.mdr_fp64")) return OPT.mdr_fp64;
+    if (!strcmp(key, "ks.fold")) return OPT.ks_fold;
+    if (!strcmp(key, "stat.ks_fold_jobs")) return g_ks_fold_jobs;
     if (!strcmp(key, "fft.fma")) return OPT.fft_fma;
     if (!strcmp(key, "stat.col_wide_launches")) return g_col_wide_launches;
     if (!strcmp(key, "stat.mdr_fp64_launches")) return g_mdr_fp64_launches;
@@ -2213,7 +2220,44 @@ struct vh_gsrc {
     RNS_MLWE *keys;
 };
 
-static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<std::vector<vh_gsrc>> &srcs, bool add)
+// The fused ModUp row pass + key inner product (k_ntt16_row_mac) runs when every limb of
+// `mask` is on the FP64 path, the ring is not split, N = 2^13..2^16 and the output has at
+// most two components.
+static bool fused_mac_ok(incNTT t, uint64_t mask, uint64_t comps)
+{
+    const vh_pool *P = VPOOL(t);
+    bool ok = OPT.ks_fused && OPT.ntt_mode != 1 && t->split_degree == 1 && comps <= 2 && P->logn >= 13 &&
+              P->logn <= 16;
+    for (uint64_t i = 0; i < t->l && ok; i++) if (ACTIVE(mask, i)) ok = P->h_limbs[i].fp64;
+    return ok;
+}
+
+// {P mod q_i centred, RN(that / q_i)} for every limb i of `level`, P the product of the
+// primes of ext & ~level (the special primes of a key-switch).
+static const double2 *pmod_consts(incNTT t, uint64_t level, uint64_t ext)
+{
+    void *&d = VPOOL(t)->div_consts[std::make_tuple(ext, level, 0x200)];
+    if (d) return (const double2 *)d;
+    std::vector<double2> v(t->l, make_double2(0.0, 0.0));
+    for (uint64_t i = 0; i < t->l; i++) {
+        if (!ACTIVE(level, i)) continue;
+        const uint64_t qi = t->ntt[i]->q;
+        uint64_t pm = 1;
+        for (uint64_t k = 0; k < t->l; k++)
+            if (ACTIVE(ext, k) && !ACTIVE(level, k)) pm = mulmod_h(pm, t->ntt[k]->q % qi, qi);
+        const double c = centred(pm, qi);
+        v[i] = make_double2(c, c / (double)qi);
+    }
+    VH_CHECK(hipMalloc(&d, v.size() * sizeof(double2)));
+    VH_CHECK(hipMemcpy(d, v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice));
+    return (const double2 *)d;
+}
+
+// fold (optional, per job): NTT-form polynomials added to the output components as
+// P * fold[i][c] on the source's level limbs, inside the fused key inner product (the
+// caller has checked fused_mac_ok; null entries add nothing).
+static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<std::vector<vh_gsrc>> &srcs, bool add,
+                         const std::vector<std::vector<RNS_Polynomial>> *fold = nullptr)
 {
     const size_t B = outs.size();
     VH_REQUIRE(B == srcs.size() && B > 0);
@@ -2296,8 +2340,10 @@ static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<st
         const int logn = ilog2(P->n);
         // Fused form: column pass here, then the row pass of every digit and the key
         // inner product in one kernel (the transformed digits stay in registers).
-        const bool fused_mac = fuse && OPT.ks_fused && OPT.ntt_mode != 1 && t->split_degree == 1 && Cc <= 2 &&
-                               logn >= 13 && logn <= 16;
+        uint64_t umask = 0;
+        for (auto &jb : J) umask |= jb.ext;
+        const bool fused_mac = fuse && fused_mac_ok(t, umask, Cc);
+        VH_REQUIRE(!fold || fused_mac);
         if (fused_mac) {
             for (size_t off = 0; off < nt.size(); off += VH_MAX_GRID_Y) {
                 const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, nt.size() - off);
@@ -2335,6 +2381,18 @@ static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<st
                 jb.digits = (int)src.nd;
                 jb.next = (int)src.next;
                 for (uint64_t ti = 0; ti < src.next; ti++) jb.limb[ti] = src.E[ti];
+                if (fold) {
+                    const uint64_t level = srcs[bi][0].poly->rns_mask;
+                    jb.pass_mask = level;
+                    jb.pmod = pmod_consts(t, level, src.ext);
+                    for (uint64_t c = 0; c < Cc; c++) {
+                        RNS_Polynomial f = (*fold)[bi][c];
+                        if (!f) continue;
+                        dev_ready(f);
+                        VH_REQUIRE(f->ntt == t && (f->rns_mask & level) == level);
+                        jb.pass[c] = VP(f)->d;
+                    }
+                }
             }
             for (size_t off = 0; off < B; off += VH_MAX_GRID_Y) {
                 const unsigned cnt = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, B - off);
@@ -2380,6 +2438,7 @@ static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<st
         }
         vh_dfree(modup);
     } else {
+        VH_REQUIRE(!fold);
         std::vector<vh_item> z;
         for (size_t bi = 0; bi < B; bi++)
             for (auto p : comps(outs[bi]))
@@ -2396,8 +2455,13 @@ static void gadget_batch(const std::vector<RNS_MLWE> &outs, const std::vector<st
 // primes (the CKKS rescale of mlwe_multiply's caller, ckks.py:128-148).  A job
 // whose rescale is one prime is finished by k_moddown_rescale: ModDown, the
 // pass-through add and the rescale in one pass over its rows.
+// pass_ntt: the pass-through components of ins are in NTT form and go into the fused key
+// inner product as P * d (ModDown(y + P d) == ModDown(y) + d exactly, limb by limb: every
+// round division by a special prime p maps y + p P' d to ModDown_p(y) + P' d); the
+// key-switched components are in coefficient form as always.
 static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &ins,
-                            const std::vector<RNS_MLWE **> &ksks, const std::vector<uint64_t> &rescale = {})
+                            const std::vector<RNS_MLWE **> &ksks, const std::vector<uint64_t> &rescale = {},
+                            bool pass_ntt = false)
 {
     const size_t B = outs.size();
     VH_REQUIRE(B == ins.size() && B == ksks.size() && B > 0);
@@ -2409,7 +2473,23 @@ static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector
         VH_REQUIRE(!srcs[bi].empty());
         for (auto p : comps(ins[bi])) dev_ready(p);
     }
-    gadget_batch(outs, srcs, false);
+    // Pass-through components keep their key: added after the ModDown (mlwe.c:588-597), or
+    // folded into the key inner product.  pass[bi][c]: the input added to output component c.
+    std::vector<std::vector<RNS_Polynomial>> pass(B);
+    for (size_t bi = 0; bi < B; bi++) {
+        const auto oc = comps(outs[bi]);
+        pass[bi].assign(oc.size(), nullptr);
+        uint64_t keep = 0;
+        for (uint64_t i = 0; i < ins[bi]->r; i++) if (!ksks[bi][i]) pass[bi][keep++] = ins[bi]->a[i];
+        pass[bi][oc.size() - 1] = ins[bi]->b;
+    }
+    if (pass_ntt) {
+        gadget_batch(outs, srcs, false, &pass);
+        for (auto &pv : pass) pv.assign(pv.size(), nullptr);
+        g_ks_fold_jobs += (long)B;
+    } else {
+        gadget_batch(outs, srcs, false);
+    }
     const uint64_t l = outs[0]->b->ntt->l, N = outs[0]->b->ntt->N;
     vh_pool *P = VPOOL(outs[0]->b->ntt);
 
@@ -2423,16 +2503,6 @@ static void keyswitch_batch(const std::vector<RNS_MLWE> &outs, const std::vector
         }
     ntt_rows(pairs, 1);
 
-    // Pass-through components keep their key: added after the ModDown (mlwe.c:588-597).
-    // pass[bi][c]: the input added to output component c (null: none).
-    std::vector<std::vector<RNS_Polynomial>> pass(B);
-    for (size_t bi = 0; bi < B; bi++) {
-        const auto oc = comps(outs[bi]);
-        pass[bi].assign(oc.size(), nullptr);
-        uint64_t keep = 0;
-        for (uint64_t i = 0; i < ins[bi]->r; i++) if (!ksks[bi][i]) pass[bi][keep++] = ins[bi]->a[i];
-        pass[bi][oc.size() - 1] = ins[bi]->b;
-    }
     // Jobs finished in one pass: a one-prime rescale, at most VH_MAX_DIV special primes,
     // one mask for every output component, pass-through inputs covering it.
     std::vector<char> fused(B, 0);
@@ -2662,15 +2732,24 @@ static void multiply_batch(const std::vector<RNS_MLWE> &outs, const std::vector<
     }
     std::vector<RNS_MLWE> ext(B);
     std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pairs;
+    // With the fused key inner product the pass-through slots stay in NTT form and enter
+    // it as P * d: only the key-switched slots go back to coefficient form.
+    uint64_t kmask = 0;
+    for (size_t bi = 0; bi < B; bi++)
+        for (uint64_t i = 0; i < R; i++)
+            if (ksks[bi] && ksks[bi][i] && ksks[bi][i][0]) kmask |= ksks[bi][i][0]->b->rns_mask;
+    const bool fold = OPT.ks_fold && kmask && fused_mac_ok(t, kmask, r + 1);
     for (size_t bi = 0; bi < B; bi++) {
         VH_REQUIRE(ksks[bi] && outs[bi]->r == r && in1[bi]->r == r && in1[bi]->b->ntt == t);
         ext[bi] = alloc_sample_raw(N, R, in1[bi]->b->rns_mask, t);
         slots[bi] = comps(ext[bi]);
-        for (auto p : slots[bi]) pairs.push_back({p, p});
+        for (uint64_t i = 0; i < R; i++)
+            if (!fold || ksks[bi][i]) pairs.push_back({ext[bi]->a[i], ext[bi]->a[i]});
+        if (!fold) pairs.push_back({ext[bi]->b, ext[bi]->b});
     }
     tensors();
     ntt_rows(pairs, 1);
-    keyswitch_batch(outs, ext, ksks, divide_masks);   // also rescales the jobs that ask for it
+    keyswitch_batch(outs, ext, ksks, divide_masks, fold);   // also rescales the jobs that ask for it
     std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> o;
     for (size_t bi = 0; bi < B; bi++)
         if (divide_masks[bi] == 0)
@@ -2736,7 +2815,7 @@ static std::vector<uint64_t> multiply_key(const std::vector<RNS_MLWE> &outs, con
                                (uint64_t)g_lane, (uint64_t)OPT.ntt_mode, (uint64_t)OPT.ntt_chunk_mb,
                                (uint64_t)OPT.ntt_tile_grid, (uint64_t)OPT.ntt_col_wide, (uint64_t)OPT.ks_fused,
                                (uint64_t)OPT.xcd_lag, (uint64_t)OPT.xcd_occ, (uint64_t)OPT.xcd_l2,
-                               (uint64_t)OPT.mdr_fp64};
+                               (uint64_t)OPT.mdr_fp64, (uint64_t)OPT.ks_fold};
     auto add = [&](RNS_MLWE c) {
         k.push_back(c->r);
         for (auto p : comps(c)) { k.push_back(p->rns_mask); k.push_back(p->allocated_l); }

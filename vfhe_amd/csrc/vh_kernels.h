@@ -82,6 +82,11 @@ struct vh_mac_job {
     int comps, digits, next;
     int add;                      // 1: out = +sum (external product), 0: out = -sum (key-switch)
     int limb[VH_MAX_LIMBS];       // extended limb list
+    // Fused rows only: out_c += P * pass[c] on the limbs of pass_mask (pass[c] NTT-form rows,
+    // limb-major; pmod[limb] = {P mod q centred, RN(that / q)}), null pass[c]: nothing added.
+    const uint64_t *pass[VH_MAX_COMPS];
+    const double2 *pmod;
+    uint64_t pass_mask;
 };
 
 hipError_t vh_launch_ntt_pass(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,

@@ -759,11 +759,27 @@ k_ntt16_row_mac(const vh_mac_job *__restrict__ jobs, const vh_limb *__restrict__
             }
         }
     }
+    const bool fold = (jb.pass_mask >> limb) & 1;
 #pragma unroll
     for (int c = 0; c < C; c++) {
         VH_GLOBAL uint64_t *out = vh_global(jb.out[c] + (size_t)limb * n + x0);
+        if (fold && jb.pass[c]) {
+            // |acc| <= q/2 + 4 (q + 2^47) and |P d| <= 3q/4: the sum stays below 2^53.  A lane
+            // reads 16 consecutive words (ordinary loads, merged into wide ones, like the key words).
+            const VH_GLOBAL uint64_t *ps = vh_global(jb.pass[c] + (size_t)limb * n + x0);
+            const double2 pm = jb.pmod[limb];
+            uint64_t pw[16];
 #pragma unroll
-        for (int m = 0; m < 16; m++) vh_st(out + m, vh_d2u(vh_canon(jb.add ? acc[c][m] : -acc[c][m], q, qinv)));
+            for (int m = 0; m < 16; m++) pw[m] = ps[m];
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                const double y = (jb.add ? acc[c][m] : -acc[c][m]) + vh_mulmod_tw(vh_u2d(pw[m]), pm.x, pm.y, q);
+                vh_st(out + m, vh_d2u(vh_canon(y, q, qinv)));
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < 16; m++) vh_st(out + m, vh_d2u(vh_canon(jb.add ? acc[c][m] : -acc[c][m], q, qinv)));
+        }
     }
 }
 
