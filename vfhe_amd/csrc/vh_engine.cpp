@@ -190,6 +190,8 @@ struct vh_opts {
     long spin_limit = 1 << 22;
     long ks_fused = 1;        // key-switch: fuse the forward row pass with the key inner product
     long mdr_fp64 = 1;        // ModDown + rescale on the FP64 path when every limb allows it
+    long ntt_small = 1;       // N = 2^9 .. 2^12 (FP64): whole transforms in registers (k_ntt_small);
+                              // 0: the LDS radix-2 kernel
     long ks_fold = 1;         // multiply: pass-through components enter the fused key inner
                               // product as P * d (NTT form) instead of after the ModDown
     long fft_fma = 1;         // complex FFT: 1 = avx512ifma engine's fused complex product, 0 = portable's
@@ -226,6 +228,7 @@ static void vh_ensure()
     if (const char *e = getenv("VFHE_HIP_KS_FUSED")) OPT.ks_fused = atol(e);
     if (const char *e = getenv("VFHE_HIP_MDR_FP64")) OPT.mdr_fp64 = atol(e);
     if (const char *e = getenv("VFHE_HIP_KS_FOLD")) OPT.ks_fold = atol(e);
+    if (const char *e = getenv("VFHE_HIP_NTT_SMALL")) OPT.ntt_small = atol(e);
     if (const char *e = getenv("VFHE_HIP_FFT_FMA")) OPT.fft_fma = atol(e);
     VH_REQUIRE(C.device >= 0 && C.device < 64);
     VH_CHECK(vh_scratch_resolve(&g_scratch_of[C.device]));
@@ -712,6 +715,7 @@ static void vh_check_device_error()
 // (vfhe_hip_get_option("stat.col_wide_launches")).
 static long g_col_wide_launches = 0;
 static long g_mdr_fp64_launches = 0;   // launches of the FP64 ModDown + rescale kernel
+static long g_ntt_small_launches = 0;  // launches of the register NTT for N = 2^9 .. 2^12
 static long g_ks_fold_jobs = 0;        // multiply jobs whose pass-through went into the key MAC
 static int col_wide(unsigned items, int logn)
 {
@@ -768,6 +772,11 @@ static void ntt_dispatch(const vh_item *d_items, unsigned cnt, const vh_limb *d_
                 { vh_prof_scope ps_(VH_PROF_NTT_FP64); VH_LAUNCH(vh_launch_ntt16(S(), ci, cc, d_limbs, 1, logn, 0, R1, 1, 0, 1)); }
             }
         }
+    } else if (fp && logn >= 9 && logn <= 12 && OPT.ntt_small) {
+        // whole transforms in registers, 4096/N of them per workgroup
+        vh_prof_scope ps_(VH_PROF_NTT_FP64);
+        VH_LAUNCH(vh_launch_ntt_small(S(), d_items, cnt, d_limbs, dir, logn));
+        g_ntt_small_launches++;
     } else if (logn <= 12) {
         { vh_prof_scope ps_(fp ? VH_PROF_NTT_FP64 : VH_PROF_NTT_INT); VH_LAUNCH(vh_launch_ntt_pass(S(), d_items, cnt, d_limbs, fp, dir, logn, 0, logn, 0, 1, 1, 1)); }
     } else {
@@ -1168,6 +1177,7 @@ extern "C" int vfhe_hip_set_option(const char *key, long value)
     else if (!strcmp(key, "ks.fused")) OPT.ks_fused = value;
     else if (!strcmp(key, "ks.mdr_fp64")) OPT.mdr_fp64 = value;
     else if (!strcmp(key, "ks.fold")) OPT.ks_fold = value;
+    else if (!strcmp(key, "ntt.small")) OPT.ntt_small = value;
     else if (!strcmp(key, "fft.fma")) OPT.fft_fma = value;
     else return -1;
     return 0;
@@ -1190,6 +1200,8 @@ extern "C" long vfhe_hip_get_option(const char *key)
     if (!strcmp(key, "ks.fused")) return OPT.ks_fused;
     if (!strcmp(key, "ks.mdr_fp64")) return OPT.mdr_fp64;
     if (!strcmp(key, "ks.fold")) return OPT.ks_fold;
+    if (!strcmp(key, "ntt.small")) return OPT.ntt_small;
+    if (!strcmp(key, "stat.ntt_small_launches")) return g_ntt_small_launches;
     if (!strcmp(key, "stat.ks_fold_jobs")) return g_ks_fold_jobs;
     if (!strcmp(key, "fft.fma")) return OPT.fft_fma;
     if (!strcmp(key, "stat.col_wide_launches")) return g_col_wide_launches;
@@ -2815,7 +2827,7 @@ static std::vector<uint64_t> multiply_key(const std::vector<RNS_MLWE> &outs, con
                                (uint64_t)g_lane, (uint64_t)OPT.ntt_mode, (uint64_t)OPT.ntt_chunk_mb,
                                (uint64_t)OPT.ntt_tile_grid, (uint64_t)OPT.ntt_col_wide, (uint64_t)OPT.ks_fused,
                                (uint64_t)OPT.xcd_lag, (uint64_t)OPT.xcd_occ, (uint64_t)OPT.xcd_l2,
-                               (uint64_t)OPT.mdr_fp64, (uint64_t)OPT.ks_fold};
+                               (uint64_t)OPT.mdr_fp64, (uint64_t)OPT.ks_fold, (uint64_t)OPT.ntt_small};
     auto add = [&](RNS_MLWE c) {
         k.push_back(c->r);
         for (auto p : comps(c)) { k.push_back(p->rns_mask); k.push_back(p->allocated_l); }

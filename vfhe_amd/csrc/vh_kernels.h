@@ -89,6 +89,9 @@ struct vh_mac_job {
     uint64_t pass_mask;
 };
 
+// Whole FP64 transforms of N = 2^9 .. 2^12 (reads it.src, writes it.dst), registers radix-16.
+hipError_t vh_launch_ntt_small(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,
+                               int logn);
 hipError_t vh_launch_ntt_pass(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                               int fp, int dir, int logn, int s0, int R, int col, int G, int first, int last);
 // Register radix-16 pass (FP64 path): 4 <= R <= 8, G = 4096 / 2^R sub-transforms per block.

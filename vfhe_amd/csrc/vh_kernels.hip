@@ -1695,6 +1695,135 @@ static inline unsigned vh_blocks(uint64_t n, unsigned per = 256, unsigned cap = 
     return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
+// ---------------------------------------------------------------------------
+// Whole transforms of N = 2^9 .. 2^12 points on the FP64 path, 16 values per thread in
+// registers: T = N/16 threads per transform, 4096/N transforms per 256-thread workgroup.
+// The reference's butterfly network (merged CT forward, natural in, bit-reversed out;
+// GS inverse with n^-1, ntt.c / ntt50.c) in three register phases:
+//   A  stages 0..3         stride ownership   j = t + T m            (uniform twiddles)
+//   B  stages 4..7         middle ownership   j = 16U g + u + U m    (t = u + U g, U = T/16)
+//   C  stages 8..logN-1    contiguous         j = 16 t + m
+// with an LDS exchange between phases (one pad slot per 16 values: the contiguous reads
+// stride 17 words).  A stage-s butterfly on (j, j + N/2^(s+1)) takes the twiddle of index
+// 2^s + (j >> (logN - s)), so every product is the reference's and the canonical outputs
+// are identical.  Rows are loaded and stored in the stride ownership (whole 128-byte runs
+// per instruction); the forward pass regroups its output through LDS for that.
+// Bounds as in the two-pass kernels: at most four stages between centrings, |x| < 2^52.
+// ---------------------------------------------------------------------------
+template <int DIR, int LOGN>
+__global__ void __launch_bounds__(256)
+k_ntt_small(const vh_item *__restrict__ items, unsigned nitems, const vh_limb *__restrict__ limbs)
+{
+    constexpr int N = 1 << LOGN, T = N / 16, U = T / 16, PER = 256 / T, PAD = N + N / 16;
+    static_assert(LOGN >= 9 && LOGN <= 12, "k_ntt_small: N = 2^9 .. 2^12");
+    __shared__ double lds_all[PER * PAD];
+    const int tid = threadIdx.x, slot = tid / T, t = tid % T, g = t / U, u = t % U;
+    const unsigned idx = blockIdx.x * PER + slot;
+    const bool live = idx < nitems;
+    // a slot past the end transforms the last item again (every thread reaches the
+    // barriers) and stores nothing
+    const vh_item &it = items[live ? idx : nitems - 1];
+    const vh_limb &L = limbs[it.limb];
+    const double q = L.qd, qinv = L.qinv;
+    const VH_GLOBAL double *tw = vh_global(reinterpret_cast<const double *>(DIR == 0 ? L.tw_fwd : L.tw_inv));
+    auto twl = [&](int i) { return make_double2(tw[2 * i], tw[2 * i + 1]); };
+    double *lds = lds_all + slot * PAD;
+    auto li = [](int j) { return j + (j >> 4); };
+    auto jA = [&](int m) { return t + T * m; };
+    auto jB = [&](int m) { return 16 * U * g + u + U * m; };
+    auto jC = [&](int m) { return 16 * t + m; };
+    const VH_GLOBAL uint64_t *in = vh_global(it.src);
+    VH_GLOBAL uint64_t *out = vh_global(it.dst);
+    double v[16];
+
+    // forward CT stage s over register pairs (m, m + hm); blk(m) = block index of element m
+    auto ct = [&](int s, int hm, auto blk) {
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            if (m & hm) continue;
+            const double2 w = twl((1 << s) + blk(m));
+            const double x = vh_mulmod_tw(v[m + hm], w.x, w.y, q);
+            const double a = v[m];
+            v[m] = a + x;
+            v[m + hm] = a - x;
+        }
+    };
+    int c = 0;   // inverse: GS sums are re-centred after every second stage
+    auto gs = [&](int s, int hm, auto blk) {
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            if (m & hm) continue;
+            const double2 w = twl((1 << s) + blk(m));
+            const double a = v[m], x = v[m + hm];
+            double sum = a + x;
+            if (c & 1) sum = vh_center(sum, q, qinv);
+            v[m] = sum;
+            v[m + hm] = vh_mulmod_tw(a - x, w.x, w.y, q);
+        }
+        c++;
+    };
+    auto exchange = [&](auto jw, auto jr, bool centre) {
+#pragma unroll
+        for (int m = 0; m < 16; m++) lds[li(jw(m))] = centre ? vh_center(v[m], q, qinv) : v[m];
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = lds[li(jr(m))];
+        __syncthreads();
+    };
+
+    if (DIR == 0) {
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = vh_u2d(vh_ld(in + jA(m)));
+#pragma unroll
+        for (int s = 0; s < 4; s++) ct(s, 8 >> s, [&](int m) { return m >> (4 - s); });
+        exchange(jA, jB, true);
+#pragma unroll
+        for (int s = 4; s < 8; s++) ct(s, 8 >> (s - 4), [&](int m) { return (g << (s - 4)) + (m >> (8 - s)); });
+        exchange(jB, jC, true);
+#pragma unroll
+        for (int s = 8; s < LOGN; s++) ct(s, 1 << (LOGN - s - 1), [&](int m) { return (16 * t + m) >> (LOGN - s); });
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = __longlong_as_double((long long)vh_d2u(vh_canon(v[m], q, qinv)));
+        exchange(jC, jA, false);
+        if (live) {
+#pragma unroll
+            for (int m = 0; m < 16; m++) vh_st(out + jA(m), (uint64_t)__double_as_longlong(v[m]));
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = vh_u2d(vh_ld(in + jA(m)));
+        exchange(jA, jC, false);
+#pragma unroll
+        for (int s = LOGN - 1; s >= 8; s--) gs(s, 1 << (LOGN - s - 1), [&](int m) { return (16 * t + m) >> (LOGN - s); });
+        exchange(jC, jB, false);
+#pragma unroll
+        for (int s = 7; s >= 4; s--) gs(s, 8 >> (s - 4), [&](int m) { return (g << (s - 4)) + (m >> (8 - s)); });
+        exchange(jB, jA, false);
+#pragma unroll
+        for (int s = 3; s >= 0; s--) gs(s, 8 >> s, [&](int m) { return m >> (4 - s); });
+        if (live) {
+#pragma unroll
+            for (int m = 0; m < 16; m++)
+                vh_st(out + jA(m), vh_d2u(vh_canon(vh_mulmod_tw(v[m], L.ninv_d, L.ninv_q, q), q, qinv)));
+        }
+    }
+}
+
+hipError_t vh_launch_ntt_small(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,
+                               int logn)
+{
+    if (logn < 9 || logn > 12 || nitems == 0) return hipErrorInvalidValue;
+    const unsigned per = 4096u >> logn, nb = (nitems + per - 1) / per;
+#define VH_SMALL(LG)                                                                                        \
+    if (logn == LG) {                                                                                       \
+        if (dir == 0) VH_KL((k_ntt_small<0, LG>), dim3(nb), dim3(256), 0, st, items, nitems, limbs);         \
+        else VH_KL((k_ntt_small<1, LG>), dim3(nb), dim3(256), 0, st, items, nitems, limbs);                  \
+    }
+    VH_SMALL(9) VH_SMALL(10) VH_SMALL(11) VH_SMALL(12)
+#undef VH_SMALL
+    return hipGetLastError();
+}
+
 hipError_t vh_launch_ntt_pass(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                               int fp, int dir, int logn, int s0, int R, int col, int G, int first, int last)
 {
