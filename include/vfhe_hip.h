@@ -462,6 +462,17 @@ void vfhe_hip_mgsw_cmux_batch(RNS_MLWE *out, RNSc_MLWE *in1, RNSc_MLWE *in2, RNS
    NTT-form samples of one MGSW encryption), then back to coefficient form.  Results in acc. */
 void vfhe_hip_blind_rotate_batch(RNS_MLWE *acc, uint64_t count, RNS_MLWE **bk, uint64_t n, const uint64_t *steps,
                                  uint64_t ell, uint64_t special_primes);
+/* The same, each accumulator starting from one test vector (cggi16.py:70-76): acc[k] =
+   X^rot_tv[k] * tv (mlwe_RNSc_mul_by_xai; tv in coefficient form), then the blind rotation.
+   At N = 1024 on FP64 limbs with one special prime, every key index runs in one launch
+   with each accumulator resident in its workgroup's LDS. */
+void vfhe_hip_blind_rotate_tv_batch(RNS_MLWE *acc, uint64_t count, RNS_MLWE tv, const uint64_t *rot_tv,
+                                    RNS_MLWE **bk, uint64_t n, const uint64_t *steps, uint64_t ell,
+                                    uint64_t special_primes);
+/* mlwe_extract_LWE (mlwe.c:40-66, mlwe.cdef:87) for `count` coefficient-form samples of one
+   ring, rank and mask: out[k] = the LWE sample of coefficient idx of in[k] (free each with
+   free_lwe_sample); the rows are gathered on the device and read back with one copy. */
+void vfhe_hip_extract_lwe_batch(RNSc_MLWE *in, uint64_t count, uint64_t idx, LWE *out);
 /* The rotation amounts of a CGGI16 blind rotation for `count` LWE samples, replacing the
    per-sample CRT and rounding of cggi16.py:54-86 (host arithmetic): rot_tv[k] = 2N - rot_b
    mod 2N, steps[k*n + i] = round(a_i * 2N / q) mod 2N, q the product of the sample's primes.

@@ -78,6 +78,16 @@ def test_lwe_phase_and_extraction(setup):
     coeffs = ct.decrypt(key)[0].get_polynomial()
     for idx in (0, 1, 100, 255):
         assert extract(ct, idx).phases(s["out_lwe"])[0] == coeffs[idx]
+    # the batched extraction (one read-back) equals mlwe_extract_LWE sample by sample
+    cts = Batch.encrypt(chain, key, [msg, msg, msg])
+    cts.to_coeff()
+    got = extract(cts, 7)
+    lib = _native.load()
+    for k, h in enumerate(cts.handles):
+        one = LweBatch(got.ring, [lib.mlwe_extract_LWE(h, 7)], got.n, got.l)
+        a1, b1 = got.arrays(k)
+        a2, b2 = one.arrays(0)
+        assert np.array_equal(a1, a2) and np.array_equal(b1, b2), k
 
 
 def test_functional_bootstrap(setup):
@@ -153,6 +163,67 @@ def test_rotation_amounts_engine_helper(setup):
         rot, ref = rotation_amounts(*lwes.arrays(k), primes, N, tb)
         assert int(rots[k]) == rot, k
         assert [int(v) for v in steps[k * n:(k + 1) * n]] == ref, k
+
+
+@pytest.fixture(scope="module")
+def setup1024():
+    """N = 1024 (the fused one-launch blind rotation's shape), a short LWE key (n = 40)."""
+    arith.NTT_processor_instance.reset()
+    lib = _native.load()
+    lib.vfhe_prng_set_deterministic_seed(0x1024)
+    rng = np.random.default_rng(0x1024)
+    Rq = Ring(1024, prime_size=[50, 50, 50], split_degree=1)
+    in_ring = Rq.quotient_ring(ell=2)
+    lwe_key = LweKey(in_ring, [int(b) for b in rng.integers(0, 2, 40)])
+    chain = Chain.nested(Rq, special=1, levels=1)
+    out_key = SecretKey(chain, [binary_key(rng, 1024, 64)], 3.2)
+    bk = BootstrapKey(chain, lwe_key, out_key)
+    lut = list(range(1 << (MSG_PREC - 1)))
+    tv = pack_lut(chain, lut, len(lut), MSG_PREC)
+    return dict(in_ring=in_ring, lwe_key=lwe_key, chain=chain, out_key=out_key, bk=bk, lut=lut, tv=tv,
+                boot=Bootstrapper(chain, bk), out_lwe=lwe_key_of(out_key, in_ring))
+
+
+def test_fused_blind_rotation_matches_steps_and_reference_sequence(setup1024):
+    """br.fused = 1 (every key index in one launch, accumulators resident in LDS) against
+    br.fused = 0 (a monomial-product launch and a batched CMUX per key index) and against the
+    reference's one-call-at-a-time sequence, bit for bit; the outputs decrypt to lut[msg]."""
+    s = setup1024
+    lib = _native.load()
+    tb = 1 << (MSG_PREC - 1)
+    msgs = [0, 5, 9, 14, 3]
+    lwes = encrypt_msgs(s, msgs)
+    saved = lib.vfhe_hip_get_option(b"br.fused")
+    got = {}
+    try:
+        for fused in (1, 0):
+            assert lib.vfhe_hip_set_option(b"br.fused", fused) == 0
+            n0 = lib.vfhe_hip_get_option(b"stat.br_fused_launches")
+            print(f"[cggi] N=1024 blind rotation of {len(msgs)} inputs, br.fused={fused}", flush=True)
+            got[fused] = s["boot"].blind_rotate(s["tv"], lwes, tb)
+            assert (lib.vfhe_hip_get_option(b"stat.br_fused_launches") - n0) == fused
+    finally:
+        lib.vfhe_hip_set_option(b"br.fused", saved)
+    for k in range(len(msgs)):
+        for p, r in zip(got[1].polys(k), got[0].polys(k)):
+            assert p == r, k
+    # vfhe_hip_blind_rotate_batch from accumulators already rotated (the fused kernel's other start)
+    rots, steps = s["boot"].amounts(lwes, tb)
+    acc = Batch.alloc(s["chain"], 0, len(msgs), form="coeff")
+    for h, rot in zip(acc.handles, rots):
+        lib.mlwe_RNSc_mul_by_xai(h, s["tv"].handles[0], int(rot))
+    lib.vfhe_hip_blind_rotate_batch(_handles(acc.handles), len(acc), s["bk"].table, s["bk"].n,
+                                    steps.ctypes.data_as(_native.U64P), s["bk"].ell, s["chain"].special)
+    for k in range(len(msgs)):
+        for p, r in zip(acc.polys(k), got[1].polys(k)):
+            assert p == r, k
+    for k in (0, 3):
+        ref = _reference_sequence(s, lwes, k, tb)
+        for p, r in zip(got[1].polys(k), ref.polys(0)):
+            assert p == r, k
+    q = s["in_ring"].q_l
+    phases = extract(got[1], 0).phases(s["out_lwe"])
+    assert [mod_switch(ph, q, 1 << MSG_PREC) for ph in phases] == [s["lut"][v] for v in msgs]
 
 
 def test_bootstrap_batch_shape_n630_N1024():

@@ -6,8 +6,10 @@ test vector, then one CMUX per LWE coefficient, then sample extraction.  Here
 a whole batch of LWE ciphertexts moves through the bootstrapping key together:
 the rotation amounts of every input are computed on the host
 (`Bootstrapper.amounts`, exact 128-bit arithmetic in the engine), the accumulators live in HBM, and
-`vfhe_hip_blind_rotate_batch` steps all of them through key index i with one
-monomial-product launch and one batched CMUX (mgsw.c:51-71) per index.  The
+`vfhe_hip_blind_rotate_tv_batch` starts every accumulator from the test vector and
+steps all of them through the key indices: at N = 1024 in one launch that keeps each
+accumulator in LDS, otherwise with one monomial-product launch and one batched CMUX
+(mgsw.c:51-71) per index.  The
 result of each accumulator is the reference's, bit for bit, since every
 accumulator sees the same sequence of engine operations the reference makes.
 
@@ -121,10 +123,12 @@ class LweBatch:
 
 
 def extract(batch: Batch, idx: int = 0) -> LweBatch:
-    """Coefficient idx of every member's phase as an LWE sample of dimension r*N (mlwe.c:40-66)."""
+    """Coefficient idx of every member's phase as an LWE sample of dimension r*N (mlwe.c:40-66),
+    the whole batch read back at once (vfhe_hip_extract_lwe_batch)."""
     batch.to_coeff()
-    hs = [lib().mlwe_extract_LWE(h, idx) for h in batch.handles]
-    return LweBatch(batch.ring, hs, batch.rank * batch.ring.N, batch.ring.ell)
+    outs = (ctypes.c_void_p * max(1, len(batch)))()
+    lib().vfhe_hip_extract_lwe_batch(_handles(batch.handles), len(batch), idx, outs)
+    return LweBatch(batch.ring, [outs[k] for k in range(len(batch))], batch.rank * batch.ring.N, batch.ring.ell)
 
 
 # ---------------------------------------------------------------------------------------------
@@ -220,10 +224,9 @@ class Bootstrapper:
         rots, steps = self.amounts(lwes, torus_base)
         tv.to_coeff()
         acc = Batch.alloc(chain, 0, len(lwes), form="coeff")
-        for h, rot in zip(acc.handles, rots):
-            lib().mlwe_RNSc_mul_by_xai(h, tv.handles[0], int(rot))
-        lib().vfhe_hip_blind_rotate_batch(_handles(acc.handles), len(acc), self.bk.table, self.bk.n,
-                                          steps.ctypes.data_as(U64P), self.bk.ell, chain.special)
+        lib().vfhe_hip_blind_rotate_tv_batch(_handles(acc.handles), len(acc), tv.handles[0],
+                                             rots.ctypes.data_as(U64P), self.bk.table, self.bk.n,
+                                             steps.ctypes.data_as(U64P), self.bk.ell, chain.special)
         return acc
 
     def amounts(self, lwes: LweBatch, torus_base: int):

@@ -192,6 +192,7 @@ struct vh_opts {
     long mdr_fp64 = 1;        // ModDown + rescale on the FP64 path when every limb allows it
     long ntt_small = 1;       // N = 2^9 .. 2^12 (FP64): whole transforms in registers (k_ntt_small);
                               // 0: the LDS radix-2 kernel
+    long br_fused = 1;        // CGGI blind rotation at N = 1024: every key index in one launch
     long ks_fold = 1;         // multiply: pass-through components enter the fused key inner
                               // product as P * d (NTT form) instead of after the ModDown
     long fft_fma = 1;         // complex FFT: 1 = avx512ifma engine's fused complex product, 0 = portable's
@@ -229,6 +230,7 @@ static void vh_ensure()
     if (const char *e = getenv("VFHE_HIP_MDR_FP64")) OPT.mdr_fp64 = atol(e);
     if (const char *e = getenv("VFHE_HIP_KS_FOLD")) OPT.ks_fold = atol(e);
     if (const char *e = getenv("VFHE_HIP_NTT_SMALL")) OPT.ntt_small = atol(e);
+    if (const char *e = getenv("VFHE_HIP_BR_FUSED")) OPT.br_fused = atol(e);
     if (const char *e = getenv("VFHE_HIP_FFT_FMA")) OPT.fft_fma = atol(e);
     VH_REQUIRE(C.device >= 0 && C.device < 64);
     VH_CHECK(vh_scratch_resolve(&g_scratch_of[C.device]));
@@ -716,6 +718,7 @@ static void vh_check_device_error()
 static long g_col_wide_launches = 0;
 static long g_mdr_fp64_launches = 0;   // launches of the FP64 ModDown + rescale kernel
 static long g_ntt_small_launches = 0;  // launches of the register NTT for N = 2^9 .. 2^12
+static long g_br_fused_launches = 0;   // launches of the one-launch blind rotation
 static long g_ks_fold_jobs = 0;        // multiply jobs whose pass-through went into the key MAC
 static int col_wide(unsigned items, int logn)
 {
@@ -1178,6 +1181,7 @@ extern "C" int vfhe_hip_set_option(const char *key, long value)
     else if (!strcmp(key, "ks.mdr_fp64")) OPT.mdr_fp64 = value;
     else if (!strcmp(key, "ks.fold")) OPT.ks_fold = value;
     else if (!strcmp(key, "ntt.small")) OPT.ntt_small = value;
+    else if (!strcmp(key, "br.fused")) OPT.br_fused = value;
     else if (!strcmp(key, "fft.fma")) OPT.fft_fma = value;
     else return -1;
     return 0;
@@ -1202,6 +1206,8 @@ extern "C" long vfhe_hip_get_option(const char *key)
     if (!strcmp(key, "ks.fold")) return OPT.ks_fold;
     if (!strcmp(key, "ntt.small")) return OPT.ntt_small;
     if (!strcmp(key, "stat.ntt_small_launches")) return g_ntt_small_launches;
+    if (!strcmp(key, "br.fused")) return OPT.br_fused;
+    if (!strcmp(key, "stat.br_fused_launches")) return g_br_fused_launches;
     if (!strcmp(key, "stat.ks_fold_jobs")) return g_ks_fold_jobs;
     if (!strcmp(key, "fft.fma")) return OPT.fft_fma;
     if (!strcmp(key, "stat.col_wide_launches")) return g_col_wide_launches;
@@ -3142,15 +3148,30 @@ extern "C" void vfhe_hip_mgsw_cmux_batch(RNS_MLWE *out, RNSc_MLWE *in1, RNSc_MLW
 // An accumulator with a = 0 gets diff = 0, hence out = 0 and acc unchanged, which is the
 // reference's skip.  Temporaries are allocated once per call; the work lists that do not
 // depend on i are built once.  The results are left in acc (coefficient form).
-extern "C" void vfhe_hip_blind_rotate_batch(RNS_MLWE *acc, uint64_t count, RNS_MLWE **bk, uint64_t n,
-                                            const uint64_t *steps, uint64_t ell, uint64_t special_primes)
+// tv (optional): accumulator k starts as X^rot_tv[k] * tv (mlwe_RNSc_mul_by_xai, coefficient
+// form) instead of holding its start already.
+static void blind_rotate_core(RNS_MLWE *acc, uint64_t count, RNS_MLWE tv, const uint64_t *rot_tv, RNS_MLWE **bk,
+                              uint64_t n, const uint64_t *steps, uint64_t ell, uint64_t special_primes)
 {
-    VH_ENTER;
-    if (count == 0 || n == 0) return;
-    VH_REQUIRE(acc && bk && steps && bk[0] && bk[0][0]);
+    if (count == 0) return;
+    VH_REQUIRE(acc && bk && steps && bk[0] && bk[0][0] && (!tv || rot_tv));
     incNTT t = acc[0]->b->ntt;
     const uint64_t N = t->N, r = acc[0]->r, Cc = r + 1, l = t->l;
     VH_REQUIRE(t->split_degree == 1 && Cc <= VH_MAX_COMPS && count <= VH_MAX_GRID_Y);
+    if (tv) {
+        VH_REQUIRE(tv->r == r && tv->b->ntt == t);
+        const auto tc = comps(tv);
+        for (auto p : tc) dev_ready(p);
+        for (uint64_t k = 0; k < count; k++) {
+            const auto ac = comps(acc[k]);
+            for (uint64_t c = 0; c < Cc; c++) {
+                VH_REQUIRE(ac[c] != tc[c]);
+                ac[c]->rns_mask = tc[c]->rns_mask;
+                check_rows(ac[c], tc[c]->rns_mask);
+            }
+        }
+    }
+    if (n == 0 && !tv) return;
     const uint64_t m = acc[0]->b->rns_mask, ext = bk[0][0]->b->rns_mask;
     for (uint64_t k = 0; k < count; k++) {
         VH_REQUIRE(acc[k]->b->ntt == t && acc[k]->r == r);
@@ -3171,6 +3192,102 @@ extern "C" void vfhe_hip_blind_rotate_batch(RNS_MLWE *acc, uint64_t count, RNS_M
     bool fp = true;   // FP64 limbs only: the lift folds into the NTT and the inner product runs in FP64
     for (int e : E) fp = fp && P->h_limbs[e].fp64;
     for (int j : J) fp = fp && P->h_limbs[j].fp64;
+    if (fp && OPT.br_fused && N == 1024 && Cc <= 2 && divs.size() == 1 && J.size() <= VH_BR_MAXJ &&
+        next == J.size() + 1) {
+        // Every key index in one launch, each accumulator resident in its workgroup's LDS.
+        vh_br_params bp;
+        memset(&bp, 0, sizeof(bp));
+        bp.C = (int)Cc;
+        bp.nJ = (int)J.size();
+        bp.next = (int)next;
+        bp.n = n;
+        const uint64_t p = t->ntt[divs[0]]->q;
+        bp.hp = (double)(p / 2);
+        for (size_t jj = 0; jj < J.size(); jj++) bp.J[jj] = J[jj];
+        for (size_t w = 0; w < next; w++) {
+            bp.E[w] = E[w];
+            bp.jw[w] = -1;
+            for (size_t jj = 0; jj < J.size(); jj++) if (J[jj] == E[w]) bp.jw[w] = (int)jj;
+            if (E[w] == divs[0]) { bp.sp = (int)w; continue; }
+            const uint64_t qe = t->ntt[E[w]]->q, iv = inverse_mod(p % qe, qe);
+            bp.hdiv[w] = (double)((p / 2) % qe);
+            bp.inv[w] = centred(iv, qe);
+            bp.invq[w] = bp.inv[w] / (double)qe;
+        }
+        std::vector<uint64_t *> accp(count * Cc * J.size());
+        for (uint64_t k = 0; k < count; k++) {
+            const auto ac = comps(acc[k]);
+            for (uint64_t c = 0; c < Cc; c++)
+                for (size_t jj = 0; jj < J.size(); jj++) accp[(k * Cc + c) * J.size() + jj] = drow(ac[c], J[jj]);
+        }
+        std::vector<const uint64_t *> bkp(n * nd * Cc * next);
+        for (uint64_t i = 0; i < n; i++) {
+            VH_REQUIRE(bk[i]);
+            for (uint64_t d = 0; d < nd; d++) {
+                const uint64_t src = d / J.size(), jj = d % J.size();
+                RNS_MLWE e = bk[i][src * ell + jj];
+                VH_REQUIRE(e && e->r == r && e->b->ntt == t && (e->b->rns_mask & ext) == ext);
+                for (uint64_t c = 0; c < Cc; c++) {
+                    RNS_Polynomial kp = c < r ? e->a[c] : e->b;
+                    dev_ready(kp);
+                    for (uint64_t w = 0; w < next; w++) bkp[((i * nd + d) * Cc + c) * next + w] = drow(kp, E[w]);
+                }
+            }
+        }
+        std::vector<uint64_t> amounts(std::max<uint64_t>(1, count * n));
+        for (uint64_t k = 0; k < count * n; k++) amounts[k] = steps[k] & ((N << 1) - 1);
+        std::vector<void *> held;
+        auto persist = [&](const void *src, size_t bytes) -> void * {
+            void *d = vh_dmalloc(bytes ? bytes : 8);
+            if (bytes) VH_CHECK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, S()));
+            held.push_back(d);
+            return d;
+        };
+        auto *d_acc = (uint64_t *const *)persist(accp.data(), accp.size() * sizeof(uint64_t *));
+        auto *d_bk = (const uint64_t *const *)persist(bkp.data(), bkp.size() * sizeof(uint64_t *));
+        auto *d_am = (const uint64_t *)persist(amounts.data(), amounts.size() * 8);
+        const uint64_t *const *d_tv = nullptr;
+        const uint64_t *d_rot = nullptr;
+        std::vector<const uint64_t *> tvp;
+        std::vector<uint64_t> rot;
+        if (tv) {
+            const auto tc = comps(tv);
+            for (uint64_t c = 0; c < Cc; c++)
+                for (size_t jj = 0; jj < J.size(); jj++) tvp.push_back(drow(tc[c], J[jj]));
+            for (uint64_t k = 0; k < count; k++) rot.push_back(rot_tv[k] & ((N << 1) - 1));
+            d_tv = (const uint64_t *const *)persist(tvp.data(), tvp.size() * sizeof(uint64_t *));
+            d_rot = (const uint64_t *)persist(rot.data(), rot.size() * 8);
+        }
+        {
+            vh_prof_scope ps_(VH_PROF_KSMAC);
+            VH_LAUNCH(vh_launch_blind_rotate(S(), bp, P->d_limbs, d_acc, d_bk, d_am, (unsigned)count, d_tv, d_rot));
+        }
+        g_br_fused_launches++;
+        VH_CHECK(hipStreamSynchronize(S()));   // the host lists go out of scope
+        for (void *d : held) vh_dfree(d);
+        return;
+    }
+    if (tv) {   // acc[k] = X^rot_tv[k] * tv, one launch (the exponent table holds rot_tv)
+        std::vector<uint64_t> rot(count);
+        for (uint64_t k = 0; k < count; k++) rot[k] = rot_tv[k] & ((N << 1) - 1);
+        const auto tc = comps(tv);
+        std::vector<vh_item> items;
+        for (uint64_t k = 0; k < count; k++) {
+            const auto ac = comps(acc[k]);
+            for (uint64_t c = 0; c < Cc; c++)
+                for_active(tc[c]->rns_mask, l, [&](uint64_t j) {
+                    items.push_back(mk_item(drow(tc[c], j), drow(ac[c], j), (uint32_t)j, nullptr, k, 0, 0u));
+                });
+        }
+        const uint64_t *d_rot = vh_stage_vec(rot);
+        for (size_t off = 0; off < items.size(); off += VH_MAX_GRID_Y) {
+            const unsigned cn = (unsigned)std::min<size_t>(VH_MAX_GRID_Y, items.size() - off);
+            std::vector<vh_item> chunk(items.begin() + off, items.begin() + off + cn);
+            vh_prof_scope ps_(VH_PROF_OTHER);
+            VH_LAUNCH(vh_launch_mul_xai(S(), vh_stage_vec(chunk), cn, P->d_limbs, N, d_rot, 1, 0));
+        }
+        if (n == 0) return;
+    }
     // temporaries
     std::vector<RNS_MLWE> diff(count), out(count);
     for (uint64_t k = 0; k < count; k++) {
@@ -3299,6 +3416,69 @@ extern "C" void vfhe_hip_blind_rotate_batch(RNS_MLWE *acc, uint64_t count, RNS_M
     for (void *d : held) vh_dfree(d);
     vh_dfree(modup);
     for (uint64_t k = 0; k < count; k++) { free_mlwe_RNS_sample(diff[k]); free_mlwe_RNS_sample(out[k]); }
+}
+
+extern "C" void vfhe_hip_blind_rotate_batch(RNS_MLWE *acc, uint64_t count, RNS_MLWE **bk, uint64_t n,
+                                            const uint64_t *steps, uint64_t ell, uint64_t special_primes)
+{
+    VH_ENTER;
+    if (count == 0 || n == 0) return;
+    blind_rotate_core(acc, count, nullptr, nullptr, bk, n, steps, ell, special_primes);
+}
+
+extern "C" void vfhe_hip_blind_rotate_tv_batch(RNS_MLWE *acc, uint64_t count, RNS_MLWE tv, const uint64_t *rot_tv,
+                                               RNS_MLWE **bk, uint64_t n, const uint64_t *steps, uint64_t ell,
+                                               uint64_t special_primes)
+{
+    VH_ENTER;
+    if (count == 0) return;
+    VH_REQUIRE(tv && rot_tv);
+    blind_rotate_core(acc, count, tv, rot_tv, bk, n, steps, ell, special_primes);
+}
+
+// mlwe_extract_LWE (mlwe.c:40-66) for `count` samples of one ring and rank: their rows are
+// gathered on the device and read back with one copy.
+extern "C" void vfhe_hip_extract_lwe_batch(RNSc_MLWE *in, uint64_t count, uint64_t idx, LWE *out)
+{
+    VH_ENTER;
+    if (count == 0) return;
+    VH_REQUIRE(in && out && in[0]);
+    incNTT t = in[0]->a[0]->ntt;
+    const uint64_t N = t->N, r = in[0]->r, mask = in[0]->a[0]->rns_mask, Cc = r + 1;
+    VH_REQUIRE(idx < N && t->split_degree == 1);
+    std::vector<int> act;
+    for (uint64_t i = 0; i < t->l; i++) if (ACTIVE(mask, i)) act.push_back((int)i);
+    const uint64_t na = act.size(), rows = count * na * Cc;
+    uint64_t *buf = (uint64_t *)vh_dmalloc(std::max<uint64_t>(1, rows) * N * 8);
+    std::vector<vh_item> items;
+    items.reserve(rows);
+    for (uint64_t k = 0; k < count; k++) {
+        VH_REQUIRE(in[k] && in[k]->r == r && in[k]->a[0]->ntt == t);
+        const auto ic = comps(in[k]);
+        for (auto p : ic) { VH_REQUIRE(p->rns_mask == mask); dev_ready(p); }
+        for (uint64_t j = 0; j < na; j++)
+            for (uint64_t c = 0; c < Cc; c++)
+                items.push_back(mk_item(drow(ic[c], act[j]), buf + ((k * na + j) * Cc + c) * N, (uint32_t)act[j]));
+    }
+    run_eltwise(items, VPOOL(t)->d_limbs, VH_OP_COPY, N);
+    std::vector<uint64_t> host(rows * N);
+    VH_CHECK(hipMemcpyAsync(host.data(), buf, rows * N * 8, hipMemcpyDeviceToHost, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
+    vh_dfree(buf);
+    for (uint64_t k = 0; k < count; k++) {
+        LWE res = lwe_alloc_sample(r * N, na, t);
+        for (uint64_t j = 0; j < na; j++) {
+            const uint64_t q = t->ntt[act[j]]->q;
+            const uint64_t *base = host.data() + ((k * na + j) * Cc) * N;
+            for (uint64_t c = 0; c < r; c++) {
+                const uint64_t *a = base + c * N;
+                for (uint64_t i = 0; i <= idx; i++) res->a[j][c * N + i] = a[idx - i];
+                for (uint64_t i = idx + 1; i < N; i++) res->a[j][c * N + i] = negate_modq(a[N + idx - i], q);
+            }
+            res->b[j] = base[r * N + idx];
+        }
+        out[k] = res;
+    }
 }
 
 // ---------------------------------------------------------------------------

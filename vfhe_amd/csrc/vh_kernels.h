@@ -92,6 +92,32 @@ struct vh_mac_job {
 // Whole FP64 transforms of N = 2^9 .. 2^12 (reads it.src, writes it.dst), registers radix-16.
 hipError_t vh_launch_ntt_small(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,
                                int logn);
+// Fused CGGI blind rotation (N = 1024, FP64 limbs, one special prime): workgroup k steps
+// accumulator k through every key index with its rows resident in LDS; wave w works modulo
+// extended limb E[w].
+#define VH_BR_MAXE 4
+#define VH_BR_MAXJ 3
+struct vh_br_params {
+    int C;                      // components (rank + 1), 1 or 2
+    int nJ;                     // level limbs (digits per component)
+    int next;                   // extended limbs = waves per workgroup (nJ + 1)
+    int sp;                     // wave of the special limb
+    int E[VH_BR_MAXE];          // extended limbs, ascending
+    int J[VH_BR_MAXJ];          // level limbs, ascending
+    int jw[VH_BR_MAXE];         // wave -> index in J (-1: the special limb)
+    double hdiv[VH_BR_MAXE];    // floor(p/2) mod q_E[w]
+    double inv[VH_BR_MAXE];     // p^-1 mod q_E[w], centred
+    double invq[VH_BR_MAXE];    // RN(inv / q_E[w])
+    double hp;                  // floor(p/2)
+    uint64_t n;                 // key indices
+};
+// accp[(k * C + c) * nJ + jj]: row J[jj] of component c of accumulator k (coefficient form,
+// updated in place); bkp[((i * C * nJ + d) * C + c) * next + w]: key index i, digit d, output
+// component c, limb E[w] (NTT form); amounts[k * n + i] < 2N.  tvp (optional): the C nJ rows of a
+// test vector; accumulator k then starts as X^rot[k] * tv instead of from its own rows.
+hipError_t vh_launch_blind_rotate(hipStream_t st, const vh_br_params &P, const vh_limb *limbs, uint64_t *const *accp,
+                                  const uint64_t *const *bkp, const uint64_t *amounts, unsigned count,
+                                  const uint64_t *const *tvp = nullptr, const uint64_t *rot = nullptr);
 hipError_t vh_launch_ntt_pass(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                               int fp, int dir, int logn, int s0, int R, int col, int G, int first, int last);
 // Register radix-16 pass (FP64 path): 4 <= R <= 8, G = 4096 / 2^R sub-transforms per block.

@@ -1710,14 +1710,99 @@ static inline unsigned vh_blocks(uint64_t n, unsigned per = 256, unsigned cap = 
 // per instruction); the forward pass regroups its output through LDS for that.
 // Bounds as in the two-pass kernels: at most four stages between centrings, |x| < 2^52.
 // ---------------------------------------------------------------------------
+// Register cores of the transform for T = N/16 threads (thread index t < T); lds holds
+// N + N/16 doubles private to the transform.  WAVE: the transform's threads are one wave
+// (or part of one), so the exchanges need only the wave's own ordering.
+template <int LOGN> struct vh_reg_ntt {
+    static constexpr int N = 1 << LOGN, T = N / 16, U = T / 16;
+    static __device__ __forceinline__ int li(int j) { return j + (j >> 4); }
+    static __device__ __forceinline__ int jA(int t, int m) { return t + T * m; }
+    static __device__ __forceinline__ int jB(int t, int m) { return 16 * U * (t / U) + t % U + U * m; }
+    static __device__ __forceinline__ int jC(int t, int m) { return 16 * t + m; }
+    template <bool WAVE> static __device__ __forceinline__ void sync()
+    {
+        if (WAVE) vh_wave_sync();
+        else __syncthreads();
+    }
+    template <bool WAVE, typename FW, typename FR>
+    static __device__ __forceinline__ void exchange(double (&v)[16], double *lds, FW jw, FR jr, bool centre, double q,
+                                                    double qinv)
+    {
+#pragma unroll
+        for (int m = 0; m < 16; m++) lds[li(jw(m))] = centre ? vh_center(v[m], q, qinv) : v[m];
+        sync<WAVE>();
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = lds[li(jr(m))];
+        sync<WAVE>();
+    }
+    // forward: v in the stride ownership (any |v| < 2^51) -> contiguous ownership, |v| < 2^52
+    template <bool WAVE>
+    static __device__ __forceinline__ void fwd(double (&v)[16], double *lds, int t, const VH_GLOBAL double *tw,
+                                               double q, double qinv)
+    {
+        const int g = t / U;
+        auto ct = [&](int s, int hm, auto blk) {
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                if (m & hm) continue;
+                const int i = (1 << s) + blk(m);
+                const double x = vh_mulmod_tw(v[m + hm], tw[2 * i], tw[2 * i + 1], q);
+                const double a = v[m];
+                v[m] = a + x;
+                v[m + hm] = a - x;
+            }
+        };
+#pragma unroll
+        for (int s = 0; s < 4; s++) ct(s, 8 >> s, [&](int m) { return m >> (4 - s); });
+        exchange<WAVE>(v, lds, [&](int m) { return jA(t, m); }, [&](int m) { return jB(t, m); }, true, q, qinv);
+#pragma unroll
+        for (int s = 4; s < 8; s++) ct(s, 8 >> (s - 4), [&](int m) { return (g << (s - 4)) + (m >> (8 - s)); });
+        exchange<WAVE>(v, lds, [&](int m) { return jB(t, m); }, [&](int m) { return jC(t, m); }, true, q, qinv);
+#pragma unroll
+        for (int s = 8; s < LOGN; s++) ct(s, 1 << (LOGN - s - 1), [&](int m) { return (16 * t + m) >> (LOGN - s); });
+    }
+    // inverse (before the n^-1 factor): v in the contiguous ownership (|v| <= q) -> stride
+    // ownership, |v| < 2^52; GS sums re-centred after every second stage
+    template <bool WAVE>
+    static __device__ __forceinline__ void inv(double (&v)[16], double *lds, int t, const VH_GLOBAL double *tw,
+                                               double q, double qinv)
+    {
+        const int g = t / U;
+        int c = 0;
+        auto gs = [&](int s, int hm, auto blk) {
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                if (m & hm) continue;
+                const int i = (1 << s) + blk(m);
+                const double a = v[m], x = v[m + hm];
+                double sum = a + x;
+                if (c & 1) sum = vh_center(sum, q, qinv);
+                v[m] = sum;
+                v[m + hm] = vh_mulmod_tw(a - x, tw[2 * i], tw[2 * i + 1], q);
+            }
+            c++;
+        };
+#pragma unroll
+        for (int s = LOGN - 1; s >= 8; s--) gs(s, 1 << (LOGN - s - 1), [&](int m) { return (16 * t + m) >> (LOGN - s); });
+        exchange<WAVE>(v, lds, [&](int m) { return jC(t, m); }, [&](int m) { return jB(t, m); }, false, q, qinv);
+#pragma unroll
+        for (int s = 7; s >= 4; s--) gs(s, 8 >> (s - 4), [&](int m) { return (g << (s - 4)) + (m >> (8 - s)); });
+        exchange<WAVE>(v, lds, [&](int m) { return jB(t, m); }, [&](int m) { return jA(t, m); }, false, q, qinv);
+#pragma unroll
+        for (int s = 3; s >= 0; s--) gs(s, 8 >> s, [&](int m) { return m >> (4 - s); });
+    }
+};
+
 template <int DIR, int LOGN>
 __global__ void __launch_bounds__(256)
 k_ntt_small(const vh_item *__restrict__ items, unsigned nitems, const vh_limb *__restrict__ limbs)
 {
-    constexpr int N = 1 << LOGN, T = N / 16, U = T / 16, PER = 256 / T, PAD = N + N / 16;
+    using R = vh_reg_ntt<LOGN>;
+    constexpr int N = R::N, T = R::T, PER = 256 / T, PAD = N + N / 16;
+    constexpr bool WAVE = T <= 64;
     static_assert(LOGN >= 9 && LOGN <= 12, "k_ntt_small: N = 2^9 .. 2^12");
     __shared__ double lds_all[PER * PAD];
-    const int tid = threadIdx.x, slot = tid / T, t = tid % T, g = t / U, u = t % U;
+    const int tid = threadIdx.x, slot = tid / T, t = tid % T;
     const unsigned idx = blockIdx.x * PER + slot;
     const bool live = idx < nitems;
     // a slot past the end transforms the last item again (every thread reaches the
@@ -1726,87 +1811,179 @@ k_ntt_small(const vh_item *__restrict__ items, unsigned nitems, const vh_limb *_
     const vh_limb &L = limbs[it.limb];
     const double q = L.qd, qinv = L.qinv;
     const VH_GLOBAL double *tw = vh_global(reinterpret_cast<const double *>(DIR == 0 ? L.tw_fwd : L.tw_inv));
-    auto twl = [&](int i) { return make_double2(tw[2 * i], tw[2 * i + 1]); };
     double *lds = lds_all + slot * PAD;
-    auto li = [](int j) { return j + (j >> 4); };
-    auto jA = [&](int m) { return t + T * m; };
-    auto jB = [&](int m) { return 16 * U * g + u + U * m; };
-    auto jC = [&](int m) { return 16 * t + m; };
     const VH_GLOBAL uint64_t *in = vh_global(it.src);
     VH_GLOBAL uint64_t *out = vh_global(it.dst);
     double v[16];
-
-    // forward CT stage s over register pairs (m, m + hm); blk(m) = block index of element m
-    auto ct = [&](int s, int hm, auto blk) {
 #pragma unroll
-        for (int m = 0; m < 16; m++) {
-            if (m & hm) continue;
-            const double2 w = twl((1 << s) + blk(m));
-            const double x = vh_mulmod_tw(v[m + hm], w.x, w.y, q);
-            const double a = v[m];
-            v[m] = a + x;
-            v[m + hm] = a - x;
-        }
-    };
-    int c = 0;   // inverse: GS sums are re-centred after every second stage
-    auto gs = [&](int s, int hm, auto blk) {
-#pragma unroll
-        for (int m = 0; m < 16; m++) {
-            if (m & hm) continue;
-            const double2 w = twl((1 << s) + blk(m));
-            const double a = v[m], x = v[m + hm];
-            double sum = a + x;
-            if (c & 1) sum = vh_center(sum, q, qinv);
-            v[m] = sum;
-            v[m + hm] = vh_mulmod_tw(a - x, w.x, w.y, q);
-        }
-        c++;
-    };
-    auto exchange = [&](auto jw, auto jr, bool centre) {
-#pragma unroll
-        for (int m = 0; m < 16; m++) lds[li(jw(m))] = centre ? vh_center(v[m], q, qinv) : v[m];
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < 16; m++) v[m] = lds[li(jr(m))];
-        __syncthreads();
-    };
-
+    for (int m = 0; m < 16; m++) v[m] = vh_u2d(vh_ld(in + R::jA(t, m)));
     if (DIR == 0) {
-#pragma unroll
-        for (int m = 0; m < 16; m++) v[m] = vh_u2d(vh_ld(in + jA(m)));
-#pragma unroll
-        for (int s = 0; s < 4; s++) ct(s, 8 >> s, [&](int m) { return m >> (4 - s); });
-        exchange(jA, jB, true);
-#pragma unroll
-        for (int s = 4; s < 8; s++) ct(s, 8 >> (s - 4), [&](int m) { return (g << (s - 4)) + (m >> (8 - s)); });
-        exchange(jB, jC, true);
-#pragma unroll
-        for (int s = 8; s < LOGN; s++) ct(s, 1 << (LOGN - s - 1), [&](int m) { return (16 * t + m) >> (LOGN - s); });
+        R::template fwd<WAVE>(v, lds, t, tw, q, qinv);
 #pragma unroll
         for (int m = 0; m < 16; m++) v[m] = __longlong_as_double((long long)vh_d2u(vh_canon(v[m], q, qinv)));
-        exchange(jC, jA, false);
+        R::template exchange<WAVE>(v, lds, [&](int m) { return R::jC(t, m); }, [&](int m) { return R::jA(t, m); },
+                                   false, q, qinv);
         if (live) {
 #pragma unroll
-            for (int m = 0; m < 16; m++) vh_st(out + jA(m), (uint64_t)__double_as_longlong(v[m]));
+            for (int m = 0; m < 16; m++) vh_st(out + R::jA(t, m), (uint64_t)__double_as_longlong(v[m]));
         }
     } else {
-#pragma unroll
-        for (int m = 0; m < 16; m++) v[m] = vh_u2d(vh_ld(in + jA(m)));
-        exchange(jA, jC, false);
-#pragma unroll
-        for (int s = LOGN - 1; s >= 8; s--) gs(s, 1 << (LOGN - s - 1), [&](int m) { return (16 * t + m) >> (LOGN - s); });
-        exchange(jC, jB, false);
-#pragma unroll
-        for (int s = 7; s >= 4; s--) gs(s, 8 >> (s - 4), [&](int m) { return (g << (s - 4)) + (m >> (8 - s)); });
-        exchange(jB, jA, false);
-#pragma unroll
-        for (int s = 3; s >= 0; s--) gs(s, 8 >> s, [&](int m) { return m >> (4 - s); });
+        R::template exchange<WAVE>(v, lds, [&](int m) { return R::jA(t, m); }, [&](int m) { return R::jC(t, m); },
+                                   false, q, qinv);
+        R::template inv<WAVE>(v, lds, t, tw, q, qinv);
         if (live) {
 #pragma unroll
             for (int m = 0; m < 16; m++)
-                vh_st(out + jA(m), vh_d2u(vh_canon(vh_mulmod_tw(v[m], L.ninv_d, L.ninv_q, q), q, qinv)));
+                vh_st(out + R::jA(t, m), vh_d2u(vh_canon(vh_mulmod_tw(v[m], L.ninv_d, L.ninv_q, q), q, qinv)));
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// CGGI16 blind rotation of a batch, every key index in one launch (cggi16.py:70-90 with
+// mgsw.c:51-71 per index).  N = 1024, FP64 limbs, one special prime.  Workgroup k holds
+// accumulator k's level rows in LDS (coefficient form) for the whole loop; wave w works
+// modulo extended limb E[w].  Per key index i with a = amounts[k n + i] != 0:
+//   digit d = (component c, level limb J[jj]):  (X^a - 1) acc_c  at limb J[jj], canonical,
+//     read from LDS straight into the forward transform's registers (the reference's
+//     mlwe_RNSc_mul_by_xai, subtraction and gadget lift; the lift is the identity on
+//     values below q_J[jj] < 2^50, which the FP64 transform accepts for any E[w]);
+//   out_c' += NTT_E[w](digit) * bk[i][d] component c' (FP64 products, centred every 4th
+//     digit), the 16 key words per lane contiguous;
+//   then per c': inverse transform, the special wave publishes t = (x + floor(p/2)) mod p
+//     and every level wave adds (x + floor(p/2) - t) p^-1 mod q to its accumulator row
+//     (round division by p, then the CMUX's  acc + ExtProd(X^a acc - acc)).
+// A zero exponent leaves the accumulator as it is (the reference skips that index).
+// Every wave takes part in every barrier; the exponent is uniform over the workgroup.
+// ---------------------------------------------------------------------------
+template <int C, int NJ>
+__global__ void __launch_bounds__(256, 2)
+k_blind_rotate(const vh_br_params P, const vh_limb *__restrict__ limbs, uint64_t *const *__restrict__ accp,
+               const uint64_t *const *__restrict__ bkp, const uint64_t *__restrict__ amounts,
+               const uint64_t *const *__restrict__ tvp, const uint64_t *__restrict__ rot)
+{
+    using R = vh_reg_ntt<10>;
+    constexpr int N = 1024, PAD = N + N / 16, nJ = NJ, next = NJ + 1, nd = C * NJ;
+    // C NJ rows of the accumulator, one exchange buffer per wave, the special limb's t:
+    // 65.5 KiB for C = 2, NJ = 2 (two workgroups per CU)
+    __shared__ double dyn[C * NJ * N + next * PAD + N];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned k = blockIdx.x;
+    double *accd = dyn;                           // [C][nJ][N]
+    double *scratch = accd + C * nJ * N + w * PAD; // this wave's exchange buffer
+    double *spd = dyn + C * nJ * N + next * PAD;   // [N]: the special limb's t of one component
+    // the accumulator's rows (every wave helps): its own, or X^rot[k] * tv (the initial
+    // rotation of the test vector, mlwe_RNSc_mul_by_xai: coefficient u = (x - a) mod 2N
+    // of tv, negated when u >= N)
+    for (int r = 0; r < C * nJ; r++) {
+        if (tvp) {
+            const VH_GLOBAL uint64_t *src = vh_global(tvp[r]);
+            const uint32_t a = (uint32_t)(rot[k] & (2 * N - 1));
+            const double qr = limbs[P.J[r % nJ]].qd;
+            for (int x = threadIdx.x; x < N; x += blockDim.x) {
+                const uint32_t u = (uint32_t)(x + 2 * N - a) & (2 * N - 1);
+                const double sv = vh_u2d(src[u & (N - 1)]);
+                accd[r * N + x] = (u < N || sv == 0.0) ? sv : qr - sv;
+            }
+        } else {
+            const VH_GLOBAL uint64_t *src = vh_global(accp[(size_t)k * C * nJ + r]);
+            for (int x = threadIdx.x; x < N; x += blockDim.x) accd[r * N + x] = vh_u2d(src[x]);
+        }
+    }
+    __syncthreads();
+    const vh_limb &L = limbs[P.E[w]];
+    const double q = L.qd, qinv = L.qinv;
+    const VH_GLOBAL double *twf = vh_global(reinterpret_cast<const double *>(L.tw_fwd));
+    const VH_GLOBAL double *twi = vh_global(reinterpret_cast<const double *>(L.tw_inv));
+    const VH_GLOBAL uint64_t *am = vh_global(amounts + (size_t)k * P.n);
+    for (uint64_t i = 0; i < P.n; i++) {
+        const uint32_t a = (uint32_t)(am[i] & (2 * N - 1));
+        if (a == 0) continue;
+        double out[C][16];
+#pragma unroll
+        for (int c = 0; c < C; c++)
+#pragma unroll
+            for (int m = 0; m < 16; m++) out[c][m] = 0.0;
+        for (int d = 0; d < nd; d++) {
+            const int c = d / nJ, jj = d - c * nJ;
+            const double qj = limbs[P.J[jj]].qd;
+            const double *row = accd + (c * nJ + jj) * N;
+            double v[16];
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                const int x = R::jA(lane, m);
+                const uint32_t u = (uint32_t)(x + 2 * N - a) & (2 * N - 1);
+                const double s = row[u & (N - 1)];
+                double r = (u < N ? s : -s) - row[x];     // in (-2 q_j, q_j)
+                r = r < 0.0 ? r + qj : r;
+                v[m] = r < 0.0 ? r + qj : r;
+            }
+            R::template fwd<true>(v, scratch, lane, twf, q, qinv);
+            const uint64_t *const *kp = bkp + ((i * nd + d) * C) * next + w;
+#pragma unroll
+            for (int c2 = 0; c2 < C; c2++) {
+                const VH_GLOBAL uint64_t *kr = vh_global(kp[(size_t)c2 * next]) + 16 * lane;
+                uint64_t kw[16];
+#pragma unroll
+                for (int m = 0; m < 16; m++) kw[m] = kr[m];
+#pragma unroll
+                for (int m = 0; m < 16; m++) {
+                    const double x = vh_center(v[m], q, qinv);
+                    const double kk = vh_u2d(kw[m]);
+                    const double hi = x * kk;
+                    const double lo = __fma_rn(x, kk, -hi);
+                    const double qt = __fma_rn(x, kk * qinv, VH_MAGIC) - VH_MAGIC;
+                    const double r = __fma_rn(-qt, q, hi) + lo;   // |r| <= q + 2^47
+                    out[c2][m] = (d & 3) == 3 ? vh_center(out[c2][m] + r, q, qinv) : out[c2][m] + r;
+                }
+            }
+        }
+        __syncthreads();   // every wave has read the accumulator rows of this index
+#pragma unroll
+        for (int c2 = 0; c2 < C; c2++) {
+            double v[16];
+#pragma unroll
+            for (int m = 0; m < 16; m++) v[m] = vh_center(out[c2][m], q, qinv);
+            R::template inv<true>(v, scratch, lane, twi, q, qinv);
+#pragma unroll
+            for (int m = 0; m < 16; m++) v[m] = vh_canon(vh_mulmod_tw(v[m], L.ninv_d, L.ninv_q, q), q, qinv);
+            if (w == P.sp) {
+#pragma unroll
+                for (int m = 0; m < 16; m++) spd[R::jA(lane, m)] = vh_canon(v[m] + P.hp, q, qinv);
+            }
+            __syncthreads();
+            if (w != P.sp) {
+                double *row = accd + (c2 * nJ + P.jw[w]) * N;
+                const double h = P.hdiv[w], iv = P.inv[w], ivq = P.invq[w];
+#pragma unroll
+                for (int m = 0; m < 16; m++) {
+                    const int x = R::jA(lane, m);
+                    const double y = vh_mulmod_tw(v[m] + h - spd[x], iv, ivq, q);
+                    row[x] = vh_canon(row[x] + y, q, qinv);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int r = 0; r < C * nJ; r++) {
+        VH_GLOBAL uint64_t *dst = vh_global(accp[(size_t)k * C * nJ + r]);
+        for (int x = threadIdx.x; x < N; x += blockDim.x) dst[x] = vh_d2u(accd[r * N + x]);
+    }
+}
+
+hipError_t vh_launch_blind_rotate(hipStream_t st, const vh_br_params &P, const vh_limb *limbs, uint64_t *const *accp,
+                                  const uint64_t *const *bkp, const uint64_t *amounts, unsigned count,
+                                  const uint64_t *const *tvp, const uint64_t *rot)
+{
+    if (P.C < 1 || P.C > 2 || P.nJ < 1 || P.nJ > VH_BR_MAXJ || P.next != P.nJ + 1 || P.next > VH_BR_MAXE ||
+        P.sp < 0 || P.sp >= P.next || count == 0 || (tvp && !rot))
+        return hipErrorInvalidValue;
+    const dim3 grid(count), block(64 * P.next);
+#define VH_BR(CC, JJ)                                                                                      \
+    if (P.C == CC && P.nJ == JJ) VH_KL((k_blind_rotate<CC, JJ>), grid, block, 0, st, P, limbs, accp, bkp, amounts, tvp, rot);
+    VH_BR(1, 1) VH_BR(1, 2) VH_BR(1, 3) VH_BR(2, 1) VH_BR(2, 2) VH_BR(2, 3)
+#undef VH_BR
+    return hipGetLastError();
 }
 
 hipError_t vh_launch_ntt_small(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,
