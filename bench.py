@@ -528,8 +528,21 @@ def bench_cggi(lib, args):
         times.append(t2 - t0)
         del acc, out
     t = float(np.median(times))
+    # where a batch's time goes: the host's rotation amounts, the device's kernels, the extraction
+    t0 = time.perf_counter()
+    boot.amounts(lwes, 1 << (prec - 1))
+    amounts_ms = (time.perf_counter() - t0) * 1e3
+    with Prof(lib) as prof:
+        acc = boot.blind_rotate(tv, lwes, 1 << (prec - 1))
+        lib.vfhe_hip_synchronize()
+    t0 = time.perf_counter()
+    extract(acc, 0)
+    extract_ms = (time.perf_counter() - t0) * 1e3
+    del acc
     res = {"unit": "bootstraps/s", "value": round(count / t, 2), "ms_per_batch": round(t * 1e3, 2),
            "blind_rotation_ms_per_batch": round(float(np.median(rot_times)) * 1e3, 2), "batch": count,
+           "split_ms": {"host_rotation_amounts": round(amounts_ms, 2), "device_kernels": round(sum(prof.ms), 2),
+                        "extraction": round(extract_ms, 2)},
            "decrypt_errors": errors, "bootstrap_key_s": round(keygen_s, 2),
            "workload": "CGGI16 functional bootstrap (RNS variant of the reference, cggi16.py:46-140): LWE n=630 "
                        "(binary key), GLWE N=1024 rank 1 over two 50-bit primes + one special, gadget = one digit "

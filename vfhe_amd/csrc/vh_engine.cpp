@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <thread>
 #include <map>
 #include <tuple>
 #include <mutex>
@@ -4516,57 +4517,74 @@ extern "C" int64_t vfhe_hip_cggi_rotations(LWE *in, uint64_t count, uint64_t N, 
                                            uint64_t *rot_tv, uint64_t *steps)
 {
     VH_REQUIRE(in && rot_tv && steps && torus_base > 0 && N > 0);
-    int64_t marked = 0;
-    for (uint64_t k = 0; k < count; k++) {
-        LWE c = in[k];
-        VH_REQUIRE(c && c->l >= 1 && c->ntt && c->l <= c->ntt->l);
-        // q and the CRT basis of the sample's limbs
-        u128 q = 1;
-        int bits = 0;
-        for (uint64_t j = 0; j < c->l; j++) {
-            const uint64_t p = c->ntt->ntt[j]->q;
-            bits += 64 - __builtin_clzll(p);
-            // (value + q/4) * 2N must stay below 2^128
-            if (bits + 1 + (64 - __builtin_clzll(2 * N)) > 127) return -1;
-            q *= p;
-        }
-        // Garner's mixed-radix recomposition: x = sum_j c_j R_j, R_j = q_0 ... q_{j-1},
-        // c_j = (x_j - (sum_{i<j} c_i R_i mod q_j)) * R_j^-1 mod q_j; every product fits 128 bits
-        std::vector<uint64_t> qs(c->l), rinv(c->l);
-        std::vector<u128> R(c->l);
-        for (uint64_t j = 0; j < c->l; j++) qs[j] = c->ntt->ntt[j]->q;
-        for (uint64_t j = 0; j < c->l; j++) {
-            R[j] = j ? R[j - 1] * qs[j - 1] : 1;
-            rinv[j] = inverse_mod((uint64_t)(R[j] % qs[j]), qs[j]);
-        }
-        std::vector<uint64_t> cj(c->l);
-        auto crt = [&](auto limb) -> u128 {
-            u128 x = 0;
+    for (uint64_t k = 0; k < count; k++) VH_REQUIRE(in[k] && in[k]->l >= 1 && in[k]->ntt && in[k]->l <= in[k]->ntt->l);
+    // samples are independent: split over host threads (exact integer work, any order)
+    auto range = [&](uint64_t k0, uint64_t k1) -> int64_t {
+        int64_t marked = 0;
+        for (uint64_t k = k0; k < k1; k++) {
+            LWE c = in[k];
+            // q and the CRT basis of the sample's limbs
+            u128 q = 1;
+            int bits = 0;
             for (uint64_t j = 0; j < c->l; j++) {
-                const uint64_t qj = qs[j];
-                u128 vmod = 0, w = 1;
-                for (uint64_t i = 0; i < j; i++) {
-                    vmod = (vmod + (u128)cj[i] * (uint64_t)w) % qj;
-                    w = w * qs[i] % qj;
-                }
-                const uint64_t d = (uint64_t)(((u128)(limb(j) % qj) + qj - (uint64_t)vmod) % qj);
-                cj[j] = (uint64_t)((u128)d * rinv[j] % qj);
-                x += (u128)cj[j] * R[j];
+                const uint64_t p = c->ntt->ntt[j]->q;
+                bits += 64 - __builtin_clzll(p);
+                // (value + q/4) * 2N must stay below 2^128
+                if (bits + 1 + (64 - __builtin_clzll(2 * N)) > 127) return -1;
+                q *= p;
             }
-            return x;
-        };
-        bool amb = false;
-        const u128 b = crt([&](uint64_t j) { return c->b[j]; });
-        const u128 off = q / (4 * torus_base);
-        const uint64_t rb = round_2n_over_q(b + off, q, N, &amb);
-        rot_tv[k] = amb ? UINT64_MAX : (2 * N - rb) % (2 * N);
-        marked += amb;
-        for (uint64_t i = 0; i < c->n; i++) {
-            bool ai = false;
-            const u128 a = crt([&](uint64_t j) { return c->a[j][i]; });
-            steps[k * c->n + i] = round_2n_over_q(a, q, N, &ai);
-            marked += ai;
+            // Garner's mixed-radix recomposition: x = sum_j c_j R_j, R_j = q_0 ... q_{j-1},
+            // c_j = (x_j - (sum_{i<j} c_i R_i mod q_j)) * R_j^-1 mod q_j; every product fits 128 bits
+            std::vector<uint64_t> qs(c->l), rinv(c->l);
+            std::vector<u128> R(c->l);
+            for (uint64_t j = 0; j < c->l; j++) qs[j] = c->ntt->ntt[j]->q;
+            for (uint64_t j = 0; j < c->l; j++) {
+                R[j] = j ? R[j - 1] * qs[j - 1] : 1;
+                rinv[j] = inverse_mod((uint64_t)(R[j] % qs[j]), qs[j]);
+            }
+            std::vector<uint64_t> cj(c->l);
+            auto crt = [&](auto limb) -> u128 {
+                u128 x = 0;
+                for (uint64_t j = 0; j < c->l; j++) {
+                    const uint64_t qj = qs[j];
+                    u128 vmod = 0, w = 1;
+                    for (uint64_t i = 0; i < j; i++) {
+                        vmod = (vmod + (u128)cj[i] * (uint64_t)w) % qj;
+                        w = w * qs[i] % qj;
+                    }
+                    const uint64_t d = (uint64_t)(((u128)(limb(j) % qj) + qj - (uint64_t)vmod) % qj);
+                    cj[j] = (uint64_t)((u128)d * rinv[j] % qj);
+                    x += (u128)cj[j] * R[j];
+                }
+                return x;
+            };
+            bool amb = false;
+            const u128 b = crt([&](uint64_t j) { return c->b[j]; });
+            const u128 off = q / (4 * torus_base);
+            const uint64_t rb = round_2n_over_q(b + off, q, N, &amb);
+            rot_tv[k] = amb ? UINT64_MAX : (2 * N - rb) % (2 * N);
+            marked += amb;
+            for (uint64_t i = 0; i < c->n; i++) {
+                bool ai = false;
+                const u128 a = crt([&](uint64_t j) { return c->a[j][i]; });
+                steps[k * c->n + i] = round_2n_over_q(a, q, N, &ai);
+                marked += ai;
+            }
         }
+        return marked;
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const uint64_t nt = std::min<uint64_t>(hw, (count + 31) / 32);
+    if (nt <= 1) return range(0, count);
+    std::vector<int64_t> res(nt, 0);
+    std::vector<std::thread> th;
+    for (uint64_t w = 0; w < nt; w++)
+        th.emplace_back([&, w] { res[w] = range(count * w / nt, count * (w + 1) / nt); });
+    for (auto &x : th) x.join();
+    int64_t marked = 0;
+    for (int64_t r : res) {
+        if (r < 0) return -1;
+        marked += r;
     }
     return marked;
 }
