@@ -322,6 +322,10 @@ def main():
 
     serve = None
     if not args.no_serving:
+        # the earlier legs' device blocks (ModUp buffers, bootstrap temporaries) go back to the
+        # allocator, so the serving mix starts as it would in its own process
+        lib.vfhe_hip_synchronize()
+        lib.vfhe_hip_release_cached()
         serve = bench_serving(lib, args)
 
     cpu = None

@@ -517,6 +517,13 @@ void *vfhe_hip_current_stream(void);
    (1..7) engine lanes; lanes beyond the fourth take a batch only while at most 8
    requests wait (heavy load grows the batches on the first four instead). */
 void *vfhe_hip_server_new(uint64_t memory_budget, uint32_t lanes, uint32_t max_batch);
+/* Returns every device block parked in the engine's block cache to the allocator (each
+   freed in order after its lane's last use) and the bytes released.  The cache keeps at
+   most the option `cache.max_mb` (default 32 GiB), evicting the least recently parked
+   blocks first, so blocks of sizes an earlier workload used give way to the current
+   one's; this call empties it at once and trims the device's stream-ordered pool (it waits
+   for every lane), e.g. before a long-lived request mix. */
+uint64_t vfhe_hip_release_cached(void);
 /* Thread-safe; returns the request id (>= 1), or 0 for an invalid request.
    Inputs and keys must stay alive and unchanged until the request completes.
    Inputs may still be being written by work the caller enqueued on its current

@@ -351,19 +351,25 @@ struct vh_prof_scope {
 // at once (the reuse is ordered after the free by the stream, as with
 // hipFreeAsync + hipMallocAsync on one stream), from another lane once the
 // free's event has completed.  Size classes: powers of two up to 1 MiB, then
-// multiples of 1 MiB.  At most VH_CACHE_MAX bytes stay parked; beyond that a
-// freed block goes back to the allocator.  (hipMallocAsync / hipFreeAsync cost
-// tens to hundreds of microseconds per call in a serving loop.)
-static const size_t VH_CACHE_MAX = (size_t)32 << 30;
+// multiples of 1 MiB.  At most `cache.max_mb` (default 32 GiB) stays parked; a
+// free that would exceed it first returns the least recently parked blocks to
+// the allocator (hipFreeAsync on the lane that parked them, so the free stays
+// ordered after that lane's last use), so a workload that changes its block
+// sizes does not keep missing a cache full of the old sizes.  (hipMallocAsync /
+// hipFreeAsync cost tens to hundreds of microseconds per call in a serving loop.)
+static size_t g_cache_max = (size_t)32 << 30;
 struct vh_parked {
     void *p;
     hipEvent_t ev;
+    uint64_t seq;                       // park order (least recently parked first out)
 };
 struct vh_block_cache {
     std::map<size_t, std::vector<vh_parked>> free_[VH_MAX_LANES];
     std::map<void *, size_t> size_of;   // every live or parked block -> its class size
+    std::map<uint64_t, std::pair<int, size_t>> order;   // seq -> (lane, class) of a parked block
     std::vector<hipEvent_t> events;     // idle events
     size_t cached = 0;
+    uint64_t seq = 0;
 };
 static vh_block_cache BC;
 // Allocation counters (vfhe_hip_get_option "stat.cache_*"): blocks handed out from the
@@ -398,12 +404,36 @@ static void *vh_take_parked(int ln, size_t c)
             }
             const vh_parked b = v[k];
             v.erase(v.begin() + (long)k);
+            BC.order.erase(b.seq);
             BC.events.push_back(b.ev);
             BC.cached -= it->first;
             return b.p;
         }
     }
     return nullptr;
+}
+
+// Returns the least recently parked block to the allocator; false when nothing is parked.
+static bool vh_evict_oldest()
+{
+    if (BC.order.empty()) return false;
+    const auto o = BC.order.begin();
+    const uint64_t seq = o->first;
+    const int ln = o->second.first;
+    const size_t c = o->second.second;
+    BC.order.erase(o);
+    auto &v = BC.free_[ln][c];
+    for (size_t k = 0; k < v.size(); k++) {
+        if (v[k].seq != seq) continue;
+        VH_CHECK(hipFreeAsync(v[k].p, C.lanes[ln].stream));   // after the lane's last use of it
+        BC.size_of.erase(v[k].p);
+        BC.events.push_back(v[k].ev);
+        v.erase(v.begin() + (long)k);
+        break;
+    }
+    if (v.empty()) BC.free_[ln].erase(c);
+    BC.cached -= c;
+    return true;
 }
 
 static void *vh_dmalloc(size_t bytes)
@@ -443,6 +473,7 @@ static void *vh_dmalloc(size_t bytes)
             for (auto &kv : lane)
                 for (auto &b : kv.second) { VH_CHECK(hipFree(b.p)); BC.size_of.erase(b.p); BC.events.push_back(b.ev); }
         for (auto &lane : BC.free_) lane.clear();
+        BC.order.clear();
         BC.cached = 0;
         VH_CHECK(hipMallocAsync(&p, c, S()));
     }
@@ -457,12 +488,15 @@ static void vh_dfree(void *p)
     auto it = BC.size_of.find(p);
     VH_REQUIRE(it != BC.size_of.end());
     const size_t c = it->second;
-    if (BC.cached + c <= VH_CACHE_MAX) {
+    while (c <= g_cache_max && BC.cached + c > g_cache_max && vh_evict_oldest()) {}
+    if (BC.cached + c <= g_cache_max) {
         hipEvent_t ev;
         if (!BC.events.empty()) { ev = BC.events.back(); BC.events.pop_back(); }
         else VH_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         VH_CHECK(hipEventRecord(ev, S()));
-        BC.free_[g_lane][c].push_back({p, ev});
+        const uint64_t seq = ++BC.seq;
+        BC.free_[g_lane][c].push_back({p, ev, seq});
+        BC.order[seq] = {g_lane, c};
         BC.cached += c;
     } else {
         BC.size_of.erase(it);
@@ -1183,9 +1217,28 @@ extern "C" int vfhe_hip_set_option(const char *key, long value)
     else if (!strcmp(key, "ks.fold")) OPT.ks_fold = value;
     else if (!strcmp(key, "ntt.small")) OPT.ntt_small = value;
     else if (!strcmp(key, "br.fused")) OPT.br_fused = value;
+    else if (!strcmp(key, "cache.max_mb")) {
+        if (value < 0) return -1;
+        g_cache_max = (size_t)value << 20;
+        while (BC.cached > g_cache_max && vh_evict_oldest()) {}
+    }
     else if (!strcmp(key, "fft.fma")) OPT.fft_fma = value;
     else return -1;
     return 0;
+}
+
+extern "C" uint64_t vfhe_hip_release_cached(void)
+{
+    VH_ENTER;
+    const uint64_t bytes = BC.cached;
+    while (vh_evict_oldest()) {}
+    // and the stream-ordered pool's unused memory: after large, varied earlier work its free
+    // space is fragmented and every later allocation searches it (measured ~1.6 ms per miss)
+    sync_all_lanes();
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, C.device) == hipSuccess) VH_CHECK(hipMemPoolTrimTo(pool, 0));
+    (void)hipGetLastError();
+    return bytes;
 }
 
 extern "C" long vfhe_hip_get_option(const char *key)
@@ -1209,6 +1262,7 @@ extern "C" long vfhe_hip_get_option(const char *key)
     if (!strcmp(key, "stat.ntt_small_launches")) return g_ntt_small_launches;
     if (!strcmp(key, "br.fused")) return OPT.br_fused;
     if (!strcmp(key, "stat.br_fused_launches")) return g_br_fused_launches;
+    if (!strcmp(key, "cache.max_mb")) return (long)(g_cache_max >> 20);
     if (!strcmp(key, "stat.ks_fold_jobs")) return g_ks_fold_jobs;
     if (!strcmp(key, "fft.fma")) return OPT.fft_fma;
     if (!strcmp(key, "stat.col_wide_launches")) return g_col_wide_launches;
