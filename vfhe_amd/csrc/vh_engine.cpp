@@ -194,6 +194,7 @@ struct vh_opts {
     long ntt_small = 1;       // N = 2^9 .. 2^12 (FP64): whole transforms in registers (k_ntt_small);
                               // 0: the LDS radix-2 kernel
     long br_fused = 1;        // CGGI blind rotation at N = 1024: every key index in one launch
+                              // (0: a launch sequence per index)
     long ks_fold = 1;         // multiply: pass-through components enter the fused key inner
                               // product as P * d (NTT form) instead of after the ModDown
     long fft_fma = 1;         // complex FFT: 1 = avx512ifma engine's fused complex product, 0 = portable's
