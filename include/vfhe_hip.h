@@ -505,6 +505,27 @@ RNS_MLWE vfhe_hip_mlwe_alloc_RNS_sample_uninit(uint64_t N, uint64_t r, uint64_t 
 void *vfhe_hip_current_stream(void);
 
 /* ------------------------------------------------------------------------ */
+/* Ciphertext and key format (no counterpart in the reference, which keeps    */
+/* keys and ciphertexts in memory only)                                      */
+/* ------------------------------------------------------------------------ */
+/* `count` samples of one ring, rank and limb mask as one little-endian block:
+   "VFHEHIP1", version 1, form (0 unstated, 1 NTT, 2 coefficient), count, N, rank,
+   split degree, limb mask, the mask's primes, the rows (sample, component, limb), and a
+   64-bit check word over everything before it.  Size in bytes of that block: */
+uint64_t vfhe_hip_mlwe_serialized_size(RNS_MLWE *in, uint64_t count);
+/* Writes the block into buf (8-byte aligned, size >= the size above): the rows are
+   gathered on the device and read back with one copy.  Returns the bytes written, or -1
+   (samples of different rings, ranks or masks; buffer too small or unaligned). */
+int64_t vfhe_hip_mlwe_serialize(RNS_MLWE *in, uint64_t count, uint32_t form, void *buf, uint64_t size);
+/* Reads a block onto the ring `ntt`: allocates out[0 .. count-1] (free each with
+   free_mlwe_RNS_sample), uploads the rows with one copy, sets *form (if not null) and
+   returns count; or -1 (not such a block, other version, check word wrong), -2 (other
+   ring: N, split degree or a prime differs), -3 (more than max_count samples), -4
+   (truncated or unaligned).  Nothing is allocated on an error. */
+int64_t vfhe_hip_mlwe_deserialize(const void *buf, uint64_t size, incNTT ntt, RNS_MLWE *out, uint64_t max_count,
+                                  uint32_t *form);
+
+/* ------------------------------------------------------------------------ */
 /* Serving runtime: continuous batching of homomorphic requests              */
 /* ------------------------------------------------------------------------ */
 #define VFHE_HIP_REQ_MULTIPLY 0      /* in1*in2, relinearize with ksk (rlk), rescale by divide_mask if != 0 */

@@ -401,3 +401,39 @@ def test_multiplication_with_rational_rescale():
     assert res.lvl == 2
     dec = ck.decrypt(key, res)[0]
     assert all(abs(a * b * cc * d - x) < 0.05 for a, b, cc, d, x in zip(*v, dec))
+
+
+# --- ciphertext and key files (the engine's block format; the reference has none) --------------
+def test_ciphertext_and_key_files_round_trip(tmp_path):
+    """A ciphertext batch and a relinearization key written to files and read back: the rows are
+    equal bit for bit, the domain is kept, and a product with the loaded key and ciphertexts
+    equals the product with the originals; a corrupted, truncated or other-ring block is refused."""
+    chain, ck = ckks(1, 2**40)
+    key = keygen(chain)
+    rlk = relinearization_key(chain, key)
+    vals = [rand_values(N // 2) for _ in range(3)]
+    cts = ck.encrypt(key, vals)
+    cts.to_ntt()
+    cts.save(str(tmp_path / "cts.bin"))
+    rlk.save(str(tmp_path / "rlk.bin"))
+    back = Batch.load(str(tmp_path / "cts.bin"), chain, cts.lvl, delta=cts.delta)
+    assert back.form == "ntt" and len(back) == 3
+    for k in range(3):
+        assert back.polys(k) == cts.polys(k)
+    rlk2 = type(rlk).load(str(tmp_path / "rlk.bin"), chain)
+    a = ck.multiply(cts, cts, rlk)
+    b = ck.multiply(back, back, rlk2)
+    for k in range(3):
+        assert a.polys(k) == b.polys(k)
+    dec = ck.decrypt(key, b)
+    for v, d in zip(vals, dec):
+        assert all(abs(x * x - y) < 0.05 for x, y in zip(v, d))
+    data = bytearray(cts.to_bytes())
+    data[-20] ^= 1
+    with pytest.raises(ValueError, match="check word"):
+        Batch.from_bytes(bytes(data), chain, cts.lvl)
+    with pytest.raises(ValueError, match="truncated"):
+        Batch.from_bytes(cts.to_bytes()[:-8], chain, cts.lvl)
+    other = Ring(2 * N, prime_size=[40, 40], split_degree=1)   # another prime pool (another N)
+    with pytest.raises(ValueError, match="another ring"):
+        Batch.from_bytes(cts.to_bytes(), chain, cts.lvl, ring=other)

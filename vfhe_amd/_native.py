@@ -192,6 +192,9 @@ _SIGNATURES = {
     "vfhe_hip_blind_rotate_tv_batch": (None, [VPP, U64, VP, U64P, VPP, U64, U64P, U64, U64]),
     "vfhe_hip_extract_lwe_batch": (None, [VPP, U64, U64, VPP]),
     "vfhe_hip_release_cached": (U64, []),
+    "vfhe_hip_mlwe_serialized_size": (U64, [VPP, U64]),
+    "vfhe_hip_mlwe_serialize": (ctypes.c_int64, [VPP, U64, ctypes.c_uint32, VP, U64]),
+    "vfhe_hip_mlwe_deserialize": (ctypes.c_int64, [VP, U64, VP, VPP, U64, ctypes.POINTER(ctypes.c_uint32)]),
     "vfhe_hip_ntt_batch_noalloc": (I32, [VPP, U64, I32]),
     "vfhe_hip_mlwe_multiply_batch_multi": (None, [VPP, VPP, VPP, U64, VPP, U64P]),
     "polynomial_RNSc_mul_by_xai": (None, [VP, VP, U64]),

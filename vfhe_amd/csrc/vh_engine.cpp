@@ -3538,6 +3538,113 @@ extern "C" void vfhe_hip_extract_lwe_batch(RNSc_MLWE *in, uint64_t count, uint64
 }
 
 // ---------------------------------------------------------------------------
+// Ciphertext and key format.  The reference has none (keys and ciphertexts live only
+// in memory); this one lets a server keep evaluation keys and ciphertexts across runs.
+// Little-endian words:
+//   "VFHEHIP1" | u32 version = 1 | u32 form (0 unstated, 1 NTT, 2 coefficient) |
+//   u64 count | u64 N | u64 rank r | u64 split degree | u64 limb mask | u64 limbs n |
+//   n x u64 prime (the mask's limbs, ascending) |
+//   count x (r + 1) x n x N u64 rows (sample, component a_0 .. a_{r-1}, b, limb) |
+//   u64 check: h = 0x9E3779B97F4A7C15; per preceding word w: h = ((h ^ w) * 0x100000001B3) ^ (h >> 29)
+// ---------------------------------------------------------------------------
+static const uint64_t VH_FMT_MAGIC = 0x3150494845484656ULL;   // "VFHEHIP1"
+static uint64_t vh_fmt_check(const uint64_t *w, uint64_t n)
+{
+    uint64_t h = 0x9E3779B97F4A7C15ULL;
+    for (uint64_t i = 0; i < n; i++) h = ((h ^ w[i]) * 0x100000001B3ULL) ^ (h >> 29);
+    return h;
+}
+
+static uint64_t vh_fmt_words(uint64_t count, uint64_t r, uint64_t nact, uint64_t N)
+{
+    return 8 + nact + count * (r + 1) * nact * N + 1;
+}
+
+extern "C" uint64_t vfhe_hip_mlwe_serialized_size(RNS_MLWE *in, uint64_t count)
+{
+    VH_ENTER;
+    if (count == 0 || !in || !in[0]) return 0;
+    const uint64_t nact = (uint64_t)__builtin_popcountll(in[0]->b->rns_mask);
+    return 8 * vh_fmt_words(count, in[0]->r, nact, in[0]->b->ntt->N);
+}
+
+extern "C" int64_t vfhe_hip_mlwe_serialize(RNS_MLWE *in, uint64_t count, uint32_t form, void *buf, uint64_t size)
+{
+    VH_ENTER;
+    if (count == 0 || !in || !in[0] || !buf) return -1;
+    incNTT t = in[0]->b->ntt;
+    const uint64_t N = t->N, r = in[0]->r, mask = in[0]->b->rns_mask, Cc = r + 1;
+    std::vector<int> act;
+    for (uint64_t i = 0; i < t->l; i++) if (ACTIVE(mask, i)) act.push_back((int)i);
+    const uint64_t nact = act.size(), words = vh_fmt_words(count, r, nact, N);
+    if (size < 8 * words || ((uintptr_t)buf & 7)) return -1;
+    for (uint64_t k = 0; k < count; k++) {
+        if (!in[k] || in[k]->r != r || in[k]->b->ntt != t) return -1;
+        for (auto p : comps(in[k])) if (p->rns_mask != mask) return -1;
+    }
+    uint64_t *w = (uint64_t *)buf;
+    w[0] = VH_FMT_MAGIC;
+    w[1] = 1ULL | ((uint64_t)form << 32);
+    w[2] = count; w[3] = N; w[4] = r; w[5] = t->split_degree; w[6] = mask; w[7] = nact;
+    for (uint64_t j = 0; j < nact; j++) w[8 + j] = t->ntt[act[j]]->q;
+    const uint64_t rows = count * Cc * nact;
+    uint64_t *dbuf = (uint64_t *)vh_dmalloc(rows * N * 8);
+    std::vector<vh_item> items;
+    items.reserve(rows);
+    for (uint64_t k = 0; k < count; k++) {
+        const auto ic = comps(in[k]);
+        for (uint64_t c = 0; c < Cc; c++) {
+            dev_ready(ic[c]);
+            for (uint64_t j = 0; j < nact; j++)
+                items.push_back(mk_item(drow(ic[c], act[j]), dbuf + ((k * Cc + c) * nact + j) * N, (uint32_t)act[j]));
+        }
+    }
+    run_eltwise(items, VPOOL(t)->d_limbs, VH_OP_COPY, N);
+    VH_CHECK(hipMemcpyAsync(w + 8 + nact, dbuf, rows * N * 8, hipMemcpyDeviceToHost, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
+    vh_dfree(dbuf);
+    w[words - 1] = vh_fmt_check(w, words - 1);
+    return (int64_t)(8 * words);
+}
+
+extern "C" int64_t vfhe_hip_mlwe_deserialize(const void *buf, uint64_t size, incNTT t, RNS_MLWE *out,
+                                             uint64_t max_count, uint32_t *form)
+{
+    VH_ENTER;
+    if (!buf || !t || ((uintptr_t)buf & 7) || size < 8 * 9) return -4;
+    const uint64_t *w = (const uint64_t *)buf;
+    if (w[0] != VH_FMT_MAGIC || (w[1] & 0xFFFFFFFFULL) != 1) return -1;
+    const uint64_t count = w[2], N = w[3], r = w[4], split = w[5], mask = w[6], nact = w[7];
+    if (nact != (uint64_t)__builtin_popcountll(mask) || r > 64 || N == 0 || N > (1ULL << 24) || count > (1ULL << 32))
+        return -1;
+    const uint64_t words = vh_fmt_words(count, r, nact, N);
+    if (size < 8 * words) return -4;
+    if (vh_fmt_check(w, words - 1) != w[words - 1]) return -1;
+    if (N != t->N || split != t->split_degree || (t->l < 64 && (mask >> t->l))) return -2;
+    std::vector<int> act;
+    for (uint64_t i = 0; i < t->l; i++) if (ACTIVE(mask, i)) act.push_back((int)i);
+    for (uint64_t j = 0; j < nact; j++) if (t->ntt[act[j]]->q != w[8 + j]) return -2;
+    if (count > max_count || !out) return -3;
+    if (form) *form = (uint32_t)(w[1] >> 32);
+    const uint64_t Cc = r + 1, rows = count * Cc * nact;
+    uint64_t *dbuf = (uint64_t *)vh_dmalloc(std::max<uint64_t>(1, rows) * N * 8);
+    VH_CHECK(hipMemcpyAsync(dbuf, w + 8 + nact, rows * N * 8, hipMemcpyHostToDevice, S()));
+    std::vector<vh_item> items;
+    items.reserve(rows);
+    for (uint64_t k = 0; k < count; k++) {
+        out[k] = vfhe_hip_mlwe_alloc_RNS_sample_uninit(N, r, mask, t);
+        const auto oc = comps(out[k]);
+        for (uint64_t c = 0; c < Cc; c++)
+            for (uint64_t j = 0; j < nact; j++)
+                items.push_back(mk_item(dbuf + ((k * Cc + c) * nact + j) * N, drow(oc[c], act[j]), (uint32_t)act[j]));
+    }
+    run_eltwise(items, VPOOL(t)->d_limbs, VH_OP_COPY, N);
+    VH_CHECK(hipStreamSynchronize(S()));   // the caller's buffer may go once this returns
+    vh_dfree(dbuf);
+    return (int64_t)count;
+}
+
+// ---------------------------------------------------------------------------
 // Keys and phase (mlwe.c:8-38, 243-252)
 // ---------------------------------------------------------------------------
 extern "C" RNS_MLWE_Key mlwe_new_RNS_key_from_array(uint64_t *array, uint64_t N, uint64_t r, uint64_t l, incNTT ntt, double sigma)

@@ -187,6 +187,41 @@ class SecretKey:
         return [v for c in self.coeffs for v in c]
 
 
+# ---------------------------------------------------------------------------------------------
+# Ciphertext and key files (the engine's block format, vfhe_hip_mlwe_serialize)
+# ---------------------------------------------------------------------------------------------
+_FORM_CODES = {"ntt": 1, "coeff": 2}
+_KEY_MAGIC = 0x3159454B45484656   # "VFHEKEY1"
+
+
+def _samples_to_bytes(handles, form: str | None) -> bytes:
+    arr = _handles(handles)
+    size = int(lib().vfhe_hip_mlwe_serialized_size(arr, len(handles)))
+    buf = np.empty(size // 8, dtype=np.uint64)
+    n = lib().vfhe_hip_mlwe_serialize(arr, len(handles), _FORM_CODES.get(form, 0),
+                                      buf.ctypes.data_as(ctypes.c_void_p), size)
+    if n != size:
+        raise ValueError("samples of different rings, ranks or limb masks cannot share one block")
+    return buf.tobytes()
+
+
+def _samples_from_bytes(data: bytes, ring: Ring, max_count: int = 1 << 20):
+    """(handles, form) of a block read onto `ring`; ValueError names what is wrong."""
+    if len(data) % 8 or len(data) < 72:
+        raise ValueError("truncated block")
+    words = np.frombuffer(data, dtype=np.uint64).copy()   # 8-byte aligned for the engine
+    count = int(words[2])
+    out = (ctypes.c_void_p * max(1, min(count, max_count)))()
+    form = ctypes.c_uint32(0)
+    n = lib().vfhe_hip_mlwe_deserialize(words.ctypes.data_as(ctypes.c_void_p), words.nbytes, ring.NTT, out,
+                                        max(1, min(count, max_count)), ctypes.byref(form))
+    if n < 0:
+        raise ValueError({-1: "not a vfhe-hip sample block, or its check word does not match",
+                          -2: "the block belongs to another ring (N, split degree or primes differ)",
+                          -3: "more samples than allowed", -4: "truncated block"}.get(int(n), "bad block"))
+    return [out[k] for k in range(n)], {1: "ntt", 2: "coeff"}.get(form.value)
+
+
 class SwitchKey:
     """Gadget key-switching key from the sources to `target`.
 
@@ -235,6 +270,47 @@ class SwitchKey:
                 entries.append(ctypes.cast(v, ctypes.c_void_p).value)
             self._views[lvl] = (_handles(entries), arrays)
         return self._views[lvl][0]
+
+    def to_bytes(self) -> bytes:
+        """"VFHEKEY1", the source count, then per source a length (0: a pass-through source) and
+        the engine block of its digits (NTT form, on the chain's span)."""
+        parts = [np.array([_KEY_MAGIC, len(self.digits)], dtype=np.uint64).tobytes()]
+        for d in self.digits:
+            blob = b"" if d is None else _samples_to_bytes(d.handles, "ntt")
+            parts.append(np.array([len(blob)], dtype=np.uint64).tobytes())
+            parts.append(blob)
+        return b"".join(parts)
+
+    def save(self, path: str):
+        with open(path, "wb") as f:
+            f.write(self.to_bytes())
+
+    @classmethod
+    def from_bytes(cls, data: bytes, chain: Chain) -> "SwitchKey":
+        """The key written by to_bytes, on `chain` (whose span must hold the same primes)."""
+        head = np.frombuffer(data[:16], dtype=np.uint64)
+        if head.size != 2 or int(head[0]) != _KEY_MAGIC:
+            raise ValueError("not a vfhe-hip switching key")
+        self = cls.__new__(cls)
+        self.chain, self.digits, self._views = chain, [], {}
+        off = 16
+        for _ in range(int(head[1])):
+            if off + 8 > len(data):
+                raise ValueError("truncated switching key")
+            n = int(np.frombuffer(data[off:off + 8], dtype=np.uint64)[0])
+            off += 8
+            if n == 0:
+                self.digits.append(None)
+                continue
+            hs, _ = _samples_from_bytes(data[off:off + n], chain.span)
+            off += n
+            self.digits.append(Batch(chain, 0, hs, "ntt", chain.rank, chain.span))
+        return self
+
+    @classmethod
+    def load(cls, path: str, chain: Chain) -> "SwitchKey":
+        with open(path, "rb") as f:
+            return cls.from_bytes(f.read(), chain)
 
     def __del__(self):
         try:
@@ -352,6 +428,29 @@ class Batch:
         return out
 
     # ---- decryption ---------------------------------------------------------------------
+    def to_bytes(self) -> bytes:
+        """The batch as one engine block (vfhe_hip_mlwe_serialize), its domain recorded."""
+        return _samples_to_bytes(self.handles, self.form)
+
+    def save(self, path: str):
+        with open(path, "wb") as f:
+            f.write(self.to_bytes())
+
+    @classmethod
+    def from_bytes(cls, data: bytes, chain: Chain, lvl: int = 0, ring: Ring | None = None,
+                   delta: float | None = None) -> "Batch":
+        """A batch written by to_bytes, read onto `ring` (the level ring unless stated)."""
+        ring = ring if ring is not None else chain.levels[lvl]
+        hs, form = _samples_from_bytes(data, ring)
+        rank = int(np.frombuffer(data[32:40], dtype=np.uint64)[0])
+        return cls(chain, lvl, hs, form or "ntt", rank, ring, delta)
+
+    @classmethod
+    def load(cls, path: str, chain: Chain, lvl: int = 0, ring: Ring | None = None,
+             delta: float | None = None) -> "Batch":
+        with open(path, "rb") as f:
+            return cls.from_bytes(f.read(), chain, lvl, ring, delta)
+
     def decrypt(self, key: SecretKey) -> list[Polynomial]:
         """The phases b - sum_i a_i s_i (mlwe.c:243-252), NTT form, one polynomial per member."""
         self.to_ntt()
