@@ -227,7 +227,9 @@ def main():
         ax, ay, az = _native.voidp_array(polys), _native.voidp_array(ys), _native.voidp_array(zs)
         with Prof(lib) as fprof:
             fused_dt = timed(lib, lambda: lib.vfhe_hip_ntt_mul_intt_batch(az, ax, ay, POLYS), args.steps, args.warmup)
+        graph_dt = timed(lib, lambda: lib.vfhe_hip_ntt_mul_intt_batch_graph(az, ax, ay, POLYS), args.steps, args.warmup)
         print(json.dumps({"fused_ms_per_batch": fused_dt * 1e3, "products_per_s": POLYS / fused_dt,
+                          "graph_ms_per_batch": graph_dt * 1e3, "graph_products_per_s": POLYS / graph_dt,
                           "ntt_ms": fprof.ms[0], "launches": fprof.launches[0]}), flush=True)
         return
 
@@ -286,6 +288,12 @@ def main():
     ax, ay, az = _native.voidp_array(xs), _native.voidp_array(ys), _native.voidp_array(zs)
     fused_dt = timed(lib, lambda: lib.vfhe_hip_ntt_mul_intt_batch(az, ax, ay, POLYS), args.steps, 2)
     log(f"fused product: {fused_dt * 1e3:.3f} ms per 64 products")
+    # the same through a graph replay (one host launch per batch), checked against the eager output
+    zg = [pool.poly() for _ in range(POLYS)]
+    azg = _native.voidp_array(zg)
+    fused_graph_dt = timed(lib, lambda: lib.vfhe_hip_ntt_mul_intt_batch_graph(azg, ax, ay, POLYS), args.steps, 2)
+    fused_graph_equal = all(np.array_equal(pool.read(a), pool.read(b)) for a, b in zip(zg[:4], zs[:4]))
+    log(f"fused product (graph): {fused_graph_dt * 1e3:.3f} ms per 64 products, equal: {fused_graph_equal}")
 
     # plain one-call-per-polynomial forms (device resident, and host resident)
     def plain_dev():
@@ -359,6 +367,8 @@ def main():
                      "algorithmic_bytes_per_launch": int(bytes_per_launch)},
         "cpu_baseline": cpu,
         "fused": {"unit": "products/s", "value": round(POLYS / fused_dt, 2), "ms_per_batch": round(fused_dt * 1e3, 4),
+                  "graph": {"value": round(POLYS / fused_graph_dt, 2), "ms_per_batch": round(fused_graph_dt * 1e3, 4),
+                            "equals_eager": fused_graph_equal},
                   "workload": "z=INTT(NTT(x)*NTT(y)), 64 pairs x 24 limbs, N=2^16"},
         "plain": {"device_resident_round_trips_per_s": round(POLYS / plain_dev_dt, 2),
                   "host_resident_round_trips_per_s": round(POLYS / plain_host_dt, 2),

@@ -419,6 +419,11 @@ void vfhe_hip_ntt_batch(RNS_Polynomial *polys, uint64_t count, int direction);
 /* z[i] = INTT(NTT(x[i]) * NTT(y[i])) on the active limbs of x[i] & y[i]
    (coefficient form in, coefficient form out); x and y are left unchanged. */
 void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polynomial *y, uint64_t count);
+/* vfhe_hip_ntt_mul_intt_batch through a HIP graph: the first call of a batch shape (ring,
+   count, every operand's limb mask and rows, lane, schedule options) runs eagerly and
+   captures its launches; later calls of the shape rebuild the operands' work lists and
+   replay the graph (one host launch).  Results equal the eager entry point's. */
+void vfhe_hip_ntt_mul_intt_batch_graph(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polynomial *y, uint64_t count);
 /* CKKS product for `count` ciphertext pairs sharing one relinearization key:
    out[i] = rescale(relinearize(in1[i] * in2[i])) when divide_mask != 0 (outputs
    in coefficient form on the rescaled mask), or mlwe_multiply semantics (NTT form)

@@ -126,3 +126,38 @@ def test_reference_cdefs_are_covered():
                 arity.append(f"{name}: reference {n}, here {ours[name]}")
     assert not missing, missing
     assert not arity, arity
+
+
+def test_scalar_modq_helpers_against_bigint_and_reference():
+    """add_modq / sub_modq / negate_modq (mod.c:54-62) are host arithmetic: the engine's exports
+    against Python integers and, where oracle/_ref is built, the reference's own functions, over
+    edge values (0, 1, q - 1) and random residues of 30..62-bit moduli.  mul_modq needs an NTT_proc
+    and is checked with the device tests."""
+    import random
+    lib = ctypes.CDLL(_native.LIB_PATH)
+    U = ctypes.c_uint64
+    for name, nargs in (("add_modq", 3), ("sub_modq", 3), ("negate_modq", 2)):
+        getattr(lib, name).restype = U
+        getattr(lib, name).argtypes = [U] * nargs
+    refs = []
+    try:
+        from oracle import oracle as O
+        for eng in O.ref_engines():
+            R = O.ref_lib(eng)
+            for name, nargs in (("add_modq", 3), ("sub_modq", 3), ("negate_modq", 2)):
+                getattr(R, name).restype = U
+                getattr(R, name).argtypes = [U] * nargs
+            refs.append(R)
+    except Exception:
+        refs = []
+    rnd = random.Random(54)
+    for bits in (30, 50, 60, 62):
+        q = (1 << bits) - rnd.randrange(1, 1 << 20) * 2 - 1
+        vals = [0, 1, q - 1, q // 2] + [rnd.randrange(q) for _ in range(200)]
+        for a in vals:
+            b = rnd.choice(vals)
+            want = ((a + b) % q, (a - b) % q, (-a) % q)
+            got = (lib.add_modq(a, b, q), lib.sub_modq(a, b, q), lib.negate_modq(a, q))
+            assert got == want, (bits, a, b)
+            for R in refs:
+                assert (R.add_modq(a, b, q), R.sub_modq(a, b, q), R.negate_modq(a, q)) == want

@@ -114,3 +114,41 @@ def test_graph_new_shape_captures_again(lib):
         lib.free_RNS_mlwe_array(lvl, view)
     assert _stat(lib, "graph_captures") - cap0 == 2
     lib.vfhe_hip_graph_clear()
+
+
+@pytest.mark.parametrize("N,L,P", [(16384, 6, 5), (65536, 24, 4)])
+def test_fused_product_graph_replays_match_eager(lib, N, L, P):
+    """vfhe_hip_ntt_mul_intt_batch_graph: a capture, then replays on new operands, each equal bit for
+    bit to the eager vfhe_hip_ntt_mul_intt_batch and (first replay, one pair) to the oracle, with eager
+    work and synchronisations between the replays."""
+    primes = O.gen_primes(N, [50] * L)
+    pool = rt.Pool(N, primes)
+    octx = O.Ctx(N, primes)
+    rng = np.random.default_rng(N + P)
+    cap0, rep0 = _stat(lib, "graph_captures"), _stat(lib, "graph_replays")
+    for call in range(3):
+        xs = [rt.random_rows(rng, primes, N) for _ in range(P)]
+        ys = [rt.random_rows(rng, primes, N) for _ in range(P)]
+        px, py = [pool.poly(x) for x in xs], [pool.poly(y) for y in ys]
+        zg, ze = [pool.poly() for _ in range(P)], [pool.poly() for _ in range(P)]
+        print(f"[fused graph] N={N} L={L} pairs={P} call={call}", flush=True)
+        lib.vfhe_hip_ntt_mul_intt_batch_graph(_native.voidp_array(zg), _native.voidp_array(px),
+                                              _native.voidp_array(py), P)
+        lib.vfhe_hip_ntt_mul_intt_batch(_native.voidp_array(ze), _native.voidp_array(px),
+                                        _native.voidp_array(py), P)
+        lib.vfhe_hip_synchronize()
+        for a, b in zip(zg, ze):
+            assert np.array_equal(pool.read(a), pool.read(b)), call
+        if call == 1:
+            x, y = xs[0].copy(), ys[0].copy()
+            m = pool.full_mask()
+            octx.to_ntt(x, x, m)
+            octx.to_ntt(y, y, m)
+            z = np.zeros_like(x)
+            octx.mul(z, x, m, y, m, 0)
+            octx.to_coeff(z, z, m)
+            assert np.array_equal(pool.read(zg[0]), z)
+        for p in px + py + zg + ze:
+            pool.free(p)
+    assert _stat(lib, "graph_captures") - cap0 == 1
+    assert _stat(lib, "graph_replays") - rep0 == 2

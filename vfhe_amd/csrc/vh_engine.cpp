@@ -2905,6 +2905,93 @@ static std::vector<uint64_t> multiply_key(const std::vector<RNS_MLWE> &outs, con
     return k;
 }
 
+// Runs body() through the graph plan of `key` (capturing it on the first call of the
+// key, replaying it afterwards).  get_state / set_state read and restore the host-side
+// state body() changes (output masks), so the eager run, the capture and every replay
+// start from the caller's state.  Inputs must be device-resident before the call.
+static void plan_run(const std::vector<uint64_t> &key, const std::function<void()> &body,
+                     const std::function<std::vector<uint64_t>()> &get_state,
+                     const std::function<void(const std::vector<uint64_t> &)> &set_state)
+{
+    vh_graph_plan *pl = nullptr;
+    for (auto q : g_plans) if (q->key == key) { pl = q; break; }
+    const std::vector<uint64_t> state0 = get_state();
+    if (pl) {
+        // Replay: rebuild the work lists, then launch the graph.
+        if (pl->launched) VH_CHECK(hipEventSynchronize(pl->done));
+        g_cap = VH_CAP_DRY; g_plan = pl; g_cap_stage = g_cap_off = g_cap_block = 0; g_cap_launches = 0;
+        g_vh_suppress_launch = 1;
+        try {
+            body();
+            if (g_cap_launches != pl->launches) g_graph_why = 4000000L + std::min(999999L, std::labs(g_cap_launches - pl->launches));
+            else if (g_cap_stage != pl->stage_sz.size()) g_graph_why = 5000000L + (long)g_cap_stage;
+            else if (g_cap_block != pl->block_sz.size()) g_graph_why = 6000000L + (long)g_cap_block;
+            if (g_cap_stage != pl->stage_sz.size() || g_cap_block != pl->block_sz.size() ||
+                g_cap_launches != pl->launches)
+                throw vh_plan_mismatch();
+        } catch (const vh_plan_mismatch &) {
+            g_vh_suppress_launch = 0;
+            g_cap = VH_CAP_OFF; g_plan = nullptr;
+            g_graph_fallbacks++;
+            set_state(state0);
+            body();
+            return;
+        }
+        g_vh_suppress_launch = 0;
+        g_cap = VH_CAP_OFF; g_plan = nullptr;
+        VH_CHECK(hipGraphLaunch(pl->exec, S()));
+        VH_CHECK(hipEventRecord(pl->done, S()));
+        pl->launched = true;
+        g_graph_replays++;
+        return;
+    }
+    // First call of this shape: run eagerly, logging sizes, then capture.
+    pl = new vh_graph_plan();
+    pl->key = key;
+    g_cap = VH_CAP_LOG; g_plan = pl;
+    body();
+    g_cap = VH_CAP_OFF; g_plan = nullptr;
+    const std::vector<uint64_t> state1 = get_state();
+    for (size_t b : pl->stage_sz) pl->wl_bytes += b;
+    if (pl->wl_bytes) {
+        VH_CHECK(hipHostMalloc((void **)&pl->h_wl, pl->wl_bytes, hipHostMallocDefault));
+        VH_CHECK(hipMalloc((void **)&pl->d_wl, pl->wl_bytes));
+    }
+    for (size_t b : pl->block_sz) {
+        void *p = nullptr;
+        VH_CHECK(hipMalloc(&p, b));
+        pl->blocks.push_back(p);
+    }
+    VH_CHECK(hipEventCreateWithFlags(&pl->done, hipEventDisableTiming));
+    set_state(state0);
+    VH_CHECK(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
+    g_cap = VH_CAP_RECORD; g_plan = pl; g_cap_stage = g_cap_off = g_cap_block = 0; g_cap_launches = 0;
+    bool ok = true;
+    try {
+        if (pl->wl_bytes)
+            VH_CHECK(hipMemcpyAsync(pl->d_wl, pl->h_wl, pl->wl_bytes, hipMemcpyHostToDevice, S()));
+        body();
+        ok = g_cap_stage == pl->stage_sz.size() && g_cap_block == pl->block_sz.size();
+    } catch (const vh_plan_mismatch &) {
+        ok = false;
+    }
+    pl->launches = g_cap_launches;
+    g_cap = VH_CAP_OFF; g_plan = nullptr;
+    hipGraph_t graph = nullptr;
+    VH_CHECK(hipStreamEndCapture(S(), &graph));
+    pl->graph = graph;
+    set_state(state1);
+    if (!ok) {
+        // The host logic did not repeat itself: keep the eager result, drop the plan.
+        g_graph_fallbacks++;
+        plan_free(pl);
+        return;
+    }
+    VH_CHECK(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
+    g_plans.push_back(pl);
+    g_graph_captures++;
+}
+
 static void multiply_batch_eager_or_plan(const std::vector<RNS_MLWE> &outs, const std::vector<RNS_MLWE> &in1,
                                          const std::vector<RNS_MLWE> &in2, RNS_MLWE **rlk, uint64_t divide_mask)
 {
@@ -2921,90 +3008,16 @@ static void multiply_batch_eager_or_plan(const std::vector<RNS_MLWE> &outs, cons
     const uint64_t nd = (uint64_t)__builtin_popcountll(in1[0]->b->rns_mask);
     for (uint64_t i = 0; i < R; i++)
         if (rlk[i]) for (uint64_t d = 0; d < nd; d++) for (auto p : comps(rlk[i][d])) dev_ready(p);
-    const std::vector<uint64_t> key = multiply_key(outs, in1, in2, rlk, divide_mask);
-    vh_graph_plan *pl = nullptr;
-    for (auto q : g_plans) if (q->key == key) { pl = q; break; }
-    std::vector<uint64_t> masks0;
-    for (auto o : outs) for (auto p : comps(o)) masks0.push_back(p->rns_mask);
-    auto restore = [&](const std::vector<uint64_t> &m) {
-        size_t k = 0;
-        for (auto o : outs) for (auto p : comps(o)) p->rns_mask = m[k++];
-    };
-    if (pl) {
-        // Replay: rebuild the work lists, then launch the graph.
-        if (pl->launched) VH_CHECK(hipEventSynchronize(pl->done));
-        g_cap = VH_CAP_DRY; g_plan = pl; g_cap_stage = g_cap_off = g_cap_block = 0; g_cap_launches = 0;
-        g_vh_suppress_launch = 1;
-        try {
-            multiply_batch(outs, in1, in2, ks, dm);
-            if (g_cap_launches != pl->launches) g_graph_why = 4000000L + std::min(999999L, std::labs(g_cap_launches - pl->launches));
-            else if (g_cap_stage != pl->stage_sz.size()) g_graph_why = 5000000L + (long)g_cap_stage;
-            else if (g_cap_block != pl->block_sz.size()) g_graph_why = 6000000L + (long)g_cap_block;
-            if (g_cap_stage != pl->stage_sz.size() || g_cap_block != pl->block_sz.size() ||
-                g_cap_launches != pl->launches)
-                throw vh_plan_mismatch();
-        } catch (const vh_plan_mismatch &) {
-            g_vh_suppress_launch = 0;
-            g_cap = VH_CAP_OFF; g_plan = nullptr;
-            g_graph_fallbacks++;
-            restore(masks0);
-            multiply_batch(outs, in1, in2, ks, dm);
-            return;
-        }
-        g_vh_suppress_launch = 0;
-        g_cap = VH_CAP_OFF; g_plan = nullptr;
-        VH_CHECK(hipGraphLaunch(pl->exec, S()));
-        VH_CHECK(hipEventRecord(pl->done, S()));
-        pl->launched = true;
-        g_graph_replays++;
-        return;
-    }
-    // First call of this shape: run eagerly, logging sizes, then capture.
-    pl = new vh_graph_plan();
-    pl->key = key;
-    g_cap = VH_CAP_LOG; g_plan = pl;
-    multiply_batch(outs, in1, in2, ks, dm);
-    g_cap = VH_CAP_OFF; g_plan = nullptr;
-    std::vector<uint64_t> masks1;
-    for (auto o : outs) for (auto p : comps(o)) masks1.push_back(p->rns_mask);
-    for (size_t b : pl->stage_sz) pl->wl_bytes += b;
-    if (pl->wl_bytes) {
-        VH_CHECK(hipHostMalloc((void **)&pl->h_wl, pl->wl_bytes, hipHostMallocDefault));
-        VH_CHECK(hipMalloc((void **)&pl->d_wl, pl->wl_bytes));
-    }
-    for (size_t b : pl->block_sz) {
-        void *p = nullptr;
-        VH_CHECK(hipMalloc(&p, b));
-        pl->blocks.push_back(p);
-    }
-    VH_CHECK(hipEventCreateWithFlags(&pl->done, hipEventDisableTiming));
-    restore(masks0);
-    VH_CHECK(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
-    g_cap = VH_CAP_RECORD; g_plan = pl; g_cap_stage = g_cap_off = g_cap_block = 0; g_cap_launches = 0;
-    bool ok = true;
-    try {
-        if (pl->wl_bytes)
-            VH_CHECK(hipMemcpyAsync(pl->d_wl, pl->h_wl, pl->wl_bytes, hipMemcpyHostToDevice, S()));
-        multiply_batch(outs, in1, in2, ks, dm);
-        ok = g_cap_stage == pl->stage_sz.size() && g_cap_block == pl->block_sz.size();
-    } catch (const vh_plan_mismatch &) {
-        ok = false;
-    }
-    pl->launches = g_cap_launches;
-    g_cap = VH_CAP_OFF; g_plan = nullptr;
-    hipGraph_t graph = nullptr;
-    VH_CHECK(hipStreamEndCapture(S(), &graph));
-    pl->graph = graph;
-    restore(masks1);
-    if (!ok) {
-        // The host logic did not repeat itself: keep the eager result, drop the plan.
-        g_graph_fallbacks++;
-        plan_free(pl);
-        return;
-    }
-    VH_CHECK(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
-    g_plans.push_back(pl);
-    g_graph_captures++;
+    plan_run(multiply_key(outs, in1, in2, rlk, divide_mask), [&] { multiply_batch(outs, in1, in2, ks, dm); },
+             [&] {
+                 std::vector<uint64_t> m;
+                 for (auto o : outs) for (auto p : comps(o)) m.push_back(p->rns_mask);
+                 return m;
+             },
+             [&](const std::vector<uint64_t> &m) {
+                 size_t k = 0;
+                 for (auto o : outs) for (auto p : comps(o)) p->rns_mask = m[k++];
+             });
 }
 
 extern "C" void vfhe_hip_mlwe_multiply_batch_graph(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLWE *in2, uint64_t count,
@@ -3768,10 +3781,8 @@ static void fused_product_launches(std::vector<vh_item> &col, std::vector<vh_ite
     launch(inv, 2);
 }
 
-extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polynomial *y, uint64_t count)
+static void ntt_mul_intt(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polynomial *y, uint64_t count)
 {
-    VH_ENTER;
-    if (count == 0) return;
     incNTT t = z[0]->ntt;
     const uint64_t N = t->N;
     std::vector<RNS_Polynomial> tmp(count);
@@ -3837,6 +3848,41 @@ extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x
     for (uint64_t i = 0; i < count; i++) inv.push_back({z[i], z[i]});
     ntt_rows(inv, 1);
     for (auto p : tmp) free_RNS_polynomial(p);
+}
+
+extern "C" void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polynomial *y, uint64_t count)
+{
+    VH_ENTER;
+    if (count == 0) return;
+    ntt_mul_intt(z, x, y, count);
+}
+
+// The same through a graph plan (plan_run): the three launches of a batch shape are
+// captured once and replayed with the operands' work lists rebuilt; the temporaries
+// are the plan's own blocks.
+extern "C" void vfhe_hip_ntt_mul_intt_batch_graph(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polynomial *y,
+                                                  uint64_t count)
+{
+    VH_ENTER;
+    if (count == 0) return;
+    VH_REQUIRE(z && x && y);
+    for (uint64_t i = 0; i < count; i++) { dev_ready(x[i]); dev_ready(y[i]); dev_ready(z[i]); }
+    std::vector<uint64_t> key = {2, (uint64_t)(uintptr_t)z[0]->ntt, count, (uint64_t)g_lane, (uint64_t)OPT.ntt_mode,
+                                 (uint64_t)OPT.ntt_chunk_mb, (uint64_t)OPT.ntt_tile_grid, (uint64_t)OPT.ntt_col_wide,
+                                 (uint64_t)OPT.ntt_small};
+    for (uint64_t i = 0; i < count; i++) {
+        VH_REQUIRE(z[i]->ntt == z[0]->ntt);
+        for (RNS_Polynomial p : {x[i], y[i], z[i]}) { key.push_back(p->rns_mask); key.push_back(p->allocated_l); }
+    }
+    plan_run(key, [&] { ntt_mul_intt(z, x, y, count); },
+             [&] {
+                 std::vector<uint64_t> m;
+                 for (uint64_t i = 0; i < count; i++) m.push_back(z[i]->rns_mask);
+                 return m;
+             },
+             [&](const std::vector<uint64_t> &m) {
+                 for (uint64_t i = 0; i < count; i++) z[i]->rns_mask = m[i];
+             });
 }
 
 
