@@ -248,6 +248,16 @@ def main():
         lib.vfhe_hip_synchronize()
         dt = (time.perf_counter() - t0) / args.steps
     assert np.array_equal(pool.read(polys[0]), ref0), "round trip changed the data"
+
+    def round_trip_graph():
+        lib.vfhe_hip_ntt_batch_graph(arr, POLYS, 0)
+        lib.vfhe_hip_ntt_batch_graph(arr, POLYS, 1)
+
+    graph_dt = None
+    if not args.only_ntt:   # (the profiled --only-ntt command runs the eager passes alone)
+        graph_dt = timed(lib, round_trip_graph, args.steps, args.warmup)
+        assert np.array_equal(pool.read(polys[0]), ref0), "graph round trip changed the data"
+        log(f"round trip (graph replay): {graph_dt * 1e3:.3f} ms/step, {POLYS / graph_dt:.1f} round-trips/s")
     value = POLYS / dt
     ntt_ms, ntt_launches = prof.ms[0], prof.launches[0]
     avg_launch_s = ntt_ms / 1e3 / max(1, ntt_launches)
@@ -366,6 +376,8 @@ def main():
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                      "algorithmic_bytes_per_launch": int(bytes_per_launch)},
         "cpu_baseline": cpu,
+        "graph": {"round_trips_per_s": round(POLYS / graph_dt, 2), "ms_per_step": round(graph_dt * 1e3, 4),
+                  "note": "the same batches through vfhe_hip_ntt_batch_graph (one graph replay per direction)"},
         "fused": {"unit": "products/s", "value": round(POLYS / fused_dt, 2), "ms_per_batch": round(fused_dt * 1e3, 4),
                   "graph": {"value": round(POLYS / fused_graph_dt, 2), "ms_per_batch": round(fused_graph_dt * 1e3, 4),
                             "equals_eager": fused_graph_equal},

@@ -416,6 +416,11 @@ void vfhe_hip_profile_stop(double *ms, uint64_t *launches);
 /* Batched hot path: one launch sequence for many polynomials.
    direction 0 = forward (RNSc -> RNS), 1 = inverse. polys[i] is transformed in place. */
 void vfhe_hip_ntt_batch(RNS_Polynomial *polys, uint64_t count, int direction);
+/* vfhe_hip_ntt_batch through a HIP graph: the launches of a batch shape (ring, count,
+   direction, every polynomial's limb mask and rows, lane, schedule options) are captured
+   on its first call and replayed afterwards with the polynomials' work lists rebuilt.
+   Results equal vfhe_hip_ntt_batch's. */
+void vfhe_hip_ntt_batch_graph(RNS_Polynomial *polys, uint64_t count, int direction);
 /* z[i] = INTT(NTT(x[i]) * NTT(y[i])) on the active limbs of x[i] & y[i]
    (coefficient form in, coefficient form out); x and y are left unchanged. */
 void vfhe_hip_ntt_mul_intt_batch(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polynomial *y, uint64_t count);

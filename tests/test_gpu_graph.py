@@ -152,3 +152,32 @@ def test_fused_product_graph_replays_match_eager(lib, N, L, P):
             pool.free(p)
     assert _stat(lib, "graph_captures") - cap0 == 1
     assert _stat(lib, "graph_replays") - rep0 == 2
+
+
+@pytest.mark.parametrize("N,L,P", [(4096, 4, 6), (65536, 24, 3)])
+def test_ntt_batch_graph_replays_match_eager(lib, N, L, P):
+    """vfhe_hip_ntt_batch_graph: forward and inverse batches captured once and replayed on new
+    rows, each equal to the oracle's transform and the round trip the identity."""
+    primes = O.gen_primes(N, [50] * L)
+    pool = rt.Pool(N, primes)
+    octx = O.Ctx(N, primes)
+    rng = np.random.default_rng(N + L)
+    rep0 = _stat(lib, "graph_replays")
+    for call in range(3):
+        xs = [rt.random_rows(rng, primes, N) for _ in range(P)]
+        ps = [pool.poly(x) for x in xs]
+        arr = _native.voidp_array(ps)
+        print(f"[ntt graph] N={N} L={L} polys={P} call={call}", flush=True)
+        lib.vfhe_hip_ntt_batch_graph(arr, P, 0)
+        lib.vfhe_hip_synchronize()
+        for p, x in zip(ps, xs):
+            r = x.copy()
+            octx.to_ntt(r, r, pool.full_mask())
+            assert np.array_equal(pool.read(p), r), call
+        lib.vfhe_hip_ntt_batch_graph(arr, P, 1)
+        lib.vfhe_hip_synchronize()
+        for p, x in zip(ps, xs):
+            assert np.array_equal(pool.read(p), x), call
+        for p in ps:
+            pool.free(p)
+    assert _stat(lib, "graph_replays") - rep0 == 4

@@ -252,6 +252,7 @@ _SIGNATURES = {
     "lwe_phase": (None, [U64P, VP, VP]),
     "lwe_subto": (None, [VP, VP]),
     "vfhe_hip_ntt_batch": (None, [VPP, U64, I32]),
+    "vfhe_hip_ntt_batch_graph": (None, [VPP, U64, I32]),
     "vfhe_hip_ntt_mul_intt_batch": (None, [VPP, VPP, VPP, U64]),
     "vfhe_hip_ntt_mul_intt_batch_graph": (None, [VPP, VPP, VPP, U64]),
     "vfhe_hip_mlwe_multiply_batch": (None, [VPP, VPP, VPP, U64, VPP, U64]),

@@ -3705,6 +3705,34 @@ extern "C" void vfhe_hip_ntt_batch(RNS_Polynomial *polys, uint64_t count, int di
     ntt_rows(pairs, direction ? 1 : 0);
 }
 
+static void plan_run(const std::vector<uint64_t> &key, const std::function<void()> &body,
+                     const std::function<std::vector<uint64_t>()> &get_state,
+                     const std::function<void(const std::vector<uint64_t> &)> &set_state);
+
+// vfhe_hip_ntt_batch through a graph plan: the launches of a batch shape (ring, count,
+// direction, every polynomial's limb mask and rows, lane, schedule options) are captured
+// once and replayed with the polynomials' work lists rebuilt.
+extern "C" void vfhe_hip_ntt_batch_graph(RNS_Polynomial *polys, uint64_t count, int direction)
+{
+    VH_ENTER;
+    if (count == 0) return;
+    VH_REQUIRE(polys);
+    std::vector<std::pair<RNS_Polynomial, RNS_Polynomial>> pairs;
+    std::vector<uint64_t> key = {3, (uint64_t)(uintptr_t)polys[0]->ntt, count, (uint64_t)(direction ? 1 : 0),
+                                 (uint64_t)g_lane, (uint64_t)OPT.ntt_mode, (uint64_t)OPT.ntt_chunk_mb,
+                                 (uint64_t)OPT.ntt_tile_grid, (uint64_t)OPT.ntt_col_wide, (uint64_t)OPT.ntt_small,
+                                 (uint64_t)OPT.xcd_lag, (uint64_t)OPT.xcd_occ, (uint64_t)OPT.xcd_l2};
+    for (uint64_t i = 0; i < count; i++) {
+        dev_ready(polys[i]);
+        pairs.push_back({polys[i], polys[i]});
+        key.push_back((uint64_t)(uintptr_t)polys[i]->ntt);
+        key.push_back(polys[i]->rns_mask);
+        key.push_back(polys[i]->allocated_l);
+    }
+    plan_run(key, [&] { ntt_rows(pairs, direction ? 1 : 0); }, [] { return std::vector<uint64_t>(); },
+             [](const std::vector<uint64_t> &) {});
+}
+
 // Allocation-free form of vfhe_hip_ntt_batch: the same transforms, run by the
 // persistent per-XCD kernel (one launch per direction) wherever it applies (FP64
 // limbs, N = 2^13..2^16, XCD placement probe passed), with that kernel's work list
