@@ -296,6 +296,12 @@ def main():
     ys = [pool.poly(rt.random_rows(rng, primes, N)) for _ in range(POLYS)]
     zs = [pool.poly() for _ in range(POLYS)]
     ax, ay, az = _native.voidp_array(xs), _native.voidp_array(ys), _native.voidp_array(zs)
+    fused_tuned = None
+    if not args.no_autotune:
+        fus = (ctypes.c_double * 6)()
+        fbest = lib.vfhe_hip_autotune_product(az, ax, ay, POLYS, 3, fus)
+        fused_tuned = {"candidate": int(fbest), "us": [round(u, 1) for u in fus]}
+        log("fused product autotune:", fused_tuned)
     fused_dt = timed(lib, lambda: lib.vfhe_hip_ntt_mul_intt_batch(az, ax, ay, POLYS), args.steps, 2)
     log(f"fused product: {fused_dt * 1e3:.3f} ms per 64 products")
     # the same through a graph replay (one host launch per batch), checked against the eager output
@@ -381,6 +387,7 @@ def main():
         "fused": {"unit": "products/s", "value": round(POLYS / fused_dt, 2), "ms_per_batch": round(fused_dt * 1e3, 4),
                   "graph": {"value": round(POLYS / fused_graph_dt, 2), "ms_per_batch": round(fused_graph_dt * 1e3, 4),
                             "equals_eager": fused_graph_equal},
+                  "autotune": fused_tuned,
                   "workload": "z=INTT(NTT(x)*NTT(y)), 64 pairs x 24 limbs, N=2^16"},
         "plain": {"device_resident_round_trips_per_s": round(POLYS / plain_dev_dt, 2),
                   "host_resident_round_trips_per_s": round(POLYS / plain_host_dt, 2),
@@ -434,6 +441,17 @@ def bench_ckks(lib, _native, params, rt, args):
             enq.append(t1 - t0)
         return np.array(lat) * 1e3, np.array(enq) * 1e3
 
+    # the multiply's schedule (column-pass width, inverse-pass grid, pass-through fold), timed on
+    # these operands by the engine's tuner (every candidate is bit-exact)
+    tuned = None
+    if not args.no_autotune:
+        us = (ctypes.c_double * 12)()
+        best = lib.vfhe_hip_autotune_multiply(ao, a1, a2, B, rlk, top, 2, us)
+        tuned = {"candidate": int(best), "us": [round(u, 1) for u in us],
+                 "ntt.col_wide": lib.vfhe_hip_get_option(b"ntt.col_wide"),
+                 "ntt.tile_grid": lib.vfhe_hip_get_option(b"ntt.tile_grid"),
+                 "ks.fold": lib.vfhe_hip_get_option(b"ks.fold")}
+        log("ckks autotune:", tuned)
     run(lib.vfhe_hip_mlwe_multiply_batch, 2)
     with Prof(lib) as prof:
         lat, enq = run(lib.vfhe_hip_mlwe_multiply_batch, args.ckks_steps)
@@ -499,6 +517,7 @@ def bench_ckks(lib, _native, params, rt, args):
         "unbatched_mults_per_s": round(1.0 / float(np.median(one)), 2),
         "kernel_ms_per_batch": {n: round(m / args.ckks_steps, 3) for n, m in zip(names, prof.ms)},
         "parity": parity,
+        "autotune": tuned,
         "cpu_baseline": cpu,
         "workload": "multiply+relinearize+rescale, N=2^16, L=24+4 special, rank 1, batch of 32 pairs, "
                     "random rlk (throughput does not depend on key validity)",

@@ -404,6 +404,18 @@ void vfhe_hip_chacha20_blocks(const uint32_t *key, uint64_t nonce, uint64_t ctr0
    us_out (9 entries, may be NULL) receives each candidate's microseconds per
    round trip, -1 when not applicable. */
 long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, double *us_out);
+/* The same for the fused product (vfhe_hip_ntt_mul_intt_batch): its row kernel one tile per
+   workgroup or persistent x its column passes at 256 / 512-by-rule / 512 threads, 6
+   candidates timed on the caller's operands (z is overwritten); the fastest is adopted and
+   its index returned, us_out[k] (if not null) the microseconds per call of candidate k. */
+long vfhe_hip_autotune_product(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polynomial *y, uint64_t count, int reps,
+                               double *us_out);
+/* The same for the CKKS multiply batch (vfhe_hip_mlwe_multiply_batch): column-pass width x
+   inverse passes one tile per workgroup or persistent x the pass-through components folded
+   into the key inner product or added after the ModDown, 12 candidates timed on the
+   caller's operands (out is overwritten).  Every candidate is bit-exact. */
+long vfhe_hip_autotune_multiply(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLWE *in2, uint64_t count, RNS_MLWE **rlk,
+                                uint64_t divide_mask, int reps, double *us_out);
 
 /* Launch profiler: between start and stop every kernel launch is bracketed by
    HIP events on the engine stream.  stop() waits, then writes the summed

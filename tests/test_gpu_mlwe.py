@@ -331,3 +331,41 @@ def test_mlwe_mul_by_poly(lib, N, L, split, mode, fn):
     lib.free_mlwe_RNS_sample(cin)
     lib.free_mlwe_RNS_sample(cout)
     s.pool.free(poly)
+
+
+def test_multiply_autotuner_adopts_a_timed_schedule(lib):
+    """vfhe_hip_autotune_multiply times its 12 schedules on a batch, adopts the fastest (only the
+    options it tunes change), and a multiply afterwards still equals the oracle."""
+    import ctypes
+    s = Setup(8192, 4, 2, 91)
+    r, m = 1, s.lvl_mask
+    rlk = [s.key(s.L, r), None]
+    rlk_arr = s.outer(rlk)
+    cts = [(s.ct(r, m), s.ct(r, m)) for _ in range(2)]
+    top = 1 << (s.L - 1)
+    keys = (b"ntt.col_wide", b"ntt.tile_grid", b"ks.fold", b"ntt.mode", b"ks.fused", b"ks.mdr_fp64")
+    before = {k: lib.vfhe_hip_get_option(k) for k in keys}
+    outs = [s.pool.sample(r, s.ext_mask) for _ in cts]
+    us = (ctypes.c_double * 12)()
+    try:
+        best = lib.vfhe_hip_autotune_multiply(_native.voidp_array(outs), _native.voidp_array([c[0][1] for c in cts]),
+                                              _native.voidp_array([c[1][1] for c in cts]), len(cts), rlk_arr, top, 1, us)
+        print("[autotune multiply] best", best, [round(u, 1) for u in us], flush=True)
+        assert 0 <= best < 12 and all(u > 0 for u in us)
+        after = {k: lib.vfhe_hip_get_option(k) for k in keys}
+        for k in (b"ntt.mode", b"ks.fused", b"ks.mdr_fp64"):
+            assert after[k] == before[k]
+        assert after[b"ntt.col_wide"] == best % 3 and after[b"ks.fold"] == (0 if best // 6 else 1)
+        o = s.pool.sample(r, s.ext_mask)
+        lib.vfhe_hip_mlwe_multiply_batch(_native.voidp_array([o]), _native.voidp_array([cts[0][0][1]]),
+                                         _native.voidp_array([cts[0][1][1]]), 1, rlk_arr, top)
+        (d1, _), (d2, _) = cts[0]
+        ref = np.zeros((r + 1, s.L + s.K, s.N), dtype=np.uint64)
+        mref = s.octx.multiply(ref, d1, d2, r, m, [rlk[0][0], None], s.ext_mask)
+        for c in range(r + 1):
+            s.octx.to_coeff(ref[c], ref[c], mref)
+        m2 = s.octx.mlwe_round_division(ref, r, mref, top)
+        s.check(s.pool.read_sample(o), ref, m2)
+    finally:
+        for k, v in before.items():
+            lib.vfhe_hip_set_option(k, v)

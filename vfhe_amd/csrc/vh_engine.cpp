@@ -4396,6 +4396,94 @@ extern "C" long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, doubl
     return best;
 }
 
+// Times `run` under each candidate setting of the options `apply` sets (two interleaved
+// rounds of `reps` calls after a warm-up, the faster round kept), adopts the fastest and
+// returns its index; us_out[k] = microseconds per call of candidate k.  Options the
+// candidates do not set keep the caller's values.
+static long autotune_generic(long nc, const std::function<void(long)> &apply, const std::function<void()> &run,
+                             int reps, double *us_out)
+{
+    VH_REQUIRE(nc > 0 && reps > 0);
+    const vh_opts saved = OPT;
+    hipEvent_t e0, e1;
+    VH_CHECK(hipEventCreate(&e0));
+    VH_CHECK(hipEventCreate(&e1));
+    std::vector<double> us(nc, -1.0);
+    for (int round = 0; round < 2; round++)
+        for (long k = 0; k < nc; k++) {
+            OPT = saved;
+            apply(k);
+            for (int w = 0; w < (round == 0 ? 2 : 1); w++) run();   // warm-up (tables, caches)
+            VH_CHECK(hipEventRecord(e0, S()));
+            for (int r = 0; r < reps; r++) run();
+            VH_CHECK(hipEventRecord(e1, S()));
+            VH_CHECK(hipEventSynchronize(e1));
+            vh_check_device_error();
+            float ms = 0;
+            VH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            const double t = ms * 1e3 / reps;
+            if (us[k] < 0 || t < us[k]) us[k] = t;
+        }
+    VH_CHECK(hipEventDestroy(e0));
+    VH_CHECK(hipEventDestroy(e1));
+    long best = 0;
+    for (long k = 0; k < nc; k++) {
+        if (us_out) us_out[k] = us[k];
+        if (us[k] < us[best]) best = k;
+    }
+    OPT = saved;
+    apply(best);
+    return best;
+}
+
+// The fused product's schedule: its row kernel launched one tile per workgroup or
+// persistent (ntt.tile_grid bit 4) x its column passes at 256 threads, 512 threads by the
+// occupancy rule, or 512 threads always (ntt.col_wide 0 / 1 / 2): 6 candidates, timed on
+// the caller's operands (z is overwritten).
+extern "C" long vfhe_hip_autotune_product(RNS_Polynomial *z, RNS_Polynomial *x, RNS_Polynomial *y, uint64_t count,
+                                          int reps, double *us_out)
+{
+    VH_ENTER;
+    VH_REQUIRE(z && x && y && count > 0);
+    return autotune_generic(
+        6,
+        [&](long k) {
+            OPT.ntt_tile_grid = (OPT.ntt_tile_grid & ~16L) | ((k & 1) ? 0L : 16L);
+            OPT.ntt_col_wide = k >> 1;
+        },
+        [&] { ntt_mul_intt(z, x, y, count); }, reps, us_out);
+}
+
+// The CKKS multiply batch's schedule: column passes (the ModUp pass and the inverse ones)
+// at 256 / 512-by-rule / 512 threads (ntt.col_wide), the inverse passes one tile per
+// workgroup or persistent (ntt.tile_grid bits 2 and 3), and the pass-through components
+// folded into the key inner product or added after the ModDown (ks.fold): 12 candidates,
+// timed on the caller's operands (out is overwritten).  Every candidate is bit-exact.
+extern "C" long vfhe_hip_autotune_multiply(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLWE *in2, uint64_t count,
+                                           RNS_MLWE **rlk, uint64_t divide_mask, int reps, double *us_out)
+{
+    VH_ENTER;
+    VH_REQUIRE(out && in1 && in2 && rlk && count > 0);
+    const std::vector<RNS_MLWE> o(out, out + count), a(in1, in1 + count), b(in2, in2 + count);
+    const std::vector<RNS_MLWE **> ks(count, rlk);
+    const std::vector<uint64_t> dm(count, divide_mask);
+    std::vector<uint64_t> masks;
+    for (auto c : o) for (auto p : comps(c)) masks.push_back(p->rns_mask);
+    return autotune_generic(
+        12,
+        [&](long k) {
+            OPT.ntt_col_wide = k % 3;
+            OPT.ntt_tile_grid = (OPT.ntt_tile_grid & ~12L) | (((k / 3) & 1) ? 0L : 12L);
+            OPT.ks_fold = (k / 6) ? 0 : 1;
+        },
+        [&] {
+            size_t i = 0;
+            for (auto c : o) for (auto p : comps(c)) p->rns_mask = masks[i++];   // every call starts alike
+            multiply_batch(o, a, b, ks, dm);
+        },
+        reps, us_out);
+}
+
 // ---------------------------------------------------------------------------
 // Profiler API
 // ---------------------------------------------------------------------------
