@@ -434,6 +434,10 @@ def test_ciphertext_and_key_files_round_trip(tmp_path):
         Batch.from_bytes(bytes(data), chain, cts.lvl)
     with pytest.raises(ValueError, match="truncated"):
         Batch.from_bytes(cts.to_bytes()[:-8], chain, cts.lvl)
+    forged = np.frombuffer(cts.to_bytes(), dtype=np.uint64).copy()
+    forged[2] = 1 << 31                        # a header whose sample count the data cannot hold
+    with pytest.raises(ValueError, match="truncated"):
+        Batch.from_bytes(forged.tobytes(), chain, cts.lvl)
     other = Ring(2 * N, prime_size=[40, 40], split_degree=1)   # another prime pool (another N)
     with pytest.raises(ValueError, match="another ring"):
         Batch.from_bytes(cts.to_bytes(), chain, cts.lvl, ring=other)

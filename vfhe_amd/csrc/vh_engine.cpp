@@ -3568,9 +3568,12 @@ static uint64_t vh_fmt_check(const uint64_t *w, uint64_t n)
     return h;
 }
 
+// Words of a block; UINT64_MAX when that does not fit 64 bits (a header read from a file
+// may hold any values).
 static uint64_t vh_fmt_words(uint64_t count, uint64_t r, uint64_t nact, uint64_t N)
 {
-    return 8 + nact + count * (r + 1) * nact * N + 1;
+    const u128 w = (u128)8 + nact + (u128)count * (r + 1) * nact * N + 1;
+    return w >> 61 ? UINT64_MAX : (uint64_t)w;   // also keeps 8 * words below 2^64
 }
 
 extern "C" uint64_t vfhe_hip_mlwe_serialized_size(RNS_MLWE *in, uint64_t count)
@@ -3578,7 +3581,8 @@ extern "C" uint64_t vfhe_hip_mlwe_serialized_size(RNS_MLWE *in, uint64_t count)
     VH_ENTER;
     if (count == 0 || !in || !in[0]) return 0;
     const uint64_t nact = (uint64_t)__builtin_popcountll(in[0]->b->rns_mask);
-    return 8 * vh_fmt_words(count, in[0]->r, nact, in[0]->b->ntt->N);
+    const uint64_t words = vh_fmt_words(count, in[0]->r, nact, in[0]->b->ntt->N);
+    return words == UINT64_MAX ? 0 : 8 * words;
 }
 
 extern "C" int64_t vfhe_hip_mlwe_serialize(RNS_MLWE *in, uint64_t count, uint32_t form, void *buf, uint64_t size)
@@ -3590,7 +3594,7 @@ extern "C" int64_t vfhe_hip_mlwe_serialize(RNS_MLWE *in, uint64_t count, uint32_
     std::vector<int> act;
     for (uint64_t i = 0; i < t->l; i++) if (ACTIVE(mask, i)) act.push_back((int)i);
     const uint64_t nact = act.size(), words = vh_fmt_words(count, r, nact, N);
-    if (size < 8 * words || ((uintptr_t)buf & 7)) return -1;
+    if (words == UINT64_MAX || size / 8 < words || ((uintptr_t)buf & 7)) return -1;
     for (uint64_t k = 0; k < count; k++) {
         if (!in[k] || in[k]->r != r || in[k]->b->ntt != t) return -1;
         for (auto p : comps(in[k])) if (p->rns_mask != mask) return -1;
@@ -3631,7 +3635,7 @@ extern "C" int64_t vfhe_hip_mlwe_deserialize(const void *buf, uint64_t size, inc
     if (nact != (uint64_t)__builtin_popcountll(mask) || r > 64 || N == 0 || N > (1ULL << 24) || count > (1ULL << 32))
         return -1;
     const uint64_t words = vh_fmt_words(count, r, nact, N);
-    if (size < 8 * words) return -4;
+    if (words == UINT64_MAX || size / 8 < words) return -4;
     if (vh_fmt_check(w, words - 1) != w[words - 1]) return -1;
     if (N != t->N || split != t->split_degree || (t->l < 64 && (mask >> t->l))) return -2;
     std::vector<int> act;
