@@ -352,6 +352,17 @@ def main():
         lib.vfhe_hip_release_cached()
         serve = bench_serving(lib, args)
 
+    # per-operation rates behind the choice of FP64 modular arithmetic (q < 2^50) over a
+    # 64-bit integer Shoup product: each kind is 8 dependent-free chains per thread on
+    # 8 workgroups of 256 threads per CU
+    rates = (ctypes.c_double * 6)()
+    lib.vfhe_hip_probe_alu_rates(rates)
+    alu = {"unit": "Gop/s", "fp64_fma": round(rates[0] / 1e9, 1), "u32_mul_lo_add": round(rates[1] / 1e9, 1),
+           "u32_mul_hi": round(rates[2] / 1e9, 1), "u32x32_to_64_mad": round(rates[3] / 1e9, 1),
+           "modmul_fp64": round(rates[4] / 1e9, 1), "modmul_shoup_u64": round(rates[5] / 1e9, 1),
+           "modmul_fp64_over_shoup": round(rates[4] / rates[5], 2)}
+    log("alu rates (Gop/s):", alu)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline:", args.cpu_seconds, "s")
@@ -396,6 +407,7 @@ def main():
         "ckks": ckks,
         "cggi": cggi,
         "serving": serve,
+        "alu": alu,
     }
     print(json.dumps(line), flush=True)
 

@@ -403,6 +403,10 @@ void vfhe_hip_chacha20_blocks(const uint32_t *key, uint64_t nonce, uint64_t ctr0
    available), keeps the fastest in the options above and returns its index;
    us_out (9 entries, may be NULL) receives each candidate's microseconds per
    round trip, -1 when not applicable. */
+/* Measured device-wide operation rates (operations per second): [0] FP64 FMA, [1] 32-bit
+   multiply-low + add, [2] 32-bit multiply-high, [3] 32x32->64 multiply-add, [4] FP64 modular
+   products (q < 2^50), [5] 64-bit Shoup modular products. */
+void vfhe_hip_probe_alu_rates(double *rates);
 long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, double *us_out);
 /* The same for the fused product (vfhe_hip_ntt_mul_intt_batch): its row kernel one tile per
    workgroup or persistent x its column passes at 256 / 512-by-rule / 512 threads, 6

@@ -173,6 +173,7 @@ _SIGNATURES = {
     "vfhe_hip_synchronize": (None, []),
     "vfhe_hip_set_lane": (ctypes.c_int, [ctypes.c_int]),
     "vfhe_hip_autotune_ntt": (ctypes.c_long, [VP, U64, I32, ctypes.POINTER(ctypes.c_double)]),
+    "vfhe_hip_probe_alu_rates": (None, [ctypes.POINTER(ctypes.c_double)]),
     "vfhe_hip_autotune_product": (ctypes.c_long, [VPP, VPP, VPP, U64, I32, ctypes.POINTER(ctypes.c_double)]),
     "vfhe_hip_autotune_multiply": (ctypes.c_long, [VPP, VPP, VPP, U64, VPP, U64, I32,
                                                    ctypes.POINTER(ctypes.c_double)]),

@@ -4488,6 +4488,36 @@ extern "C" long vfhe_hip_autotune_multiply(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLW
         reps, us_out);
 }
 
+// Measured per-lane operation rates of the device (operations per second over the whole
+// chip): rates[0] FP64 FMA, [1] 32-bit multiply-low + add, [2] 32-bit multiply-high,
+// [3] 32x32->64 multiply-add, [4] FP64 modular products (vh_mulmod_tw, q < 2^50),
+// [5] 64-bit Shoup modular products.  Each kind runs on 8 workgroups of 256 threads per
+// CU, timed with events after one warm-up launch.
+extern "C" void vfhe_hip_probe_alu_rates(double *rates)
+{
+    VH_ENTER;
+    VH_REQUIRE(rates);
+    const unsigned blocks = (unsigned)C.cus * 8;
+    const int iters = 4096;
+    uint64_t *d = (uint64_t *)vh_dmalloc((size_t)blocks * 256 * 8);
+    hipEvent_t e0, e1;
+    VH_CHECK(hipEventCreate(&e0));
+    VH_CHECK(hipEventCreate(&e1));
+    for (int kind = 0; kind < 6; kind++) {
+        VH_LAUNCH(vh_launch_alu_rate(S(), kind, iters, blocks, d));
+        VH_CHECK(hipEventRecord(e0, S()));
+        VH_LAUNCH(vh_launch_alu_rate(S(), kind, iters, blocks, d));
+        VH_CHECK(hipEventRecord(e1, S()));
+        VH_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        VH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        rates[kind] = (double)blocks * 256 * 8 * iters / (ms * 1e-3);
+    }
+    VH_CHECK(hipEventDestroy(e0));
+    VH_CHECK(hipEventDestroy(e1));
+    vh_dfree(d);
+}
+
 // ---------------------------------------------------------------------------
 // Profiler API
 // ---------------------------------------------------------------------------

@@ -118,6 +118,8 @@ struct vh_br_params {
 hipError_t vh_launch_blind_rotate(hipStream_t st, const vh_br_params &P, const vh_limb *limbs, uint64_t *const *accp,
                                   const uint64_t *const *bkp, const uint64_t *amounts, unsigned count,
                                   const uint64_t *const *tvp = nullptr, const uint64_t *rot = nullptr);
+// ALU rate probe (see k_alu_rate): blocks x 256 threads, 8 chains x iters operations each.
+hipError_t vh_launch_alu_rate(hipStream_t st, int kind, int iters, unsigned blocks, uint64_t *out);
 hipError_t vh_launch_ntt_pass(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,
                               int fp, int dir, int logn, int s0, int R, int col, int G, int first, int last);
 // Register radix-16 pass (FP64 path): 4 <= R <= 8, G = 4096 / 2^R sub-transforms per block.

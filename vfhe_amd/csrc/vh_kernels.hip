@@ -1986,6 +1986,68 @@ hipError_t vh_launch_blind_rotate(hipStream_t st, const vh_br_params &P, const v
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// ALU rate probes: each thread runs 8 independent chains of `iters` operations of one
+// kind (the chains keep the pipeline full), then stores a value that depends on all of
+// them so nothing is optimised away.  kind: 0 FP64 FMA, 1 32-bit multiply (low half) +
+// add, 2 32-bit multiply high half, 3 32x32->64 multiply-add, 4 FP64 modular product
+// (vh_mulmod_tw), 5 64-bit Shoup modular product (vh_shoup).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_alu_rate(int kind, int iters, uint64_t *out)
+{
+    const unsigned id = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t acc = 0;
+    if (kind == 0) {
+        double x[8];
+        for (int k = 0; k < 8; k++) x[k] = 1.0 + 1e-9 * (id + k);
+        const double a = 0.999999999, b = 1e-12;
+        for (int i = 0; i < iters; i++)
+#pragma unroll
+            for (int k = 0; k < 8; k++) x[k] = __fma_rn(x[k], a, b);
+        for (int k = 0; k < 8; k++) acc += (uint64_t)__double_as_longlong(x[k]);
+    } else if (kind == 1 || kind == 2) {
+        uint32_t x[8];
+        for (int k = 0; k < 8; k++) x[k] = 0x9E3779B9u * (id + k) + 7u;
+        const uint32_t a = 0x85EBCA6Bu, b = 0xC2B2AE35u;
+        for (int i = 0; i < iters; i++)
+#pragma unroll
+            for (int k = 0; k < 8; k++) x[k] = kind == 1 ? x[k] * a + b : __umulhi(x[k], a) ^ b;
+        for (int k = 0; k < 8; k++) acc += x[k];
+    } else if (kind == 3) {
+        uint64_t x[8];
+        for (int k = 0; k < 8; k++) x[k] = 0x9E3779B97F4A7C15ull * (id + k);
+        const uint32_t a = 0x85EBCA6Bu;
+        for (int i = 0; i < iters; i++)
+#pragma unroll
+            for (int k = 0; k < 8; k++) x[k] = (uint64_t)(uint32_t)x[k] * a + (x[k] >> 32);
+        for (int k = 0; k < 8; k++) acc += x[k];
+    } else if (kind == 4) {
+        const double q = 1125899906842597.0, w = 392817365281.0, wq = w / q;   // a prime below 2^50
+        double x[8];
+        for (int k = 0; k < 8; k++) x[k] = (double)((id * 8 + k) % 1000003);
+        for (int i = 0; i < iters; i++)
+#pragma unroll
+            for (int k = 0; k < 8; k++) x[k] = vh_mulmod_tw(x[k], w, wq, q);
+        for (int k = 0; k < 8; k++) acc += (uint64_t)__double_as_longlong(x[k]);
+    } else {
+        const uint64_t q = 1125899906842597ull, w = 392817365281ull;
+        const uint64_t wsh = (uint64_t)(((unsigned __int128)w << 64) / q);
+        uint64_t x[8];
+        for (int k = 0; k < 8; k++) x[k] = (id * 8ull + k) % q;
+        for (int i = 0; i < iters; i++)
+#pragma unroll
+            for (int k = 0; k < 8; k++) x[k] = vh_shoup(x[k], w, wsh, q);
+        for (int k = 0; k < 8; k++) acc += x[k];
+    }
+    out[id] = acc;
+}
+
+hipError_t vh_launch_alu_rate(hipStream_t st, int kind, int iters, unsigned blocks, uint64_t *out)
+{
+    VH_KL(k_alu_rate, dim3(blocks), dim3(256), 0, st, kind, iters, out);
+    return hipGetLastError();
+}
+
 hipError_t vh_launch_ntt_small(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs, int dir,
                                int logn)
 {
