@@ -403,11 +403,11 @@ void vfhe_hip_chacha20_blocks(const uint32_t *key, uint64_t nonce, uint64_t ctr0
    available), keeps the fastest in the options above and returns its index;
    us_out (9 entries, may be NULL) receives each candidate's microseconds per
    round trip, -1 when not applicable. */
+long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, double *us_out);
 /* Measured device-wide operation rates (operations per second): [0] FP64 FMA, [1] 32-bit
    multiply-low + add, [2] 32-bit multiply-high, [3] 32x32->64 multiply-add, [4] FP64 modular
    products (q < 2^50), [5] 64-bit Shoup modular products. */
 void vfhe_hip_probe_alu_rates(double *rates);
-long vfhe_hip_autotune_ntt(incNTT ntt, uint64_t rows, int reps, double *us_out);
 /* The same for the fused product (vfhe_hip_ntt_mul_intt_batch): its row kernel one tile per
    workgroup or persistent x its column passes at 256 / 512-by-rule / 512 threads, 6
    candidates timed on the caller's operands (z is overwritten); the fastest is adopted and
@@ -592,6 +592,43 @@ uint64_t vfhe_hip_server_poll_timed(void *srv, uint64_t *ids, RNS_MLWE *outs, do
 void vfhe_hip_server_stats(void *srv, double *stats);
 /* Drains every submitted request, then stops the worker. */
 void vfhe_hip_server_free(void *srv);
+
+/* ---- CGGI over the 32-bit torus with an FP64 FFT ----
+   No counterpart in the reference, whose CGGI16 (modules/fhe/python/src/vfhe/fhe/cggi16.py:
+   46-140) bootstraps over RNS primes with the NTT.  The same functional bootstrap (rotation
+   amounts of cggi16.py:54-86 with q = 2^32, the test vector rotated by -rot_b, one CMUX per
+   key index with a non-zero amount, extraction of coefficient 0) on the discretised torus
+   Z/2^32: LWE samples of dimension n, one GLWE ring of N = 1024 (rank 1), signed gadget
+   decomposition in base 2^base_log with ell levels, external products through a negacyclic
+   FP64 FFT of N/2 points.  Torus values are uint32; samples are stored a_0 .. a_(dim-2), b
+   with phase b - <a, s> mod 2^32.  Entry points return NULL / a negative code on bad
+   arguments instead of aborting. */
+typedef struct vfhe_torus_key vfhe_torus_key;
+typedef struct vfhe_torus_batch vfhe_torus_batch;
+/* A device-resident batch of `count` rows of `dim` torus values (samples, accumulators
+   or a test vector), copied from `samples` (count x dim) or zeroed when it is NULL. */
+vfhe_torus_batch *vfhe_hip_torus_batch_new(const uint32_t *samples, uint64_t count, uint64_t dim);
+/* Copies the batch to `samples` (count x dim) after the work queued on the calling lane. */
+int vfhe_hip_torus_batch_read(const vfhe_torus_batch *b, uint32_t *samples);
+void vfhe_hip_torus_batch_free(vfhe_torus_batch *b);
+/* The bootstrapping key: n GGSW encryptions of the LWE key bits, bsk laid out
+   [n][2 ell rows][2 components][N] (row c * ell + l encrypts s_i * 2^(32 - base_log (l+1))
+   added to component c: 0 the mask, 1 the body), moved to the transform domain on the
+   device.  N must be 1024; ell * base_log <= 32, base_log <= 16. */
+vfhe_torus_key *vfhe_hip_torus_key_new(const uint32_t *bsk, uint64_t n, uint64_t N, uint64_t ell, uint32_t base_log);
+void vfhe_hip_torus_key_free(vfhe_torus_key *key);
+/* Functional bootstrap of every sample of `in` (dim n + 1) with test vector `tv` (1 x N):
+   out (count x (N + 1)) receives the extracted samples (under the GLWE key's coefficients),
+   acc_out (count x 2N, may be NULL) the accumulators (mask then body).  torus_base as
+   cggi16.py:60: b is offset by floor(2^32 / (4 torus_base)) before its rounding.  One
+   launch, queued on the calling lane.  Returns 0, or -1 / -2 / -3 for bad arguments,
+   shapes of `in` / `tv`, shapes of the outputs. */
+int vfhe_hip_torus_bootstrap(vfhe_torus_batch *out, vfhe_torus_batch *acc_out, const vfhe_torus_key *key,
+                             const vfhe_torus_batch *in, const vfhe_torus_batch *tv, uint64_t torus_base);
+/* The negacyclic transform of `count` polynomials (N = 1024 coefficients read as signed
+   32-bit integers): out[k] = p(zeta w^k), zeta = e^(i pi / N), w = e^(-2 pi i / (N/2)),
+   k < N/2, as (re, im) pairs, natural order.  Returns 0 or -1. */
+int vfhe_hip_torus_fft(const uint32_t *in, uint64_t count, uint64_t N, double *out);
 
 #ifdef __cplusplus
 }

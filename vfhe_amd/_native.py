@@ -174,6 +174,13 @@ _SIGNATURES = {
     "vfhe_hip_set_lane": (ctypes.c_int, [ctypes.c_int]),
     "vfhe_hip_autotune_ntt": (ctypes.c_long, [VP, U64, I32, ctypes.POINTER(ctypes.c_double)]),
     "vfhe_hip_probe_alu_rates": (None, [ctypes.POINTER(ctypes.c_double)]),
+    "vfhe_hip_torus_batch_new": (VP, [VP, U64, U64]),
+    "vfhe_hip_torus_batch_read": (I32, [VP, VP]),
+    "vfhe_hip_torus_batch_free": (None, [VP]),
+    "vfhe_hip_torus_key_new": (VP, [VP, U64, U64, U64, ctypes.c_uint32]),
+    "vfhe_hip_torus_key_free": (None, [VP]),
+    "vfhe_hip_torus_bootstrap": (I32, [VP, VP, VP, VP, VP, U64]),
+    "vfhe_hip_torus_fft": (I32, [VP, U64, U64, VP]),
     "vfhe_hip_autotune_product": (ctypes.c_long, [VPP, VPP, VPP, U64, I32, ctypes.POINTER(ctypes.c_double)]),
     "vfhe_hip_autotune_multiply": (ctypes.c_long, [VPP, VPP, VPP, U64, VPP, U64, I32,
                                                    ctypes.POINTER(ctypes.c_double)]),

@@ -19,6 +19,16 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// An exchange through LDS among the lanes of one wave: a wave's LDS instructions
+// execute in order, so only the compiler has to be kept from moving loads and stores
+// across the exchange, which the fences do.
+__device__ __forceinline__ void vh_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Per-limb constants, device resident (one per prime of a pool).
 struct vh_limb {
     uint64_t q;

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -4486,6 +4487,147 @@ extern "C" long vfhe_hip_autotune_multiply(RNS_MLWE *out, RNS_MLWE *in1, RNS_MLW
             multiply_batch(o, a, b, ks, dm);
         },
         reps, us_out);
+}
+
+// ---------------------------------------------------------------------------
+// CGGI over the 32-bit torus with an FP64 FFT (vh_torus.hip).  No counterpart in
+// the reference (its CGGI16 runs over RNS primes, cggi16.py:46-140); the entry
+// points return an error code instead of aborting on bad arguments.
+// ---------------------------------------------------------------------------
+struct vfhe_torus_key {
+    uint64_t n, N, ell;
+    uint32_t base_log;
+    double2 *bsk;
+};
+struct vfhe_torus_batch {
+    uint64_t count, dim;
+    uint32_t *d;
+};
+static double2 *g_torus_consts = nullptr;
+
+// The transform constants (twiddles and twist), from long double cos / sin rounded once.
+static const double2 *torus_consts()
+{
+    if (g_torus_consts) return g_torus_consts;
+    std::vector<double2> h(1088);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    auto root = [&](long double num, long double den) {
+        const long double a = pi * num / den;
+        return make_double2((double)cosl(a), (double)sinl(a));
+    };
+    for (int r = 0; r < 64; r++)
+        for (int s = 0; s < 8; s++) h[r * 8 + s] = root(-2.0L * ((r * s) % 512), 512);
+    for (int a = 0; a < 8; a++)
+        for (int s = 0; s < 8; s++) h[512 + a * 8 + s] = root(-2.0L * ((8 * a * s) % 512), 512);
+    for (int j = 0; j < 512; j++) h[576 + j] = root(j, 1024);
+    double2 *d = nullptr;
+    VH_CHECK(hipMalloc(&d, h.size() * sizeof(double2)));
+    VH_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+    g_torus_consts = d;
+    return d;
+}
+
+extern "C" vfhe_torus_batch *vfhe_hip_torus_batch_new(const uint32_t *samples, uint64_t count, uint64_t dim)
+{
+    VH_ENTER;
+    if (!count || !dim || dim > (1ull << 32) || count > (1ull << 38) / dim) return nullptr;
+    auto *b = new vfhe_torus_batch{count, dim, nullptr};
+    VH_CHECK(hipMalloc(&b->d, count * dim * 4));
+    if (samples) VH_CHECK(hipMemcpyAsync(b->d, samples, count * dim * 4, hipMemcpyHostToDevice, S()));
+    else VH_CHECK(hipMemsetAsync(b->d, 0, count * dim * 4, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
+    return b;
+}
+
+extern "C" int vfhe_hip_torus_batch_read(const vfhe_torus_batch *b, uint32_t *samples)
+{
+    VH_ENTER;
+    if (!b || !samples) return -1;
+    VH_CHECK(hipMemcpyAsync(samples, b->d, b->count * b->dim * 4, hipMemcpyDeviceToHost, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
+    return 0;
+}
+
+extern "C" void vfhe_hip_torus_batch_free(vfhe_torus_batch *b)
+{
+    VH_ENTER;
+    if (!b) return;
+    sync_all_lanes();
+    VH_CHECK(hipFree(b->d));
+    delete b;
+}
+
+extern "C" vfhe_torus_key *vfhe_hip_torus_key_new(const uint32_t *bsk, uint64_t n, uint64_t N, uint64_t ell,
+                                                  uint32_t base_log)
+{
+    VH_ENTER;
+    if (!bsk || N != 1024 || !n || n > (1u << 20) || !ell || !base_log || base_log > 16 || ell * base_log > 32)
+        return nullptr;
+    const uint64_t polys = n * 4 * ell;
+    uint32_t *d_in = nullptr;
+    auto *k = new vfhe_torus_key{n, N, ell, base_log, nullptr};
+    VH_CHECK(hipMalloc(&d_in, polys * N * 4));
+    VH_CHECK(hipMalloc(&k->bsk, polys * (N / 2) * sizeof(double2)));
+    VH_CHECK(hipMemcpyAsync(d_in, bsk, polys * N * 4, hipMemcpyHostToDevice, S()));
+    VH_LAUNCH(vh_launch_torus_fft(S(), d_in, k->bsk, torus_consts(), polys, 0));
+    VH_CHECK(hipStreamSynchronize(S()));
+    VH_CHECK(hipFree(d_in));
+    return k;
+}
+
+extern "C" void vfhe_hip_torus_key_free(vfhe_torus_key *k)
+{
+    VH_ENTER;
+    if (!k) return;
+    sync_all_lanes();
+    VH_CHECK(hipFree(k->bsk));
+    delete k;
+}
+
+extern "C" int vfhe_hip_torus_bootstrap(vfhe_torus_batch *out, vfhe_torus_batch *acc_out, const vfhe_torus_key *key,
+                                        const vfhe_torus_batch *in, const vfhe_torus_batch *tv, uint64_t torus_base)
+{
+    VH_ENTER;
+    if (!key || !in || !tv || (!out && !acc_out) || !torus_base || torus_base > (1ull << 30)) return -1;
+    const uint64_t N = key->N;
+    if (in->dim != key->n + 1 || tv->count != 1 || tv->dim != N) return -2;
+    if (out && (out->count != in->count || out->dim != N + 1)) return -3;
+    if (acc_out && (acc_out->count != in->count || acc_out->dim != 2 * N)) return -3;
+    if (in->count > 0x7fffffffull) return -1;
+    uint32_t dec_off = 0;
+    for (uint64_t l = 1; l <= key->ell; l++) dec_off += 1u << (32 - key->base_log * l + key->base_log - 1);
+    if (key->base_log * key->ell < 32) dec_off += 1u << (32 - key->base_log * key->ell - 1);
+    vt_br_args A{};
+    A.lwe = in->d;
+    A.tv = tv->d;
+    A.bsk = key->bsk;
+    A.consts = torus_consts();
+    A.out = out ? out->d : nullptr;
+    A.acc_out = acc_out ? acc_out->d : nullptr;
+    A.n = (uint32_t)key->n;
+    A.ell = (uint32_t)key->ell;
+    A.base_log = key->base_log;
+    A.offset = (uint32_t)((1ull << 32) / (4 * torus_base));
+    A.dec_off = dec_off;
+    VH_LAUNCH(vh_launch_torus_blind_rotate(S(), A, in->count));
+    return 0;
+}
+
+extern "C" int vfhe_hip_torus_fft(const uint32_t *in, uint64_t count, uint64_t N, double *out)
+{
+    VH_ENTER;
+    if (!in || !out || N != 1024 || !count || count > (1ull << 24)) return -1;
+    uint32_t *d_in = nullptr;
+    double2 *d_out = nullptr;
+    VH_CHECK(hipMalloc(&d_in, count * N * 4));
+    VH_CHECK(hipMalloc(&d_out, count * (N / 2) * sizeof(double2)));
+    VH_CHECK(hipMemcpyAsync(d_in, in, count * N * 4, hipMemcpyHostToDevice, S()));
+    VH_LAUNCH(vh_launch_torus_fft(S(), d_in, d_out, torus_consts(), count, 1));
+    VH_CHECK(hipMemcpyAsync(out, d_out, count * (N / 2) * sizeof(double2), hipMemcpyDeviceToHost, S()));
+    VH_CHECK(hipStreamSynchronize(S()));
+    VH_CHECK(hipFree(d_in));
+    VH_CHECK(hipFree(d_out));
+    return 0;
 }
 
 // Measured per-lane operation rates of the device (operations per second over the whole

@@ -118,6 +118,24 @@ struct vh_br_params {
 hipError_t vh_launch_blind_rotate(hipStream_t st, const vh_br_params &P, const vh_limb *limbs, uint64_t *const *accp,
                                   const uint64_t *const *bkp, const uint64_t *amounts, unsigned count,
                                   const uint64_t *const *tvp = nullptr, const uint64_t *rot = nullptr);
+// CGGI over the 32-bit torus (vh_torus.hip).  consts: 1088 double2 -- [0, 512) the
+// pass-1 twiddles w^(r s) (r < 64, s < 8), [512, 576) the pass-2 twiddles w^(8 a s)
+// (a, s < 8), [576, 1088) the twist zeta^j (j < 512); w = e^(-2 pi i / 512),
+// zeta = e^(i pi / 1024).
+struct vt_br_args {
+    const uint32_t *lwe;      // count x (n + 1): a_0 .. a_(n-1), b
+    const uint32_t *tv;       // N test-vector coefficients
+    const double2 *bsk;       // n x (2 ell) x 2 x [slot][lane], transform domain
+    const double2 *consts;
+    uint32_t *out;            // count x (N + 1) extracted samples, or null
+    uint32_t *acc_out;        // count x 2N accumulators (a then b), or null
+    uint32_t n, ell, base_log;
+    uint32_t offset;          // floor(2^32 / (4 torus_base)), added to b before its rounding
+    uint32_t dec_off;         // rounding and digit offsets of the signed decomposition
+};
+hipError_t vh_launch_torus_fft(hipStream_t st, const uint32_t *in, double2 *out, const double2 *consts, uint64_t count,
+                               int natural);
+hipError_t vh_launch_torus_blind_rotate(hipStream_t st, const vt_br_args &A, uint64_t count);
 // ALU rate probe (see k_alu_rate): blocks x 256 threads, 8 chains x iters operations each.
 hipError_t vh_launch_alu_rate(hipStream_t st, int kind, int iters, unsigned blocks, uint64_t *out);
 hipError_t vh_launch_ntt_pass(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,

@@ -145,14 +145,7 @@ k_ntt_pass(const vh_item *__restrict__ items, const vh_limb *__restrict__ limbs,
 // 2-way minimum for 64-bit lanes.
 // Row passes keep each row's 16 lanes inside one wave and each row's LDS slice
 // private to it, so their exchanges only need the wave's own LDS operations to
-// stay in program order (they do: a wave's LDS instructions execute in order);
-// the fences stop the compiler from moving loads and stores across the exchange.
-__device__ __forceinline__ void vh_wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+// stay in program order (vh_wave_sync).
 
 // Streaming accesses of the passes: every row is read once and written once per
 // pass, so loads and stores are non-temporal (measured: ~5% on a round trip

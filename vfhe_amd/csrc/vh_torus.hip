@@ -1,0 +1,385 @@
+// vh_torus.hip -- CGGI functional bootstrapping over the 32-bit discretised torus
+// (Z / 2^32) with the external product computed through an FP64 negacyclic FFT.
+//
+// No counterpart in the reference: its CGGI16 (modules/fhe/python/src/vfhe/fhe/
+// cggi16.py:46-140) runs over RNS primes and the NTT.  This keeps that bootstrap's
+// structure -- rotation amounts rounded as cggi16.py:54-86 does, the test vector
+// rotated by -rot_b, one CMUX per key index with a non-zero amount (cggi16.py:76-90),
+// extraction of coefficient 0 (cggi16.py:118-140) -- on the torus, with the external
+// product and signed gadget decomposition of TFHE (Chillotti, Gama, Georgieva,
+// Izabachene, "TFHE: fast fully homomorphic encryption over the torus",
+// J. Cryptology 33 (2020): Definition 3.6 / Lemma 3.7 (decomposition), Theorem 3.13
+// (external product), Algorithm 3 (blind rotation)).
+//
+// Negacyclic FFT.  A real polynomial p of N = 1024 coefficients is evaluated at the
+// N/2 roots x of X^N = -1 with x^(N/2) = i: x = zeta * w^m, zeta = e^(i pi / N),
+// w = e^(-2 pi i / M), M = N/2.  Since x^(N/2) = i,
+//     p(zeta w^m) = sum_{j < M} (p_j + i p_{j+M}) zeta^j w^(jm),
+// an M-point DFT of the folded and twisted sequence z_j = (p_j + i p_{j+M}) zeta^j.
+// Real polynomials take conjugate values at the other N/2 roots, so products of
+// these M values determine the negacyclic product; its inverse gives
+// (r_j + i r_{j+M}) zeta^j, untwisted and unfolded.
+//
+// One wave computes one M = 512-point transform, 8 points per lane, as three
+// radix-8 passes (input index j = 64 j2 + 8 j1 + j0, output k = k2 + 8 k1 + 64 k0):
+//   layout A  lane = j mod 64, slot = j2       forward input, inverse output
+//   layout B  lane = 8 k2 + j0, slot = j1      between the passes
+//   layout C  lane = 8 k2 + k1, slot = k0      forward output (digit-reversed order),
+//                                              the key's storage order, inverse input
+// pass 1: DFT-8 over j2, twiddle w^((8 j1 + j0) k2); pass 2: DFT-8 over j1, twiddle
+// w^(8 j0 k1); pass 3: DFT-8 over j0.  The two exchanges go through a private LDS
+// slice of the wave with row pitches chosen so that every access spreads evenly over
+// the banks (pitch 72 for the A<->B exchange, 65 for B<->C).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vh_kernels.h"
+
+namespace {
+
+constexpr int VT_N = 1024, VT_M = 512;
+constexpr int VT_P1 = 72, VT_P2 = 65;          // exchange row pitches (doubles)
+constexpr int VT_XS = 8 * VT_P1;               // doubles per exchange array (>= 8 * VT_P2)
+
+// S = -1: X_k = sum_j x_j e^(-2 pi i jk / 8); S = +1: the same with +i (no scaling).
+template <int S>
+__device__ __forceinline__ void vt_dft4(double &r0, double &i0, double &r1, double &i1, double &r2, double &i2,
+                                        double &r3, double &i3)
+{
+    const double c0r = r0 + r2, c0i = i0 + i2, d0r = r0 - r2, d0i = i0 - i2;
+    const double c1r = r1 + r3, c1i = i1 + i3, er = r1 - r3, ei = i1 - i3;
+    const double d1r = S > 0 ? -ei : ei, d1i = S > 0 ? er : -er;   // (r1 - r3) * (S i)
+    r0 = c0r + c1r;
+    i0 = c0i + c1i;
+    r2 = c0r - c1r;
+    i2 = c0i - c1i;
+    r1 = d0r + d1r;
+    i1 = d0i + d1i;
+    r3 = d0r - d1r;
+    i3 = d0i - d1i;
+}
+
+template <int S>
+__device__ __forceinline__ void vt_dft8(double *re, double *im)
+{
+    const double h = 0.70710678118654752440;
+    double ar[4], ai[4], br[4], bi[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        ar[j] = re[j] + re[j + 4];
+        ai[j] = im[j] + im[j + 4];
+    }
+    const double d0r = re[0] - re[4], d0i = im[0] - im[4];
+    const double d1r = re[1] - re[5], d1i = im[1] - im[5];
+    const double d2r = re[2] - re[6], d2i = im[2] - im[6];
+    const double d3r = re[3] - re[7], d3i = im[3] - im[7];
+    br[0] = d0r;
+    bi[0] = d0i;
+    br[1] = (S > 0 ? d1r - d1i : d1r + d1i) * h;          // * (1 + S i) / sqrt 2
+    bi[1] = (S > 0 ? d1i + d1r : d1i - d1r) * h;
+    br[2] = S > 0 ? -d2i : d2i;                           // * (S i)
+    bi[2] = S > 0 ? d2r : -d2r;
+    br[3] = (S > 0 ? -d3r - d3i : d3i - d3r) * h;         // * (-1 + S i) / sqrt 2
+    bi[3] = (S > 0 ? d3r - d3i : -d3r - d3i) * h;
+    vt_dft4<S>(ar[0], ai[0], ar[1], ai[1], ar[2], ai[2], ar[3], ai[3]);
+    vt_dft4<S>(br[0], bi[0], br[1], bi[1], br[2], bi[2], br[3], bi[3]);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        re[2 * m] = ar[m];
+        im[2 * m] = ai[m];
+        re[2 * m + 1] = br[m];
+        im[2 * m + 1] = bi[m];
+    }
+}
+
+__device__ __forceinline__ void vt_cmul(double &r, double &i, double wr, double wi)
+{
+    const double t = r * wr - i * wi;
+    i = r * wi + i * wr;
+    r = t;
+}
+
+// The lane's twiddles: t1[s] = w^(lane s), t2[s] = w^(8 (lane mod 8) s), s = 0..7.
+struct vt_tw {
+    double2 t1[8], t2[8];
+};
+
+__device__ __forceinline__ void vt_load_tw(vt_tw &T, const double2 *consts, int lane)
+{
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        T.t1[s] = consts[lane * 8 + s];
+        T.t2[s] = consts[512 + (lane & 7) * 8 + s];
+    }
+}
+
+// Forward transform: layout A in, layout C out.
+__device__ __forceinline__ void vt_fwd(double *re, double *im, double *xr, double *xi, const vt_tw &T, int lane)
+{
+    const int hi = lane >> 3, lo = lane & 7;
+    vt_dft8<-1>(re, im);
+#pragma unroll
+    for (int s = 1; s < 8; s++) vt_cmul(re[s], im[s], T.t1[s].x, T.t1[s].y);
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        xr[VT_P1 * s + lane] = re[s];
+        xi[VT_P1 * s + lane] = im[s];
+    }
+    vh_wave_sync();
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        re[s] = xr[VT_P1 * hi + 8 * s + lo];
+        im[s] = xi[VT_P1 * hi + 8 * s + lo];
+    }
+    vh_wave_sync();
+    vt_dft8<-1>(re, im);
+#pragma unroll
+    for (int s = 1; s < 8; s++) vt_cmul(re[s], im[s], T.t2[s].x, T.t2[s].y);
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        xr[VT_P2 * lo + 8 * hi + s] = re[s];
+        xi[VT_P2 * lo + 8 * hi + s] = im[s];
+    }
+    vh_wave_sync();
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        re[s] = xr[VT_P2 * s + lane];
+        im[s] = xi[VT_P2 * s + lane];
+    }
+    vh_wave_sync();
+    vt_dft8<-1>(re, im);
+}
+
+// Inverse transform (unscaled): layout C in, layout A out.
+__device__ __forceinline__ void vt_inv(double *re, double *im, double *xr, double *xi, const vt_tw &T, int lane)
+{
+    const int hi = lane >> 3, lo = lane & 7;
+    vt_dft8<1>(re, im);
+#pragma unroll
+    for (int s = 1; s < 8; s++) vt_cmul(re[s], im[s], T.t2[s].x, -T.t2[s].y);
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        xr[VT_P2 * s + lane] = re[s];
+        xi[VT_P2 * s + lane] = im[s];
+    }
+    vh_wave_sync();
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        re[s] = xr[VT_P2 * lo + 8 * hi + s];
+        im[s] = xi[VT_P2 * lo + 8 * hi + s];
+    }
+    vh_wave_sync();
+    vt_dft8<1>(re, im);
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        xr[VT_P1 * hi + 8 * s + lo] = re[s];
+        xi[VT_P1 * hi + 8 * s + lo] = im[s];
+    }
+    vh_wave_sync();
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        re[s] = xr[VT_P1 * s + lane];
+        im[s] = xi[VT_P1 * s + lane];
+    }
+    vh_wave_sync();
+#pragma unroll
+    for (int s = 1; s < 8; s++) vt_cmul(re[s], im[s], T.t1[s].x, -T.t1[s].y);
+    vt_dft8<1>(re, im);
+}
+
+// Layout A input of a polynomial given by its coefficients as signed 32-bit values:
+// lane r, slot s holds z_j, j = 64 s + r.
+__device__ __forceinline__ void vt_fold(double *re, double *im, const uint32_t *p, const double2 *twist, int lane)
+{
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int j = 64 * s + lane;
+        const double a = (double)(int32_t)p[j], b = (double)(int32_t)p[j + VT_M];
+        const double2 z = twist[j];
+        re[s] = a * z.x - b * z.y;
+        im[s] = a * z.y + b * z.x;
+    }
+}
+
+// x mod 2^32 of an integer-valued double with |x| < 2^51 (1.5 * 2^52 puts the
+// rounded integer's low bits at the bottom of the mantissa).
+__device__ __forceinline__ uint32_t vt_round_u32(double x)
+{
+    return (uint32_t)__double_as_longlong(x + 6755399441055744.0);
+}
+
+// round(x * 2N / 2^32) with ties to even, as a double quotient rounds in Python
+// (cggi16.py:62,77; exact here since 2N / 2^32 is a power of two), mod 2N.
+__device__ __forceinline__ uint32_t vt_mod_switch(uint64_t x, int shift, uint32_t two_n)
+{
+    const uint64_t q = x >> shift, rem = x & ((1ull << shift) - 1), half = 1ull << (shift - 1);
+    const uint64_t r = q + ((rem > half || (rem == half && (q & 1))) ? 1 : 0);
+    return (uint32_t)(r & (two_n - 1));
+}
+
+// coefficient j of X^a * p (a in [0, 2N)), mod 2^32
+__device__ __forceinline__ uint32_t vt_rot(const uint32_t *p, int j, uint32_t a)
+{
+    const uint32_t t = ((uint32_t)j - a) & (2 * VT_N - 1);
+    return t < VT_N ? p[t] : 0u - p[t - VT_N];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Key transform: one wave per polynomial (signed 32-bit coefficients) to layout C,
+// stored [poly][slot][lane] (double2).  natural != 0: stored in natural order
+// [poly][k] instead (the transform check).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_torus_fft(const uint32_t *__restrict__ in, double2 *__restrict__ out,
+                                                   const double2 *__restrict__ consts, uint64_t count, int natural)
+{
+    __shared__ double lds[4][2][VT_XS];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t poly = (uint64_t)blockIdx.x * 4 + w;
+    if (poly >= count) return;   // whole waves leave; the exchanges are wave-private
+    vt_tw T;
+    vt_load_tw(T, consts, lane);
+    double re[8], im[8];
+    vt_fold(re, im, in + poly * VT_N, consts + 576, lane);
+    vt_fwd(re, im, lds[w][0], lds[w][1], T, lane);
+    double2 *o = out + poly * VT_M;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int idx = natural ? (lane >> 3) + 8 * (lane & 7) + 64 * s : s * 64 + lane;
+        o[idx] = make_double2(re[s], im[s]);
+    }
+}
+
+hipError_t vh_launch_torus_fft(hipStream_t st, const uint32_t *in, double2 *out, const double2 *consts, uint64_t count,
+                               int natural)
+{
+    if (!count) return hipSuccess;
+    VH_KL(k_torus_fft, dim3((unsigned)((count + 3) / 4)), dim3(256), 0, st, in, out, consts, count, natural);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Blind rotation + extraction: one workgroup of two waves per LWE sample; wave c owns
+// component c of the accumulator (0: the mask a, 1: the body b), kept in LDS as
+// 32-bit words.  Per key index i with a non-zero amount:
+//   diff = X^a_i * acc_c - acc_c  (registers, 16 coefficients per lane)
+//   for each level l: digit polynomial -> forward transform -> accumulate its products
+//     with key rows (c * ell + l, output component 0 and 1) into two partial sums;
+//   the waves swap the partial of the other's component through LDS (two barriers),
+//   so wave c holds the sum over all 2 * ell rows for its component in a fixed order
+//   (wave 0's rows, then wave 1's); inverse transform, untwist, scale by 1/M,
+//   round, and add into acc_c.
+// The key: n x (2 ell rows) x (2 components) x [slot][lane] double2 (layout C).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(128, 4) k_torus_blind_rotate(vt_br_args A)
+{
+    __shared__ uint32_t acc[2][VT_N];
+    __shared__ double lds[2][2][VT_XS];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t k = blockIdx.x;
+    const uint32_t *in = A.lwe + k * (uint64_t)(A.n + 1);
+    const uint32_t two_n = 2 * VT_N;
+    const int shift = 32 - 11;   // log2(2^32 / 2N)
+    double *xr = lds[w][0], *xi = lds[w][1];
+
+    vt_tw T;
+    vt_load_tw(T, A.consts, lane);
+    double2 tz[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) tz[s] = A.consts[576 + 64 * s + lane];
+
+    // acc = (0, X^-rot_b * tv): rot_b from b + floor(2^32 / (4 torus_base)) (cggi16.py:60-71)
+    const uint32_t rot_b = vt_mod_switch((uint64_t)in[A.n] + A.offset, shift, two_n);
+    const uint32_t rot_tv = (two_n - rot_b) & (two_n - 1);
+    uint32_t *mine = acc[w];
+    for (int j = lane; j < VT_N; j += 64) mine[j] = w == 0 ? 0u : vt_rot(A.tv, j, rot_tv);
+    vh_wave_sync();
+
+    const int rows = 2 * A.ell;
+    const uint32_t bmask = (1u << A.base_log) - 1, bhalf = 1u << (A.base_log - 1);
+    for (uint32_t i = 0; i < A.n; i++) {
+        const uint32_t ai = vt_mod_switch(in[i], shift, two_n);
+        if (ai == 0) continue;
+        uint32_t dlo[8], dhi[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int j = 64 * s + lane;
+            dlo[s] = vt_rot(mine, j, ai) - mine[j] + A.dec_off;
+            dhi[s] = vt_rot(mine, j + VT_M, ai) - mine[j + VT_M] + A.dec_off;
+        }
+        double p0r[8], p0i[8], p1r[8], p1i[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) p0r[s] = p0i[s] = p1r[s] = p1i[s] = 0.0;
+        for (uint32_t l = 0; l < A.ell; l++) {
+            const int sh = 32 - (int)(A.base_log * (l + 1));
+            double re[8], im[8];
+#pragma unroll
+            for (int s = 0; s < 8; s++) {
+                const double a = (double)(int)(((dlo[s] >> sh) & bmask) - bhalf);
+                const double b = (double)(int)(((dhi[s] >> sh) & bmask) - bhalf);
+                re[s] = a * tz[s].x - b * tz[s].y;
+                im[s] = a * tz[s].y + b * tz[s].x;
+            }
+            vt_fwd(re, im, xr, xi, T, lane);
+            const double2 *K = A.bsk + (((uint64_t)i * rows + (uint64_t)w * A.ell + l) * 2) * VT_M + lane;
+#pragma unroll
+            for (int s = 0; s < 8; s++) {
+                const double2 k0 = K[s * 64], k1 = K[VT_M + s * 64];
+                p0r[s] = __fma_rn(re[s], k0.x, p0r[s]);
+                p0r[s] = __fma_rn(-im[s], k0.y, p0r[s]);
+                p0i[s] = __fma_rn(re[s], k0.y, p0i[s]);
+                p0i[s] = __fma_rn(im[s], k0.x, p0i[s]);
+                p1r[s] = __fma_rn(re[s], k1.x, p1r[s]);
+                p1r[s] = __fma_rn(-im[s], k1.y, p1r[s]);
+                p1i[s] = __fma_rn(re[s], k1.y, p1i[s]);
+                p1i[s] = __fma_rn(im[s], k1.x, p1i[s]);
+            }
+        }
+        // swap partials: wave 0 gives its component-1 partial, wave 1 its component-0 one
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            xr[s * 64 + lane] = w == 0 ? p1r[s] : p0r[s];
+            xi[s * 64 + lane] = w == 0 ? p1i[s] : p0i[s];
+        }
+        __syncthreads();
+        const double *yr = lds[1 - w][0], *yi = lds[1 - w][1];
+        double re[8], im[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            re[s] = (w == 0 ? p0r[s] : p1r[s]) + yr[s * 64 + lane];
+            im[s] = (w == 0 ? p0i[s] : p1i[s]) + yi[s * 64 + lane];
+        }
+        __syncthreads();
+        vt_inv(re, im, xr, xi, T, lane);
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int j = 64 * s + lane;
+            const double cr = (re[s] * tz[s].x + im[s] * tz[s].y) * (1.0 / VT_M);
+            const double ci = (im[s] * tz[s].x - re[s] * tz[s].y) * (1.0 / VT_M);
+            mine[j] += vt_round_u32(cr);
+            mine[j + VT_M] += vt_round_u32(ci);
+        }
+        vh_wave_sync();
+    }
+
+    if (A.acc_out) {
+        uint32_t *o = A.acc_out + k * 2 * VT_N + (uint64_t)w * VT_N;
+        for (int j = lane; j < VT_N; j += 64) o[j] = mine[j];
+    }
+    if (A.out) {
+        // coefficient 0 of the phase b - a s: a'_0 = a_0, a'_j = -a_(N-j); b' = b_0
+        uint32_t *o = A.out + k * (VT_N + 1);
+        if (w == 0)
+            for (int j = lane; j < VT_N; j += 64) o[j] = j == 0 ? mine[0] : 0u - mine[VT_N - j];
+        else if (lane == 0)
+            o[VT_N] = mine[0];
+    }
+}
+
+hipError_t vh_launch_torus_blind_rotate(hipStream_t st, const vt_br_args &A, uint64_t count)
+{
+    if (!count) return hipSuccess;
+    VH_KL(k_torus_blind_rotate, dim3((unsigned)count), dim3(128), 0, st, A);
+    return hipGetLastError();
+}
