@@ -43,6 +43,7 @@ if ROOT not in sys.path:
 
 N, L, POLYS = 1 << 16, 24, 64
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6    # MI355X vector FP64
 PROF_TAGS = 6
 
 
@@ -63,6 +64,9 @@ def parse():
     ap.add_argument("--no-cggi", action="store_true")
     ap.add_argument("--cggi-steps", type=int, default=3)
     ap.add_argument("--only-cggi", action="store_true", help="run the bootstrap leg alone (profiling)")
+    ap.add_argument("--no-torus", action="store_true")
+    ap.add_argument("--torus-steps", type=int, default=20)
+    ap.add_argument("--only-torus", action="store_true", help="run the 32-bit-torus bootstrap leg alone (profiling)")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=1.5, help="wall time per offered load")
     ap.add_argument("--serving-lanes", type=int, default=7,
@@ -199,6 +203,9 @@ def main():
         return
     if args.only_cggi:
         print(json.dumps(bench_cggi(lib, args)), flush=True)
+        return
+    if args.only_torus:
+        print(json.dumps(bench_cggi_torus(lib, args)), flush=True)
         return
     primes = params.gen_primes(N, [50] * L)
     log("device", lib.vfhe_hip_device_name().decode(), "primes", primes[0], "...", primes[-1])
@@ -344,6 +351,10 @@ def main():
     if not args.no_cggi:
         cggi = bench_cggi(lib, args)
 
+    torus = None
+    if not args.no_torus:
+        torus = bench_cggi_torus(lib, args)
+
     serve = None
     if not args.no_serving:
         # the earlier legs' device blocks (ModUp buffers, bootstrap temporaries) go back to the
@@ -406,6 +417,7 @@ def main():
                   "batched_over_host_plain": round(plain_host_dt / dt, 2)},
         "ckks": ckks,
         "cggi": cggi,
+        "cggi_torus": torus,
         "serving": serve,
         "alu": alu,
     }
@@ -606,6 +618,78 @@ def bench_cggi(lib, args):
                        "per limb (ell=2), 1024 LWE ciphertexts of uniform 4-bit messages, identity table; the "
                        "32-bit-torus FFT variant has no counterpart in the reference"}
     log("cggi:", res)
+    return res
+
+
+def torus_flops_per_index(p) -> int:
+    """FP64 operations of one sample at one key index in k_torus_blind_rotate, counted from
+    its code: per transform of M = N/2 points three DFT-8 passes (56 per 8 points) and two
+    twiddle passes (7 complex products of 6 per 8 points); per digit polynomial the twist
+    (6 per point); per key row and component one complex multiply-add (8 per point); the
+    partial swap (2 per point per component); per output component the untwist and scale
+    (8 per point)."""
+    M = p.N // 2
+    fft = M // 8 * (3 * 56 + 2 * 42)
+    rows = 2 * p.ell
+    return rows * (6 * M + fft) + rows * 2 * 8 * M + 2 * 2 * M + 2 * (fft + 8 * M)
+
+
+def bench_cggi_torus(lib, args):
+    """CGGI over the 32-bit torus with an FP64 FFT: 1024 LWE samples (n = 630) of uniform 4-bit
+    messages through the identity table into N = 1024, base 2^7, 3 levels; inputs, test vector
+    and outputs resident on the device; bootstraps/s over --torus-steps batches."""
+    from vfhe_amd import torus as VT
+
+    prec, count = 5, 1024
+    p = VT.TorusParams()
+    t0 = time.perf_counter()
+    keys = VT.TorusKeys(p, seed=630)
+    bsk = keys.bootstrap_key()
+    boot = VT.TorusBootstrapper(bsk, p)
+    keygen_s = time.perf_counter() - t0
+    msgs = np.random.default_rng(631).integers(0, 16, size=count)
+    samples = keys.encrypt(msgs, prec)
+    tv_h = VT.pack_lut(list(range(16)), 16, prec, p.N)
+    lw, tv = VT.TorusBatch.upload(samples), VT.TorusBatch.upload(tv_h)
+    out = VT.TorusBatch.zeros(count, p.N + 1)
+    boot.bootstrap(lw, tv, 1 << (prec - 1), out=out)
+    errors = int((keys.decrypt(out.read(), prec) != msgs).sum())
+
+    def step():
+        boot.bootstrap(lw, tv, 1 << (prec - 1), out=out)
+
+    t = timed(lib, step, args.torus_steps, 2)
+    with Prof(lib) as prof:
+        for _ in range(5):
+            step()
+        lib.vfhe_hip_synchronize()
+    kernel_ms = prof.ms[5] / max(prof.launches[5], 1)
+    flops = torus_flops_per_index(p) * p.n * count
+    tflops = flops / (kernel_ms * 1e-3) / 1e12
+    res = {"unit": "bootstraps/s", "value": round(count / t, 2), "ms_per_batch": round(t * 1e3, 3),
+           "kernel_ms": round(kernel_ms, 3), "batch": count, "decrypt_errors": errors,
+           "fp64": {"achieved": round(tflops, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(tflops / FP64_PEAK_TFLOPS, 4), "flops_per_batch": int(flops),
+                    "kernel": "k_torus_blind_rotate"},
+           "bootstrap_key_s": round(keygen_s, 2),
+           "workload": "CGGI functional bootstrap over the 32-bit torus, FP64 negacyclic FFT of N/2 points: "
+                       "LWE n=630 (binary key, sigma 2^-15), GLWE N=1024 rank 1 (binary key, sigma 2^-25), "
+                       "gadget base 2^7 x 3 levels, 1024 samples of uniform 4-bit messages, identity table; "
+                       "no counterpart in the reference (its CGGI16 is the RNS variant)"}
+    if not args.no_cpu:
+        from oracle import torus as OT
+        kf = OT.key_fft(bsk)
+        t0 = time.perf_counter()
+        k = 0
+        while k < 64 and (k < 2 or time.perf_counter() - t0 < args.cpu_seconds):
+            OT.extract(OT.blind_rotate(samples[k].astype(np.int64), tv_h.astype(np.int64), kf, p.ell, p.base_log,
+                                       1 << (prec - 1)))
+            k += 1
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(k / dt, 2), "unit": "bootstraps/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} samples of the batch through oracle/torus.py (numpy, one thread)"}
+    log("cggi torus:", res)
+    lw.free(), tv.free(), out.free(), boot.free()
     return res
 
 

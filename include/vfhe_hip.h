@@ -614,7 +614,7 @@ void vfhe_hip_torus_batch_free(vfhe_torus_batch *b);
 /* The bootstrapping key: n GGSW encryptions of the LWE key bits, bsk laid out
    [n][2 ell rows][2 components][N] (row c * ell + l encrypts s_i * 2^(32 - base_log (l+1))
    added to component c: 0 the mask, 1 the body), moved to the transform domain on the
-   device.  N must be 1024; ell * base_log <= 32, base_log <= 16. */
+   device.  N must be 1024; ell <= 4, base_log <= 16, ell * base_log <= 32. */
 vfhe_torus_key *vfhe_hip_torus_key_new(const uint32_t *bsk, uint64_t n, uint64_t N, uint64_t ell, uint32_t base_log);
 void vfhe_hip_torus_key_free(vfhe_torus_key *key);
 /* Functional bootstrap of every sample of `in` (dim n + 1) with test vector `tv` (1 x N):

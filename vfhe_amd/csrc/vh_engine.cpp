@@ -4561,7 +4561,8 @@ extern "C" vfhe_torus_key *vfhe_hip_torus_key_new(const uint32_t *bsk, uint64_t 
                                                   uint32_t base_log)
 {
     VH_ENTER;
-    if (!bsk || N != 1024 || !n || n > (1u << 20) || !ell || !base_log || base_log > 16 || ell * base_log > 32)
+    if (!bsk || N != 1024 || !n || n > (1u << 20) || !ell || ell > 4 || !base_log || base_log > 16 ||
+        ell * base_log > 32)
         return nullptr;
     const uint64_t polys = n * 4 * ell;
     uint32_t *d_in = nullptr;
@@ -4609,6 +4610,7 @@ extern "C" int vfhe_hip_torus_bootstrap(vfhe_torus_batch *out, vfhe_torus_batch 
     A.base_log = key->base_log;
     A.offset = (uint32_t)((1ull << 32) / (4 * torus_base));
     A.dec_off = dec_off;
+    vh_prof_scope ps_(VH_PROF_OTHER);
     VH_LAUNCH(vh_launch_torus_blind_rotate(S(), A, in->count));
     return 0;
 }

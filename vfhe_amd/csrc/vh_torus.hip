@@ -94,8 +94,8 @@ __device__ __forceinline__ void vt_dft8(double *re, double *im)
 
 __device__ __forceinline__ void vt_cmul(double &r, double &i, double wr, double wi)
 {
-    const double t = r * wr - i * wi;
-    i = r * wi + i * wr;
+    const double t = __fma_rn(r, wr, -(i * wi));
+    i = __fma_rn(r, wi, i * wr);
     r = t;
 }
 
@@ -196,8 +196,8 @@ __device__ __forceinline__ void vt_fold(double *re, double *im, const uint32_t *
         const int j = 64 * s + lane;
         const double a = (double)(int32_t)p[j], b = (double)(int32_t)p[j + VT_M];
         const double2 z = twist[j];
-        re[s] = a * z.x - b * z.y;
-        im[s] = a * z.y + b * z.x;
+        re[s] = __fma_rn(a, z.x, -(b * z.y));
+        im[s] = __fma_rn(a, z.y, b * z.x);
     }
 }
 
@@ -221,15 +221,17 @@ __device__ __forceinline__ uint32_t vt_mod_switch(uint64_t x, int shift, uint32_
 __device__ __forceinline__ uint32_t vt_rot(const uint32_t *p, int j, uint32_t a)
 {
     const uint32_t t = ((uint32_t)j - a) & (2 * VT_N - 1);
-    return t < VT_N ? p[t] : 0u - p[t - VT_N];
+    const uint32_t v = p[t & (VT_N - 1)], neg = 0u - (t >> 10);   // t >> 10: 1 when t >= N
+    return (v ^ neg) - neg;
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // Key transform: one wave per polynomial (signed 32-bit coefficients) to layout C,
-// stored [poly][slot][lane] (double2).  natural != 0: stored in natural order
-// [poly][k] instead (the transform check).
+// stored [poly][slot][lane] (double2) and scaled by 1/M (exact: a power of two), so
+// the blind rotation's inverse transforms need no scaling.  natural != 0: unscaled,
+// in natural order [poly][k] (the transform check).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_torus_fft(const uint32_t *__restrict__ in, double2 *__restrict__ out,
                                                    const double2 *__restrict__ consts, uint64_t count, int natural)
@@ -244,10 +246,11 @@ __global__ void __launch_bounds__(256) k_torus_fft(const uint32_t *__restrict__ 
     vt_fold(re, im, in + poly * VT_N, consts + 576, lane);
     vt_fwd(re, im, lds[w][0], lds[w][1], T, lane);
     double2 *o = out + poly * VT_M;
+    const double scale = natural ? 1.0 : 1.0 / VT_M;
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         const int idx = natural ? (lane >> 3) + 8 * (lane & 7) + 64 * s : s * 64 + lane;
-        o[idx] = make_double2(re[s], im[s]);
+        o[idx] = make_double2(re[s] * scale, im[s] * scale);
     }
 }
 
@@ -268,15 +271,16 @@ hipError_t vh_launch_torus_fft(hipStream_t st, const uint32_t *in, double2 *out,
 //     with key rows (c * ell + l, output component 0 and 1) into two partial sums;
 //   the waves swap the partial of the other's component through LDS (two barriers),
 //   so wave c holds the sum over all 2 * ell rows for its component in a fixed order
-//   (wave 0's rows, then wave 1's); inverse transform, untwist, scale by 1/M,
-//   round, and add into acc_c.
+//   (wave 0's rows, then wave 1's); inverse transform, untwist (the key rows carry
+//   the 1/M of the inverse), round, and add into acc_c.
 // The key: n x (2 ell rows) x (2 components) x [slot][lane] double2 (layout C).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(128, 4) k_torus_blind_rotate(vt_br_args A)
+template <int ELL>   // levels (1..4)
+__global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
 {
     __shared__ uint32_t acc[2][VT_N];
     __shared__ double lds[2][2][VT_XS];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const uint64_t k = blockIdx.x;
     const uint32_t *in = A.lwe + k * (uint64_t)(A.n + 1);
     const uint32_t two_n = 2 * VT_N;
@@ -296,8 +300,14 @@ __global__ void __launch_bounds__(128, 4) k_torus_blind_rotate(vt_br_args A)
     for (int j = lane; j < VT_N; j += 64) mine[j] = w == 0 ? 0u : vt_rot(A.tv, j, rot_tv);
     vh_wave_sync();
 
-    const int rows = 2 * A.ell;
-    const uint32_t bmask = (1u << A.base_log) - 1, bhalf = 1u << (A.base_log - 1);
+    // Digits: with y = x + dec_off, field_l = (y >> sh_l) mod 2^B and d_l = field_l - 2^(B-1),
+    // which is the B-bit sign extension of field_l ^ 2^(B-1): one xor per coefficient, then
+    // one signed bit-field extract per level.
+    const uint32_t B = A.base_log;
+    uint32_t xmask = 0;
+#pragma unroll
+    for (int l = 1; l <= ELL; l++) xmask |= (1u << (B - 1)) << (32 - B * l);
+    const int rows = 2 * ELL;
     for (uint32_t i = 0; i < A.n; i++) {
         const uint32_t ai = vt_mod_switch(in[i], shift, two_n);
         if (ai == 0) continue;
@@ -305,58 +315,61 @@ __global__ void __launch_bounds__(128, 4) k_torus_blind_rotate(vt_br_args A)
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int j = 64 * s + lane;
-            dlo[s] = vt_rot(mine, j, ai) - mine[j] + A.dec_off;
-            dhi[s] = vt_rot(mine, j + VT_M, ai) - mine[j + VT_M] + A.dec_off;
+            dlo[s] = (vt_rot(mine, j, ai) - mine[j] + A.dec_off) ^ xmask;
+            dhi[s] = (vt_rot(mine, j + VT_M, ai) - mine[j + VT_M] + A.dec_off) ^ xmask;
         }
-        double p0r[8], p0i[8], p1r[8], p1i[8];
+        // own: this wave's output component (w), other: the other wave's (1 - w)
+        double por[8], poi[8], pxr[8], pxi[8];
 #pragma unroll
-        for (int s = 0; s < 8; s++) p0r[s] = p0i[s] = p1r[s] = p1i[s] = 0.0;
-        for (uint32_t l = 0; l < A.ell; l++) {
-            const int sh = 32 - (int)(A.base_log * (l + 1));
+        for (int s = 0; s < 8; s++) por[s] = poi[s] = pxr[s] = pxi[s] = 0.0;
+#pragma unroll
+        for (int l = 0; l < ELL; l++) {
+            const uint32_t sh = 32 - B * (l + 1);
             double re[8], im[8];
 #pragma unroll
             for (int s = 0; s < 8; s++) {
-                const double a = (double)(int)(((dlo[s] >> sh) & bmask) - bhalf);
-                const double b = (double)(int)(((dhi[s] >> sh) & bmask) - bhalf);
-                re[s] = a * tz[s].x - b * tz[s].y;
-                im[s] = a * tz[s].y + b * tz[s].x;
+                const double a = (double)(int)__builtin_amdgcn_sbfe((int)dlo[s], sh, B);
+                const double b = (double)(int)__builtin_amdgcn_sbfe((int)dhi[s], sh, B);
+                re[s] = __fma_rn(a, tz[s].x, -(b * tz[s].y));
+                im[s] = __fma_rn(a, tz[s].y, b * tz[s].x);
             }
             vt_fwd(re, im, xr, xi, T, lane);
-            const double2 *K = A.bsk + (((uint64_t)i * rows + (uint64_t)w * A.ell + l) * 2) * VT_M + lane;
+            const double2 *K = A.bsk + (((uint64_t)i * rows + (uint64_t)w * ELL + l) * 2) * VT_M + lane;
+            const double2 *Ko = K + (w ? VT_M : 0), *Kx = K + (w ? 0 : VT_M);
 #pragma unroll
             for (int s = 0; s < 8; s++) {
-                const double2 k0 = K[s * 64], k1 = K[VT_M + s * 64];
-                p0r[s] = __fma_rn(re[s], k0.x, p0r[s]);
-                p0r[s] = __fma_rn(-im[s], k0.y, p0r[s]);
-                p0i[s] = __fma_rn(re[s], k0.y, p0i[s]);
-                p0i[s] = __fma_rn(im[s], k0.x, p0i[s]);
-                p1r[s] = __fma_rn(re[s], k1.x, p1r[s]);
-                p1r[s] = __fma_rn(-im[s], k1.y, p1r[s]);
-                p1i[s] = __fma_rn(re[s], k1.y, p1i[s]);
-                p1i[s] = __fma_rn(im[s], k1.x, p1i[s]);
+                const double2 ko = Ko[s * 64], kx = Kx[s * 64];
+                por[s] = __fma_rn(re[s], ko.x, por[s]);
+                por[s] = __fma_rn(-im[s], ko.y, por[s]);
+                poi[s] = __fma_rn(re[s], ko.y, poi[s]);
+                poi[s] = __fma_rn(im[s], ko.x, poi[s]);
+                pxr[s] = __fma_rn(re[s], kx.x, pxr[s]);
+                pxr[s] = __fma_rn(-im[s], kx.y, pxr[s]);
+                pxi[s] = __fma_rn(re[s], kx.y, pxi[s]);
+                pxi[s] = __fma_rn(im[s], kx.x, pxi[s]);
             }
         }
-        // swap partials: wave 0 gives its component-1 partial, wave 1 its component-0 one
+        // swap partials; each output is wave 0's partial plus wave 1's (addition commutes)
 #pragma unroll
         for (int s = 0; s < 8; s++) {
-            xr[s * 64 + lane] = w == 0 ? p1r[s] : p0r[s];
-            xi[s * 64 + lane] = w == 0 ? p1i[s] : p0i[s];
+            xr[s * 64 + lane] = pxr[s];
+            xi[s * 64 + lane] = pxi[s];
         }
         __syncthreads();
         const double *yr = lds[1 - w][0], *yi = lds[1 - w][1];
         double re[8], im[8];
 #pragma unroll
         for (int s = 0; s < 8; s++) {
-            re[s] = (w == 0 ? p0r[s] : p1r[s]) + yr[s * 64 + lane];
-            im[s] = (w == 0 ? p0i[s] : p1i[s]) + yi[s * 64 + lane];
+            re[s] = por[s] + yr[s * 64 + lane];
+            im[s] = poi[s] + yi[s * 64 + lane];
         }
         __syncthreads();
         vt_inv(re, im, xr, xi, T, lane);
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int j = 64 * s + lane;
-            const double cr = (re[s] * tz[s].x + im[s] * tz[s].y) * (1.0 / VT_M);
-            const double ci = (im[s] * tz[s].x - re[s] * tz[s].y) * (1.0 / VT_M);
+            const double cr = __fma_rn(re[s], tz[s].x, im[s] * tz[s].y);   // the key carries 1/M
+            const double ci = __fma_rn(im[s], tz[s].x, -(re[s] * tz[s].y));
             mine[j] += vt_round_u32(cr);
             mine[j + VT_M] += vt_round_u32(ci);
         }
@@ -380,6 +393,12 @@ __global__ void __launch_bounds__(128, 4) k_torus_blind_rotate(vt_br_args A)
 hipError_t vh_launch_torus_blind_rotate(hipStream_t st, const vt_br_args &A, uint64_t count)
 {
     if (!count) return hipSuccess;
-    VH_KL(k_torus_blind_rotate, dim3((unsigned)count), dim3(128), 0, st, A);
+    switch (A.ell) {
+    case 1: VH_KL(k_torus_blind_rotate<1>, dim3((unsigned)count), dim3(128), 0, st, A); break;
+    case 2: VH_KL(k_torus_blind_rotate<2>, dim3((unsigned)count), dim3(128), 0, st, A); break;
+    case 3: VH_KL(k_torus_blind_rotate<3>, dim3((unsigned)count), dim3(128), 0, st, A); break;
+    case 4: VH_KL(k_torus_blind_rotate<4>, dim3((unsigned)count), dim3(128), 0, st, A); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
