@@ -99,27 +99,35 @@ __device__ __forceinline__ void vt_cmul(double &r, double &i, double wr, double 
     r = t;
 }
 
-// The lane's twiddles: t1[s] = w^(lane s), t2[s] = w^(8 (lane mod 8) s), s = 0..7.
-struct vt_tw {
-    double2 t1[8], t2[8];
-};
+// Twiddle tables in LDS, filled once per workgroup: t1[(s-1) 64 + r] = w^(r s) (pass 1,
+// lane r), t2[(s-1) 8 + a] = w^(8 a s) (pass 2, a = lane mod 8), s = 1..7.  Per lane the
+// reads are consecutive 16-byte words (t1) or broadcasts (t2).
+constexpr int VT_T1 = 7 * 64, VT_T2 = 7 * 8;
 
-__device__ __forceinline__ void vt_load_tw(vt_tw &T, const double2 *consts, int lane)
+__device__ __forceinline__ void vt_fill_tw(double2 *t1, double2 *t2, const double2 *consts, int tid, int nthreads)
 {
-#pragma unroll
-    for (int s = 0; s < 8; s++) {
-        T.t1[s] = consts[lane * 8 + s];
-        T.t2[s] = consts[512 + (lane & 7) * 8 + s];
+    for (int e = tid; e < VT_T1 + VT_T2; e += nthreads) {
+        if (e < VT_T1) {
+            const int s = e / 64 + 1, r = e % 64;
+            t1[e] = consts[r * 8 + s];
+        } else {
+            const int s = (e - VT_T1) / 8 + 1, a = (e - VT_T1) % 8;
+            t2[e - VT_T1] = consts[512 + a * 8 + s];
+        }
     }
 }
 
 // Forward transform: layout A in, layout C out.
-__device__ __forceinline__ void vt_fwd(double *re, double *im, double *xr, double *xi, const vt_tw &T, int lane)
+__device__ __forceinline__ void vt_fwd(double *re, double *im, double *xr, double *xi, const double2 *t1,
+                                       const double2 *t2, int lane)
 {
     const int hi = lane >> 3, lo = lane & 7;
     vt_dft8<-1>(re, im);
 #pragma unroll
-    for (int s = 1; s < 8; s++) vt_cmul(re[s], im[s], T.t1[s].x, T.t1[s].y);
+    for (int s = 1; s < 8; s++) {
+        const double2 t = t1[(s - 1) * 64 + lane];
+        vt_cmul(re[s], im[s], t.x, t.y);
+    }
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         xr[VT_P1 * s + lane] = re[s];
@@ -134,7 +142,10 @@ __device__ __forceinline__ void vt_fwd(double *re, double *im, double *xr, doubl
     vh_wave_sync();
     vt_dft8<-1>(re, im);
 #pragma unroll
-    for (int s = 1; s < 8; s++) vt_cmul(re[s], im[s], T.t2[s].x, T.t2[s].y);
+    for (int s = 1; s < 8; s++) {
+        const double2 t = t2[(s - 1) * 8 + lo];
+        vt_cmul(re[s], im[s], t.x, t.y);
+    }
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         xr[VT_P2 * lo + 8 * hi + s] = re[s];
@@ -151,12 +162,16 @@ __device__ __forceinline__ void vt_fwd(double *re, double *im, double *xr, doubl
 }
 
 // Inverse transform (unscaled): layout C in, layout A out.
-__device__ __forceinline__ void vt_inv(double *re, double *im, double *xr, double *xi, const vt_tw &T, int lane)
+__device__ __forceinline__ void vt_inv(double *re, double *im, double *xr, double *xi, const double2 *t1,
+                                       const double2 *t2, int lane)
 {
     const int hi = lane >> 3, lo = lane & 7;
     vt_dft8<1>(re, im);
 #pragma unroll
-    for (int s = 1; s < 8; s++) vt_cmul(re[s], im[s], T.t2[s].x, -T.t2[s].y);
+    for (int s = 1; s < 8; s++) {
+        const double2 t = t2[(s - 1) * 8 + lo];
+        vt_cmul(re[s], im[s], t.x, -t.y);
+    }
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         xr[VT_P2 * s + lane] = re[s];
@@ -183,7 +198,10 @@ __device__ __forceinline__ void vt_inv(double *re, double *im, double *xr, doubl
     }
     vh_wave_sync();
 #pragma unroll
-    for (int s = 1; s < 8; s++) vt_cmul(re[s], im[s], T.t1[s].x, -T.t1[s].y);
+    for (int s = 1; s < 8; s++) {
+        const double2 t = t1[(s - 1) * 64 + lane];
+        vt_cmul(re[s], im[s], t.x, -t.y);
+    }
     vt_dft8<1>(re, im);
 }
 
@@ -237,14 +255,15 @@ __global__ void __launch_bounds__(256) k_torus_fft(const uint32_t *__restrict__ 
                                                    const double2 *__restrict__ consts, uint64_t count, int natural)
 {
     __shared__ double lds[4][2][VT_XS];
+    __shared__ double2 t1[VT_T1], t2[VT_T2];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    vt_fill_tw(t1, t2, consts, threadIdx.x, 256);
+    __syncthreads();
     const uint64_t poly = (uint64_t)blockIdx.x * 4 + w;
     if (poly >= count) return;   // whole waves leave; the exchanges are wave-private
-    vt_tw T;
-    vt_load_tw(T, consts, lane);
     double re[8], im[8];
     vt_fold(re, im, in + poly * VT_N, consts + 576, lane);
-    vt_fwd(re, im, lds[w][0], lds[w][1], T, lane);
+    vt_fwd(re, im, lds[w][0], lds[w][1], t1, t2, lane);
     double2 *o = out + poly * VT_M;
     const double scale = natural ? 1.0 : 1.0 / VT_M;
 #pragma unroll
@@ -280,6 +299,7 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
 {
     __shared__ uint32_t acc[2][VT_N];
     __shared__ double lds[2][2][VT_XS];
+    __shared__ double2 t1[VT_T1], t2[VT_T2];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const uint64_t k = blockIdx.x;
     const uint32_t *in = A.lwe + k * (uint64_t)(A.n + 1);
@@ -287,8 +307,7 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
     const int shift = 32 - 11;   // log2(2^32 / 2N)
     double *xr = lds[w][0], *xi = lds[w][1];
 
-    vt_tw T;
-    vt_load_tw(T, A.consts, lane);
+    vt_fill_tw(t1, t2, A.consts, threadIdx.x, 128);
     double2 tz[8];
 #pragma unroll
     for (int s = 0; s < 8; s++) tz[s] = A.consts[576 + 64 * s + lane];
@@ -298,7 +317,7 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
     const uint32_t rot_tv = (two_n - rot_b) & (two_n - 1);
     uint32_t *mine = acc[w];
     for (int j = lane; j < VT_N; j += 64) mine[j] = w == 0 ? 0u : vt_rot(A.tv, j, rot_tv);
-    vh_wave_sync();
+    __syncthreads();
 
     // Digits: with y = x + dec_off, field_l = (y >> sh_l) mod 2^B and d_l = field_l - 2^(B-1),
     // which is the B-bit sign extension of field_l ^ 2^(B-1): one xor per coefficient, then
@@ -324,6 +343,15 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
         for (int s = 0; s < 8; s++) por[s] = poi[s] = pxr[s] = pxi[s] = 0.0;
 #pragma unroll
         for (int l = 0; l < ELL; l++) {
+            // the level's key rows are loaded first, their latency covered by the transform
+            const double2 *K = A.bsk + (((uint64_t)i * rows + (uint64_t)w * ELL + l) * 2) * VT_M + lane;
+            const double2 *Ko = K + (w ? VT_M : 0), *Kx = K + (w ? 0 : VT_M);
+            double2 kor[8], kxr[8];
+#pragma unroll
+            for (int s = 0; s < 8; s++) {
+                kor[s] = Ko[s * 64];
+                kxr[s] = Kx[s * 64];
+            }
             const uint32_t sh = 32 - B * (l + 1);
             double re[8], im[8];
 #pragma unroll
@@ -333,12 +361,10 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
                 re[s] = __fma_rn(a, tz[s].x, -(b * tz[s].y));
                 im[s] = __fma_rn(a, tz[s].y, b * tz[s].x);
             }
-            vt_fwd(re, im, xr, xi, T, lane);
-            const double2 *K = A.bsk + (((uint64_t)i * rows + (uint64_t)w * ELL + l) * 2) * VT_M + lane;
-            const double2 *Ko = K + (w ? VT_M : 0), *Kx = K + (w ? 0 : VT_M);
+            vt_fwd(re, im, xr, xi, t1, t2, lane);
 #pragma unroll
             for (int s = 0; s < 8; s++) {
-                const double2 ko = Ko[s * 64], kx = Kx[s * 64];
+                const double2 ko = kor[s], kx = kxr[s];
                 por[s] = __fma_rn(re[s], ko.x, por[s]);
                 por[s] = __fma_rn(-im[s], ko.y, por[s]);
                 poi[s] = __fma_rn(re[s], ko.y, poi[s]);
@@ -364,7 +390,7 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
             im[s] = poi[s] + yi[s * 64 + lane];
         }
         __syncthreads();
-        vt_inv(re, im, xr, xi, T, lane);
+        vt_inv(re, im, xr, xi, t1, t2, lane);
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int j = 64 * s + lane;
