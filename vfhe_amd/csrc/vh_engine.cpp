@@ -4509,17 +4509,17 @@ static double2 *g_torus_consts = nullptr;
 static const double2 *torus_consts()
 {
     if (g_torus_consts) return g_torus_consts;
-    std::vector<double2> h(1088);
+    std::vector<double2> h(584);
     const long double pi = 3.141592653589793238462643383279502884L;
     auto root = [&](long double num, long double den) {
         const long double a = pi * num / den;
         return make_double2((double)cosl(a), (double)sinl(a));
     };
-    for (int r = 0; r < 64; r++)
-        for (int s = 0; s < 8; s++) h[r * 8 + s] = root(-2.0L * ((r * s) % 512), 512);
+    for (int k = 0; k < 8; k++)   // zeta^r w^(r k): angle pi r / 1024 - 2 pi r k / 512
+        for (int r = 0; r < 64; r++) h[k * 64 + r] = root((long double)r - 4.0L * ((r * k) % 512), 1024);
     for (int a = 0; a < 8; a++)
         for (int s = 0; s < 8; s++) h[512 + a * 8 + s] = root(-2.0L * ((8 * a * s) % 512), 512);
-    for (int j = 0; j < 512; j++) h[576 + j] = root(j, 1024);
+    for (int s = 0; s < 8; s++) h[576 + s] = root(64.0L * s, 1024);
     double2 *d = nullptr;
     VH_CHECK(hipMalloc(&d, h.size() * sizeof(double2)));
     VH_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));

@@ -30,6 +30,10 @@
 // w^(8 j0 k1); pass 3: DFT-8 over j0.  The two exchanges go through a private LDS
 // slice of the wave with row pitches chosen so that every access spreads evenly over
 // the banks (pitch 72 for the A<->B exchange, 65 for B<->C).
+// The twist zeta^j, j = 64 j2 + r, splits as zeta^r zeta^(64 j2): the slot factor
+// c_j2 = zeta^(64 j2) multiplies the inputs (uniform constants), and the lane factor
+// zeta^r commutes with the DFT-8 over j2 and joins pass 1's twiddle, which becomes
+// zeta^r w^(r k2) for every k2 (k2 = 0 included); the inverse mirrors both.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -99,17 +103,17 @@ __device__ __forceinline__ void vt_cmul(double &r, double &i, double wr, double 
     r = t;
 }
 
-// Twiddle tables in LDS, filled once per workgroup: t1[(s-1) 64 + r] = w^(r s) (pass 1,
-// lane r), t2[(s-1) 8 + a] = w^(8 a s) (pass 2, a = lane mod 8), s = 1..7.  Per lane the
-// reads are consecutive 16-byte words (t1) or broadcasts (t2).
-constexpr int VT_T1 = 7 * 64, VT_T2 = 7 * 8;
+// Twiddle tables in LDS, filled once per workgroup: t1[k 64 + r] = zeta^r w^(r k)
+// (pass 1 with the lane's twist factor, lane r, k = 0..7), t2[(s-1) 8 + a] = w^(8 a s)
+// (pass 2, a = lane mod 8, s = 1..7).  Per lane the reads are consecutive 16-byte words
+// (t1) or broadcasts (t2).
+constexpr int VT_T1 = 8 * 64, VT_T2 = 7 * 8;
 
 __device__ __forceinline__ void vt_fill_tw(double2 *t1, double2 *t2, const double2 *consts, int tid, int nthreads)
 {
     for (int e = tid; e < VT_T1 + VT_T2; e += nthreads) {
         if (e < VT_T1) {
-            const int s = e / 64 + 1, r = e % 64;
-            t1[e] = consts[r * 8 + s];
+            t1[e] = consts[e];
         } else {
             const int s = (e - VT_T1) / 8 + 1, a = (e - VT_T1) % 8;
             t2[e - VT_T1] = consts[512 + a * 8 + s];
@@ -124,8 +128,8 @@ __device__ __forceinline__ void vt_fwd(double *re, double *im, double *xr, doubl
     const int hi = lane >> 3, lo = lane & 7;
     vt_dft8<-1>(re, im);
 #pragma unroll
-    for (int s = 1; s < 8; s++) {
-        const double2 t = t1[(s - 1) * 64 + lane];
+    for (int s = 0; s < 8; s++) {
+        const double2 t = t1[s * 64 + lane];
         vt_cmul(re[s], im[s], t.x, t.y);
     }
 #pragma unroll
@@ -198,25 +202,33 @@ __device__ __forceinline__ void vt_inv(double *re, double *im, double *xr, doubl
     }
     vh_wave_sync();
 #pragma unroll
-    for (int s = 1; s < 8; s++) {
-        const double2 t = t1[(s - 1) * 64 + lane];
+    for (int s = 0; s < 8; s++) {
+        const double2 t = t1[s * 64 + lane];
         vt_cmul(re[s], im[s], t.x, -t.y);
     }
     vt_dft8<1>(re, im);
 }
 
+// The slot factors c_s = zeta^(64 s) (uniform): applied to the forward input, and
+// conjugated to the inverse output.
+__device__ __forceinline__ void vt_slot_twist(double *re, double *im, const double2 *c, bool conj)
+{
+#pragma unroll
+    for (int s = 1; s < 8; s++) vt_cmul(re[s], im[s], c[s].x, conj ? -c[s].y : c[s].y);
+}
+
 // Layout A input of a polynomial given by its coefficients as signed 32-bit values:
-// lane r, slot s holds z_j, j = 64 s + r.
-__device__ __forceinline__ void vt_fold(double *re, double *im, const uint32_t *p, const double2 *twist, int lane)
+// lane r, slot s holds (p_j + i p_(j+M)) c_s, j = 64 s + r (the lane's twist factor
+// comes with pass 1).
+__device__ __forceinline__ void vt_fold(double *re, double *im, const uint32_t *p, const double2 *c, int lane)
 {
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         const int j = 64 * s + lane;
-        const double a = (double)(int32_t)p[j], b = (double)(int32_t)p[j + VT_M];
-        const double2 z = twist[j];
-        re[s] = __fma_rn(a, z.x, -(b * z.y));
-        im[s] = __fma_rn(a, z.y, b * z.x);
+        re[s] = (double)(int32_t)p[j];
+        im[s] = (double)(int32_t)p[j + VT_M];
     }
+    vt_slot_twist(re, im, c, false);
 }
 
 // x mod 2^32 of an integer-valued double with |x| < 2^51 (1.5 * 2^52 puts the
@@ -262,7 +274,7 @@ __global__ void __launch_bounds__(256) k_torus_fft(const uint32_t *__restrict__ 
     const uint64_t poly = (uint64_t)blockIdx.x * 4 + w;
     if (poly >= count) return;   // whole waves leave; the exchanges are wave-private
     double re[8], im[8];
-    vt_fold(re, im, in + poly * VT_N, consts + 576, lane);
+    vt_fold(re, im, in + poly * VT_N, consts + 576, lane);   // c_s
     vt_fwd(re, im, lds[w][0], lds[w][1], t1, t2, lane);
     double2 *o = out + poly * VT_M;
     const double scale = natural ? 1.0 : 1.0 / VT_M;
@@ -308,9 +320,7 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
     double *xr = lds[w][0], *xi = lds[w][1];
 
     vt_fill_tw(t1, t2, A.consts, threadIdx.x, 128);
-    double2 tz[8];
-#pragma unroll
-    for (int s = 0; s < 8; s++) tz[s] = A.consts[576 + 64 * s + lane];
+    const double2 *cs = A.consts + 576;   // the slot factors c_s (uniform loads)
 
     // acc = (0, X^-rot_b * tv): rot_b from b + floor(2^32 / (4 torus_base)) (cggi16.py:60-71)
     const uint32_t rot_b = vt_mod_switch((uint64_t)in[A.n] + A.offset, shift, two_n);
@@ -356,11 +366,10 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
             double re[8], im[8];
 #pragma unroll
             for (int s = 0; s < 8; s++) {
-                const double a = (double)(int)__builtin_amdgcn_sbfe((int)dlo[s], sh, B);
-                const double b = (double)(int)__builtin_amdgcn_sbfe((int)dhi[s], sh, B);
-                re[s] = __fma_rn(a, tz[s].x, -(b * tz[s].y));
-                im[s] = __fma_rn(a, tz[s].y, b * tz[s].x);
+                re[s] = (double)(int)__builtin_amdgcn_sbfe((int)dlo[s], sh, B);
+                im[s] = (double)(int)__builtin_amdgcn_sbfe((int)dhi[s], sh, B);
             }
+            vt_slot_twist(re, im, cs, false);
             vt_fwd(re, im, xr, xi, t1, t2, lane);
 #pragma unroll
             for (int s = 0; s < 8; s++) {
@@ -391,13 +400,12 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
         }
         __syncthreads();
         vt_inv(re, im, xr, xi, t1, t2, lane);
+        vt_slot_twist(re, im, cs, true);
 #pragma unroll
-        for (int s = 0; s < 8; s++) {
+        for (int s = 0; s < 8; s++) {   // the key carries the 1/M of the inverse
             const int j = 64 * s + lane;
-            const double cr = __fma_rn(re[s], tz[s].x, im[s] * tz[s].y);   // the key carries 1/M
-            const double ci = __fma_rn(im[s], tz[s].x, -(re[s] * tz[s].y));
-            mine[j] += vt_round_u32(cr);
-            mine[j + VT_M] += vt_round_u32(ci);
+            mine[j] += vt_round_u32(re[s]);
+            mine[j + VT_M] += vt_round_u32(im[s]);
         }
         vh_wave_sync();
     }
