@@ -115,3 +115,43 @@ def test_bootstrap_batch_n630_N1024_1024_samples():
         want = T.extract(T.blind_rotate(lw[k].astype(np.int64), tv.astype(np.int64), kf, p.ell, p.base_log,
                                         TORUS_BASE))
         assert np.array_equal(out[k], want), k
+
+
+@pytest.mark.parametrize("ell,base_log", [(1, 10), (2, 9), (4, 8)])
+def test_other_gadgets_bit_exact(ell, base_log):
+    """Every compiled level count (the kernel is instantiated for ell = 1..4) against the oracle,
+    with digit widths whose products keep the float sums far inside the exact-integer range.
+    One 10-bit level keeps too few bits for these messages to decrypt (its rounding noise is
+    about 2^-6 of the torus), so that case checks the bits only."""
+    p = TorusParams(n=10, ell=ell, base_log=base_log)
+    keys = TorusKeys(p, seed=20 + ell)
+    bsk = keys.bootstrap_key()
+    kf = T.key_fft(bsk)
+    boot = TorusBootstrapper(bsk, p)
+    tv = VT.pack_lut([(3 * m + 1) % 16 for m in range(16)], SIZE, PREC, p.N)
+    msgs = np.array([0, 5, 11, 15])
+    lw = keys.encrypt(msgs, PREC)
+    print(f"[torus] ell={ell} base_log={base_log}: 4 samples, n=10", flush=True)
+    acc = TorusBatch.zeros(len(msgs), 2 * p.N)
+    out = boot.bootstrap(TorusBatch.upload(lw), TorusBatch.upload(tv), TORUS_BASE, acc=acc).read()
+    got = acc.read()
+    for k in range(len(msgs)):
+        want = T.blind_rotate(lw[k].astype(np.int64), tv.astype(np.int64), kf, ell, base_log, TORUS_BASE)
+        assert np.array_equal(got[k].reshape(2, p.N), want), k
+        assert np.array_equal(out[k], T.extract(want)), k
+    if ell > 1:
+        assert np.array_equal(keys.decrypt(out, PREC), (3 * msgs + 1) % 16)
+
+
+def test_single_sample_and_other_torus_base():
+    """One sample, 3-bit messages (torus_base 8, prec 4): the rounding offset follows torus_base."""
+    p, keys, bsk, kf, boot = _setup(16, 31)
+    prec, size = 4, 8
+    tv = VT.pack_lut([7 - m for m in range(8)], size, prec, p.N)
+    lw = keys.encrypt([6], prec)
+    print("[torus] one sample, torus_base 8", flush=True)
+    acc = TorusBatch.zeros(1, 2 * p.N)
+    out = boot.bootstrap(TorusBatch.upload(lw), TorusBatch.upload(tv), 8, acc=acc).read()
+    want = T.blind_rotate(lw[0].astype(np.int64), tv.astype(np.int64), kf, p.ell, p.base_log, 8)
+    assert np.array_equal(acc.read()[0].reshape(2, p.N), want)
+    assert keys.decrypt(out, prec)[0] == 1
