@@ -527,6 +527,10 @@ RNS_MLWE *vfhe_hip_mlwe_view_array(RNS_MLWE *in, uint64_t count, uint64_t mask);
    NULL), so an output that is only passed on to other engine calls never
    costs a host block. */
 RNS_MLWE vfhe_hip_mlwe_alloc_RNS_sample_uninit(uint64_t N, uint64_t r, uint64_t mask, incNTT ntt);
+/* The same for `count` samples in one call: out[k] receives each (free each with
+   free_mlwe_RNS_sample or all with vfhe_hip_free_mlwe_samples). */
+void vfhe_hip_mlwe_alloc_RNS_samples_uninit(RNS_MLWE *out, uint64_t count, uint64_t N, uint64_t r, uint64_t mask,
+                                            incNTT ntt);
 /* The calling thread's current lane stream (a hipStream_t). */
 void *vfhe_hip_current_stream(void);
 

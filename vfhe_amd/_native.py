@@ -174,6 +174,7 @@ _SIGNATURES = {
     "vfhe_hip_set_lane": (ctypes.c_int, [ctypes.c_int]),
     "vfhe_hip_autotune_ntt": (ctypes.c_long, [VP, U64, I32, ctypes.POINTER(ctypes.c_double)]),
     "vfhe_hip_probe_alu_rates": (None, [ctypes.POINTER(ctypes.c_double)]),
+    "vfhe_hip_mlwe_alloc_RNS_samples_uninit": (None, [VPP, U64, U64, U64, U64, VP]),
     "vfhe_hip_torus_batch_new": (VP, [VP, U64, U64]),
     "vfhe_hip_torus_batch_read": (I32, [VP, VP]),
     "vfhe_hip_torus_batch_free": (None, [VP]),

@@ -223,7 +223,7 @@ class Bootstrapper:
         assert lwes.n == self.bk.n
         rots, steps = self.amounts(lwes, torus_base)
         tv.to_coeff()
-        acc = Batch.alloc(chain, 0, len(lwes), form="coeff")
+        acc = Batch.alloc(chain, 0, len(lwes), form="coeff", uninit=True)   # every row is written
         lib().vfhe_hip_blind_rotate_tv_batch(_handles(acc.handles), len(acc), tv.handles[0],
                                              rots.ctypes.data_as(U64P), self.bk.table, self.bk.n,
                                              steps.ctypes.data_as(U64P), self.bk.ell, chain.special)

@@ -2031,6 +2031,14 @@ extern "C" RNS_MLWE vfhe_hip_mlwe_alloc_RNS_sample_uninit(uint64_t N, uint64_t r
     return alloc_sample_raw(N, r, mask, ntt);
 }
 
+extern "C" void vfhe_hip_mlwe_alloc_RNS_samples_uninit(RNS_MLWE *out, uint64_t count, uint64_t N, uint64_t r,
+                                                       uint64_t mask, incNTT ntt)
+{
+    VH_ENTER;
+    VH_REQUIRE(out || !count);
+    for (uint64_t k = 0; k < count; k++) out[k] = alloc_sample_raw(N, r, mask, ntt);
+}
+
 extern "C" RNS_MLWE mlwe_alloc_RNS_sample(uint64_t N, uint64_t r, uint64_t mask, incNTT ntt)
 {
     RNS_MLWE c = (RNS_MLWE)malloc(sizeof(*c));
@@ -3508,6 +3516,19 @@ extern "C" void vfhe_hip_blind_rotate_tv_batch(RNS_MLWE *acc, uint64_t count, RN
 
 // mlwe_extract_LWE (mlwe.c:40-66) for `count` samples of one ring and rank: their rows are
 // gathered on the device and read back with one copy.
+// Pinned host staging for batched read-backs (grows, kept for the process).
+static void *g_pin = nullptr;
+static size_t g_pin_bytes = 0;
+static void *pinned(size_t bytes)
+{
+    if (bytes > g_pin_bytes) {
+        if (g_pin) VH_CHECK(hipHostFree(g_pin));
+        g_pin_bytes = std::max<size_t>(bytes, 2 * g_pin_bytes);
+        VH_CHECK(hipHostMalloc(&g_pin, g_pin_bytes, hipHostMallocDefault));
+    }
+    return g_pin;
+}
+
 extern "C" void vfhe_hip_extract_lwe_batch(RNSc_MLWE *in, uint64_t count, uint64_t idx, LWE *out)
 {
     VH_ENTER;
@@ -3518,34 +3539,33 @@ extern "C" void vfhe_hip_extract_lwe_batch(RNSc_MLWE *in, uint64_t count, uint64
     VH_REQUIRE(idx < N && t->split_degree == 1);
     std::vector<int> act;
     for (uint64_t i = 0; i < t->l; i++) if (ACTIVE(mask, i)) act.push_back((int)i);
-    const uint64_t na = act.size(), rows = count * na * Cc;
-    uint64_t *buf = (uint64_t *)vh_dmalloc(std::max<uint64_t>(1, rows) * N * 8);
-    std::vector<vh_item> items;
-    items.reserve(rows);
+    const uint64_t na = act.size(), width = r * N + 1;
+    // the device forms every sample's rows (mlwe.c:40-66's layout: a'_i = a_(idx-i), then
+    // -a_(N+idx-i) mod q, b' = b_idx) into one buffer, read back with one copy
+    std::vector<const uint64_t *> src(count * na * Cc);
+    std::vector<uint64_t> qs(na);
+    for (uint64_t j = 0; j < na; j++) qs[j] = t->ntt[act[j]]->q;
     for (uint64_t k = 0; k < count; k++) {
         VH_REQUIRE(in[k] && in[k]->r == r && in[k]->a[0]->ntt == t);
         const auto ic = comps(in[k]);
         for (auto p : ic) { VH_REQUIRE(p->rns_mask == mask); dev_ready(p); }
         for (uint64_t j = 0; j < na; j++)
-            for (uint64_t c = 0; c < Cc; c++)
-                items.push_back(mk_item(drow(ic[c], act[j]), buf + ((k * na + j) * Cc + c) * N, (uint32_t)act[j]));
+            for (uint64_t c = 0; c < Cc; c++) src[(k * na + j) * Cc + c] = drow(ic[c], act[j]);
     }
-    run_eltwise(items, VPOOL(t)->d_limbs, VH_OP_COPY, N);
-    std::vector<uint64_t> host(rows * N);
-    VH_CHECK(hipMemcpyAsync(host.data(), buf, rows * N * 8, hipMemcpyDeviceToHost, S()));
+    const size_t bytes = count * na * width * 8;
+    uint64_t *buf = (uint64_t *)vh_dmalloc(bytes);
+    VH_LAUNCH(vh_launch_extract_lwe(S(), vh_stage_vec(src), vh_stage_vec(qs), buf, (uint32_t)N, (uint32_t)r,
+                                    (uint32_t)na, (uint32_t)idx, count));
+    uint64_t *host = (uint64_t *)pinned(bytes);
+    VH_CHECK(hipMemcpyAsync(host, buf, bytes, hipMemcpyDeviceToHost, S()));
     VH_CHECK(hipStreamSynchronize(S()));
     vh_dfree(buf);
     for (uint64_t k = 0; k < count; k++) {
         LWE res = lwe_alloc_sample(r * N, na, t);
         for (uint64_t j = 0; j < na; j++) {
-            const uint64_t q = t->ntt[act[j]]->q;
-            const uint64_t *base = host.data() + ((k * na + j) * Cc) * N;
-            for (uint64_t c = 0; c < r; c++) {
-                const uint64_t *a = base + c * N;
-                for (uint64_t i = 0; i <= idx; i++) res->a[j][c * N + i] = a[idx - i];
-                for (uint64_t i = idx + 1; i < N; i++) res->a[j][c * N + i] = negate_modq(a[N + idx - i], q);
-            }
-            res->b[j] = base[r * N + idx];
+            const uint64_t *row = host + (k * na + j) * width;
+            memcpy(res->a[j], row, r * N * 8);
+            res->b[j] = row[r * N];
         }
         out[k] = res;
     }

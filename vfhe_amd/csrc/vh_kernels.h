@@ -136,6 +136,11 @@ struct vt_br_args {
 hipError_t vh_launch_torus_fft(hipStream_t st, const uint32_t *in, double2 *out, const double2 *consts, uint64_t count,
                                int natural);
 hipError_t vh_launch_torus_blind_rotate(hipStream_t st, const vt_br_args &A, uint64_t count);
+// Batched sample extraction (mlwe_extract_LWE, mlwe.c:40-66, at coefficient idx): src[(k na + j)
+// (r + 1) + c] = row of limb j of component c of sample k (coefficient form), qs[j] its prime;
+// out[(k na + j) (r N + 1) + e] = the extracted sample's mask words (e < r N) and body (e = r N).
+hipError_t vh_launch_extract_lwe(hipStream_t st, const uint64_t *const *src, const uint64_t *qs, uint64_t *out,
+                                 uint32_t N, uint32_t r, uint32_t na, uint32_t idx, uint64_t count);
 // ALU rate probe (see k_alu_rate): blocks x 256 threads, 8 chains x iters operations each.
 hipError_t vh_launch_alu_rate(hipStream_t st, int kind, int iters, unsigned blocks, uint64_t *out);
 hipError_t vh_launch_ntt_pass(hipStream_t st, const vh_item *items, unsigned nitems, const vh_limb *limbs,

@@ -1980,6 +1980,44 @@ hipError_t vh_launch_blind_rotate(hipStream_t st, const vh_br_params &P, const v
 }
 
 // ---------------------------------------------------------------------------
+// Batched sample extraction: one thread per output word (grid-stride).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_extract_lwe(const uint64_t *const *__restrict__ src,
+                                                     const uint64_t *__restrict__ qs, uint64_t *__restrict__ out,
+                                                     uint32_t N, uint32_t r, uint32_t na, uint32_t idx, uint64_t total)
+{
+    const uint64_t width = (uint64_t)r * N + 1;
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t rowi = g / width, e = g - rowi * width;   // rowi = k na + j
+        const uint32_t j = (uint32_t)(rowi % na);
+        const uint64_t *const *rows = src + rowi * (r + 1);
+        uint64_t v;
+        if (e == (uint64_t)r * N) {
+            v = rows[r][idx];
+        } else {
+            const uint32_t c = (uint32_t)(e / N), i = (uint32_t)(e - (uint64_t)c * N);
+            if (i <= idx) {
+                v = rows[c][idx - i];
+            } else {
+                const uint64_t x = rows[c][N + idx - i];
+                v = x ? qs[j] - x : 0;
+            }
+        }
+        out[g] = v;
+    }
+}
+
+hipError_t vh_launch_extract_lwe(hipStream_t st, const uint64_t *const *src, const uint64_t *qs, uint64_t *out,
+                                 uint32_t N, uint32_t r, uint32_t na, uint32_t idx, uint64_t count)
+{
+    const uint64_t total = count * na * ((uint64_t)r * N + 1);
+    if (!total) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 65536);
+    VH_KL(k_extract_lwe, dim3(blocks), dim3(256), 0, st, src, qs, out, N, r, na, idx, total);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // ALU rate probes: each thread runs 8 independent chains of `iters` operations of one
 // kind (the chains keep the pipeline full), then stores a value that depends on all of
 // them so nothing is optimised away.  kind: 0 FP64 FMA, 1 32-bit multiply (low half) +

@@ -351,10 +351,17 @@ class Batch:
 
     @classmethod
     def alloc(cls, chain: Chain, lvl: int, count: int, rank: int | None = None, ring: Ring | None = None,
-              form: str = "ntt", delta: float | None = None) -> "Batch":
+              form: str = "ntt", delta: float | None = None, uninit: bool = False) -> "Batch":
+        """`count` samples on the level's ring, zeroed; uninit: not cleared (outputs written
+        before they are read), allocated in one engine call."""
         ring = ring if ring is not None else chain.levels[lvl]
         rank = rank if rank is not None else chain.rank
-        hs = [lib().mlwe_alloc_RNS_sample(ring.N, rank, ring.mask, ring.NTT) for _ in range(count)]
+        if uninit:
+            arr = (ctypes.c_void_p * max(1, count))()
+            lib().vfhe_hip_mlwe_alloc_RNS_samples_uninit(arr, count, ring.N, rank, ring.mask, ring.NTT)
+            hs = [arr[k] for k in range(count)]
+        else:
+            hs = [lib().mlwe_alloc_RNS_sample(ring.N, rank, ring.mask, ring.NTT) for _ in range(count)]
         return cls(chain, lvl, hs, form, rank, ring, delta)
 
     @classmethod
