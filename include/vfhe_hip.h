@@ -16,8 +16,10 @@
  * polynomial_sync_to_host(), and host writes into `coeffs` become visible to
  * the device after polynomial_mark_host_dirty().  Those two functions, the
  * vfhe_hip_* functions and the batched entry points have no counterpart in the
- * reference: INTEGRATION.md lists the two-line change they need on the Python
- * side (modules/arith/python/src/vfhe/arith/polynomial.py:437-469).
+ * reference; its Python side needs the two residency calls around its direct
+ * reads and writes of `coeffs` (modules/arith/python/src/vfhe/arith/polynomial.py:
+ * 437-469): polynomial_sync_to_host before reading, polynomial_mark_host_dirty
+ * after writing.
  *
  * Standalone single-limb kernels (ntt_forward, ntt_reverse, mod_eltwise_*)
  * accept host pointers (staged through the device, synchronous, as the
