@@ -426,9 +426,10 @@ def main():
 
 def bench_ckks(lib, _native, params, rt, args):
     """CKKS multiply + relinearize + rescale, N=2^16, L=24 level limbs + 4 special limbs, r=1, batch of
-    32 pairs: the eager batched entry point and its graph-replayed form, p50/p99 over --ckks-steps
-    batches, parity of two products against the reference kernels' composition, and the same
-    composition timed on the host's CPUs."""
+    32 pairs: the eager batched entry point and its graph-replayed form; throughput with --ckks-steps
+    batches issued back to back, and p50/p99 latency with each batch waited for; parity of two
+    products against the reference kernels' composition, and the same composition timed on the
+    host's CPUs."""
     K, B = 4, args.ckks_batch
     primes = params.gen_primes(N, [50] * (L + K))
     pool = rt.Pool(N, primes)
@@ -465,6 +466,18 @@ def bench_ckks(lib, _native, params, rt, args):
             enq.append(t1 - t0)
         return np.array(lat) * 1e3, np.array(enq) * 1e3
 
+    def pipelined(fn, steps):
+        """mults/s with the batches issued back to back on the lane (one wait at the end): the
+        host's enqueue of batch i+1 overlaps batch i on the device, as in a serving loop."""
+        lib.vfhe_hip_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for c in outs:
+                pool.set_sample_mask(c, ext)
+            fn(ao, a1, a2, B, rlk, top)
+        lib.vfhe_hip_synchronize()
+        return B * steps / (time.perf_counter() - t0)
+
     # the multiply's schedule (column-pass width, inverse-pass grid, pass-through fold), timed on
     # these operands by the engine's tuner (every candidate is bit-exact)
     tuned = None
@@ -479,6 +492,7 @@ def bench_ckks(lib, _native, params, rt, args):
     run(lib.vfhe_hip_mlwe_multiply_batch, 2)
     with Prof(lib) as prof:
         lat, enq = run(lib.vfhe_hip_mlwe_multiply_batch, args.ckks_steps)
+    piped = pipelined(lib.vfhe_hip_mlwe_multiply_batch, args.ckks_steps)
     # parity of the eager result: two products against the reference kernels (oracle/_ref)
     parity = None
     cpu = None
@@ -512,6 +526,7 @@ def bench_ckks(lib, _native, params, rt, args):
     g0 = {k: int(lib.vfhe_hip_get_option(f"stat.graph_{k}".encode())) for k in ("captures", "replays", "fallbacks")}
     run(lib.vfhe_hip_mlwe_multiply_batch_graph, 2)
     glat, genq = run(lib.vfhe_hip_mlwe_multiply_batch_graph, args.ckks_steps)
+    gpiped = pipelined(lib.vfhe_hip_mlwe_multiply_batch_graph, args.ckks_steps)
     g1 = {k: int(lib.vfhe_hip_get_option(f"stat.graph_{k}".encode())) - g0[k] for k in g0}
     ggot = [pool.read_sample(outs[k]) for k in range(len(keep))]
     graph_same = all(np.array_equal(ggot[k], got[k]) for k in range(len(keep)))
@@ -527,14 +542,16 @@ def bench_ckks(lib, _native, params, rt, args):
     names = ["ntt_fp64", "ntt_int", "eltwise", "division", "keyswitch_mac", "other"]
     res = {
         "unit": "mults/s",
-        "value": round(B / float(np.median(lat) / 1e3), 2),
+        "value": round(piped, 2),
+        "value_one_batch_at_a_time": round(B / float(np.median(lat) / 1e3), 2),
         "batch": B,
         "batches_timed": int(args.ckks_steps),
         "batch_ms_p50": round(float(np.percentile(lat, 50)), 3),
         "batch_ms_p99": round(float(np.percentile(lat, 99)), 3),
         "request_latency_p99_ms": round(float(np.percentile(lat, 99)), 3),
         "host_enqueue_ms_p50": round(float(np.percentile(enq, 50)), 3),
-        "graph": {"value": round(B / float(np.median(glat) / 1e3), 2), "batch_ms_p50": round(float(np.percentile(glat, 50)), 3),
+        "graph": {"value": round(gpiped, 2), "value_one_batch_at_a_time": round(B / float(np.median(glat) / 1e3), 2),
+                  "batch_ms_p50": round(float(np.percentile(glat, 50)), 3),
                   "batch_ms_p99": round(float(np.percentile(glat, 99)), 3),
                   "host_enqueue_ms_p50": round(float(np.percentile(genq, 50)), 3), "counters": g1,
                   "equals_eager": bool(graph_same)},
