@@ -4529,7 +4529,7 @@ static double2 *g_torus_consts = nullptr;
 static const double2 *torus_consts()
 {
     if (g_torus_consts) return g_torus_consts;
-    std::vector<double2> h(584);
+    std::vector<double2> h(576);
     const long double pi = 3.141592653589793238462643383279502884L;
     auto root = [&](long double num, long double den) {
         const long double a = pi * num / den;
@@ -4539,7 +4539,6 @@ static const double2 *torus_consts()
         for (int r = 0; r < 64; r++) h[k * 64 + r] = root((long double)r - 4.0L * ((r * k) % 512), 1024);
     for (int a = 0; a < 8; a++)
         for (int s = 0; s < 8; s++) h[512 + a * 8 + s] = root(-2.0L * ((8 * a * s) % 512), 512);
-    for (int s = 0; s < 8; s++) h[576 + s] = root(64.0L * s, 1024);
     double2 *d = nullptr;
     VH_CHECK(hipMalloc(&d, h.size() * sizeof(double2)));
     VH_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));

@@ -118,10 +118,10 @@ struct vh_br_params {
 hipError_t vh_launch_blind_rotate(hipStream_t st, const vh_br_params &P, const vh_limb *limbs, uint64_t *const *accp,
                                   const uint64_t *const *bkp, const uint64_t *amounts, unsigned count,
                                   const uint64_t *const *tvp = nullptr, const uint64_t *rot = nullptr);
-// CGGI over the 32-bit torus (vh_torus.hip).  consts: 584 double2 -- [0, 512) pass 1's
+// CGGI over the 32-bit torus (vh_torus.hip).  consts: 576 double2 -- [0, 512) pass 1's
 // twiddles with the lane's twist factor, zeta^r w^(r k) at k 64 + r (r < 64, k < 8),
-// [512, 576) the pass-2 twiddles w^(8 a s) at a 8 + s (a, s < 8), [576, 584) the slot
-// factors zeta^(64 s); w = e^(-2 pi i / 512), zeta = e^(i pi / 1024).
+// [512, 576) the pass-2 twiddles w^(8 a s) at a 8 + s (a, s < 8); w = e^(-2 pi i / 512),
+// zeta = e^(i pi / 1024).  The slot factors zeta^(64 s) are constants in the kernels.
 struct vt_br_args {
     const uint32_t *lwe;      // count x (n + 1): a_0 .. a_(n-1), b
     const uint32_t *tv;       // N test-vector coefficients

@@ -209,18 +209,29 @@ __device__ __forceinline__ void vt_inv(double *re, double *im, double *xr, doubl
     vt_dft8<1>(re, im);
 }
 
-// The slot factors c_s = zeta^(64 s) (uniform): applied to the forward input, and
-// conjugated to the inverse output.
-__device__ __forceinline__ void vt_slot_twist(double *re, double *im, const double2 *c, bool conj)
+// The slot factors c_s = zeta^(64 s) = e^(i pi s / 16), correctly rounded (uniform
+// constants, no loads): applied to the forward input, conjugated to the inverse output.
+__device__ constexpr double VT_CS[8][2] = {
+    {1.0, 0.0},
+    {0x1.f6297cff75cb0p-1, 0x1.8f8b83c69a60bp-3},
+    {0x1.d906bcf328d46p-1, 0x1.87de2a6aea963p-2},
+    {0x1.a9b66290ea1a3p-1, 0x1.1c73b39ae68c8p-1},
+    {0x1.6a09e667f3bcdp-1, 0x1.6a09e667f3bcdp-1},
+    {0x1.1c73b39ae68c8p-1, 0x1.a9b66290ea1a3p-1},
+    {0x1.87de2a6aea963p-2, 0x1.d906bcf328d46p-1},
+    {0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1},
+};
+
+__device__ __forceinline__ void vt_slot_twist(double *re, double *im, bool conj)
 {
 #pragma unroll
-    for (int s = 1; s < 8; s++) vt_cmul(re[s], im[s], c[s].x, conj ? -c[s].y : c[s].y);
+    for (int s = 1; s < 8; s++) vt_cmul(re[s], im[s], VT_CS[s][0], conj ? -VT_CS[s][1] : VT_CS[s][1]);
 }
 
 // Layout A input of a polynomial given by its coefficients as signed 32-bit values:
 // lane r, slot s holds (p_j + i p_(j+M)) c_s, j = 64 s + r (the lane's twist factor
 // comes with pass 1).
-__device__ __forceinline__ void vt_fold(double *re, double *im, const uint32_t *p, const double2 *c, int lane)
+__device__ __forceinline__ void vt_fold(double *re, double *im, const uint32_t *p, int lane)
 {
 #pragma unroll
     for (int s = 0; s < 8; s++) {
@@ -228,7 +239,7 @@ __device__ __forceinline__ void vt_fold(double *re, double *im, const uint32_t *
         re[s] = (double)(int32_t)p[j];
         im[s] = (double)(int32_t)p[j + VT_M];
     }
-    vt_slot_twist(re, im, c, false);
+    vt_slot_twist(re, im, false);
 }
 
 // x mod 2^32 of an integer-valued double with |x| < 2^51 (1.5 * 2^52 puts the
@@ -274,7 +285,7 @@ __global__ void __launch_bounds__(256) k_torus_fft(const uint32_t *__restrict__ 
     const uint64_t poly = (uint64_t)blockIdx.x * 4 + w;
     if (poly >= count) return;   // whole waves leave; the exchanges are wave-private
     double re[8], im[8];
-    vt_fold(re, im, in + poly * VT_N, consts + 576, lane);   // c_s
+    vt_fold(re, im, in + poly * VT_N, lane);
     vt_fwd(re, im, lds[w][0], lds[w][1], t1, t2, lane);
     double2 *o = out + poly * VT_M;
     const double scale = natural ? 1.0 : 1.0 / VT_M;
@@ -320,7 +331,6 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
     double *xr = lds[w][0], *xi = lds[w][1];
 
     vt_fill_tw(t1, t2, A.consts, threadIdx.x, 128);
-    const double2 *cs = A.consts + 576;   // the slot factors c_s (uniform loads)
 
     // acc = (0, X^-rot_b * tv): rot_b from b + floor(2^32 / (4 torus_base)) (cggi16.py:60-71)
     const uint32_t rot_b = vt_mod_switch((uint64_t)in[A.n] + A.offset, shift, two_n);
@@ -369,7 +379,7 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
                 re[s] = (double)(int)__builtin_amdgcn_sbfe((int)dlo[s], sh, B);
                 im[s] = (double)(int)__builtin_amdgcn_sbfe((int)dhi[s], sh, B);
             }
-            vt_slot_twist(re, im, cs, false);
+            vt_slot_twist(re, im, false);
             vt_fwd(re, im, xr, xi, t1, t2, lane);
 #pragma unroll
             for (int s = 0; s < 8; s++) {
@@ -400,7 +410,7 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
         }
         __syncthreads();
         vt_inv(re, im, xr, xi, t1, t2, lane);
-        vt_slot_twist(re, im, cs, true);
+        vt_slot_twist(re, im, true);
 #pragma unroll
         for (int s = 0; s < 8; s++) {   // the key carries the 1/M of the inverse
             const int j = 64 * s + lane;
