@@ -155,3 +155,22 @@ def test_single_sample_and_other_torus_base():
     want = T.blind_rotate(lw[0].astype(np.int64), tv.astype(np.int64), kf, p.ell, p.base_log, 8)
     assert np.array_equal(acc.read()[0].reshape(2, p.N), want)
     assert keys.decrypt(out, prec)[0] == 1
+
+
+def test_batch_larger_than_one_residency_round():
+    """2,500 samples (more workgroups than the device holds at once) into one reused output batch:
+    every output decrypts, spot samples equal the oracle, and a second call overwrites the first."""
+    p, keys, bsk, kf, boot = _setup(6, 41)
+    rng = np.random.default_rng(42)
+    tv = TorusBatch.upload(VT.pack_lut(list(range(16)), SIZE, PREC, p.N))
+    out = TorusBatch.zeros(2500, p.N + 1)
+    for rep in range(2):
+        msgs = rng.integers(0, 16, size=2500)
+        lw = keys.encrypt(msgs, PREC)
+        print(f"[torus] 2500 samples, call {rep}", flush=True)
+        got = boot.bootstrap(TorusBatch.upload(lw), tv, TORUS_BASE, out=out).read()
+        assert np.array_equal(keys.decrypt(got, PREC), msgs)
+        tvh = VT.pack_lut(list(range(16)), SIZE, PREC, p.N).astype(np.int64)
+        for k in (0, 1023, 1024, 2499):
+            want = T.extract(T.blind_rotate(lw[k].astype(np.int64), tvh, kf, p.ell, p.base_log, TORUS_BASE))
+            assert np.array_equal(got[k], want), (rep, k)
