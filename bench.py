@@ -642,15 +642,15 @@ def bench_cggi(lib, args):
 
 def torus_flops_per_index(p) -> int:
     """FP64 operations of one sample at one key index in k_torus_blind_rotate, counted from
-    its code: per transform of M = N/2 points three DFT-8 passes (56 per 8 points) and two
-    twiddle passes (7 complex products of 6 per 8 points); per digit polynomial the twist
-    (6 per point); per key row and component one complex multiply-add (8 per point); the
-    partial swap (2 per point per component); per output component the untwist and scale
-    (8 per point)."""
+    its code (an FMA counts two; the pass-1 twiddles are formed once per kernel, outside the
+    loop).  Per transform, per 8 points (one lane): three DFT-8 (56 each), the seven slot
+    factors, pass 1's eight and pass 2's seven twiddle products (complex products of 6): 300.
+    Per key row and output component one complex multiply-add per point (8); per output
+    component the partial swap's complex add and the rounding adds (2 + 2 per point)."""
     M = p.N // 2
-    fft = M // 8 * (3 * 56 + 2 * 42)
+    fft = M // 8 * (3 * 56 + (7 + 8 + 7) * 6)
     rows = 2 * p.ell
-    return rows * (6 * M + fft) + rows * 2 * 8 * M + 2 * 2 * M + 2 * (fft + 8 * M)
+    return (rows + 2) * fft + rows * 2 * 8 * M + 2 * (2 + 2) * M
 
 
 def bench_cggi_torus(lib, args):

@@ -122,15 +122,20 @@ __device__ __forceinline__ void vt_fill_tw(double2 *t1, double2 *t2, const doubl
 }
 
 // Forward transform: layout A in, layout C out.
-__device__ __forceinline__ void vt_fwd(double *re, double *im, double *xr, double *xi, const double2 *t1,
+// Pass 1's twiddles zeta^r w^(r k) are formed by powers of the lane's w^r from zeta^r
+// (z = {zeta^r, w^r}): seven products instead of eight LDS reads of 16 bytes.
+__device__ __forceinline__ void vt_fwd(double *re, double *im, double *xr, double *xi, const double2 (&z)[2],
                                        const double2 *t2, int lane)
 {
     const int hi = lane >> 3, lo = lane & 7;
     vt_dft8<-1>(re, im);
+    {
+        double cr = z[0].x, ci = z[0].y;
 #pragma unroll
-    for (int s = 0; s < 8; s++) {
-        const double2 t = t1[s * 64 + lane];
-        vt_cmul(re[s], im[s], t.x, t.y);
+        for (int s = 0; s < 8; s++) {
+            vt_cmul(re[s], im[s], cr, ci);
+            if (s < 7) vt_cmul(cr, ci, z[1].x, z[1].y);
+        }
     }
 #pragma unroll
     for (int s = 0; s < 8; s++) {
@@ -166,7 +171,7 @@ __device__ __forceinline__ void vt_fwd(double *re, double *im, double *xr, doubl
 }
 
 // Inverse transform (unscaled): layout C in, layout A out.
-__device__ __forceinline__ void vt_inv(double *re, double *im, double *xr, double *xi, const double2 *t1,
+__device__ __forceinline__ void vt_inv(double *re, double *im, double *xr, double *xi, const double2 (&z)[2],
                                        const double2 *t2, int lane)
 {
     const int hi = lane >> 3, lo = lane & 7;
@@ -201,12 +206,25 @@ __device__ __forceinline__ void vt_inv(double *re, double *im, double *xr, doubl
         im[s] = xi[VT_P1 * s + lane];
     }
     vh_wave_sync();
+    {
+        double cr = z[0].x, ci = z[0].y;
 #pragma unroll
-    for (int s = 0; s < 8; s++) {
-        const double2 t = t1[s * 64 + lane];
-        vt_cmul(re[s], im[s], t.x, -t.y);
+        for (int s = 0; s < 8; s++) {
+            vt_cmul(re[s], im[s], cr, -ci);
+            if (s < 7) vt_cmul(cr, ci, z[1].x, z[1].y);
+        }
     }
     vt_dft8<1>(re, im);
+}
+
+// {zeta^r, w^r} of lane r from the pass-1 table (w^r = t1[64 + r] conj(t1[r]), |zeta^r| = 1).
+__device__ __forceinline__ void vt_lane_roots(double2 (&z)[2], const double2 *t1, int lane)
+{
+    const double2 a = t1[lane], b = t1[64 + lane];
+    z[0] = a;
+    double r = b.x, i = b.y;
+    vt_cmul(r, i, a.x, -a.y);
+    z[1] = make_double2(r, i);
 }
 
 // The slot factors c_s = zeta^(64 s) = e^(i pi s / 16), correctly rounded (uniform
@@ -284,9 +302,11 @@ __global__ void __launch_bounds__(256) k_torus_fft(const uint32_t *__restrict__ 
     __syncthreads();
     const uint64_t poly = (uint64_t)blockIdx.x * 4 + w;
     if (poly >= count) return;   // whole waves leave; the exchanges are wave-private
+    double2 z[2];
+    vt_lane_roots(z, t1, lane);
     double re[8], im[8];
     vt_fold(re, im, in + poly * VT_N, lane);
-    vt_fwd(re, im, lds[w][0], lds[w][1], t1, t2, lane);
+    vt_fwd(re, im, lds[w][0], lds[w][1], z, t2, lane);
     double2 *o = out + poly * VT_M;
     const double scale = natural ? 1.0 : 1.0 / VT_M;
 #pragma unroll
@@ -338,6 +358,8 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
     uint32_t *mine = acc[w];
     for (int j = lane; j < VT_N; j += 64) mine[j] = w == 0 ? 0u : vt_rot(A.tv, j, rot_tv);
     __syncthreads();
+    double2 z[2];
+    vt_lane_roots(z, t1, lane);
 
     // Digits: with y = x + dec_off, field_l = (y >> sh_l) mod 2^B and d_l = field_l - 2^(B-1),
     // which is the B-bit sign extension of field_l ^ 2^(B-1): one xor per coefficient, then
@@ -380,7 +402,7 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
                 im[s] = (double)(int)__builtin_amdgcn_sbfe((int)dhi[s], sh, B);
             }
             vt_slot_twist(re, im, false);
-            vt_fwd(re, im, xr, xi, t1, t2, lane);
+            vt_fwd(re, im, xr, xi, z, t2, lane);
 #pragma unroll
             for (int s = 0; s < 8; s++) {
                 const double2 ko = kor[s], kx = kxr[s];
@@ -409,7 +431,7 @@ __global__ void __launch_bounds__(128, 2) k_torus_blind_rotate(vt_br_args A)
             im[s] = poi[s] + yi[s * 64 + lane];
         }
         __syncthreads();
-        vt_inv(re, im, xr, xi, t1, t2, lane);
+        vt_inv(re, im, xr, xi, z, t2, lane);
         vt_slot_twist(re, im, true);
 #pragma unroll
         for (int s = 0; s < 8; s++) {   // the key carries the 1/M of the inverse
