@@ -29,6 +29,9 @@ the bootstrap by decryption of every LUT entry.
 """
 from __future__ import annotations
 
+import ctypes
+import os
+
 import numpy as np
 
 Q = 1 << 32
@@ -181,3 +184,59 @@ def phase(sample: np.ndarray, key: np.ndarray) -> int:
     """b - <a, s> mod 2^32 of one LWE sample."""
     a = np.asarray(sample[:-1], dtype=np.int64)
     return int((int(sample[-1]) - int((a * np.asarray(key, dtype=np.int64)).sum())) % Q)
+
+
+_port = None
+
+
+def _port_lib():
+    global _port
+    if _port is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liboracle_torus.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle_torus.so is not built (run `make -C oracle`)")
+        _port = ctypes.CDLL(path)
+        VP, U64, I = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
+        _port.torus_port_new.restype = VP
+        _port.torus_port_new.argtypes = [VP, I, I, I, I]
+        _port.torus_port_free.restype = None
+        _port.torus_port_free.argtypes = [VP]
+        _port.torus_port_run.restype = I
+        _port.torus_port_run.argtypes = [VP, VP, U64, VP, U64, VP, I]
+    return _port
+
+
+class PortKey:
+    """The key rows transformed once by torus_port.c (the C form of this bootstrap)."""
+
+    def __init__(self, bsk: np.ndarray, ell: int, base_log: int):
+        key = np.ascontiguousarray(np.asarray(bsk, dtype=np.int64) & (Q - 1), dtype=np.uint32)
+        self.n, self.N = key.shape[0], key.shape[-1]
+        assert key.shape == (self.n, 2 * ell, 2, self.N)
+        self.h = _port_lib().torus_port_new(key.ctypes.data, self.n, self.N, ell, base_log)
+        assert self.h
+
+    def bootstrap(self, lwe: np.ndarray, tv: np.ndarray, torus_base: int, threads: int = 1) -> np.ndarray:
+        """Extracted samples (count, N + 1) of every sample (count, n + 1), on `threads` threads."""
+        lw = np.ascontiguousarray(np.asarray(lwe, dtype=np.int64) & (Q - 1), dtype=np.uint32)
+        if lw.ndim == 1:
+            lw = lw[None, :]
+        assert lw.shape[1] == self.n + 1
+        tvu = np.ascontiguousarray(np.asarray(tv, dtype=np.int64) & (Q - 1), dtype=np.uint32)
+        out = np.empty((lw.shape[0], self.N + 1), dtype=np.uint32)
+        rc = _port_lib().torus_port_run(self.h, lw.ctypes.data, lw.shape[0], tvu.ctypes.data, torus_base,
+                                        out.ctypes.data, threads)
+        assert rc == 0, rc
+        return out
+
+    def __del__(self):
+        try:
+            _port_lib().torus_port_free(self.h)
+        except Exception:
+            pass
+
+
+def port_bootstrap(lwe: np.ndarray, tv: np.ndarray, bsk: np.ndarray, ell: int, base_log: int, torus_base: int,
+                   threads: int = 1) -> np.ndarray:
+    """PortKey(bsk).bootstrap(...) in one call."""
+    return PortKey(bsk, ell, base_log).bootstrap(lwe, tv, torus_base, threads)

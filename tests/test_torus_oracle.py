@@ -137,3 +137,18 @@ def test_oracle_bootstrap_decrypts_every_lut_entry():
         ex = T.extract(T.blind_rotate(lw[k].astype(np.int64), tv, kf, p.ell, p.base_log, 1 << (prec - 1)))
         ph = T.phase(ex, keys.glwe)
         assert ((ph * (1 << prec) + (1 << 31)) >> 32) % (1 << prec) == lut[m], m
+
+
+def test_c_port_equals_numpy_oracle():
+    """oracle/torus_port.c (the CPU baseline) computes the same bootstrap bit for bit, threaded."""
+    p = TorusParams(n=24)
+    keys = TorusKeys(p, seed=51)
+    bsk = keys.bootstrap_key()
+    kf = T.key_fft(bsk)
+    tv = T.pack_lut([(5 * m + 2) % 16 for m in range(16)], 16, 5, p.N)
+    lw = keys.encrypt(list(range(16)), 5)
+    lw[3, :-1] = 1 << 20                          # exact halves in every amount
+    got = T.port_bootstrap(lw, tv, bsk, p.ell, p.base_log, 16, threads=4)
+    for k in range(len(lw)):
+        want = T.extract(T.blind_rotate(lw[k].astype(np.int64), tv, kf, p.ell, p.base_log, 16))
+        assert np.array_equal(got[k].astype(np.int64), want), k
