@@ -227,6 +227,32 @@ def test_fused_blind_rotation_matches_steps_and_reference_sequence(setup1024):
     assert [mod_switch(ph, q, 1 << MSG_PREC) for ph in phases] == [s["lut"][v] for v in msgs]
 
 
+def test_blind_rotations_queued_back_to_back(setup1024):
+    """Two fused blind rotations queued without a wait between them (precomputed amounts, given
+    accumulators, the second batch's host work done while the first runs) equal the same
+    rotations run one at a time, bit for bit."""
+    s = setup1024
+    lib = _native.load()
+    tb = 1 << (MSG_PREC - 1)
+    boot = s["boot"]
+    a_msgs, b_msgs = [1, 6, 11], [2, 15, 7, 4]
+    la, lb = encrypt_msgs(s, a_msgs), encrypt_msgs(s, b_msgs)
+    print("[cggi] two blind rotations queued back to back", flush=True)
+    acc_a = boot.alloc_accumulators(len(a_msgs))
+    boot.blind_rotate(s["tv"], la, tb, amounts=boot.amounts(la, tb), acc=acc_a)
+    acc_b = boot.blind_rotate(s["tv"], lb, tb, amounts=boot.amounts(lb, tb))
+    lib.vfhe_hip_synchronize()
+    one_a = boot.blind_rotate(s["tv"], la, tb)
+    one_b = boot.blind_rotate(s["tv"], lb, tb)
+    for got, want, n in ((acc_a, one_a, len(a_msgs)), (acc_b, one_b, len(b_msgs))):
+        for k in range(n):
+            for p, r in zip(got.polys(k), want.polys(k)):
+                assert p == r, k
+    q = s["in_ring"].q_l
+    phases = extract(acc_b, 0).phases(s["out_lwe"])
+    assert [mod_switch(ph, q, 1 << MSG_PREC) for ph in phases] == [s["lut"][v] for v in b_msgs]
+
+
 def test_bootstrap_batch_shape_n630_N1024():
     """The bootstrap benchmark's shape: LWE n = 630 (binary key), GLWE N = 1024 (rank 1, two
     50-bit level primes + one special), 1024 LWE ciphertexts of uniform 4-bit messages, identity

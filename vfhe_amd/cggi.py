@@ -216,18 +216,26 @@ class Bootstrapper:
     def __init__(self, chain: Chain, bk: BootstrapKey):
         self.chain, self.bk = chain, bk
 
-    def blind_rotate(self, tv: Batch, lwes: LweBatch, torus_base: int) -> Batch:
+    def blind_rotate(self, tv: Batch, lwes: LweBatch, torus_base: int, amounts=None, acc: Batch | None = None) -> Batch:
         """The accumulators X^(phase-dependent) * tv of every input (cggi16.py:46-90), coefficient
-        form on the level-0 ring."""
+        form on the level-0 ring.  amounts: (rot_tv, steps) from `amounts` when already computed;
+        acc: an output batch of len(lwes) samples (every row is written).  The work is queued on
+        the device; the call does not wait for it (at N = 1024)."""
         chain, N = self.chain, self.chain.N
         assert lwes.n == self.bk.n
-        rots, steps = self.amounts(lwes, torus_base)
+        rots, steps = amounts if amounts is not None else self.amounts(lwes, torus_base)
         tv.to_coeff()
-        acc = Batch.alloc(chain, 0, len(lwes), form="coeff", uninit=True)   # every row is written
+        if acc is None:
+            acc = self.alloc_accumulators(len(lwes))
+        assert len(acc) == len(lwes)
         lib().vfhe_hip_blind_rotate_tv_batch(_handles(acc.handles), len(acc), tv.handles[0],
                                              rots.ctypes.data_as(U64P), self.bk.table, self.bk.n,
                                              steps.ctypes.data_as(U64P), self.bk.ell, chain.special)
         return acc
+
+    def alloc_accumulators(self, count: int) -> Batch:
+        """`count` uninitialised accumulators on the level-0 ring, coefficient form."""
+        return Batch.alloc(self.chain, 0, count, form="coeff", uninit=True)
 
     def amounts(self, lwes: LweBatch, torus_base: int):
         """(rot_tv per input, steps as count x n) of cggi16.py:54-86, computed by the engine's

@@ -3314,16 +3314,24 @@ static void blind_rotate_core(RNS_MLWE *acc, uint64_t count, RNS_MLWE tv, const 
         }
         std::vector<uint64_t> amounts(std::max<uint64_t>(1, count * n));
         for (uint64_t k = 0; k < count * n; k++) amounts[k] = steps[k] & ((N << 1) - 1);
+        // The tables go through the lane's pinned staging ring (copied in stream order, the
+        // ring's reuse ordered by its events), so the call returns once the launch is queued;
+        // tables too large for the ring are copied into blocks of their own and the call waits
+        // for the launch before their host copies go out of scope.
+        const size_t table_bytes = (accp.size() + bkp.size()) * sizeof(void *) + amounts.size() * 8 +
+                                   (tv ? (Cc * J.size() + count) * 8 : 0);
+        const bool staged = table_bytes + 4096 <= LN().ring_size / 4;
         std::vector<void *> held;
-        auto persist = [&](const void *src, size_t bytes) -> void * {
+        auto put = [&](const void *src, size_t bytes) -> void * {
+            if (staged) return vh_stage(src, bytes);
             void *d = vh_dmalloc(bytes ? bytes : 8);
             if (bytes) VH_CHECK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, S()));
             held.push_back(d);
             return d;
         };
-        auto *d_acc = (uint64_t *const *)persist(accp.data(), accp.size() * sizeof(uint64_t *));
-        auto *d_bk = (const uint64_t *const *)persist(bkp.data(), bkp.size() * sizeof(uint64_t *));
-        auto *d_am = (const uint64_t *)persist(amounts.data(), amounts.size() * 8);
+        auto *d_acc = (uint64_t *const *)put(accp.data(), accp.size() * sizeof(uint64_t *));
+        auto *d_bk = (const uint64_t *const *)put(bkp.data(), bkp.size() * sizeof(uint64_t *));
+        auto *d_am = (const uint64_t *)put(amounts.data(), amounts.size() * 8);
         const uint64_t *const *d_tv = nullptr;
         const uint64_t *d_rot = nullptr;
         std::vector<const uint64_t *> tvp;
@@ -3333,16 +3341,18 @@ static void blind_rotate_core(RNS_MLWE *acc, uint64_t count, RNS_MLWE tv, const 
             for (uint64_t c = 0; c < Cc; c++)
                 for (size_t jj = 0; jj < J.size(); jj++) tvp.push_back(drow(tc[c], J[jj]));
             for (uint64_t k = 0; k < count; k++) rot.push_back(rot_tv[k] & ((N << 1) - 1));
-            d_tv = (const uint64_t *const *)persist(tvp.data(), tvp.size() * sizeof(uint64_t *));
-            d_rot = (const uint64_t *)persist(rot.data(), rot.size() * 8);
+            d_tv = (const uint64_t *const *)put(tvp.data(), tvp.size() * sizeof(uint64_t *));
+            d_rot = (const uint64_t *)put(rot.data(), rot.size() * 8);
         }
         {
             vh_prof_scope ps_(VH_PROF_KSMAC);
             VH_LAUNCH(vh_launch_blind_rotate(S(), bp, P->d_limbs, d_acc, d_bk, d_am, (unsigned)count, d_tv, d_rot));
         }
         g_br_fused_launches++;
-        VH_CHECK(hipStreamSynchronize(S()));   // the host lists go out of scope
-        for (void *d : held) vh_dfree(d);
+        if (!held.empty()) {
+            VH_CHECK(hipStreamSynchronize(S()));   // the host tables go out of scope
+            for (void *d : held) vh_dfree(d);
+        }
         return;
     }
     if (tv) {   // acc[k] = X^rot_tv[k] * tv, one launch (the exponent table holds rot_tv)
